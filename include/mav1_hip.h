@@ -1,0 +1,164 @@
+/* mav1 HIP analysis backend: C ABI.
+ *
+ * libmav1hip.so runs the mav1 encoder's analysis path (everything behind
+ * the encoder's `Backend` vtable, src/backend.h:10-24) on one MI355X GPU
+ * (gfx950). It takes plain host pointers and sizes; the records are the
+ * layout-identical structs of mav1_types.h. A mav1 build binds it through
+ * host/backend_hip.c (a `Backend` whose slots call these functions).
+ *
+ * Two ways to drive a context:
+ *  - stage calls with host buffers (mav1hip_motion_search, ...): inputs are
+ *    copied to the device, the stage runs, outputs are copied back and the
+ *    call returns when they are in the host buffers; the semantics of the
+ *    matching Backend slot;
+ *  - the device-resident form (mav1hip_upload / mav1hip_launch_* /
+ *    mav1hip_download / mav1hip_sync): stages are enqueued on the context's
+ *    HIP stream against device buffers and nothing crosses PCIe until asked.
+ *
+ * Return values: 0 on success, a negative MAV1HIP_E* code on failure, with a
+ * message in mav1hip_error(ctx). A context serves one caller at a time; use
+ * one context per concurrent caller (the encoder's frame engines each own a
+ * Backend, src/encoder.c:196-208).
+ */
+#ifndef MAV1_HIP_H
+#define MAV1_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "mav1_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MAV1HIP_OK 0
+#define MAV1HIP_EINVAL (-1)    /* bad argument or size */
+#define MAV1HIP_EDEVICE (-2)   /* HIP runtime / device error */
+#define MAV1HIP_ENOMEM (-3)    /* device allocation failed */
+#define MAV1HIP_EUNSUP (-4)    /* configuration this build does not run on the GPU */
+
+/* ------------------------------------------------------------------------
+ * Encoder configuration and per-frame parameters (host only; no device).
+ * ------------------------------------------------------------------------ */
+
+/* The encoder configuration (Mav1Config, src/mav1.h:20-42) plus the testing
+ * knobs the reference reads from the environment; a knob of -1 takes the
+ * reference's default. */
+typedef struct {
+  int32_t width, height;
+  int32_t qindex;
+  int32_t keyint;
+  int32_t tx_search, tx_depth, loop_filter, cdef, restoration;
+  int32_t tile_cols_log2, intra_edge_filter;
+  int32_t nonref_odd, fast, multi_ref, altref, leaf, switchable_interp;
+  int32_t me_range;
+  int32_t turbo;
+  int32_t knob_turbo_mask;  /* MAV1_TURBO (default 0x47CBF1) */
+  int32_t knob_dt;          /* MAV1_DT (default 320 when sb_rows >= 16, else 352) */
+  int32_t knob_dt_bias;     /* MAV1_DTB (default 0) */
+  int32_t knob_cdef_preset; /* MAV1_CP (default 2) */
+  int32_t knob_leafopt;     /* MAV1_LEAFOPT (default 0x3F) */
+} Mav1HipConfig;
+
+/* Frame-level state the parameter setup reads (src/encoder.c:1004-1030). */
+typedef struct {
+  int32_t is_key;
+  int32_t frame_kind;       /* 0 regular, 1 hidden ALTREF, 2 inside a mini-GOP, 3 pyramid middle */
+  int32_t nonref;           /* not kept as a reference (leaf frame) */
+  int32_t order_hint;
+  int32_t reference_select; /* compound LAST + ALTREF allowed */
+  int32_t gold_differs;     /* the GOLDEN slot holds another picture than LAST */
+} Mav1HipFrame;
+
+/* mav1_default_config (src/encoder.c:51-66) followed by the CLI's speed
+ * presets (tools/mav1enc.c:98-100): turbo implies fast and 4 tile columns,
+ * fast turns off tx search, tx depth, restoration and switchable filters. */
+void mav1hip_config_default(Mav1HipConfig *cfg, int32_t width, int32_t height, int32_t turbo);
+/* The adjustments mav1_create applies to a configuration
+ * (src/encoder.c:75-92): turbo => fast, altref >= 2, leaf; clamps. */
+void mav1hip_config_normalize(Mav1HipConfig *cfg);
+/* Frame geometry (src/encoder.c:93-120): fills mi/sb counts, tile width,
+ * strides, offsets; returns the padded frame size in bytes (0: bad size). */
+size_t mav1hip_geometry(const Mav1HipConfig *cfg, int32_t *mi_cols, int32_t *mi_rows, int32_t *sb_cols,
+                        int32_t *sb_rows, int32_t *tile_w_sb, int32_t stride[3], int32_t offset[3],
+                        int32_t plane_h[3]);
+/* Frame quantizer cascade (src/encoder.c:585-596). */
+int32_t mav1hip_frame_qindex(const Mav1HipConfig *cfg, const Mav1HipFrame *fr);
+/* setup_params (src/encoder.c:407-510) for a frame coded at base_q_idx.
+ * Returns the number of CDEF presets the filter stage evaluates. */
+int32_t mav1hip_setup_params(const Mav1HipConfig *cfg, const Mav1HipFrame *fr, int32_t base_q_idx,
+                             Mav1FrameParams *fp);
+
+/* ------------------------------------------------------------------------
+ * Context
+ * ------------------------------------------------------------------------ */
+typedef struct mav1hip_ctx mav1hip_ctx;
+
+/* Creates a context with device buffers for frames of cfg's size (the
+ * convention of backend_metal_create, src/backend.h:37: NULL and a message
+ * in err when there is no usable device). */
+mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen);
+/* Backend.destroy (src/backend.h:22). */
+void mav1hip_destroy(mav1hip_ctx *c);
+const char *mav1hip_error(const mav1hip_ctx *c);
+/* Name of the device the context runs on. */
+const char *mav1hip_device_name(const mav1hip_ctx *c);
+
+/* ------------------------------------------------------------------------
+ * Stage calls with host buffers (Backend slots, src/backend.h:10-24)
+ * ------------------------------------------------------------------------ */
+
+/* Backend.motion_search (src/backend_cpu.c:23-56): quarter-pel and
+ * decimated planes, then the 4-level search of every 64/32/16/8 node.
+ * src, ref, gold: padded frames (frame_bytes); gold is read when
+ * fp->golden. mvfield / mvgold: num_sbs * 85 * 2 int32 (mvgold written
+ * when fp->golden, may be NULL otherwise). */
+int mav1hip_motion_search(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *ref,
+                          const uint8_t *gold, int32_t *mvfield, int32_t *mvgold);
+
+/* ------------------------------------------------------------------------
+ * Device-resident form
+ * ------------------------------------------------------------------------ */
+enum {
+  MAV1HIP_BUF_SRC = 0,   /* padded source frame */
+  MAV1HIP_BUF_REF = 1,   /* LAST reference picture */
+  MAV1HIP_BUF_GOLD = 2,  /* second reference picture */
+  MAV1HIP_BUF_MVFIELD = 3,
+  MAV1HIP_BUF_MVGOLD = 4,
+  MAV1HIP_BUF_PLANES = 5 /* motion-search planes: per reference 16 quarter-pel planes + decimated
+                            plane (mav1hip_planes_entry_bytes each), LAST then second reference,
+                            then the decimated source */
+};
+/* Size in bytes of a device buffer. */
+size_t mav1hip_buffer_bytes(const mav1hip_ctx *c, int32_t buf);
+size_t mav1hip_planes_entry_bytes(const mav1hip_ctx *c);
+int mav1hip_upload(mav1hip_ctx *c, int32_t buf, const void *host, size_t bytes);
+int mav1hip_download(mav1hip_ctx *c, int32_t buf, void *host, size_t bytes);
+/* Parameters of the next launches (copied; the pointer is not kept). */
+int mav1hip_set_params(mav1hip_ctx *c, const Mav1FrameParams *fp);
+int mav1hip_launch_motion_search(mav1hip_ctx *c);
+/* Waits for everything enqueued on the context's stream. */
+int mav1hip_sync(mav1hip_ctx *c);
+
+/* A pool of device-resident frames (the benchmark keeps its whole input
+ * sequence in HBM): mav1hip_pool_alloc reserves n padded frames,
+ * mav1hip_pool_upload fills one, mav1hip_bind_frames points the SRC / REF /
+ * GOLD buffers at pool frames (-1: the context's own buffer). */
+int mav1hip_pool_alloc(mav1hip_ctx *c, int32_t n);
+int mav1hip_pool_upload(mav1hip_ctx *c, int32_t idx, const uint8_t *frame);
+int mav1hip_bind_frames(mav1hip_ctx *c, int32_t src_idx, int32_t ref_idx, int32_t gold_idx);
+
+/* Per-kernel timing with HIP events recorded on the context's stream
+ * around every launch while enabled. mav1hip_timing_get returns the number
+ * of launches of the named kernel and their total duration in ms since the
+ * last reset (synchronises the stream). */
+int mav1hip_timing_enable(mav1hip_ctx *c, int32_t on);
+int mav1hip_timing_reset(mav1hip_ctx *c);
+int mav1hip_timing_get(mav1hip_ctx *c, const char *kernel, int32_t *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,379 @@
+/* C-ABI of the HIP analysis backend (include/mav1_hip.h): context, device
+ * buffers, stage launches on the context's stream, host-buffer stage calls
+ * and per-kernel event timing. */
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mav1_hip.h"
+#include "internal.h"
+
+namespace {
+
+struct TimedLaunch {
+  const char *name;
+  hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct mav1hip_ctx {
+  Mav1HipConfig cfg;
+  int mi_cols, mi_rows, sb_cols, sb_rows, num_sbs, tile_w_sb;
+  int stride[3], offset[3], plane_h[3];
+  size_t frame_bytes;
+  int device;
+  hipStream_t stream;
+  char err[512];
+  char devname[256];
+  Mav1FrameParams fp;
+  int fp_set;
+  /* own device buffers */
+  uint8_t *d_src, *d_ref, *d_gold;
+  int32_t *d_mvfield, *d_mvgold;
+  uint8_t *d_planes;
+  long entry_bytes;
+  size_t planes_bytes;
+  /* frames the stages read (own buffers or pool frames) */
+  uint8_t *b_src, *b_ref, *b_gold;
+  uint8_t *d_pool;
+  int pool_n;
+  /* timing */
+  int timing;
+  std::vector<TimedLaunch> pending;
+  std::vector<hipEvent_t> free_events;
+  std::map<std::string, std::pair<int, double>> stats;
+};
+
+static int fail(mav1hip_ctx *c, int code, const char *fmt, ...) {
+  if (c) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(c->err, sizeof c->err, fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                                   \
+  do {                                                                                                    \
+    hipError_t e_ = (call);                                                                               \
+    if (e_ != hipSuccess) return fail(c, MAV1HIP_EDEVICE, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                                                \
+  } while (0)
+
+static hipEvent_t take_event(mav1hip_ctx *c) {
+  if (!c->free_events.empty()) {
+    hipEvent_t e = c->free_events.back();
+    c->free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+/* Records the launch of `name` between two events when timing is on. */
+template <class F>
+static int timed(mav1hip_ctx *c, const char *name, F &&launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = take_event(c);
+    b = take_event(c);
+    if (a && b) HIPCHK(c, hipEventRecord(a, c->stream));
+  }
+  hipError_t e = launch();
+  if (e != hipSuccess) return fail(c, MAV1HIP_EDEVICE, "launch of %s: %s", name, hipGetErrorString(e));
+  if (c->timing && a && b) {
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    c->pending.push_back({name, a, b});
+  }
+  return MAV1HIP_OK;
+}
+
+static int collect_timing(mav1hip_ctx *c) {
+  if (c->pending.empty()) return MAV1HIP_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &t : c->pending) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, t.start, t.stop));
+    auto &s = c->stats[t.name];
+    s.first += 1;
+    s.second += ms;
+    c->free_events.push_back(t.start);
+    c->free_events.push_back(t.stop);
+  }
+  c->pending.clear();
+  return MAV1HIP_OK;
+}
+
+static int dmalloc(mav1hip_ctx *c, void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+  if (e != hipSuccess) return fail(c, MAV1HIP_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return MAV1HIP_OK;
+}
+
+extern "C" {
+
+mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) {
+  if (!cfg) {
+    if (err && errlen) snprintf(err, errlen, "no configuration");
+    return nullptr;
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1) {
+    if (err && errlen) snprintf(err, errlen, "no HIP device available (%s)", hipGetErrorString(e));
+    return nullptr;
+  }
+  mav1hip_ctx *c = new mav1hip_ctx();
+  c->cfg = *cfg;
+  mav1hip_config_normalize(&c->cfg);
+  c->frame_bytes = mav1hip_geometry(&c->cfg, &c->mi_cols, &c->mi_rows, &c->sb_cols, &c->sb_rows, &c->tile_w_sb,
+                                    c->stride, c->offset, c->plane_h);
+  if (!c->frame_bytes) {
+    if (err && errlen) snprintf(err, errlen, "unsupported frame size %dx%d", cfg->width, cfg->height);
+    delete c;
+    return nullptr;
+  }
+  c->num_sbs = c->sb_cols * c->sb_rows;
+  int rc = MAV1HIP_OK;
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    if (err && errlen) snprintf(err, errlen, "cannot create a HIP stream");
+    delete c;
+    return nullptr;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess)
+    snprintf(c->devname, sizeof c->devname, "%s (%s)", prop.name, prop.gcnArchName);
+  long W = c->cfg.width, H = c->cfg.height;
+  /* entries start 256-byte aligned (the plane build stores 32-bit words) */
+  c->entry_bytes = (16 * W * H + ((W + 3) >> 2) * ((H + 3) >> 2) + 255) & ~255L;
+  c->planes_bytes = (size_t)c->entry_bytes * 2 + (size_t)((W + 3) >> 2) * ((H + 3) >> 2);
+  size_t mvb = sizeof(int32_t) * 2 * MAV1_NUM_NODES * (size_t)c->num_sbs;
+  rc |= dmalloc(c, (void **)&c->d_src, c->frame_bytes);
+  rc |= dmalloc(c, (void **)&c->d_ref, c->frame_bytes);
+  rc |= dmalloc(c, (void **)&c->d_gold, c->frame_bytes);
+  rc |= dmalloc(c, (void **)&c->d_mvfield, mvb);
+  rc |= dmalloc(c, (void **)&c->d_mvgold, mvb);
+  rc |= dmalloc(c, (void **)&c->d_planes, c->planes_bytes);
+  if (rc != MAV1HIP_OK) {
+    if (err && errlen) snprintf(err, errlen, "%s", c->err);
+    mav1hip_destroy(c);
+    return nullptr;
+  }
+  /* deterministic contents before the first frame (the reference's buffers
+   * are zero-filled pages) */
+  (void)hipMemsetAsync(c->d_planes, 0, c->planes_bytes, c->stream);
+  (void)hipMemsetAsync(c->d_mvfield, 0, mvb, c->stream);
+  (void)hipMemsetAsync(c->d_mvgold, 0, mvb, c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  c->b_src = c->d_src;
+  c->b_ref = c->d_ref;
+  c->b_gold = c->d_gold;
+  return c;
+}
+
+void mav1hip_destroy(mav1hip_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &t : c->pending) {
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
+  for (auto e : c->free_events) (void)hipEventDestroy(e);
+  (void)hipFree(c->d_src);
+  (void)hipFree(c->d_ref);
+  (void)hipFree(c->d_gold);
+  (void)hipFree(c->d_mvfield);
+  (void)hipFree(c->d_mvgold);
+  (void)hipFree(c->d_planes);
+  (void)hipFree(c->d_pool);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *mav1hip_error(const mav1hip_ctx *c) { return c ? c->err : "no context"; }
+const char *mav1hip_device_name(const mav1hip_ctx *c) { return c ? c->devname : ""; }
+
+static void *buf_ptr(mav1hip_ctx *c, int buf, size_t *bytes) {
+  size_t mvb = sizeof(int32_t) * 2 * MAV1_NUM_NODES * (size_t)c->num_sbs;
+  switch (buf) {
+    case MAV1HIP_BUF_SRC: *bytes = c->frame_bytes; return c->b_src;
+    case MAV1HIP_BUF_REF: *bytes = c->frame_bytes; return c->b_ref;
+    case MAV1HIP_BUF_GOLD: *bytes = c->frame_bytes; return c->b_gold;
+    case MAV1HIP_BUF_MVFIELD: *bytes = mvb; return c->d_mvfield;
+    case MAV1HIP_BUF_MVGOLD: *bytes = mvb; return c->d_mvgold;
+    case MAV1HIP_BUF_PLANES: *bytes = c->planes_bytes; return c->d_planes;
+    default: *bytes = 0; return nullptr;
+  }
+}
+
+size_t mav1hip_buffer_bytes(const mav1hip_ctx *c, int32_t buf) {
+  size_t b = 0;
+  if (c) buf_ptr(const_cast<mav1hip_ctx *>(c), buf, &b);
+  return b;
+}
+size_t mav1hip_planes_entry_bytes(const mav1hip_ctx *c) { return c ? (size_t)c->entry_bytes : 0; }
+
+int mav1hip_upload(mav1hip_ctx *c, int32_t buf, const void *host, size_t bytes) {
+  if (!c || !host) return MAV1HIP_EINVAL;
+  size_t cap;
+  void *d = buf_ptr(c, buf, &cap);
+  if (!d || bytes > cap) return fail(c, MAV1HIP_EINVAL, "upload: buffer %d holds %zu bytes, %zu given", buf, cap, bytes);
+  HIPCHK(c, hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_download(mav1hip_ctx *c, int32_t buf, void *host, size_t bytes) {
+  if (!c || !host) return MAV1HIP_EINVAL;
+  size_t cap;
+  void *d = buf_ptr(c, buf, &cap);
+  if (!d || bytes > cap) return fail(c, MAV1HIP_EINVAL, "download: buffer %d holds %zu bytes, %zu asked", buf, cap, bytes);
+  HIPCHK(c, hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+/* The parameters must describe the context's geometry: the kernels index
+ * the device buffers with them. */
+static int check_params(mav1hip_ctx *c, const Mav1FrameParams *fp) {
+  if (fp->frame_w != c->cfg.width || fp->frame_h != c->cfg.height || fp->sb_cols != c->sb_cols ||
+      fp->sb_rows != c->sb_rows || fp->mi_cols != c->mi_cols || fp->mi_rows != c->mi_rows)
+    return fail(c, MAV1HIP_EINVAL, "frame parameters (%dx%d) do not match the context (%dx%d)", fp->frame_w,
+                fp->frame_h, c->cfg.width, c->cfg.height);
+  for (int p = 0; p < 3; p++)
+    if (fp->stride[p] != c->stride[p] || fp->offset[p] != c->offset[p])
+      return fail(c, MAV1HIP_EINVAL, "frame parameters: plane %d layout does not match the context", p);
+  if (fp->me_range < 0 || fp->me_range > 256) return fail(c, MAV1HIP_EINVAL, "me_range %d out of range", fp->me_range);
+  return MAV1HIP_OK;
+}
+
+int mav1hip_set_params(mav1hip_ctx *c, const Mav1FrameParams *fp) {
+  if (!c || !fp) return MAV1HIP_EINVAL;
+  int rc = check_params(c, fp);
+  if (rc) return rc;
+  c->fp = *fp;
+  c->fp_set = 1;
+  return MAV1HIP_OK;
+}
+
+int mav1hip_launch_motion_search(mav1hip_ctx *c) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "motion search: no frame parameters set");
+  const Mav1FrameParams &fp = c->fp;
+  if (!(fp.turbo & MAV1_T_ME_HPEL))
+    return fail(c, MAV1HIP_EUNSUP, "motion search without quarter-pel planes (turbo bit 16) is not built yet");
+  int nref = fp.golden ? 2 : 1;
+  uint8_t *srcDec = c->d_planes + 2 * c->entry_bytes;
+  size_t n16 = (size_t)16 * fp.frame_w * fp.frame_h;
+  int rc;
+  for (int r = 0; r < nref; r++) {
+    const uint8_t *refp = r ? c->b_gold : c->b_ref;
+    uint8_t *ent = c->d_planes + r * c->entry_bytes;
+    if ((rc = timed(c, "qpel_build", [&] { return launch_qpel_build(c->stream, fp, refp, ent); }))) return rc;
+    if (fp.turbo & MAV1_T_ME_DEC)
+      if ((rc = timed(c, "dec_build", [&] { return launch_dec_build(c->stream, fp, refp, ent + n16); }))) return rc;
+  }
+  if (fp.turbo & MAV1_T_ME_DEC)
+    if ((rc = timed(c, "dec_build", [&] { return launch_dec_build(c->stream, fp, c->b_src, srcDec); }))) return rc;
+  static const char *names[4] = {"me_level0", "me_level1", "me_level2", "me_level3"};
+  for (int level = 0; level < 4; level++)
+    if ((rc = timed(c, names[level], [&] {
+           return launch_me_level(c->stream, level, fp, c->b_src, c->b_ref, c->b_gold, c->d_planes, c->entry_bytes,
+                                  srcDec, c->d_mvfield, c->d_mvgold, nref);
+         })))
+      return rc;
+  return MAV1HIP_OK;
+}
+
+int mav1hip_sync(mav1hip_ctx *c) {
+  if (!c) return MAV1HIP_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_motion_search(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *ref,
+                          const uint8_t *gold, int32_t *mvfield, int32_t *mvgold) {
+  if (!c || !fp || !src || !ref || !mvfield) return fail(c, MAV1HIP_EINVAL, "motion search: missing buffer");
+  if (fp->golden && (!gold || !mvgold)) return fail(c, MAV1HIP_EINVAL, "motion search: second reference missing");
+  int rc;
+  if ((rc = mav1hip_set_params(c, fp))) return rc;
+  c->b_src = c->d_src;
+  c->b_ref = c->d_ref;
+  c->b_gold = c->d_gold;
+  HIPCHK(c, hipMemcpyAsync(c->d_src, src, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_ref, ref, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  if (fp->golden) HIPCHK(c, hipMemcpyAsync(c->d_gold, gold, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  if ((rc = mav1hip_launch_motion_search(c))) return rc;
+  size_t mvb = sizeof(int32_t) * 2 * MAV1_NUM_NODES * (size_t)c->num_sbs;
+  HIPCHK(c, hipMemcpyAsync(mvfield, c->d_mvfield, mvb, hipMemcpyDeviceToHost, c->stream));
+  if (fp->golden) HIPCHK(c, hipMemcpyAsync(mvgold, c->d_mvgold, mvb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_pool_alloc(mav1hip_ctx *c, int32_t n) {
+  if (!c || n < 0) return MAV1HIP_EINVAL;
+  if (c->d_pool) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->d_pool);
+    c->d_pool = nullptr;
+    c->pool_n = 0;
+    c->b_src = c->d_src;
+    c->b_ref = c->d_ref;
+    c->b_gold = c->d_gold;
+  }
+  if (n == 0) return MAV1HIP_OK;
+  int rc = dmalloc(c, (void **)&c->d_pool, c->frame_bytes * (size_t)n);
+  if (rc) return rc;
+  c->pool_n = n;
+  return MAV1HIP_OK;
+}
+
+int mav1hip_pool_upload(mav1hip_ctx *c, int32_t idx, const uint8_t *frame) {
+  if (!c || !frame || idx < 0 || idx >= c->pool_n) return fail(c, MAV1HIP_EINVAL, "pool_upload: bad index %d", idx);
+  HIPCHK(c, hipMemcpyAsync(c->d_pool + (size_t)idx * c->frame_bytes, frame, c->frame_bytes, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_bind_frames(mav1hip_ctx *c, int32_t s, int32_t r, int32_t g) {
+  if (!c) return MAV1HIP_EINVAL;
+  if (s >= c->pool_n || r >= c->pool_n || g >= c->pool_n) return fail(c, MAV1HIP_EINVAL, "bind_frames: bad index");
+  c->b_src = s < 0 ? c->d_src : c->d_pool + (size_t)s * c->frame_bytes;
+  c->b_ref = r < 0 ? c->d_ref : c->d_pool + (size_t)r * c->frame_bytes;
+  c->b_gold = g < 0 ? c->d_gold : c->d_pool + (size_t)g * c->frame_bytes;
+  return MAV1HIP_OK;
+}
+
+int mav1hip_timing_enable(mav1hip_ctx *c, int32_t on) {
+  if (!c) return MAV1HIP_EINVAL;
+  c->timing = on != 0;
+  return MAV1HIP_OK;
+}
+
+int mav1hip_timing_reset(mav1hip_ctx *c) {
+  if (!c) return MAV1HIP_EINVAL;
+  int rc = collect_timing(c);
+  c->stats.clear();
+  return rc;
+}
+
+int mav1hip_timing_get(mav1hip_ctx *c, const char *kernel, int32_t *launches, double *total_ms) {
+  if (!c || !kernel) return MAV1HIP_EINVAL;
+  int rc = collect_timing(c);
+  if (rc) return rc;
+  auto it = c->stats.find(kernel);
+  if (launches) *launches = it == c->stats.end() ? 0 : it->second.first;
+  if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.second;
+  return MAV1HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,24 @@
+/* Launchers of the gfx950 kernels, called by the C-ABI layer (api.hip).
+ * Each enqueues its kernels on `st` and returns the launch status. */
+#ifndef MAV1_INTERNAL_H
+#define MAV1_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mav1_types.h"
+
+/* Motion-search planes of one reference: 16 quarter-pel planes
+ * (frame_w x frame_h each) followed by the 4x4-decimated plane. */
+long me_entry_bytes(const Mav1FrameParams &fp);
+size_t me_dec_bytes(const Mav1FrameParams &fp);
+
+hipError_t launch_qpel_build(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *ref, uint8_t *planes16);
+hipError_t launch_dec_build(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *frame, uint8_t *dec);
+/* One level of the node search for nref references (LAST: ref0 / field0 /
+ * entry 0; second reference: ref1 / field1 / entry 1). */
+hipError_t launch_me_level(hipStream_t st, int level, const Mav1FrameParams &fp, const uint8_t *src,
+                           const uint8_t *ref0, const uint8_t *ref1, const uint8_t *planes, long entryBytes,
+                           const uint8_t *srcDec, int32_t *field0, int32_t *field1, int nref);
+
+#endif

@@ -1,0 +1,244 @@
+"""Python binding of libmav1hip.so (include/mav1_hip.h) over ctypes.
+
+The records are the ABI structs of include/mav1_types.h; frames and
+motion fields are numpy arrays. Loading fails loudly when the library has
+not been built: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmav1hip.so")
+
+NUM_NODES = 85
+A_SLOTS = 13
+TURBO_DEFAULT = 0x47CBF1
+BUF_SRC, BUF_REF, BUF_GOLD, BUF_MVFIELD, BUF_MVGOLD, BUF_PLANES = range(6)
+
+
+class FrameParams(C.Structure):
+    _fields_ = [
+        ("mi_cols", C.c_int32), ("mi_rows", C.c_int32), ("sb_cols", C.c_int32), ("sb_rows", C.c_int32),
+        ("plane_w", C.c_int32 * 3), ("plane_h", C.c_int32 * 3), ("stride", C.c_int32 * 3),
+        ("offset", C.c_int32 * 3), ("base_q_idx", C.c_int32), ("dc_q", C.c_int32 * 3), ("ac_q", C.c_int32 * 3),
+        ("frame_is_intra", C.c_int32), ("reduced_tx_set", C.c_int32), ("enable_intra_edge_filter", C.c_int32),
+        ("tx_search", C.c_int32), ("lambda_", C.c_int32), ("wave", C.c_int32), ("num_blocks", C.c_int32),
+        ("frame_w", C.c_int32), ("frame_h", C.c_int32), ("me_range", C.c_int32), ("lossless", C.c_int32),
+        ("tx_mode_select", C.c_int32), ("tile_w_sb", C.c_int32), ("fast", C.c_int32), ("leaf", C.c_int32),
+        ("depth_thr", C.c_int32), ("switchable_interp", C.c_int32), ("golden", C.c_int32),
+        ("ref2_frame", C.c_int32), ("compound", C.c_int32), ("lf_level", C.c_int32 * 4),
+        ("lf_sharpness", C.c_int32), ("cdef_damping", C.c_int32), ("cdef_bits", C.c_int32),
+        ("cdef_y_pri", C.c_int32 * 8), ("cdef_y_sec", C.c_int32 * 8), ("cdef_uv_pri", C.c_int32 * 8),
+        ("cdef_uv_sec", C.c_int32 * 8), ("lr_type", C.c_int32 * 3), ("lr_unit_size", C.c_int32 * 3),
+        ("lr_unit_rows", C.c_int32 * 3), ("lr_unit_cols", C.c_int32 * 3), ("lr_unit_off", C.c_int32 * 3),
+        ("turbo", C.c_int32),
+    ]
+
+
+assert C.sizeof(FrameParams) == 392
+
+
+class BlockInfo(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "mi_row", "mi_col", "bsize", "is_inter", "y_mode", "uv_mode", "angle_y", "angle_uv", "mv_row", "mv_col",
+        "tx_type")] + [("eob", C.c_int32 * 3), ("coef_off", C.c_int32 * 3)] + [(n, C.c_int32) for n in (
+            "cfl_alpha_u", "cfl_alpha_v", "tx_depth", "tx_types_lo", "tx_types_hi", "interp_filter", "ref_frame",
+            "ref_frame1", "mv1_row", "mv1_col")]
+
+
+assert C.sizeof(BlockInfo) == 108
+
+
+class SbInfo(C.Structure):
+    _fields_ = [("first_block", C.c_int32), ("num_blocks", C.c_int32), ("sb_row", C.c_int32),
+                ("sb_col", C.c_int32), ("part", C.c_int32 * 85)]
+
+
+assert C.sizeof(SbInfo) == 356
+
+
+class Config(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "width", "height", "qindex", "keyint", "tx_search", "tx_depth", "loop_filter", "cdef", "restoration",
+        "tile_cols_log2", "intra_edge_filter", "nonref_odd", "fast", "multi_ref", "altref", "leaf",
+        "switchable_interp", "me_range", "turbo", "knob_turbo_mask", "knob_dt", "knob_dt_bias", "knob_cdef_preset",
+        "knob_leafopt")]
+
+
+class Frame(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "is_key", "frame_kind", "nonref", "order_hint", "reference_select", "gold_differs")]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    i32 = C.c_int32
+    sig = {
+        "mav1hip_config_default": (None, [C.POINTER(Config), i32, i32, i32]),
+        "mav1hip_config_normalize": (None, [C.POINTER(Config)]),
+        "mav1hip_geometry": (C.c_size_t, [C.POINTER(Config)] + [C.POINTER(i32)] * 5 + [C.POINTER(i32)] * 3),
+        "mav1hip_frame_qindex": (i32, [C.POINTER(Config), C.POINTER(Frame)]),
+        "mav1hip_setup_params": (i32, [C.POINTER(Config), C.POINTER(Frame), i32, C.POINTER(FrameParams)]),
+        "mav1hip_create": (P, [C.POINTER(Config), C.c_char_p, C.c_size_t]),
+        "mav1hip_destroy": (None, [P]),
+        "mav1hip_error": (C.c_char_p, [P]),
+        "mav1hip_device_name": (C.c_char_p, [P]),
+        "mav1hip_motion_search": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, P, P]),
+        "mav1hip_buffer_bytes": (C.c_size_t, [P, i32]),
+        "mav1hip_planes_entry_bytes": (C.c_size_t, [P]),
+        "mav1hip_upload": (C.c_int, [P, i32, P, C.c_size_t]),
+        "mav1hip_download": (C.c_int, [P, i32, P, C.c_size_t]),
+        "mav1hip_set_params": (C.c_int, [P, C.POINTER(FrameParams)]),
+        "mav1hip_launch_motion_search": (C.c_int, [P]),
+        "mav1hip_sync": (C.c_int, [P]),
+        "mav1hip_pool_alloc": (C.c_int, [P, i32]),
+        "mav1hip_pool_upload": (C.c_int, [P, i32, P]),
+        "mav1hip_bind_frames": (C.c_int, [P, i32, i32, i32]),
+        "mav1hip_timing_enable": (C.c_int, [P, i32]),
+        "mav1hip_timing_reset": (C.c_int, [P]),
+        "mav1hip_timing_get": (C.c_int, [P, C.c_char_p, C.POINTER(i32), C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def default_config(width, height, turbo=1, **kw):
+    cfg = Config()
+    lib().mav1hip_config_default(C.byref(cfg), width, height, turbo)
+    for k, v in kw.items():
+        setattr(cfg, k, v)
+    lib().mav1hip_config_normalize(C.byref(cfg))
+    return cfg
+
+
+def frame_info(is_key=0, frame_kind=0, nonref=0, order_hint=0, reference_select=0, gold_differs=0):
+    return Frame(is_key, frame_kind, nonref, order_hint, reference_select, gold_differs)
+
+
+def turbo_frame(kind, order_hint):
+    """Frame state of the turbo mini-GOP of 2 (altref 2, leaf): kind 'key',
+    'arf' (hidden ALTREF, LAST only) or 'leaf' (non-reference, LAST + ALTREF
+    compound) (src/encoder.c:1004-1030, 1252-1275)."""
+    if kind == "key":
+        return frame_info(is_key=1, order_hint=order_hint)
+    if kind == "arf":
+        return frame_info(frame_kind=1, order_hint=order_hint)
+    if kind == "leaf":
+        return frame_info(frame_kind=2, nonref=1, order_hint=order_hint, reference_select=1)
+    raise ValueError(kind)
+
+
+def setup_params(cfg, fr, q=None):
+    if q is None:
+        q = lib().mav1hip_frame_qindex(C.byref(cfg), C.byref(fr))
+    fp = FrameParams()
+    n = lib().mav1hip_setup_params(C.byref(cfg), C.byref(fr), q, C.byref(fp))
+    return fp, n
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class Context:
+    """A device context for frames of one size (mav1hip_create)."""
+
+    def __init__(self, cfg):
+        err = C.create_string_buffer(512)
+        h = lib().mav1hip_create(C.byref(cfg), err, 512)
+        if not h:
+            raise HipError(err.value.decode())
+        self.h = h
+        self.cfg = cfg
+        self.frame_bytes = lib().mav1hip_buffer_bytes(h, BUF_SRC)
+        self.mv_ints = lib().mav1hip_buffer_bytes(h, BUF_MVFIELD) // 4
+
+    def close(self):
+        if self.h:
+            lib().mav1hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise HipError(f"{what}: {lib().mav1hip_error(self.h).decode()} (rc {rc})")
+
+    @property
+    def device_name(self):
+        return lib().mav1hip_device_name(self.h).decode()
+
+    def motion_search(self, fp, src, ref, gold=None):
+        mv = np.zeros(self.mv_ints, np.int32)
+        mvg = np.zeros(self.mv_ints, np.int32) if fp.golden else None
+        rc = lib().mav1hip_motion_search(self.h, C.byref(fp), ptr(src), ptr(ref), ptr(gold), ptr(mv), ptr(mvg))
+        self._chk(rc, "motion_search")
+        return mv, mvg
+
+    def upload(self, buf, arr):
+        arr = np.ascontiguousarray(arr)
+        self._chk(lib().mav1hip_upload(self.h, buf, ptr(arr), arr.nbytes), "upload")
+
+    def download(self, buf, dtype=np.uint8, count=None):
+        nbytes = lib().mav1hip_buffer_bytes(self.h, buf)
+        itemsize = np.dtype(dtype).itemsize
+        out = np.empty(count if count is not None else nbytes // itemsize, dtype)
+        self._chk(lib().mav1hip_download(self.h, buf, ptr(out), out.nbytes), "download")
+        return out
+
+    def set_params(self, fp):
+        self._chk(lib().mav1hip_set_params(self.h, C.byref(fp)), "set_params")
+
+    def launch_motion_search(self):
+        self._chk(lib().mav1hip_launch_motion_search(self.h), "launch_motion_search")
+
+    def sync(self):
+        self._chk(lib().mav1hip_sync(self.h), "sync")
+
+    def pool_alloc(self, n):
+        self._chk(lib().mav1hip_pool_alloc(self.h, n), "pool_alloc")
+
+    def pool_upload(self, idx, frame):
+        frame = np.ascontiguousarray(frame, np.uint8)
+        self._chk(lib().mav1hip_pool_upload(self.h, idx, ptr(frame)), "pool_upload")
+
+    def bind_frames(self, s, r, g):
+        self._chk(lib().mav1hip_bind_frames(self.h, s, r, g), "bind_frames")
+
+    def timing(self, on=True):
+        self._chk(lib().mav1hip_timing_enable(self.h, 1 if on else 0), "timing_enable")
+
+    def timing_reset(self):
+        self._chk(lib().mav1hip_timing_reset(self.h), "timing_reset")
+
+    def timing_get(self, kernel):
+        n = C.c_int32()
+        ms = C.c_double()
+        self._chk(lib().mav1hip_timing_get(self.h, kernel.encode(), C.byref(n), C.byref(ms)), "timing_get")
+        return n.value, ms.value

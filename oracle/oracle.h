@@ -1,0 +1,54 @@
+/* ORACLE — test infrastructure only.
+ *
+ * A plain-C, single-threaded restatement of the mav1 analysis path
+ * (the reference's src/kernels/k_*.h, backend_cpu.c and the host searches
+ * of encoder.c / lr_search.c), written for this project to check the HIP
+ * backend against. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker: the product path
+ * (libmav1hip.so) never links or calls it.
+ *
+ * Every function cites the reference file:line whose behaviour it follows.
+ * The restatement is pinned against the reference itself: oracle/Makefile
+ * also compiles the reference's kernel headers, unchanged, into
+ * oracle/_ref/libmav1ref.so, and tests/test_oracle_vs_ref.py compares the
+ * two on seeded synthetic frames; closed forms (Hadamard SATD, SAD, the
+ * quantiser's division formula) are checked independently.
+ *
+ * Buffers use the encoder's padded planar layout (Mav1FrameParams.stride /
+ * offset); records are the ABI structs of include/mav1_types.h.
+ */
+#ifndef MAV1_ORACLE_H
+#define MAV1_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/mav1_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* --- motion search (src/kernels/k_me.h, src/backend_cpu.c:23-56) --- */
+/* Bytes of one quarter-pel plane entry: 16 W x H planes + the decimated plane. */
+long oracle_qp_entry_bytes(const Mav1FrameParams *fp);
+/* The 16 EIGHTTAP quarter-pel planes of a reference (k_me.h:289-313). */
+void oracle_qpel_build(const Mav1FrameParams *fp, const uint8_t *ref, uint8_t *planes16);
+/* 4x4-mean decimated luma plane, ceil(W/4) x ceil(H/4) (k_me.h:112-118). */
+void oracle_dec_build(const Mav1FrameParams *fp, const uint8_t *frame, uint8_t *dec);
+/* Whole-frame motion search of every node of every level, LAST into
+ * mvfield and (fp->golden) the second reference into mvgold; both
+ * NUM_SBS * 85 * 2 int32 (k_me.h:343-379 per node). */
+void oracle_motion_search(const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *ref, const uint8_t *gold,
+                          int32_t *mvfield, int32_t *mvgold);
+
+/* --- shared prediction primitives --- */
+int oracle_satd4x4(const int32_t *d);
+void oracle_predict_inter(const uint8_t *ref, int refStride, int refW, int refH, uint8_t *dst, int dstStride, int x,
+                          int y, int w, int h, int mvRow, int mvCol, int subX, int subY, int filter);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,74 @@
+/* ORACLE — test infrastructure only.
+ *
+ * Entry points into the mav1 reference's own kernel code, compiled from the
+ * reference sources where they lie (-I <reference>/src) into
+ * oracle/_ref/libmav1ref.so by oracle/Makefile. Nothing here restates an
+ * algorithm: each function is the serial loop of the reference's CPU
+ * backend (src/backend_cpu.c, which itself needs libdispatch and cannot be
+ * built here) around the reference's unchanged kernel functions, so the
+ * outputs are the reference's. Used to pin the oracle restatement and as
+ * the reference CPU baseline.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "kernels/k_analysis.h"
+#include "kernels/k_cdef.h"
+#include "kernels/k_lf.h"
+#include "kernels/k_lr.h"
+#include "kernels/k_me.h"
+#include "kernels/k_recon.h"
+#include "fwd_matrices.h"
+
+_Static_assert(sizeof(FrameParams) == 392, "FrameParams layout");
+_Static_assert(sizeof(BlockInfo) == 108, "BlockInfo layout");
+_Static_assert(sizeof(SbInfo) == 356, "SbInfo layout");
+_Static_assert(sizeof(LrUnit) == 48, "LrUnit layout");
+
+long ref_qp_entry_bytes(const FrameParams *fp) { return qp_entry_bytes(fp); }
+
+void ref_qpel_build(const FrameParams *fp, const uint8_t *ref, uint8_t *planes16) {
+  for (int y = 0; y < fp->frame_h; y++)
+    for (int x = 0; x < fp->frame_w; x++) qpel_build(fp, ref, planes16, y, x);
+}
+
+void ref_dec_build(const FrameParams *fp, const uint8_t *frame, uint8_t *dec) {
+  for (int y = 0; y < dec_h(fp); y++)
+    for (int x = 0; x < dec_w(fp); x++) dec_build(fp, frame, dec, y, x);
+}
+
+/* The serial form of cpu_motion_search (src/backend_cpu.c:23-56). */
+void ref_motion_search(const FrameParams *fp, const uint8_t *src, const uint8_t *ref, const uint8_t *gold,
+                       int32_t *mvfield, int32_t *mvgold) {
+  int nref = fp->golden ? 2 : 1, nsb = fp->sb_cols * fp->sb_rows;
+  long eb = qp_entry_bytes(fp);
+  uint8_t *qp = calloc((size_t)(ME_QP_ENTRIES * eb + eb), 1);
+  uint8_t *srcDec = qp + ME_QP_ENTRIES * eb;
+  int32_t *scratch = calloc((size_t)nsb * SCRATCH_INTS, sizeof(int32_t));
+  if (fp->turbo & T_ME_HPEL) {
+    int W = fp->frame_w, H = fp->frame_h;
+    for (int g = 0; g < H * (nref + 1); g++) {
+      int r = g / H, y = g % H;
+      for (int x = 0; x < W; x++) {
+        int dec = (fp->turbo & T_ME_DEC) && y < dec_h(fp) && x < dec_w(fp);
+        if (r == nref) {
+          if (dec) dec_build(fp, src, srcDec, y, x);
+          continue;
+        }
+        uint8_t *ent = qp + r * eb;
+        qpel_build(fp, r ? gold : ref, ent, y, x);
+        if (dec) dec_build(fp, r ? gold : ref, ent + 16L * W * H, y, x);
+      }
+    }
+  }
+  for (int level = 0; level < 4; level++) {
+    int nn = 1 << (2 * level), nodes = nsb * nn;
+    for (int g = 0; g < nodes * nref; g++) {
+      int r = g / nodes, sb = (g % nodes) / nn, i = g % nn;
+      me_node_serial(fp, sb, level, i, src, r ? gold : ref, qp + r * eb, srcDec, qp_dec(fp, qp + r * eb),
+                     r ? mvgold : mvfield, scratch + (size_t)sb * SCRATCH_INTS + r * (SCRATCH_INTS / 2), r);
+    }
+  }
+  free(scratch);
+  free(qp);
+}
