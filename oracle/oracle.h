@@ -42,7 +42,24 @@ void oracle_dec_build(const Mav1FrameParams *fp, const uint8_t *frame, uint8_t *
 void oracle_motion_search(const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *ref, const uint8_t *gold,
                           int32_t *mvfield, int32_t *mvgold);
 
+/* --- analysis (src/kernels/k_analysis.h, src/backend_cpu.c:58-74) --- */
+/* Pass 1 over every (SB, level, node, slot) item into costs / infos
+ * (NUM_SBS * 1105 entries each; cost -1: not evaluated), then pass 2 into
+ * blocks (NUM_SBS * 64) and sbs (NUM_SBS). */
+void oracle_analyze(const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *ref, const uint8_t *gold,
+                    const int32_t *mvfield, const int32_t *mvgold, int64_t *costs, Mav1BlockInfo *infos,
+                    Mav1BlockInfo *blocks, Mav1SbInfo *sbs);
+
 /* --- shared prediction primitives --- */
+/* Intra prediction arguments (the IntraArgs record of src/kernels/k_intra.h:79-85). */
+typedef struct {
+  int32_t haveLeft, haveAbove, haveAboveRight, haveBelowLeft;
+  int32_t mode, angleDelta, log2W, log2H;
+  int32_t maxX, maxY;
+  int32_t edgeFilter, filterType;
+} OracleIntraArgs;
+void oracle_predict_intra(const uint8_t *px, int stride, int x, int y, const OracleIntraArgs *a, uint8_t *dst,
+                          int dstStride);
 int oracle_satd4x4(const int32_t *d);
 void oracle_predict_inter(const uint8_t *ref, int refStride, int refW, int refH, uint8_t *dst, int dstStride, int x,
                           int y, int w, int h, int mvRow, int mvCol, int subX, int subY, int filter);

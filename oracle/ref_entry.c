@@ -72,3 +72,25 @@ void ref_motion_search(const FrameParams *fp, const uint8_t *src, const uint8_t 
   free(scratch);
   free(qp);
 }
+
+/* The serial form of cpu_backend_analyze (src/backend_cpu.c:58-74). */
+void ref_analyze(const FrameParams *fp, const uint8_t *src, const uint8_t *ref, const uint8_t *gold,
+                 const int32_t *mvfield, const int32_t *mvgold, long *costs, BlockInfo *infos, BlockInfo *blocks,
+                 SbInfo *sbs) {
+  int nsb = fp->sb_cols * fp->sb_rows;
+  int32_t *scratch = calloc(SCRATCH_INTS, sizeof(int32_t));
+  for (int i = 0; i < nsb; i++) {
+    for (int level = 0; level < 4; level++) {
+      int ns = level_slots(fp, level);
+      for (int node = 0; node < (1 << (2 * level)); node++)
+        for (int slot = 0; slot < ns; slot++)
+          analyze_item(fp, i, level, node, slot, src, ref, gold, mvfield, mvgold, costs, infos, scratch);
+    }
+    analyze_sb_table(fp, i, costs, infos, blocks, sbs);
+  }
+  free(scratch);
+}
+
+void ref_predict_intra(const uint8_t *px, int stride, int x, int y, const IntraArgs *a, uint8_t *dst, int dstStride) {
+  predict_intra(px, stride, x, y, *a, dst, dstStride);
+}

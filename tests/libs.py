@@ -22,6 +22,8 @@ def _bind(lib, prefix):
         "qpel_build": (None, [FP, P, P]),
         "dec_build": (None, [FP, P, P]),
         "motion_search": (None, [FP, P, P, P, P, P]),
+        "analyze": (None, [FP, P, P, P, P, P, P, P, P, P]),
+        "predict_intra": (None, [P, C.c_int, C.c_int, C.c_int, P, P, C.c_int]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, prefix + name, None)
@@ -75,6 +77,25 @@ class Impl:
                                  mav1hip.ptr(mvg))
         return mv, (mvg if fp.golden else None)
 
+    def analyze(self, fp, src, ref_, gold, mvf, mvg):
+        nsb = fp.sb_cols * fp.sb_rows
+        costs = np.zeros(nsb * mav1hip.NUM_NODES * mav1hip.A_SLOTS, np.int64)
+        infos = np.zeros((nsb * mav1hip.NUM_NODES * mav1hip.A_SLOTS, 27), np.int32)
+        blocks = np.zeros((nsb * 64, 27), np.int32)
+        sbs = np.zeros((nsb, 89), np.int32)
+        if mvf is None:
+            mvf = np.zeros(nsb * mav1hip.NUM_NODES * 2, np.int32)
+        self.fn("analyze")(C.byref(fp), mav1hip.ptr(src), mav1hip.ptr(ref_), mav1hip.ptr(gold), mav1hip.ptr(mvf),
+                           mav1hip.ptr(mvg), mav1hip.ptr(costs), mav1hip.ptr(infos), mav1hip.ptr(blocks),
+                           mav1hip.ptr(sbs))
+        return costs, infos, blocks, sbs
+
+    def predict_intra(self, px, stride, x, y, args, w, h):
+        a = np.asarray(args, np.int32)
+        dst = np.zeros((h, w), np.uint8)
+        self.fn("predict_intra")(mav1hip.ptr(px), stride, x, y, mav1hip.ptr(a), mav1hip.ptr(dst), w)
+        return dst
+
     def qpel_build(self, fp, ref_):
         out = np.zeros(16 * fp.frame_w * fp.frame_h, np.uint8)
         self.fn("qpel_build")(C.byref(fp), mav1hip.ptr(ref_), mav1hip.ptr(out))
@@ -98,3 +119,22 @@ def turbo_params(width, height, kind, q=125, order_hint=1, **cfgkw):
     fr = mav1hip.turbo_frame(kind, order_hint)
     fp, npresets = mav1hip.setup_params(cfg, fr)
     return cfg, fp
+
+
+# BlockInfo columns (int32) that eval_block defines; eob / coef_off (11..16)
+# are filled by pass 2 only.
+INFO_COLS = [i for i in range(27) if not 11 <= i <= 16]
+
+
+def compare_analysis(a, b, nsb):
+    """Equal pass-1 costs everywhere, equal decisions where evaluated, equal
+    blocks and superblock records."""
+    ca, ia, ba, sa = a
+    cb, ib, bb, sb = b
+    assert np.array_equal(ca, cb), f"pass-1 costs differ at {np.nonzero(ca != cb)[0][:8]}"
+    ev = ca >= 0
+    assert np.array_equal(ia[ev][:, INFO_COLS], ib[ev][:, INFO_COLS]), "pass-1 decisions differ"
+    assert np.array_equal(sa, sb), "superblock records differ"
+    for s in range(nsb):
+        n = sa[s, 1]
+        assert np.array_equal(ba[s * 64:s * 64 + n], bb[s * 64:s * 64 + n]), f"blocks of SB {s} differ"
