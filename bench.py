@@ -64,6 +64,13 @@ def algorithmic_bytes(kernel, fp, nref):
         level = int(kernel[-1])
         field = nref * nsb * (4 ** level) * 8 + (nref * nsb * (4 ** (level - 1)) * 8 if level else 0)
         return nref * (Y + Y) + field
+    F = Y + 2 * (((fp.frame_w + 1) >> 1) * ((fp.frame_h + 1) >> 1))
+    if kernel.startswith("analyze_level"):
+        level = int(kernel[-1])
+        items = nsb * (4 ** level)  # turbo: one slot per node
+        return F * (1 + nref) + items * (nref * 8 + 8 + 108)
+    if kernel == "analyze_table":
+        return nsb * 85 * (8 + 108) + nsb * (64 * 108 + 356)
     return 0
 
 
@@ -101,6 +108,7 @@ def main():
         ctx.bind_frames(f % uniq, last % uniq, alt % uniq if alt >= 0 else -1)
         ctx.set_params(fp)
         ctx.launch_motion_search()
+        ctx.launch_analyze()
 
     for s in order[:args.warmup]:
         step(*s)
@@ -113,7 +121,8 @@ def main():
     ctx.sync()
     elapsed = time.perf_counter() - t0
     ctx.timing(False)
-    kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3"]
+    kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3", "analyze_level0",
+               "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table"]
     stats = {k: ctx.timing_get(k) for k in kernels}
     fps = args.steps / elapsed
 
@@ -149,7 +158,7 @@ def main():
         "data": "synthetic (seeded generator: 6 sinusoids + 12 moving rectangles + noise)",
         "config": {
             "workload": "1920x1080 8-bit 4:2:0, --turbo -q 125, turbo GOP (ARF + leaf), plain HIP path",
-            "stages": ["motion_search"],
+            "stages": ["motion_search", "analyze"],
             "frame_kinds": "alternating hidden ALTREF (1 ref) / leaf (2 refs)",
             "references": "source pictures (reconstruction not in the step yet)",
             "device": ctx.device_name,
@@ -177,23 +186,27 @@ def cpu_baseline(seconds, frames, params, uniq):
     """The reference's own motion-search kernels (oracle/_ref), serial on one
     host core, over the same frame sequence until `seconds` have passed."""
     path = os.path.join(ROOT, "oracle", "_ref", "libmav1ref.so")
-    kind = "reference"
-    fn_name = "ref_motion_search"
+    kind, pre = "reference", "ref_"
     if not os.path.exists(path):
-        path = os.path.join(ROOT, "oracle", "liboracle.so")
-        kind = "port"
-        fn_name = "oracle_motion_search"
+        path, kind, pre = os.path.join(ROOT, "oracle", "liboracle.so"), "port", "oracle_"
     lib = C.CDLL(path)
-    fn = getattr(lib, fn_name)
-    n = W and (params["arf"].sb_cols * params["arf"].sb_rows * 85 * 2)
-    mv = np.zeros(n, np.int32)
-    mvg = np.zeros(n, np.int32)
+    me, ana = getattr(lib, pre + "motion_search"), getattr(lib, pre + "analyze")
+    nsb = params["arf"].sb_cols * params["arf"].sb_rows
+    mv = np.zeros(nsb * 170, np.int32)
+    mvg = np.zeros(nsb * 170, np.int32)
+    costs = np.zeros(nsb * 1105, np.int64)
+    infos = np.zeros(nsb * 1105 * 27, np.int32)
+    blocks = np.zeros(nsb * 64 * 27, np.int32)
+    sbs = np.zeros(nsb * 89, np.int32)
     done = 0
     t0 = time.perf_counter()
     for kind_f, f, last, alt in coding_order(64):
         fp = params[kind_f]
-        fn(C.byref(fp), mav1hip.ptr(frames[f % uniq]), mav1hip.ptr(frames[last % uniq]),
-           mav1hip.ptr(frames[alt % uniq]) if alt >= 0 else None, mav1hip.ptr(mv), mav1hip.ptr(mvg))
+        s, r, g = (mav1hip.ptr(frames[f % uniq]), mav1hip.ptr(frames[last % uniq]),
+                   mav1hip.ptr(frames[alt % uniq]) if alt >= 0 else None)
+        me(C.byref(fp), s, r, g, mav1hip.ptr(mv), mav1hip.ptr(mvg))
+        ana(C.byref(fp), s, r, g, mav1hip.ptr(mv), mav1hip.ptr(mvg), mav1hip.ptr(costs), mav1hip.ptr(infos),
+            mav1hip.ptr(blocks), mav1hip.ptr(sbs))
         done += 1
         if time.perf_counter() - t0 >= seconds and done % 2 == 0:
             break
@@ -203,7 +216,7 @@ def cpu_baseline(seconds, frames, params, uniq):
         "unit": "frames/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"{done} frames of the same 1080p turbo GOP (ARF/leaf alternating), motion-search stage, "
+        "sample": f"{done} frames of the same 1080p turbo GOP (ARF/leaf alternating), motion search + analysis, "
                   f"{dt:.1f} s on one host core ({os.cpu_count()} visible)",
     }
 

@@ -117,6 +117,17 @@ const char *mav1hip_device_name(const mav1hip_ctx *c);
 int mav1hip_motion_search(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *ref,
                           const uint8_t *gold, int32_t *mvfield, int32_t *mvgold);
 
+/* Backend.analyze (src/backend_cpu.c:58-74): pass 1 evaluates every
+ * (superblock, node, slot) table item, pass 2 picks each superblock's
+ * partition tree and writes its blocks in decode order. mvfield / mvgold:
+ * the motion search's fields (inter frames). blocks: num_sbs * 64 records,
+ * sbs: num_sbs. acosts (num_sbs * 1105 int64, -1 = not evaluated) and
+ * ainfos (num_sbs * 1105 records, defined where the cost is >= 0) receive
+ * the pass-1 table when not NULL. */
+int mav1hip_analyze(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *ref,
+                    const uint8_t *gold, const int32_t *mvfield, const int32_t *mvgold, Mav1BlockInfo *blocks,
+                    Mav1SbInfo *sbs, int64_t *acosts, Mav1BlockInfo *ainfos);
+
 /* ------------------------------------------------------------------------
  * Device-resident form
  * ------------------------------------------------------------------------ */
@@ -126,9 +137,13 @@ enum {
   MAV1HIP_BUF_GOLD = 2,  /* second reference picture */
   MAV1HIP_BUF_MVFIELD = 3,
   MAV1HIP_BUF_MVGOLD = 4,
-  MAV1HIP_BUF_PLANES = 5 /* motion-search planes: per reference 16 quarter-pel planes + decimated
+  MAV1HIP_BUF_PLANES = 5, /* motion-search planes: per reference 16 quarter-pel planes + decimated
                             plane (mav1hip_planes_entry_bytes each), LAST then second reference,
                             then the decimated source */
+  MAV1HIP_BUF_ACOSTS = 6,  /* pass-1 analysis costs, int64 per item */
+  MAV1HIP_BUF_AINFOS = 7,  /* pass-1 analysis decisions */
+  MAV1HIP_BUF_BLOCKS = 8,  /* coded blocks, 64 records per superblock */
+  MAV1HIP_BUF_SBS = 9      /* superblock records */
 };
 /* Size in bytes of a device buffer. */
 size_t mav1hip_buffer_bytes(const mav1hip_ctx *c, int32_t buf);
@@ -138,6 +153,7 @@ int mav1hip_download(mav1hip_ctx *c, int32_t buf, void *host, size_t bytes);
 /* Parameters of the next launches (copied; the pointer is not kept). */
 int mav1hip_set_params(mav1hip_ctx *c, const Mav1FrameParams *fp);
 int mav1hip_launch_motion_search(mav1hip_ctx *c);
+int mav1hip_launch_analyze(mav1hip_ctx *c);
 /* Waits for everything enqueued on the context's stream. */
 int mav1hip_sync(mav1hip_ctx *c);
 

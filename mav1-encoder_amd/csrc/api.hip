@@ -38,6 +38,9 @@ struct mav1hip_ctx {
   uint8_t *d_src, *d_ref, *d_gold;
   int32_t *d_mvfield, *d_mvgold;
   uint8_t *d_planes;
+  long long *d_acosts;
+  Mav1BlockInfo *d_ainfos, *d_blocks;
+  Mav1SbInfo *d_sbs;
   long entry_bytes;
   size_t planes_bytes;
   /* frames the stages read (own buffers or pool frames) */
@@ -163,6 +166,10 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   rc |= dmalloc(c, (void **)&c->d_mvfield, mvb);
   rc |= dmalloc(c, (void **)&c->d_mvgold, mvb);
   rc |= dmalloc(c, (void **)&c->d_planes, c->planes_bytes);
+  rc |= dmalloc(c, (void **)&c->d_acosts, sizeof(long long) * MAV1_A_ITEMS * (size_t)c->num_sbs);
+  rc |= dmalloc(c, (void **)&c->d_ainfos, sizeof(Mav1BlockInfo) * MAV1_A_ITEMS * (size_t)c->num_sbs);
+  rc |= dmalloc(c, (void **)&c->d_blocks, sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs);
+  rc |= dmalloc(c, (void **)&c->d_sbs, sizeof(Mav1SbInfo) * (size_t)c->num_sbs);
   if (rc != MAV1HIP_OK) {
     if (err && errlen) snprintf(err, errlen, "%s", c->err);
     mav1hip_destroy(c);
@@ -173,6 +180,10 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   (void)hipMemsetAsync(c->d_planes, 0, c->planes_bytes, c->stream);
   (void)hipMemsetAsync(c->d_mvfield, 0, mvb, c->stream);
   (void)hipMemsetAsync(c->d_mvgold, 0, mvb, c->stream);
+  (void)hipMemsetAsync(c->d_acosts, 0, sizeof(long long) * MAV1_A_ITEMS * (size_t)c->num_sbs, c->stream);
+  (void)hipMemsetAsync(c->d_ainfos, 0, sizeof(Mav1BlockInfo) * MAV1_A_ITEMS * (size_t)c->num_sbs, c->stream);
+  (void)hipMemsetAsync(c->d_blocks, 0, sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs, c->stream);
+  (void)hipMemsetAsync(c->d_sbs, 0, sizeof(Mav1SbInfo) * (size_t)c->num_sbs, c->stream);
   (void)hipStreamSynchronize(c->stream);
   c->b_src = c->d_src;
   c->b_ref = c->d_ref;
@@ -194,6 +205,10 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipFree(c->d_mvfield);
   (void)hipFree(c->d_mvgold);
   (void)hipFree(c->d_planes);
+  (void)hipFree(c->d_acosts);
+  (void)hipFree(c->d_ainfos);
+  (void)hipFree(c->d_blocks);
+  (void)hipFree(c->d_sbs);
   (void)hipFree(c->d_pool);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -211,6 +226,12 @@ static void *buf_ptr(mav1hip_ctx *c, int buf, size_t *bytes) {
     case MAV1HIP_BUF_MVFIELD: *bytes = mvb; return c->d_mvfield;
     case MAV1HIP_BUF_MVGOLD: *bytes = mvb; return c->d_mvgold;
     case MAV1HIP_BUF_PLANES: *bytes = c->planes_bytes; return c->d_planes;
+    case MAV1HIP_BUF_ACOSTS: *bytes = sizeof(long long) * MAV1_A_ITEMS * (size_t)c->num_sbs; return c->d_acosts;
+    case MAV1HIP_BUF_AINFOS: *bytes = sizeof(Mav1BlockInfo) * MAV1_A_ITEMS * (size_t)c->num_sbs; return c->d_ainfos;
+    case MAV1HIP_BUF_BLOCKS:
+      *bytes = sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs;
+      return c->d_blocks;
+    case MAV1HIP_BUF_SBS: *bytes = sizeof(Mav1SbInfo) * (size_t)c->num_sbs; return c->d_sbs;
     default: *bytes = 0; return nullptr;
   }
 }
@@ -289,6 +310,57 @@ int mav1hip_launch_motion_search(mav1hip_ctx *c) {
                                   srcDec, c->d_mvfield, c->d_mvgold, nref);
          })))
       return rc;
+  return MAV1HIP_OK;
+}
+
+int mav1hip_launch_analyze(mav1hip_ctx *c) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "analyze: no frame parameters set");
+  const Mav1FrameParams &fp = c->fp;
+  static const char *names[4] = {"analyze_level0", "analyze_level1", "analyze_level2", "analyze_level3"};
+  int rc;
+  for (int level = 0; level < 4; level++)
+    if ((rc = timed(c, names[level], [&] {
+           return launch_analyze(c->stream, level, fp, c->b_src, c->b_ref, c->b_gold, c->d_mvfield, c->d_mvgold,
+                                 c->d_acosts, c->d_ainfos);
+         })))
+      return rc;
+  return timed(c, "analyze_table",
+               [&] { return launch_analyze_table(c->stream, fp, c->d_acosts, c->d_ainfos, c->d_blocks, c->d_sbs); });
+}
+
+int mav1hip_analyze(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *ref,
+                    const uint8_t *gold, const int32_t *mvfield, const int32_t *mvgold, Mav1BlockInfo *blocks,
+                    Mav1SbInfo *sbs, int64_t *acosts, Mav1BlockInfo *ainfos) {
+  if (!c || !fp || !src || !blocks || !sbs) return fail(c, MAV1HIP_EINVAL, "analyze: missing buffer");
+  if (!fp->frame_is_intra && (!ref || !mvfield)) return fail(c, MAV1HIP_EINVAL, "analyze: inter frame without LAST");
+  if (!fp->frame_is_intra && fp->golden && (!gold || !mvgold))
+    return fail(c, MAV1HIP_EINVAL, "analyze: second reference missing");
+  int rc;
+  if ((rc = mav1hip_set_params(c, fp))) return rc;
+  c->b_src = c->d_src;
+  c->b_ref = c->d_ref;
+  c->b_gold = c->d_gold;
+  size_t mvb = sizeof(int32_t) * 2 * MAV1_NUM_NODES * (size_t)c->num_sbs;
+  HIPCHK(c, hipMemcpyAsync(c->d_src, src, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  if (!fp->frame_is_intra) {
+    HIPCHK(c, hipMemcpyAsync(c->d_ref, ref, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_mvfield, mvfield, mvb, hipMemcpyHostToDevice, c->stream));
+    if (fp->golden) {
+      HIPCHK(c, hipMemcpyAsync(c->d_gold, gold, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->d_mvgold, mvgold, mvb, hipMemcpyHostToDevice, c->stream));
+    }
+  }
+  if ((rc = mav1hip_launch_analyze(c))) return rc;
+  size_t nb = sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs;
+  HIPCHK(c, hipMemcpyAsync(blocks, c->d_blocks, nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(sbs, c->d_sbs, sizeof(Mav1SbInfo) * (size_t)c->num_sbs, hipMemcpyDeviceToHost, c->stream));
+  if (acosts)
+    HIPCHK(c, hipMemcpyAsync(acosts, c->d_acosts, sizeof(long long) * MAV1_A_ITEMS * (size_t)c->num_sbs,
+                             hipMemcpyDeviceToHost, c->stream));
+  if (ainfos)
+    HIPCHK(c, hipMemcpyAsync(ainfos, c->d_ainfos, sizeof(Mav1BlockInfo) * MAV1_A_ITEMS * (size_t)c->num_sbs,
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return MAV1HIP_OK;
 }
 

@@ -21,4 +21,12 @@ hipError_t launch_me_level(hipStream_t st, int level, const Mav1FrameParams &fp,
                            const uint8_t *ref0, const uint8_t *ref1, const uint8_t *planes, long entryBytes,
                            const uint8_t *srcDec, int32_t *field0, int32_t *field1, int nref);
 
+/* Analysis: pass 1 of one level (every slot of every node), then pass 2. */
+int analysis_level_slots(const Mav1FrameParams &fp, int level);
+hipError_t launch_analyze(hipStream_t st, int level, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *ref,
+                          const uint8_t *gold, const int32_t *mvf, const int32_t *mvg, long long *costs,
+                          Mav1BlockInfo *infos);
+hipError_t launch_analyze_table(hipStream_t st, const Mav1FrameParams &fp, const long long *costs,
+                                const Mav1BlockInfo *infos, Mav1BlockInfo *blocks, Mav1SbInfo *sbs);
+
 #endif

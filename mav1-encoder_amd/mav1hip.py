@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(PKG, "libmav1hip.so")
 NUM_NODES = 85
 A_SLOTS = 13
 TURBO_DEFAULT = 0x47CBF1
-BUF_SRC, BUF_REF, BUF_GOLD, BUF_MVFIELD, BUF_MVGOLD, BUF_PLANES = range(6)
+(BUF_SRC, BUF_REF, BUF_GOLD, BUF_MVFIELD, BUF_MVGOLD, BUF_PLANES, BUF_ACOSTS, BUF_AINFOS, BUF_BLOCKS,
+ BUF_SBS) = range(10)
 
 
 class FrameParams(C.Structure):
@@ -95,6 +96,8 @@ def _load():
         "mav1hip_download": (C.c_int, [P, i32, P, C.c_size_t]),
         "mav1hip_set_params": (C.c_int, [P, C.POINTER(FrameParams)]),
         "mav1hip_launch_motion_search": (C.c_int, [P]),
+        "mav1hip_analyze": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, P, P, P, P, P, P]),
+        "mav1hip_launch_analyze": (C.c_int, [P]),
         "mav1hip_sync": (C.c_int, [P]),
         "mav1hip_pool_alloc": (C.c_int, [P, i32]),
         "mav1hip_pool_upload": (C.c_int, [P, i32, P]),
@@ -200,6 +203,22 @@ class Context:
         rc = lib().mav1hip_motion_search(self.h, C.byref(fp), ptr(src), ptr(ref), ptr(gold), ptr(mv), ptr(mvg))
         self._chk(rc, "motion_search")
         return mv, mvg
+
+    def analyze(self, fp, src, ref, gold, mvf, mvg, tables=True):
+        """Blocks (num_sbs*64 x 27 int32) and superblock records (num_sbs x 89);
+        with tables, also the pass-1 costs and decisions."""
+        nsb = fp.sb_cols * fp.sb_rows
+        blocks = np.zeros((nsb * 64, 27), np.int32)
+        sbs = np.zeros((nsb, 89), np.int32)
+        costs = np.zeros(nsb * NUM_NODES * A_SLOTS, np.int64) if tables else None
+        infos = np.zeros((nsb * NUM_NODES * A_SLOTS, 27), np.int32) if tables else None
+        rc = lib().mav1hip_analyze(self.h, C.byref(fp), ptr(src), ptr(ref), ptr(gold), ptr(mvf), ptr(mvg),
+                                   ptr(blocks), ptr(sbs), ptr(costs), ptr(infos))
+        self._chk(rc, "analyze")
+        return costs, infos, blocks, sbs
+
+    def launch_analyze(self):
+        self._chk(lib().mav1hip_launch_analyze(self.h), "launch_analyze")
 
     def upload(self, buf, arr):
         arr = np.ascontiguousarray(arr)
