@@ -11,8 +11,9 @@
  * cost is a wave-uniform value and the mode decisions (first minimum, ties
  * as the serial code) are taken identically by all lanes.
  *
- * Pass 2: one thread per superblock runs the bottom-up partition choice over
- * the 85 nodes and writes the blocks in decode order.
+ * Pass 2: one wave per superblock runs the bottom-up partition choice over
+ * the 85 nodes (lanes over the nodes of a level) and writes the blocks in
+ * decode order (lanes over the 64 Z-order positions, prefix-scanned).
  */
 #include "dev_pred.h"
 #include "internal.h"
@@ -387,20 +388,34 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
   }
 }
 
-/* Pass 2: one thread per superblock (analyze_sb_table). */
+/* Pass 2 (analyze_sb_table): one wave per superblock. The bottom-up DP runs
+ * level by level with the lanes over the level's nodes (a node only reads
+ * its children, decided by the previous level) and the per-node state in
+ * LDS; the blocks are emitted with one lane per 8x8 position in Z order
+ * and an exclusive prefix scan over the lanes giving each position's block
+ * index and coefficient offset. */
+DEVI int wave_excl_scan(int v, int lane) {
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x - v;
+}
+
 __global__ __launch_bounds__(64) void k_analyze_table(const Mav1FrameParams fp, const long long *__restrict__ costs,
                                                       const Mav1BlockInfo *__restrict__ infos,
                                                       Mav1BlockInfo *__restrict__ blocks, Mav1SbInfo *__restrict__ sbs) {
-  const int sb = blockIdx.x * 64 + threadIdx.x;
-  if (sb >= fp.sb_cols * fp.sb_rows) return;
+  __shared__ long long cost[MAV1_NUM_NODES], noneCost[MAV1_NUM_NODES];
+  __shared__ int part[MAV1_NUM_NODES];
+  const int sb = blockIdx.x, lane = threadIdx.x;
   const int sbRow = sb / fp.sb_cols, sbCol = sb - sbRow * fp.sb_cols, lam = fp.ac_q[0] / 16;
-  long long cost[MAV1_NUM_NODES], noneCost[MAV1_NUM_NODES];
-  signed char part[MAV1_NUM_NODES];
   const long long *tc = costs + (long long)sb * MAV1_A_ITEMS;
   const Mav1BlockInfo *ti = infos + (long long)sb * MAV1_A_ITEMS;
   for (int level = 3; level >= 0; level--) {
     const int n = 1 << level, size4 = 16 >> level, bSize = level_bsize(level);
-    for (int i = 0; i < n * n; i++) {
+    for (int i = lane; i < n * n; i += 64) {
       const int node = d_node_base(level) + i;
       const int r = sbRow * 16 + (i / n) * size4, c = sbCol * 16 + (i % n) * size4;
       noneCost[node] = -1;
@@ -411,7 +426,7 @@ __global__ __launch_bounds__(64) void k_analyze_table(const Mav1FrameParams fp, 
       }
       const int half = size4 >> 1, hasRows = r + half < fp.mi_rows, hasCols = c + half < fp.mi_cols;
       const int cn = 2 * n, ci = (i / n) * 2 * cn + (i % n) * 2, cb = level < 3 ? d_node_base(level + 1) : 0;
-      long long bestC = -1;
+      long long bestC = -1, myNone = -1;
       int bestP = PARTITION_SPLIT;
       if (level < 3) {
         long long c0 = cost[cb + ci], c1 = cost[cb + ci + 1], c2 = cost[cb + ci + cn], c3 = cost[cb + ci + cn + 1];
@@ -434,63 +449,80 @@ __global__ __launch_bounds__(64) void k_analyze_table(const Mav1FrameParams fp, 
         }
         if (pc < 0) continue;
         pc += (long long)lam * (p == PARTITION_NONE ? 8 : 24) / 8;
-        if (p == PARTITION_NONE) noneCost[node] = pc;
+        if (p == PARTITION_NONE) myNone = pc;
         if (bestC < 0 || pc <= bestC) {
           bestC = pc;
           bestP = p;
         }
       }
+      noneCost[node] = myNone;
       cost[node] = bestC;
-      part[node] = (signed char)bestP;
+      part[node] = bestP;
     }
+    __syncthreads();
   }
   Mav1SbInfo *S = &sbs[sb];
-  for (int q = 0; q < MAV1_NUM_NODES; q++) S->part[q] = part[q];
-  int count = 0, coefOff = sb * MAV1_COEFS_PER_SB;
-  for (int z = 0; z < 64; z++) {
-    int zx = 0, zy = 0;
-    for (int b = 0; b < 3; b++) {
-      zx |= ((z >> (2 * b)) & 1) << b;
-      zy |= ((z >> (2 * b + 1)) & 1) << b;
+  for (int q = lane; q < MAV1_NUM_NODES; q += 64) S->part[q] = part[q];
+  /* emission: lane = Z-order position of an 8x8 unit */
+  const int z = lane;
+  int zx = 0, zy = 0;
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    zx |= ((z >> (2 * b)) & 1) << b;
+    zy |= ((z >> (2 * b + 1)) & 1) << b;
+  }
+  int nb = 0, lvl = 0, p = 0, node = 0;
+  for (int level = 0; level <= 3; level++) {
+    const int shift = 3 - level, n = 1 << level;
+    node = d_node_base(level) + (zy >> shift) * n + (zx >> shift);
+    p = part[node];
+    if (p < 0) break;
+    if (p == PARTITION_SPLIT) continue;
+    if ((zx & ((1 << shift) - 1)) == 0 && (zy & ((1 << shift) - 1)) == 0) {
+      const int size4 = 16 >> level;
+      nb = part_count(fp, p, sbRow * 16 + (zy >> shift) * size4, sbCol * 16 + (zx >> shift) * size4, size4 >> 1);
+      lvl = level;
     }
-    for (int level = 0; level <= 3; level++) {
-      const int shift = 3 - level, n = 1 << level;
-      const int node = d_node_base(level) + (zy >> shift) * n + (zx >> shift);
-      const int p = part[node];
-      if (p < 0) break;
-      if (p == PARTITION_SPLIT) continue;
-      if ((zx & ((1 << shift) - 1)) == 0 && (zy & ((1 << shift) - 1)) == 0) {
-        const int size4 = 16 >> level, half = size4 >> 1;
-        const int r = sbRow * 16 + (zy >> shift) * size4, c = sbCol * 16 + (zx >> shift) * size4;
-        const int nb = part_count(fp, p, r, c, half);
-        const int cn = 2 * n, ci = (zy >> shift) * 2 * cn + (zx >> shift) * 2, cb = level < 3 ? d_node_base(level + 1) : 0;
-        for (int k = 0; k < nb; k++) {
-          int s = part_slot(p, k), dr, dc;
-          Mav1BlockInfo b;
-          if (s < 0) {
-            part_block(p, level_bsize(level), k, half, dr, dc);
-            b = ti[(cb + ci + (dr ? cn : 0) + (dc ? 1 : 0)) * MAV1_A_SLOTS];
-          } else {
-            b = ti[node * MAV1_A_SLOTS + s];
-          }
-          const int w = BLOCK_WIDTH(b.bsize), h = BLOCK_HEIGHT(b.bsize);
-          b.coef_off[0] = coefOff;
-          coefOff += w * h;
-          b.coef_off[1] = coefOff;
-          coefOff += (w >> 1) * (h >> 1);
-          b.coef_off[2] = coefOff;
-          coefOff += (w >> 1) * (h >> 1);
-          b.eob[0] = b.eob[1] = b.eob[2] = 0;
-          blocks[sb * MAV1_MAX_BLOCKS_PER_SB + count++] = b;
-        }
+    break;
+  }
+  Mav1BlockInfo mine[4];
+  int ncoef = 0;
+  {
+    const int n = 1 << lvl, shift = 3 - lvl, half = (16 >> lvl) >> 1;
+    const int cn = 2 * n, ci = (zy >> shift) * 2 * cn + (zx >> shift) * 2, cb = lvl < 3 ? d_node_base(lvl + 1) : 0;
+    for (int k = 0; k < nb; k++) {
+      int s = part_slot(p, k), dr, dc;
+      if (s < 0) {
+        part_block(p, level_bsize(lvl), k, half, dr, dc);
+        mine[k] = ti[(cb + ci + (dr ? cn : 0) + (dc ? 1 : 0)) * MAV1_A_SLOTS];
+      } else {
+        mine[k] = ti[node * MAV1_A_SLOTS + s];
       }
-      break;
+      const int w = BLOCK_WIDTH(mine[k].bsize), h = BLOCK_HEIGHT(mine[k].bsize);
+      ncoef += w * h + 2 * ((w >> 1) * (h >> 1));
     }
   }
-  S->first_block = sb * MAV1_MAX_BLOCKS_PER_SB;
-  S->num_blocks = count;
-  S->sb_row = sbRow;
-  S->sb_col = sbCol;
+  const int first = wave_excl_scan(nb, lane);
+  int coefOff = sb * MAV1_COEFS_PER_SB + wave_excl_scan(ncoef, lane);
+  for (int k = 0; k < nb; k++) {
+    Mav1BlockInfo b = mine[k];
+    const int w = BLOCK_WIDTH(b.bsize), h = BLOCK_HEIGHT(b.bsize);
+    b.coef_off[0] = coefOff;
+    coefOff += w * h;
+    b.coef_off[1] = coefOff;
+    coefOff += (w >> 1) * (h >> 1);
+    b.coef_off[2] = coefOff;
+    coefOff += (w >> 1) * (h >> 1);
+    b.eob[0] = b.eob[1] = b.eob[2] = 0;
+    blocks[sb * MAV1_MAX_BLOCKS_PER_SB + first + k] = b;
+  }
+  const int count = wave_sum(nb);
+  if (lane == 0) {
+    S->first_block = sb * MAV1_MAX_BLOCKS_PER_SB;
+    S->num_blocks = count;
+    S->sb_row = sbRow;
+    S->sb_col = sbCol;
+  }
 }
 
 hipError_t launch_analyze(hipStream_t st, int level, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *ref,
@@ -510,6 +542,6 @@ hipError_t launch_analyze(hipStream_t st, int level, const Mav1FrameParams &fp, 
 hipError_t launch_analyze_table(hipStream_t st, const Mav1FrameParams &fp, const long long *costs,
                                 const Mav1BlockInfo *infos, Mav1BlockInfo *blocks, Mav1SbInfo *sbs) {
   const int nsb = fp.sb_cols * fp.sb_rows;
-  hipLaunchKernelGGL(k_analyze_table, dim3((nsb + 63) / 64), dim3(64), 0, st, fp, costs, infos, blocks, sbs);
+  hipLaunchKernelGGL(k_analyze_table, dim3(nsb), dim3(64), 0, st, fp, costs, infos, blocks, sbs);
   return hipGetLastError();
 }

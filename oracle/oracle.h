@@ -50,6 +50,20 @@ void oracle_analyze(const Mav1FrameParams *fp, const uint8_t *src, const uint8_t
                     const int32_t *mvfield, const int32_t *mvgold, int64_t *costs, Mav1BlockInfo *infos,
                     Mav1BlockInfo *blocks, Mav1SbInfo *sbs);
 
+/* --- reconstruction (src/kernels/k_recon.h, src/backend_cpu.c:81-156) --- */
+/* Edge filter types of the blocks (src/encoder.c:319-346), in place. */
+void oracle_edge_types(const Mav1FrameParams *fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs);
+/* Every superblock in raster order: prediction, transform, quantisation,
+ * reconstruction into rec, levels into coefs (NUM_SBS * 6144 int32), the
+ * block outputs into blocks. mats: the Q20 forward matrices (15 * 4096). */
+void oracle_recon(const Mav1FrameParams *fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs, const uint8_t *src,
+                  uint8_t *rec, const uint8_t *ref, const uint8_t *gold, int32_t *coefs, const int32_t *mats);
+/* Single transforms (row stride 64 for coefficient / residual blocks). */
+void oracle_itx_2d(const int32_t *dq, int32_t *res, int txSz, int txType, int lossless);
+void oracle_fwd_2d(const int32_t *resid, int32_t *out, int txSz, int txType, const int32_t *mats);
+void oracle_quantize(const int32_t *coef, int32_t *quant, int txSz, int dcQ, int acQ);
+void oracle_dequant(const int32_t *quant, int32_t *dq, int txSz, int dcQ, int acQ);
+
 /* --- shared prediction primitives --- */
 /* Intra prediction arguments (the IntraArgs record of src/kernels/k_intra.h:79-85). */
 typedef struct {

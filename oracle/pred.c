@@ -56,3 +56,33 @@ void oracle_predict_inter(const uint8_t *ref, int refStride, int refW, int refH,
     }
   free(mid);
 }
+
+/* Compound single-reference prediction (7.11.3.1 with isCompound: vertical
+ * InterRound1 = 7, not clipped), as src/kernels/k_inter.h:195-248. */
+void o_predict_inter_raw(const uint8_t *ref, int refStride, int refW, int refH, int32_t *dst, int dstStride, int x,
+                         int y, int w, int h, int mvRow, int mvCol, int subX, int subY, int filter) {
+  int posX = ((x << SUBPEL_BITS) + ((2 * mvCol) >> subX)) * 64 + 32;
+  int posY = ((y << SUBPEL_BITS) + ((2 * mvRow) >> subY)) * 64 + 32;
+  int lastX = ((refW + subX) >> subX) - 1, lastY = ((refH + subY) >> subY) - 1;
+  int fv = filter & 3, fh = (filter >> 2) & 3;
+  if (w <= 4) fh = (fh == EIGHTTAP_SMOOTH) ? 5 : (fh == EIGHTTAP || fh == EIGHTTAP_SHARP) ? 4 : fh;
+  if (h <= 4) fv = (fv == EIGHTTAP_SMOOTH) ? 5 : (fv == EIGHTTAP || fv == EIGHTTAP_SHARP) ? 4 : fv;
+  const int16_t *kh = SUBPEL_FILTER(fh, (posX >> 6) & SUBPEL_MASK), *kv = SUBPEL_FILTER(fv, (posY >> 6) & SUBPEL_MASK);
+  int ix = posX >> 10, iy = posY >> 10;
+  int *mid = (int *)malloc(sizeof(int) * (size_t)(h + 7) * w);
+  for (int r = 0; r < h + 7; r++) {
+    const uint8_t *row = ref + o_clip(0, lastY, iy + r - 3) * refStride;
+    for (int c = 0; c < w; c++) {
+      int acc = 0;
+      for (int t = 0; t < 8; t++) acc += kh[t] * row[o_clip(0, lastX, ix + c + t - 3)];
+      mid[r * w + c] = o_round2(acc, 3);
+    }
+  }
+  for (int r = 0; r < h; r++)
+    for (int c = 0; c < w; c++) {
+      int acc = 0;
+      for (int t = 0; t < 8; t++) acc += kv[t] * mid[(r + t) * w + c];
+      dst[r * dstStride + c] = o_round2(acc, 7);
+    }
+  free(mid);
+}

@@ -94,3 +94,24 @@ void ref_analyze(const FrameParams *fp, const uint8_t *src, const uint8_t *ref, 
 void ref_predict_intra(const uint8_t *px, int stride, int x, int y, const IntraArgs *a, uint8_t *dst, int dstStride) {
   predict_intra(px, stride, x, y, *a, dst, dstStride);
 }
+
+/* The serial form of cpu_backend_recon (src/backend_cpu.c:153-156):
+ * superblocks in raster order satisfy every left / above-right dependency. */
+void ref_recon(const FrameParams *fp, BlockInfo *blocks, const SbInfo *sbs, const uint8_t *src, uint8_t *rec,
+               const uint8_t *ref, const uint8_t *gold, int32_t *coefs, const int32_t *mats) {
+  int nsb = fp->sb_cols * fp->sb_rows;
+  int32_t *scratch = calloc(SCRATCH_INTS, sizeof(int32_t));
+  for (int i = 0; i < nsb; i++) recon_sb(fp, sbs[i], blocks, src, rec, ref, gold, coefs, mats, scratch);
+  free(scratch);
+}
+
+void ref_itx_2d(int32_t *dq, int32_t *res, int txSz, int txType, int lossless) { itx_2d(dq, res, txSz, txType, lossless); }
+void ref_fwd_2d(const int32_t *resid, int32_t *out, int txSz, int txType, const int32_t *mats) {
+  static int32_t tmp[64 * 64];
+  fwd_2d(resid, out, txSz, txType, mats, tmp);
+}
+void ref_quantize(const int32_t *coef, int32_t *quant, int txSz, int dcQ, int acQ) {
+  quantize_block(coef, quant, txSz, dcQ, acQ, 26, 22);
+}
+void ref_dequant(const int32_t *quant, int32_t *dq, int txSz, int dcQ, int acQ) { dequant_block(quant, dq, txSz, dcQ, acQ); }
+void ref_forward_matrices(int32_t *mats) { fwd_build_matrices(mats); }

@@ -24,6 +24,13 @@ def _bind(lib, prefix):
         "motion_search": (None, [FP, P, P, P, P, P]),
         "analyze": (None, [FP, P, P, P, P, P, P, P, P, P]),
         "predict_intra": (None, [P, C.c_int, C.c_int, C.c_int, P, P, C.c_int]),
+        "recon": (None, [FP, P, P, P, P, P, P, P, P]),
+        "edge_types": (None, [FP, P, P]),
+        "itx_2d": (None, [P, P, C.c_int, C.c_int, C.c_int]),
+        "fwd_2d": (None, [P, P, C.c_int, C.c_int, P]),
+        "quantize": (None, [P, P, C.c_int, C.c_int, C.c_int]),
+        "dequant": (None, [P, P, C.c_int, C.c_int, C.c_int]),
+        "forward_matrices": (None, [P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, prefix + name, None)
@@ -96,6 +103,16 @@ class Impl:
         self.fn("predict_intra")(mav1hip.ptr(px), stride, x, y, mav1hip.ptr(a), mav1hip.ptr(dst), w)
         return dst
 
+    def recon(self, fp, blocks, sbs, src, ref_, gold, mats, rec=None):
+        """Returns (rec, coefs, blocks) after reconstructing the frame."""
+        blocks = np.ascontiguousarray(blocks, np.int32).copy()
+        sbs = np.ascontiguousarray(sbs, np.int32)
+        rec = np.zeros_like(src) if rec is None else rec.copy()
+        coefs = np.zeros(fp.sb_cols * fp.sb_rows * 6144, np.int32)
+        self.fn("recon")(C.byref(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs), mav1hip.ptr(src), mav1hip.ptr(rec),
+                         mav1hip.ptr(ref_), mav1hip.ptr(gold), mav1hip.ptr(coefs), mav1hip.ptr(mats))
+        return rec, coefs, blocks
+
     def qpel_build(self, fp, ref_):
         out = np.zeros(16 * fp.frame_w * fp.frame_h, np.uint8)
         self.fn("qpel_build")(C.byref(fp), mav1hip.ptr(ref_), mav1hip.ptr(out))
@@ -138,3 +155,28 @@ def compare_analysis(a, b, nsb):
     for s in range(nsb):
         n = sa[s, 1]
         assert np.array_equal(ba[s * 64:s * 64 + n], bb[s * 64:s * 64 + n]), f"blocks of SB {s} differ"
+
+
+TXW = [4, 8, 16, 32, 64, 4, 8, 8, 16, 16, 32, 32, 64, 4, 16, 8, 32, 16, 64]
+TXH = [4, 8, 16, 32, 64, 8, 4, 16, 8, 32, 16, 64, 32, 16, 4, 32, 8, 64, 16]
+ROW_KIND = [0, 0, 1, 1, 0, 1, 1, 1, 1, 2, 0, 2, 1, 2, 1, 2]  # per tx type: 0 DCT, 1 ADST, 2 identity
+COL_KIND = [0, 1, 0, 1, 1, 0, 1, 1, 1, 2, 2, 0, 2, 1, 2, 1]
+
+
+def valid_tx(txsz, t):
+    """Transform types a size can carry: ADST up to 16, identity up to 32."""
+    def ok(kind, n):
+        return kind == 0 or (kind == 1 and n <= 16) or (kind == 2 and n <= 32)
+    return ok(ROW_KIND[t], TXW[txsz]) and ok(COL_KIND[t], TXH[txsz])
+
+
+def ref_mats():
+    m = np.zeros(15 * 4096, np.int32)
+    ref().ref_forward_matrices(mav1hip.ptr(m))
+    return m
+
+
+def edge_types(impl, fp, blocks, sbs):
+    b = np.ascontiguousarray(blocks, np.int32).copy()
+    impl.fn("edge_types")(C.byref(fp), mav1hip.ptr(b), mav1hip.ptr(np.ascontiguousarray(sbs, np.int32)))
+    return b
