@@ -90,6 +90,12 @@ int32_t mav1hip_frame_qindex(const Mav1HipConfig *cfg, const Mav1HipFrame *fr);
 int32_t mav1hip_setup_params(const Mav1HipConfig *cfg, const Mav1HipFrame *fr, int32_t base_q_idx,
                              Mav1FrameParams *fp);
 
+/* The forward-transform matrices in Q20 (fwd_build_matrices,
+ * src/fwd_matrices.c:57-65): 15 slots of 4096 int32, slot
+ * (kind * 5 + log2(n) - 2) for kind 0 DCT, 1 ADST, 2 identity. Every context
+ * computes them at creation. */
+void mav1hip_forward_matrices(int32_t *mats);
+
 /* ------------------------------------------------------------------------
  * Context
  * ------------------------------------------------------------------------ */

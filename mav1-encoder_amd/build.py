@@ -13,7 +13,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmav1hip.so")
 
 HIP_SOURCES = ["api.hip", "me.hip", "analysis.hip"]
-C_SOURCES = ["params.c"]
+C_SOURCES = ["params.c", "fwdmats.c"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -39,7 +39,8 @@ def build(verbose=False):
         jobs.append(([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o], o))
     for s in C_SOURCES:
         o = os.path.join(BUILD, s.replace(".c", ".o"))
-        jobs.append((["gcc", "-O2", "-fPIC", "-std=gnu11", "-Wall", "-c", os.path.join(CSRC, s), "-o", o], o))
+        jobs.append((["gcc", "-O2", "-fPIC", "-std=gnu11", "-Wall", "-ffp-contract=off", "-c", os.path.join(CSRC, s),
+                      "-o", o], o))
     workers = min(len(jobs), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(workers) as ex:
         outs = list(ex.map(lambda j: _run(j[0]), jobs))
@@ -48,7 +49,7 @@ def build(verbose=False):
             if o.strip():
                 sys.stdout.write(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for _, o in jobs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for _, o in jobs] + ["-lm"])
     os.replace(tmp, LIB)
     return LIB
 

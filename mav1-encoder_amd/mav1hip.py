@@ -85,6 +85,7 @@ def _load():
         "mav1hip_geometry": (C.c_size_t, [C.POINTER(Config)] + [C.POINTER(i32)] * 5 + [C.POINTER(i32)] * 3),
         "mav1hip_frame_qindex": (i32, [C.POINTER(Config), C.POINTER(Frame)]),
         "mav1hip_setup_params": (i32, [C.POINTER(Config), C.POINTER(Frame), i32, C.POINTER(FrameParams)]),
+        "mav1hip_forward_matrices": (None, [P]),
         "mav1hip_create": (P, [C.POINTER(Config), C.c_char_p, C.c_size_t]),
         "mav1hip_destroy": (None, [P]),
         "mav1hip_error": (C.c_char_p, [P]),
@@ -159,6 +160,12 @@ def setup_params(cfg, fr, q=None):
     fp = FrameParams()
     n = lib().mav1hip_setup_params(C.byref(cfg), C.byref(fr), q, C.byref(fp))
     return fp, n
+
+
+def forward_matrices():
+    m = np.zeros(15 * 4096, np.int32)
+    lib().mav1hip_forward_matrices(ptr(m))
+    return m
 
 
 class HipError(RuntimeError):

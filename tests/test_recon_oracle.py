@@ -199,3 +199,12 @@ def test_edge_types_known_layout():
     assert f[3] == 0x20
     # SB 1 block: left neighbour (0,15) belongs to the other tile column: not available
     assert f[64] == 0
+
+
+@needs_ref
+def test_forward_matrices_equal_reference():
+    """The backend's own FP64 derivation of the Q20 forward matrices equals
+    the reference's fwd_build_matrices entry for entry."""
+    mine = mav1hip.forward_matrices()
+    assert np.array_equal(mine, libs.ref_mats())
+    assert np.count_nonzero(mine) > 3500
