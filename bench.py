@@ -71,6 +71,13 @@ def algorithmic_bytes(kernel, fp, nref):
         return F * (1 + nref) + items * (nref * 8 + 8 + 108)
     if kernel == "analyze_table":
         return nsb * 85 * (8 + 108) + nsb * (64 * 108 + 356)
+    if kernel == "recon_inter":
+        # prediction reads of the references, source read, reconstruction and levels written, blocks in/out
+        return F * (nref + 1) + F + 4 * F + nsb * 64 * 108 * 2
+    if kernel == "recon_intra":
+        return nsb * 64 * 108 // max(1, fp.tile_w_sb + 2 * (fp.sb_rows - 1))
+    if kernel == "edge_types":
+        return nsb * 64 * 108 * 2 + 2 * fp.mi_rows * fp.mi_cols
     return 0
 
 
@@ -109,6 +116,8 @@ def main():
         ctx.set_params(fp)
         ctx.launch_motion_search()
         ctx.launch_analyze()
+        ctx.launch_edge_types()
+        ctx.launch_recon()
 
     for s in order[:args.warmup]:
         step(*s)
@@ -122,7 +131,8 @@ def main():
     elapsed = time.perf_counter() - t0
     ctx.timing(False)
     kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3", "analyze_level0",
-               "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table"]
+               "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table", "edge_types", "recon_inter",
+               "recon_intra"]
     stats = {k: ctx.timing_get(k) for k in kernels}
     fps = args.steps / elapsed
 
@@ -158,7 +168,7 @@ def main():
         "data": "synthetic (seeded generator: 6 sinusoids + 12 moving rectangles + noise)",
         "config": {
             "workload": "1920x1080 8-bit 4:2:0, --turbo -q 125, turbo GOP (ARF + leaf), plain HIP path",
-            "stages": ["motion_search", "analyze"],
+            "stages": ["motion_search", "analyze", "edge_types", "recon"],
             "frame_kinds": "alternating hidden ALTREF (1 ref) / leaf (2 refs)",
             "references": "source pictures (reconstruction not in the step yet)",
             "device": ctx.device_name,

@@ -134,6 +134,21 @@ int mav1hip_analyze(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *sr
                     const uint8_t *gold, const int32_t *mvfield, const int32_t *mvgold, Mav1BlockInfo *blocks,
                     Mav1SbInfo *sbs, int64_t *acosts, Mav1BlockInfo *ainfos);
 
+/* Intra edge filter types of the blocks (the driver's edge_types_rows,
+ * src/encoder.c:319-346): interp_filter bits 4 / 5 from the smooth modes of
+ * the above and left neighbours; blocks updated in place. */
+int mav1hip_edge_types(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs);
+
+/* Backend.recon (src/backend_cpu.c:153-156, per superblock recon_sb of
+ * src/kernels/k_recon.h): prediction, transform type / depth choice,
+ * forward transform, quantisation and reconstruction of every block in
+ * decode order. blocks are completed in place (tx_type, eob, cfl alphas,
+ * tx_depth, tx_types, interp filter); rec (frame_bytes) receives the
+ * reconstruction inside the mi area (its other bytes are kept); coefs
+ * (num_sbs * 6144 int32) the quantised levels at each block's coef_off. */
+int mav1hip_recon(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
+                  const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold, int32_t *coefs);
+
 /* ------------------------------------------------------------------------
  * Device-resident form
  * ------------------------------------------------------------------------ */
@@ -149,7 +164,9 @@ enum {
   MAV1HIP_BUF_ACOSTS = 6,  /* pass-1 analysis costs, int64 per item */
   MAV1HIP_BUF_AINFOS = 7,  /* pass-1 analysis decisions */
   MAV1HIP_BUF_BLOCKS = 8,  /* coded blocks, 64 records per superblock */
-  MAV1HIP_BUF_SBS = 9      /* superblock records */
+  MAV1HIP_BUF_SBS = 9,     /* superblock records */
+  MAV1HIP_BUF_REC = 10,    /* reconstruction */
+  MAV1HIP_BUF_COEFS = 11   /* quantised levels, 6144 int32 per superblock */
 };
 /* Size in bytes of a device buffer. */
 size_t mav1hip_buffer_bytes(const mav1hip_ctx *c, int32_t buf);
@@ -160,6 +177,8 @@ int mav1hip_download(mav1hip_ctx *c, int32_t buf, void *host, size_t bytes);
 int mav1hip_set_params(mav1hip_ctx *c, const Mav1FrameParams *fp);
 int mav1hip_launch_motion_search(mav1hip_ctx *c);
 int mav1hip_launch_analyze(mav1hip_ctx *c);
+int mav1hip_launch_edge_types(mav1hip_ctx *c);
+int mav1hip_launch_recon(mav1hip_ctx *c);
 /* Waits for everything enqueued on the context's stream. */
 int mav1hip_sync(mav1hip_ctx *c);
 

@@ -12,7 +12,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmav1hip.so")
 
-HIP_SOURCES = ["api.hip", "me.hip", "analysis.hip"]
+HIP_SOURCES = ["api.hip", "me.hip", "analysis.hip", "recon.hip"]
 C_SOURCES = ["params.c", "fwdmats.c"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

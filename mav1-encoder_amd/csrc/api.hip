@@ -41,6 +41,9 @@ struct mav1hip_ctx {
   long long *d_acosts;
   Mav1BlockInfo *d_ainfos, *d_blocks;
   Mav1SbInfo *d_sbs;
+  uint8_t *d_rec;
+  int32_t *d_coefs, *d_mats;
+  uint8_t *d_ys, *d_uvs;
   long entry_bytes;
   size_t planes_bytes;
   /* frames the stages read (own buffers or pool frames) */
@@ -170,6 +173,11 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   rc |= dmalloc(c, (void **)&c->d_ainfos, sizeof(Mav1BlockInfo) * MAV1_A_ITEMS * (size_t)c->num_sbs);
   rc |= dmalloc(c, (void **)&c->d_blocks, sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs);
   rc |= dmalloc(c, (void **)&c->d_sbs, sizeof(Mav1SbInfo) * (size_t)c->num_sbs);
+  rc |= dmalloc(c, (void **)&c->d_rec, c->frame_bytes);
+  rc |= dmalloc(c, (void **)&c->d_coefs, sizeof(int32_t) * MAV1_COEFS_PER_SB * (size_t)c->num_sbs);
+  rc |= dmalloc(c, (void **)&c->d_mats, sizeof(int32_t) * 4096 * 15);
+  rc |= dmalloc(c, (void **)&c->d_ys, (size_t)c->mi_rows * c->mi_cols);
+  rc |= dmalloc(c, (void **)&c->d_uvs, (size_t)c->mi_rows * c->mi_cols);
   if (rc != MAV1HIP_OK) {
     if (err && errlen) snprintf(err, errlen, "%s", c->err);
     mav1hip_destroy(c);
@@ -184,6 +192,19 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   (void)hipMemsetAsync(c->d_ainfos, 0, sizeof(Mav1BlockInfo) * MAV1_A_ITEMS * (size_t)c->num_sbs, c->stream);
   (void)hipMemsetAsync(c->d_blocks, 0, sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs, c->stream);
   (void)hipMemsetAsync(c->d_sbs, 0, sizeof(Mav1SbInfo) * (size_t)c->num_sbs, c->stream);
+  (void)hipMemsetAsync(c->d_rec, 0, c->frame_bytes, c->stream);
+  (void)hipMemsetAsync(c->d_coefs, 0, sizeof(int32_t) * MAV1_COEFS_PER_SB * (size_t)c->num_sbs, c->stream);
+  (void)hipMemsetAsync(c->d_ys, 0, (size_t)c->mi_rows * c->mi_cols, c->stream);
+  (void)hipMemsetAsync(c->d_uvs, 0, (size_t)c->mi_rows * c->mi_cols, c->stream);
+  {
+    static int32_t mats[4096 * 15];
+    static int have = 0;
+    if (!have) {
+      mav1hip_forward_matrices(mats);
+      have = 1;
+    }
+    (void)hipMemcpyAsync(c->d_mats, mats, sizeof mats, hipMemcpyHostToDevice, c->stream);
+  }
   (void)hipStreamSynchronize(c->stream);
   c->b_src = c->d_src;
   c->b_ref = c->d_ref;
@@ -209,6 +230,11 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipFree(c->d_ainfos);
   (void)hipFree(c->d_blocks);
   (void)hipFree(c->d_sbs);
+  (void)hipFree(c->d_rec);
+  (void)hipFree(c->d_coefs);
+  (void)hipFree(c->d_mats);
+  (void)hipFree(c->d_ys);
+  (void)hipFree(c->d_uvs);
   (void)hipFree(c->d_pool);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -232,6 +258,8 @@ static void *buf_ptr(mav1hip_ctx *c, int buf, size_t *bytes) {
       *bytes = sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs;
       return c->d_blocks;
     case MAV1HIP_BUF_SBS: *bytes = sizeof(Mav1SbInfo) * (size_t)c->num_sbs; return c->d_sbs;
+    case MAV1HIP_BUF_REC: *bytes = c->frame_bytes; return c->d_rec;
+    case MAV1HIP_BUF_COEFS: *bytes = sizeof(int32_t) * MAV1_COEFS_PER_SB * (size_t)c->num_sbs; return c->d_coefs;
     default: *bytes = 0; return nullptr;
   }
 }
@@ -360,6 +388,69 @@ int mav1hip_analyze(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *sr
   if (ainfos)
     HIPCHK(c, hipMemcpyAsync(ainfos, c->d_ainfos, sizeof(Mav1BlockInfo) * MAV1_A_ITEMS * (size_t)c->num_sbs,
                              hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_launch_edge_types(mav1hip_ctx *c) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "edge types: no frame parameters set");
+  return timed(c, "edge_types",
+               [&] { return launch_edge_types(c->stream, c->fp, c->d_blocks, c->d_sbs, c->d_ys, c->d_uvs); });
+}
+
+int mav1hip_launch_recon(mav1hip_ctx *c) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "recon: no frame parameters set");
+  const Mav1FrameParams &fp = c->fp;
+  int rc = timed(c, "recon_inter", [&] {
+    return launch_recon_inter(c->stream, fp, c->d_blocks, c->d_sbs, c->b_src, c->d_rec, c->b_ref, c->b_gold,
+                              c->d_coefs, c->d_mats);
+  });
+  if (rc) return rc;
+  const int steps = recon_intra_steps(fp);
+  for (int t = 0; t < steps; t++)
+    if ((rc = timed(c, "recon_intra", [&] {
+           return launch_recon_intra(c->stream, fp, t, c->d_blocks, c->d_sbs, c->b_src, c->d_rec, c->b_ref, c->b_gold,
+                                     c->d_coefs, c->d_mats);
+         })))
+      return rc;
+  return MAV1HIP_OK;
+}
+
+int mav1hip_edge_types(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs) {
+  if (!c || !fp || !blocks || !sbs) return fail(c, MAV1HIP_EINVAL, "edge types: missing buffer");
+  int rc;
+  if ((rc = mav1hip_set_params(c, fp))) return rc;
+  size_t nb = sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs;
+  HIPCHK(c, hipMemcpyAsync(c->d_blocks, blocks, nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_sbs, sbs, sizeof(Mav1SbInfo) * (size_t)c->num_sbs, hipMemcpyHostToDevice, c->stream));
+  if ((rc = mav1hip_launch_edge_types(c))) return rc;
+  HIPCHK(c, hipMemcpyAsync(blocks, c->d_blocks, nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_recon(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
+                  const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold, int32_t *coefs) {
+  if (!c || !fp || !blocks || !sbs || !src || !rec || !coefs) return fail(c, MAV1HIP_EINVAL, "recon: missing buffer");
+  if (!fp->frame_is_intra && !ref) return fail(c, MAV1HIP_EINVAL, "recon: inter frame without LAST");
+  int rc;
+  if ((rc = mav1hip_set_params(c, fp))) return rc;
+  c->b_src = c->d_src;
+  c->b_ref = c->d_ref;
+  c->b_gold = c->d_gold;
+  size_t nb = sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs;
+  size_t ncoef = sizeof(int32_t) * MAV1_COEFS_PER_SB * (size_t)c->num_sbs;
+  HIPCHK(c, hipMemcpyAsync(c->d_blocks, blocks, nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_sbs, sbs, sizeof(Mav1SbInfo) * (size_t)c->num_sbs, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_src, src, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_rec, rec, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_coefs, coefs, ncoef, hipMemcpyHostToDevice, c->stream));
+  if (ref) HIPCHK(c, hipMemcpyAsync(c->d_ref, ref, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  if (gold) HIPCHK(c, hipMemcpyAsync(c->d_gold, gold, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  if ((rc = mav1hip_launch_recon(c))) return rc;
+  HIPCHK(c, hipMemcpyAsync(blocks, c->d_blocks, nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(rec, c->d_rec, c->frame_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(coefs, c->d_coefs, ncoef, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MAV1HIP_OK;
 }

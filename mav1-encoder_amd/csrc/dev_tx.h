@@ -1,0 +1,412 @@
+/* Device transforms: the AV1 inverse 1-D transforms (spec 7.13.2) as fully
+ * unrolled templates whose working array stays in registers, and the
+ * wave-cooperative 2-D forward / inverse transforms and quantiser of one
+ * transform block (behaviour of src/kernels/k_itx.h and k_fwd.h). */
+#ifndef MAV1_DEV_TX_H
+#define MAV1_DEV_TX_H
+
+#include "dev_common.h"
+
+DEVI int c128(int angle) {
+  int a = angle & 255;
+  return a <= 64 ? Cos128_Lookup[a] : a <= 128 ? -Cos128_Lookup[128 - a] : a <= 192 ? -Cos128_Lookup[a - 128]
+                                                                                     : Cos128_Lookup[256 - a];
+}
+__host__ __device__ constexpr int cbrev(int bits, int x) {
+  int r = 0;
+  for (int i = 0; i < bits; i++) r |= ((x >> i) & 1) << (bits - 1 - i);
+  return r;
+}
+DEVI void tB(int32_t *T, int a, int b, int angle, int flip) {
+  const int c = c128(angle), s = c128(angle - 64);
+  long long x = (long long)T[a] * c - (long long)T[b] * s;
+  long long y = (long long)T[a] * s + (long long)T[b] * c;
+  int32_t xa = (int32_t)((x + 2048) >> 12), yb = (int32_t)((y + 2048) >> 12);
+  T[a] = flip ? yb : xa;
+  T[b] = flip ? xa : yb;
+}
+DEVI void tH(int32_t *T, int a, int b, int flip, int r) {
+  const int lo = -(1 << (r - 1)), hi = (1 << (r - 1)) - 1;
+  int32_t x = T[flip ? b : a], y = T[flip ? a : b];
+  int32_t s = d_clip(lo, hi, x + y), d = d_clip(lo, hi, x - y);
+  if (flip) {
+    T[b] = s;
+    T[a] = d;
+  } else {
+    T[a] = s;
+    T[b] = d;
+  }
+}
+
+template <int n>
+DEVI void t_idct(int32_t *T, int r) {
+  constexpr int N0 = 1 << n;
+  int32_t c[N0];
+#pragma unroll
+  for (int i = 0; i < N0; i++) c[i] = T[i];
+#pragma unroll
+  for (int i = 0; i < N0; i++) T[i] = c[cbrev(n, i)];
+  if constexpr (n == 6) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) tB(T, 32 + i, 63 - i, 63 - 4 * cbrev(4, i), 0);
+  }
+  if constexpr (n >= 5) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) tB(T, 16 + i, 31 - i, 6 + (cbrev(3, 7 - i) << 3), 0);
+  }
+  if constexpr (n == 6) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) tH(T, 32 + i * 2, 33 + i * 2, i & 1, r);
+  }
+  if constexpr (n >= 4) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) tB(T, 8 + i, 15 - i, 12 + (cbrev(2, 3 - i) << 4), 0);
+  }
+  if constexpr (n >= 5) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) tH(T, 16 + 2 * i, 17 + 2 * i, i & 1, r);
+  }
+  if constexpr (n == 6) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) tB(T, 62 - i * 4 - j, 33 + i * 4 + j, 60 - 16 * cbrev(2, i) + 64 * j, 1);
+  }
+  if constexpr (n >= 3) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) tB(T, 4 + i, 7 - i, 56 - 32 * i, 0);
+  }
+  if constexpr (n >= 4) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) tH(T, 8 + 2 * i, 9 + 2 * i, i & 1, r);
+  }
+  if constexpr (n >= 5) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) tB(T, 30 - 4 * i - j, 17 + 4 * i + j, 24 + (j << 6) + ((1 - i) << 5), 1);
+  }
+  if constexpr (n == 6) {
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) tH(T, 32 + i * 4 + j, 35 + i * 4 - j, i & 1, r);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++) tB(T, 2 * i, 2 * i + 1, 32 + 16 * i, 1 - i);
+  if constexpr (n >= 3) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) tH(T, 4 + 2 * i, 5 + 2 * i, i, r);
+  }
+  if constexpr (n >= 4) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) tB(T, 14 - i, 9 + i, 48 + 64 * i, 1);
+  }
+  if constexpr (n >= 5) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) tH(T, 16 + 4 * i + j, 19 + 4 * i - j, i & 1, r);
+  }
+  if constexpr (n == 6) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) tB(T, 61 - i * 8 - j, 34 + i * 8 + j, 56 - i * 32 + (j >> 1) * 64, 1);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++) tH(T, i, 3 - i, 0, r);
+  if constexpr (n >= 3) tB(T, 6, 5, 32, 1);
+  if constexpr (n >= 4) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) tH(T, 8 + 4 * i + j, 11 + 4 * i - j, i, r);
+  }
+  if constexpr (n >= 5) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) tB(T, 29 - i, 18 + i, 48 + (i >> 1) * 64, 1);
+  }
+  if constexpr (n == 6) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) tH(T, 32 + 8 * i + j, 39 + 8 * i - j, i & 1, r);
+  }
+  if constexpr (n >= 3) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) tH(T, i, 7 - i, 0, r);
+  }
+  if constexpr (n >= 4) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) tB(T, 13 - i, 10 + i, 32, 1);
+  }
+  if constexpr (n >= 5) {
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) tH(T, 16 + i * 8 + j, 23 + i * 8 - j, i, r);
+  }
+  if constexpr (n == 6) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) tB(T, 59 - i, 36 + i, i < 4 ? 48 : 112, 1);
+  }
+  if constexpr (n >= 4) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) tH(T, i, 15 - i, 0, r);
+  }
+  if constexpr (n >= 5) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) tB(T, 27 - i, 20 + i, 32, 1);
+  }
+  if constexpr (n == 6) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      tH(T, 32 + i, 47 - i, 0, r);
+      tH(T, 48 + i, 63 - i, 1, r);
+    }
+  }
+  if constexpr (n >= 5) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) tH(T, i, 31 - i, 0, r);
+  }
+  if constexpr (n == 6) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) tB(T, 55 - i, 40 + i, 32, 1);
+#pragma unroll
+    for (int i = 0; i < 32; i++) tH(T, i, 63 - i, 0, r);
+  }
+}
+
+template <int n>
+DEVI void t_adst_in(int32_t *T) {
+  constexpr int N0 = 1 << n;
+  int32_t c[N0];
+#pragma unroll
+  for (int i = 0; i < N0; i++) c[i] = T[i];
+#pragma unroll
+  for (int i = 0; i < N0; i++) T[i] = c[(i & 1) ? i - 1 : N0 - i - 1];
+}
+__host__ __device__ constexpr int adst_out_idx(int i, int n) {
+  return ((((i & 1) ^ ((i >> 1) & 1)) << 3) | ((((i >> 1) & 1) ^ ((i >> 2) & 1)) << 2) |
+          ((((i >> 2) & 1) ^ ((i >> 3) & 1)) << 1) | ((i >> 3) & 1)) >> (4 - n);
+}
+template <int n>
+DEVI void t_adst_out(int32_t *T) {
+  constexpr int N0 = 1 << n;
+  int32_t c[N0];
+#pragma unroll
+  for (int i = 0; i < N0; i++) c[i] = T[i];
+#pragma unroll
+  for (int i = 0; i < N0; i++) T[i] = (i & 1) ? -c[adst_out_idx(i, n)] : c[adst_out_idx(i, n)];
+}
+
+DEVI void t_adst4(int32_t *T) {
+  long long s0 = 1321LL * T[0], s1 = 2482LL * T[0], s2 = 3344LL * T[1], s3 = 3803LL * T[2], s4 = 1321LL * T[2],
+            s5 = 2482LL * T[3], s6 = 3803LL * T[3], b7 = (long long)T[0] - T[2] + T[3];
+  s0 += s3;
+  s1 -= s4;
+  s3 = s2;
+  s2 = 3344LL * b7;
+  s0 += s5;
+  s1 -= s6;
+  T[0] = (int32_t)((s0 + s3 + 2048) >> 12);
+  T[1] = (int32_t)((s1 + s3 + 2048) >> 12);
+  T[2] = (int32_t)((s2 + 2048) >> 12);
+  T[3] = (int32_t)((s0 + s1 - s3 + 2048) >> 12);
+}
+DEVI void t_adst8(int32_t *T, int r) {
+  t_adst_in<3>(T);
+#pragma unroll
+  for (int i = 0; i < 4; i++) tB(T, 2 * i, 2 * i + 1, 60 - 16 * i, 1);
+#pragma unroll
+  for (int i = 0; i < 4; i++) tH(T, i, 4 + i, 0, r);
+#pragma unroll
+  for (int i = 0; i < 2; i++) tB(T, 4 + 3 * i, 5 + i, 48 - 32 * i, 1);
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) tH(T, 4 * j + i, 2 + 4 * j + i, 0, r);
+#pragma unroll
+  for (int i = 0; i < 2; i++) tB(T, 2 + 4 * i, 3 + 4 * i, 32, 1);
+  t_adst_out<3>(T);
+}
+DEVI void t_adst16(int32_t *T, int r) {
+  t_adst_in<4>(T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) tB(T, 2 * i, 2 * i + 1, 62 - 8 * i, 1);
+#pragma unroll
+  for (int i = 0; i < 8; i++) tH(T, i, 8 + i, 0, r);
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    tB(T, 8 + 2 * i, 9 + 2 * i, 56 - 32 * i, 1);
+    tB(T, 13 + 2 * i, 12 + 2 * i, 8 + 32 * i, 1);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) tH(T, 8 * j + i, 4 + 8 * j + i, 0, r);
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) tB(T, 4 + 8 * j + 3 * i, 5 + 8 * j + i, 48 - 32 * i, 1);
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) tH(T, 4 * j + i, 2 + 4 * j + i, 0, r);
+#pragma unroll
+  for (int i = 0; i < 4; i++) tB(T, 2 + 4 * i, 3 + 4 * i, 32, 1);
+  t_adst_out<4>(T);
+}
+template <int n>
+DEVI void t_identity(int32_t *T) {
+#pragma unroll
+  for (int i = 0; i < (1 << n); i++) {
+    if constexpr (n == 2) T[i] = (int32_t)(((long long)T[i] * 5793 + 2048) >> 12);
+    else if constexpr (n == 3) T[i] = T[i] * 2;
+    else if constexpr (n == 4) T[i] = (int32_t)(((long long)T[i] * 11586 + 2048) >> 12);
+    else T[i] = T[i] * 4;
+  }
+}
+
+/* 1-D inverse transform of kind 0 DCT, 1 ADST, 2 identity, length 1 << n. */
+DEVI void t_itx1d(int32_t *T, int kind, int n, int r) {
+  if (kind == 0) {
+    switch (n) {
+      case 2: t_idct<2>(T, r); break;
+      case 3: t_idct<3>(T, r); break;
+      case 4: t_idct<4>(T, r); break;
+      case 5: t_idct<5>(T, r); break;
+      default: t_idct<6>(T, r); break;
+    }
+  } else if (kind == 1) {
+    if (n == 2) t_adst4(T);
+    else if (n == 3) t_adst8(T, r);
+    else t_adst16(T, r);
+  } else {
+    switch (n) {
+      case 2: t_identity<2>(T); break;
+      case 3: t_identity<3>(T); break;
+      case 4: t_identity<4>(T); break;
+      case 5: t_identity<5>(T); break;
+      default: t_identity<6>(T); break;
+    }
+  }
+}
+
+DEVI int row_kind(int t) {
+  if (t == DCT_DCT || t == ADST_DCT || t == FLIPADST_DCT || t == H_DCT) return 0;
+  if (t == DCT_ADST || t == ADST_ADST || t == DCT_FLIPADST || t == FLIPADST_FLIPADST || t == ADST_FLIPADST ||
+      t == FLIPADST_ADST || t == H_ADST || t == H_FLIPADST)
+    return 1;
+  return 2;
+}
+DEVI int col_kind(int t) {
+  if (t == DCT_DCT || t == DCT_ADST || t == DCT_FLIPADST || t == V_DCT) return 0;
+  if (t == ADST_DCT || t == ADST_ADST || t == FLIPADST_DCT || t == FLIPADST_FLIPADST || t == ADST_FLIPADST ||
+      t == FLIPADST_ADST || t == V_ADST || t == V_FLIPADST)
+    return 1;
+  return 2;
+}
+DEVI int flip_ud(int t) { return t == FLIPADST_DCT || t == FLIPADST_ADST || t == V_FLIPADST || t == FLIPADST_FLIPADST; }
+DEVI int flip_lr(int t) { return t == DCT_FLIPADST || t == ADST_FLIPADST || t == H_FLIPADST || t == FLIPADST_FLIPADST; }
+DEVI int dq_denom(int tx) {
+  if (tx == TX_32X32 || tx == TX_16X32 || tx == TX_32X16 || tx == TX_16X64 || tx == TX_64X16) return 2;
+  if (tx == TX_64X64 || tx == TX_32X64 || tx == TX_64X32) return 4;
+  return 1;
+}
+
+/* 2-D inverse transform (7.13.3): dq holds Min(32,h) x Min(32,w) values with
+ * row stride 32; res receives h x w residuals with row stride 64. One lane
+ * per row, then one lane per column. */
+DEVI void wave_itx2d(const int32_t *dq, int32_t *res, int txSz, int txType, int lane) {
+  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
+  const int rowShift = Transform_Row_Shift[txSz], rk = row_kind(txType), ck = col_kind(txType);
+  const bool rect = (lw - lh == 1) || (lh - lw == 1);
+  for (int i = lane; i < h; i += 64) {
+    int32_t T[64];
+#pragma unroll
+    for (int j = 0; j < 64; j++) {
+      int32_t v = (i < 32 && j < 32 && j < w) ? dq[i * 32 + j] : 0;
+      T[j] = rect ? (int32_t)(((long long)v * 2896 + 2048) >> 12) : v;
+    }
+    t_itx1d(T, rk, lw, 16);
+#pragma unroll
+    for (int j = 0; j < 64; j++)
+      if (j < w) res[i * 64 + j] = d_clip(-(1 << 15), (1 << 15) - 1, d_round2(T[j], rowShift));
+  }
+  __syncthreads();
+  for (int j = lane; j < w; j += 64) {
+    int32_t T[64];
+#pragma unroll
+    for (int i = 0; i < 64; i++) T[i] = i < h ? res[i * 64 + j] : 0;
+    t_itx1d(T, ck, lh, 16);
+#pragma unroll
+    for (int i = 0; i < 64; i++)
+      if (i < h) res[i * 64 + j] = d_round2(T[i], 4);
+  }
+  __syncthreads();
+}
+
+/* Forward 2-D transform through the Q20 matrices (k_fwd.h:35-59): resid
+ * h x w (stride 64) -> coef Min(32,h) x Min(32,w) (stride 32); tmp holds
+ * h x Min(32,w) (stride 32). */
+DEVI void wave_fwd2d(const int32_t *resid, int32_t *tmp, int32_t *coef, int txSz, int txType,
+                     const int32_t *__restrict__ mats, int lane) {
+  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
+  const int tw = d_min(32, w), th = d_min(32, h);
+  const int32_t *Mh = mats + (row_kind(txType) * 5 + lw - 2) * 4096;
+  const int32_t *Mv = mats + (col_kind(txType) * 5 + lh - 2) * 4096;
+  for (int k = lane; k < h * tw; k += 64) {
+    int i = k / tw, kk = k - i * tw;
+    const int32_t *m = Mh + kk * w, *rr = resid + i * 64;
+    long long s = 0;
+    for (int j = 0; j < w; j++) s += (long long)rr[j] * m[j];
+    tmp[i * 32 + kk] = (int32_t)((s + 32768) >> 16);
+  }
+  __syncthreads();
+  const int sh = Transform_Row_Shift[txSz] + 4;
+  const bool rect = (lw - lh == 1) || (lh - lw == 1);
+  for (int k = lane; k < th * tw; k += 64) {
+    int k2 = k / tw, kk = k - k2 * tw;
+    const int32_t *m = Mv + k2 * h;
+    long long s = 0;
+    for (int i = 0; i < h; i++) s += (long long)m[i] * tmp[i * 32 + kk];
+    s *= (1ll << sh);
+    if (rect) s = ((s + 2048) >> 12) * 5793;
+    coef[k2 * 32 + kk] = (int32_t)(s >= 0 ? (s + (1ll << 19)) >> 20 : -((-s + (1ll << 19)) >> 20));
+  }
+  __syncthreads();
+}
+
+/* Dead-zone quantiser (k_fwd.h:66-90) into the decoder layout (stride tw). */
+DEVI void wave_quantize(const int32_t *coef, int32_t *q, int txSz, int dcQ, int acQ, int lane) {
+  const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
+  const unsigned long long invDc = (1ull << 44) / (unsigned)dcQ + 1, invAc = (1ull << 44) / (unsigned)acQ + 1;
+  for (int k = lane; k < th * tw; k += 64) {
+    int i = k / tw, j = k - i * tw;
+    const bool dc = k == 0;
+    const int qv = dc ? dcQ : acQ, rnd = dc ? 26 : 22;
+    const int v = coef[i * 32 + j];
+    long long a = (long long)d_abs(v) * denom;
+    long long t = (a + (((long long)qv * rnd) >> 2)) >> 4;
+    int level = t < (1ll << 19) ? (int)(((unsigned long long)t * (dc ? invDc : invAc)) >> 44) : (int)(t / qv);
+    if (level > 0xFFFF) level = 0xFFFF;
+    q[k] = v < 0 ? -level : level;
+  }
+  __syncthreads();
+}
+
+/* Dequantisation (spec 7.12.3), output stride 32. */
+DEVI void wave_dequant(const int32_t *q, int32_t *dq, int txSz, int dcQ, int acQ, int lane) {
+  const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
+  for (int k = lane; k < th * tw; k += 64) {
+    int i = k / tw, j = k - i * tw;
+    long long d = (long long)q[k] * (k == 0 ? dcQ : acQ), ad = d < 0 ? -d : d;
+    long long d2 = (ad & 0xFFFFFF) >> (denom >> 1);
+    dq[i * 32 + j] = d_clip(-(1 << 15), (1 << 15) - 1, (int)(d < 0 ? -d2 : d2));
+  }
+  __syncthreads();
+}
+
+#endif

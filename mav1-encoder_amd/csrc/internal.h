@@ -29,4 +29,15 @@ hipError_t launch_analyze(hipStream_t st, int level, const Mav1FrameParams &fp, 
 hipError_t launch_analyze_table(hipStream_t st, const Mav1FrameParams &fp, const long long *costs,
                                 const Mav1BlockInfo *infos, Mav1BlockInfo *blocks, Mav1SbInfo *sbs);
 
+/* Reconstruction: inter blocks (all at once), then the intra wavefront. */
+hipError_t launch_edge_types(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
+                             uint8_t *ys, uint8_t *uvs);
+hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
+                              const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold,
+                              int32_t *coefs, const int32_t *mats);
+int recon_intra_steps(const Mav1FrameParams &fp);
+hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, int t, Mav1BlockInfo *blocks,
+                              const Mav1SbInfo *sbs, const uint8_t *src, uint8_t *rec, const uint8_t *ref,
+                              const uint8_t *gold, int32_t *coefs, const int32_t *mats);
+
 #endif

@@ -16,7 +16,7 @@ NUM_NODES = 85
 A_SLOTS = 13
 TURBO_DEFAULT = 0x47CBF1
 (BUF_SRC, BUF_REF, BUF_GOLD, BUF_MVFIELD, BUF_MVGOLD, BUF_PLANES, BUF_ACOSTS, BUF_AINFOS, BUF_BLOCKS,
- BUF_SBS) = range(10)
+ BUF_SBS, BUF_REC, BUF_COEFS) = range(12)
 
 
 class FrameParams(C.Structure):
@@ -99,6 +99,10 @@ def _load():
         "mav1hip_launch_motion_search": (C.c_int, [P]),
         "mav1hip_analyze": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, P, P, P, P, P, P]),
         "mav1hip_launch_analyze": (C.c_int, [P]),
+        "mav1hip_edge_types": (C.c_int, [P, C.POINTER(FrameParams), P, P]),
+        "mav1hip_recon": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, P, P, P, P]),
+        "mav1hip_launch_edge_types": (C.c_int, [P]),
+        "mav1hip_launch_recon": (C.c_int, [P]),
         "mav1hip_sync": (C.c_int, [P]),
         "mav1hip_pool_alloc": (C.c_int, [P, i32]),
         "mav1hip_pool_upload": (C.c_int, [P, i32, P]),
@@ -223,6 +227,28 @@ class Context:
                                    ptr(blocks), ptr(sbs), ptr(costs), ptr(infos))
         self._chk(rc, "analyze")
         return costs, infos, blocks, sbs
+
+    def edge_types(self, fp, blocks, sbs):
+        b = np.ascontiguousarray(blocks, np.int32).copy()
+        self._chk(lib().mav1hip_edge_types(self.h, C.byref(fp), ptr(b), ptr(np.ascontiguousarray(sbs, np.int32))),
+                  "edge_types")
+        return b
+
+    def recon(self, fp, blocks, sbs, src, ref, gold, rec=None):
+        """Returns (rec, coefs, blocks)."""
+        b = np.ascontiguousarray(blocks, np.int32).copy()
+        rec = np.zeros_like(src) if rec is None else rec.copy()
+        coefs = np.zeros(fp.sb_cols * fp.sb_rows * 6144, np.int32)
+        rc = lib().mav1hip_recon(self.h, C.byref(fp), ptr(b), ptr(np.ascontiguousarray(sbs, np.int32)), ptr(src),
+                                 ptr(rec), ptr(ref), ptr(gold), ptr(coefs))
+        self._chk(rc, "recon")
+        return rec, coefs, b
+
+    def launch_edge_types(self):
+        self._chk(lib().mav1hip_launch_edge_types(self.h), "launch_edge_types")
+
+    def launch_recon(self):
+        self._chk(lib().mav1hip_launch_recon(self.h), "launch_recon")
 
     def launch_analyze(self):
         self._chk(lib().mav1hip_launch_analyze(self.h), "launch_analyze")
