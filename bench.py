@@ -193,14 +193,20 @@ def main():
 
 
 def cpu_baseline(seconds, frames, params, uniq):
-    """The reference's own motion-search kernels (oracle/_ref), serial on one
-    host core, over the same frame sequence until `seconds` have passed."""
+    """The reference's own kernels (oracle/_ref: motion search, analysis,
+    reconstruction; the edge filter types from the oracle, as the reference's
+    are in its driver), serial on one host core, over the same frame
+    sequence until `seconds` have passed."""
     path = os.path.join(ROOT, "oracle", "_ref", "libmav1ref.so")
     kind, pre = "reference", "ref_"
     if not os.path.exists(path):
         path, kind, pre = os.path.join(ROOT, "oracle", "liboracle.so"), "port", "oracle_"
     lib = C.CDLL(path)
-    me, ana = getattr(lib, pre + "motion_search"), getattr(lib, pre + "analyze")
+    me, ana, rec_fn = getattr(lib, pre + "motion_search"), getattr(lib, pre + "analyze"), getattr(lib, pre + "recon")
+    orc = C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))  # the driver's edge filter types
+    mats = mav1hip.forward_matrices()
+    rec = np.zeros_like(frames[0])
+    coefs = np.zeros(params["arf"].sb_cols * params["arf"].sb_rows * 6144, np.int32)
     nsb = params["arf"].sb_cols * params["arf"].sb_rows
     mv = np.zeros(nsb * 170, np.int32)
     mvg = np.zeros(nsb * 170, np.int32)
@@ -217,6 +223,9 @@ def cpu_baseline(seconds, frames, params, uniq):
         me(C.byref(fp), s, r, g, mav1hip.ptr(mv), mav1hip.ptr(mvg))
         ana(C.byref(fp), s, r, g, mav1hip.ptr(mv), mav1hip.ptr(mvg), mav1hip.ptr(costs), mav1hip.ptr(infos),
             mav1hip.ptr(blocks), mav1hip.ptr(sbs))
+        orc.oracle_edge_types(C.byref(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs))
+        rec_fn(C.byref(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs), s, mav1hip.ptr(rec), r, g, mav1hip.ptr(coefs),
+               mav1hip.ptr(mats))
         done += 1
         if time.perf_counter() - t0 >= seconds and done % 2 == 0:
             break
@@ -226,7 +235,7 @@ def cpu_baseline(seconds, frames, params, uniq):
         "unit": "frames/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"{done} frames of the same 1080p turbo GOP (ARF/leaf alternating), motion search + analysis, "
+        "sample": f"{done} frames of the same 1080p turbo GOP (ARF/leaf alternating), motion search + analysis + edge types + reconstruction, "
                   f"{dt:.1f} s on one host core ({os.cpu_count()} visible)",
     }
 
