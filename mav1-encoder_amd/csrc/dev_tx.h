@@ -318,40 +318,89 @@ DEVI int dq_denom(int tx) {
 
 /* 2-D inverse transform (7.13.3): dq holds Min(32,h) x Min(32,w) values with
  * row stride 32; res receives h x w residuals with row stride 64. One lane
- * per row, then one lane per column. */
-DEVI void wave_itx2d(const int32_t *dq, int32_t *res, int txSz, int txType, int lane) {
-  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
-  const int rowShift = Transform_Row_Shift[txSz], rk = row_kind(txType), ck = col_kind(txType);
-  const bool rect = (lw - lh == 1) || (lh - lw == 1);
+ * per row, then one lane per column; each (kind, length) pass is its own
+ * non-inlined function so the unrolled butterflies exist once in the code
+ * object. */
+template <int KIND, int N>
+DEVI void t_1d(int32_t *T, int r) {
+  if constexpr (KIND == 0) t_idct<N>(T, r);
+  else if constexpr (KIND == 1) {
+    if constexpr (N == 2) t_adst4(T);
+    else if constexpr (N == 3) t_adst8(T, r);
+    else t_adst16(T, r);
+  } else
+    t_identity<N>(T);
+}
+
+template <int KIND, int N>
+__device__ __attribute__((noinline)) void itx_rows(const int32_t *dq, int32_t *res, int h, int rect, int rowShift,
+                                                    int lane) {
+  constexpr int W = 1 << N;
   for (int i = lane; i < h; i += 64) {
-    int32_t T[64];
+    int32_t T[W];
 #pragma unroll
-    for (int j = 0; j < 64; j++) {
-      int32_t v = (i < 32 && j < 32 && j < w) ? dq[i * 32 + j] : 0;
+    for (int j = 0; j < W; j++) {
+      int32_t v = (i < 32 && j < 32) ? dq[i * 32 + j] : 0;
       T[j] = rect ? (int32_t)(((long long)v * 2896 + 2048) >> 12) : v;
     }
-    t_itx1d(T, rk, lw, 16);
+    t_1d<KIND, N>(T, 16);
 #pragma unroll
-    for (int j = 0; j < 64; j++)
-      if (j < w) res[i * 64 + j] = d_clip(-(1 << 15), (1 << 15) - 1, d_round2(T[j], rowShift));
+    for (int j = 0; j < W; j++) res[i * 64 + j] = d_clip(-(1 << 15), (1 << 15) - 1, d_round2(T[j], rowShift));
   }
-  __syncthreads();
+}
+template <int KIND, int N>
+__device__ __attribute__((noinline)) void itx_cols(int32_t *res, int w, int lane) {
+  constexpr int H = 1 << N;
   for (int j = lane; j < w; j += 64) {
-    int32_t T[64];
+    int32_t T[H];
 #pragma unroll
-    for (int i = 0; i < 64; i++) T[i] = i < h ? res[i * 64 + j] : 0;
-    t_itx1d(T, ck, lh, 16);
+    for (int i = 0; i < H; i++) T[i] = res[i * 64 + j];
+    t_1d<KIND, N>(T, 16);
 #pragma unroll
-    for (int i = 0; i < 64; i++)
-      if (i < h) res[i * 64 + j] = d_round2(T[i], 4);
+    for (int i = 0; i < H; i++) res[i * 64 + j] = d_round2(T[i], 4);
   }
+}
+
+#define ITX_DISPATCH(FN, kind, n, ...)                                      \
+  do {                                                                      \
+    if (kind == 0) {                                                        \
+      switch (n) {                                                          \
+        case 2: FN<0, 2>(__VA_ARGS__); break;                               \
+        case 3: FN<0, 3>(__VA_ARGS__); break;                               \
+        case 4: FN<0, 4>(__VA_ARGS__); break;                               \
+        case 5: FN<0, 5>(__VA_ARGS__); break;                               \
+        default: FN<0, 6>(__VA_ARGS__); break;                              \
+      }                                                                     \
+    } else if (kind == 1) {                                                 \
+      switch (n) {                                                          \
+        case 2: FN<1, 2>(__VA_ARGS__); break;                               \
+        case 3: FN<1, 3>(__VA_ARGS__); break;                               \
+        default: FN<1, 4>(__VA_ARGS__); break;                              \
+      }                                                                     \
+    } else {                                                                \
+      switch (n) {                                                          \
+        case 2: FN<2, 2>(__VA_ARGS__); break;                               \
+        case 3: FN<2, 3>(__VA_ARGS__); break;                               \
+        case 4: FN<2, 4>(__VA_ARGS__); break;                               \
+        default: FN<2, 5>(__VA_ARGS__); break;                              \
+      }                                                                     \
+    }                                                                       \
+  } while (0)
+
+__device__ __attribute__((noinline)) void wave_itx2d(const int32_t *dq, int32_t *res, int txSz, int txType, int lane) {
+  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
+  const int rect = (lw - lh == 1) || (lh - lw == 1);
+  const int rk = row_kind(txType), ck = col_kind(txType);
+  ITX_DISPATCH(itx_rows, rk, lw, dq, res, 1 << lh, rect, (int)Transform_Row_Shift[txSz], lane);
+  __syncthreads();
+  ITX_DISPATCH(itx_cols, ck, lh, res, 1 << lw, lane);
   __syncthreads();
 }
 
 /* Forward 2-D transform through the Q20 matrices (k_fwd.h:35-59): resid
  * h x w (stride 64) -> coef Min(32,h) x Min(32,w) (stride 32); tmp holds
  * h x Min(32,w) (stride 32). */
-DEVI void wave_fwd2d(const int32_t *resid, int32_t *tmp, int32_t *coef, int txSz, int txType,
+__device__ __attribute__((noinline)) void wave_fwd2d(const int32_t *resid, int32_t *tmp, int32_t *coef, int txSz, int txType,
                      const int32_t *__restrict__ mats, int lane) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
   const int tw = d_min(32, w), th = d_min(32, h);

@@ -44,6 +44,12 @@ struct RLds {
   Mav1BlockInfo blk;
 };
 
+/* The working set of the wave that reconstructs a block / superblock
+ * (one per workgroup). File scope so that the non-inlined stage functions
+ * address it as LDS. */
+__shared__ RLds gL;
+#define NOINL __device__ __attribute__((noinline))
+
 /* ---- scan / tx-set helpers (spec 5.11.47 / 5.11.48) ---- */
 DEVI const int16_t *scan_of(int txSz, int kind) {
   switch (txSz) {
@@ -165,9 +171,10 @@ DEVI void wht_inv1(int32_t *T, int shift) {
 
 /* One transform block whose prediction is in px (k_recon.h:60-173).
  * Returns the eob; the chosen type and RD cost in *outType / *outCost. */
-DEVI int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ spx, uint8_t *px, int S, int x, int y,
+NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ spx, uint8_t *px, int S, int x, int y,
                        int txSz, int plane, const int *cands, int nc, int32_t *coefOut,
-                       const int32_t *__restrict__ mats, RLds &L, int lane, int *outType, long long *outCost) {
+                       const int32_t *__restrict__ mats, int lane, int *outType, long long *outCost) {
+  RLds &L = gL;
   const int w = Tx_Width[txSz], h = Tx_Height[txSz], tw = d_min(32, w), th = d_min(32, h);
   const int dcQ = fp.dc_q[plane], acQ = fp.ac_q[plane];
   int32_t *resid = L.u.fw.resid;
@@ -338,9 +345,10 @@ DEVI void wave_edge_upsample(int *e, int *dup, int numPx, int lane) {
   __syncthreads();
 }
 
-DEVI void intra_tx_predict(const Mav1FrameParams &fp, uint8_t *px, int S, int sx, int sy, int lw, int lh, int mode,
-                           int delta, int hl, int ha, int har, int hbl, int maxX, int maxY, int edgeFilter,
-                           int filterType, RLds &L, int lane) {
+NOINL void intra_tx_predict(const Mav1FrameParams &fp, uint8_t *px, int S, int sx, int sy, int lw, int lh, int mode,
+                            int delta, int hl, int ha, int har, int hbl, int maxX, int maxY, int edgeFilter,
+                            int filterType, int lane) {
+  RLds &L = gL;
   const int w = 1 << lw, h = 1 << lh;
   int *A = L.edgeA + EDGE_NEG, *Lc = L.edgeL + EDGE_NEG;
   wave_intra_edges(px, S, sx, sy, w, h, hl, ha, har, hbl, maxX, maxY, A, Lc, lane);
@@ -376,8 +384,9 @@ DEVI void intra_tx_predict(const Mav1FrameParams &fp, uint8_t *px, int S, int sx
 
 /* CfL (7.11.5) over the DC prediction in px: the alpha with the smallest
  * SSE against the source (first minimum over -16..16) unless given. */
-DEVI int cfl_tx(const uint8_t *luma, int LS, int sx, int sy, int maxLW, int maxLH, int lw, int lh, uint8_t *px, int S,
-                const uint8_t *spx, int alphaIn, int search, int excludeZero, RLds &L, int lane) {
+NOINL int cfl_tx(const uint8_t *luma, int LS, int sx, int sy, int maxLW, int maxLH, int lw, int lh, uint8_t *px, int S,
+                 const uint8_t *spx, int alphaIn, int search, int excludeZero, int lane) {
+  RLds &L = gL;
   const int w = 1 << lw, h = 1 << lh, n = w * h;
   const int dcv = px[sy * S + sx]; /* DC prediction: one value over the block */
   int s = 0;
@@ -419,9 +428,10 @@ DEVI int cfl_tx(const uint8_t *luma, int LS, int sx, int sy, int maxLW, int maxL
 
 /* code_plane (k_recon.h:190-291) for the block in L.blk. markOnly: only the
  * BlockDecoded flags (the block was coded by the inter phase). */
-DEVI long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, uint8_t *rec,
-                          int32_t *__restrict__ coefs, const int32_t *__restrict__ mats, RLds &L, int plane, int txSz,
-                          int &maxLW, int &maxLH, bool useBd, bool markOnly, int lane) {
+NOINL long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, uint8_t *rec,
+                           int32_t *__restrict__ coefs, const int32_t *__restrict__ mats, int plane, int txSz,
+                           int &maxLW, int &maxLH, bool useBd, bool markOnly, int lane) {
+  RLds &L = gL;
   Mav1BlockInfo &b = L.blk;
   const int bs = b.bsize, sub = plane > 0, S = fp.stride[plane];
   const int availU = b.mi_row > 0, availL = b.mi_col > tile_start(fp, b.mi_col);
@@ -460,15 +470,15 @@ DEVI long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict__
                            L.bd[BDI(plane, (sbr >> sub) - 1, (sbc >> sub) + stepX)],
                            L.bd[BDI(plane, (sbr >> sub) + stepY, (sbc >> sub) - 1)], ((fp.mi_cols * 4) >> sub) - 1,
                            ((fp.mi_rows * 4) >> sub) - 1, fp.enable_intra_edge_filter,
-                           (b.interp_filter >> (plane == 0 ? 4 : 5)) & 1, L, lane);
+                           (b.interp_filter >> (plane == 0 ? 4 : 5)) & 1, lane);
           if (isCfl) {
             int a;
             if (txIdx > 0)
               a = cfl_tx(rec + fp.offset[0], fp.stride[0], sx, sy, maxLW, maxLH, lw, lh, px, S, spx,
-                         plane == 1 ? b.cfl_alpha_u : b.cfl_alpha_v, 0, 0, L, lane);
+                         plane == 1 ? b.cfl_alpha_u : b.cfl_alpha_v, 0, 0, lane);
             else
               a = cfl_tx(rec + fp.offset[0], fp.stride[0], sx, sy, maxLW, maxLH, lw, lh, px, S, spx, 0, 1,
-                         plane == 2 && b.cfl_alpha_u == 0, L, lane);
+                         plane == 2 && b.cfl_alpha_u == 0, lane);
             if (txIdx == 0 && lane == 0) {
               if (plane == 1) b.cfl_alpha_u = a;
               else b.cfl_alpha_v = a;
@@ -483,7 +493,7 @@ DEVI long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict__
         int t = DCT_DCT;
         long long c = 0;
         const int eob = code_tx_block(fp, spx, px, S, sx, sy, txSz, plane, cands, nc,
-                                      coefs + b.coef_off[plane] + txIdx * Tx_Width[txSz] * Tx_Height[txSz], mats, L,
+                                      coefs + b.coef_off[plane] + txIdx * Tx_Width[txSz] * Tx_Height[txSz], mats,
                                       lane, &t, &c);
         total += c;
         if (plane == 0 && lane == 0) {
@@ -509,9 +519,17 @@ DEVI long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict__
 
 /* recon_block (k_recon.h:293-394) for L.blk. Inter blocks are fully coded
  * by the inter phase; the intra phase only marks their decoded area. */
-DEVI void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, uint8_t *rec,
-                      const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
-                      const int32_t *__restrict__ mats, RLds &L, bool useBd, bool markOnly, int lane) {
+NOINL void predict_inter_plane(const uint8_t *__restrict__ plane, int S, int refW, int refH, int x, int y, int w,
+                              int h, int mvr, int mvc, int sub, int filter, uint8_t *dst, int ds, int32_t *dst32,
+                              int round1, int lane) {
+  wave_predict_inter(plane, S, refW, refH, x, y, w, h, mvr, mvc, sub, filter, gL.u.pr.win, gL.u.pr.mid, dst, ds, dst32,
+                     round1, lane);
+}
+
+NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, uint8_t *rec,
+                       const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
+                       const int32_t *__restrict__ mats, bool useBd, bool markOnly, int lane) {
+  RLds &L = gL;
   Mav1BlockInfo &b = L.blk;
   const int bs = b.bsize, comp = b.is_inter && b.ref_frame1 > INTRA_FRAME;
   const uint8_t *r0 = (b.is_inter && b.ref_frame != LAST_FRAME) ? gold : ref;
@@ -531,8 +549,8 @@ DEVI void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ src
       int bestF = 0;
       for (int f = 0; f < 9; f++) {
         const int packed = (f % 3) | ((f / 3) << 2);
-        wave_predict_inter(r0 + fp.offset[0], S, fp.frame_w, fp.frame_h, x0, y0, w, h, b.mv_row, b.mv_col, 0, packed,
-                           L.u.pr.win, L.u.pr.mid, px, S, nullptr, 11, lane);
+        predict_inter_plane(r0 + fp.offset[0], S, fp.frame_w, fp.frame_h, x0, y0, w, h, b.mv_row, b.mv_col, 0, packed,
+                            px, S, nullptr, 11, lane);
         long long e = 0;
         for (int k = lane; k < vw * vh; k += 64) {
           int i = k / vw, j = k - i * vw;
@@ -555,14 +573,14 @@ DEVI void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ src
     const int w = BLOCK_WIDTH(bs) >> sub, h = BLOCK_HEIGHT(bs) >> sub;
     if (!markOnly) {
       if (comp) {
-        wave_predict_inter(r0 + fp.offset[plane], S, fp.frame_w, fp.frame_h, baseX, baseY, w, h, b.mv_row, b.mv_col,
-                           sub, b.interp_filter, L.u.pr.win, L.u.pr.mid, nullptr, w, L.u.pr.p0, 7, lane);
+        predict_inter_plane(r0 + fp.offset[plane], S, fp.frame_w, fp.frame_h, baseX, baseY, w, h, b.mv_row, b.mv_col,
+                            sub, b.interp_filter, nullptr, w, L.u.pr.p0, 7, lane);
         /* second prediction's vertical pass combined with the first one */
         int32_t *p1 = (int32_t *)L.coef; /* free until code_plane */
         for (int yy = 0; yy < h; yy += 16) {
           const int hh = d_min(16, h - yy);
-          wave_predict_inter(gold + fp.offset[plane], S, fp.frame_w, fp.frame_h, baseX, baseY + yy, w, hh, b.mv1_row,
-                             b.mv1_col, sub, b.interp_filter, L.u.pr.win, L.u.pr.mid, nullptr, w, p1, 7, lane);
+          predict_inter_plane(gold + fp.offset[plane], S, fp.frame_w, fp.frame_h, baseX, baseY + yy, w, hh, b.mv1_row,
+                              b.mv1_col, sub, b.interp_filter, nullptr, w, p1, 7, lane);
           uint8_t *d = rec + fp.offset[plane] + (baseY + yy) * S + baseX;
           for (int k = lane; k < w * hh; k += 64) {
             int i = k / w, j = k - i * w;
@@ -571,22 +589,21 @@ DEVI void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ src
           __syncthreads();
         }
       } else if (b.is_inter) {
-        wave_predict_inter(r0 + fp.offset[plane], S, fp.frame_w, fp.frame_h, baseX, baseY, w, h, b.mv_row, b.mv_col,
-                           sub, b.interp_filter, L.u.pr.win, L.u.pr.mid, rec + fp.offset[plane] + baseY * S + baseX, S,
-                           nullptr, 11, lane);
+        predict_inter_plane(r0 + fp.offset[plane], S, fp.frame_w, fp.frame_h, baseX, baseY, w, h, b.mv_row, b.mv_col,
+                            sub, b.interp_filter, rec + fp.offset[plane] + baseY * S + baseX, S, nullptr, 11, lane);
       }
     }
     if (fp.lossless) {
-      code_plane(fp, src, rec, coefs, mats, L, plane, TX_4X4, maxLW, maxLH, useBd, markOnly, lane);
+      code_plane(fp, src, rec, coefs, mats, plane, TX_4X4, maxLW, maxLH, useBd, markOnly, lane);
       continue;
     }
     if (plane > 0) {
-      code_plane(fp, src, rec, coefs, mats, L, plane, uv_tx_size(bs), maxLW, maxLH, useBd, markOnly, lane);
+      code_plane(fp, src, rec, coefs, mats, plane, uv_tx_size(bs), maxLW, maxLH, useBd, markOnly, lane);
       continue;
     }
     const int maxDepth = (!b.is_inter && fp.tx_mode_select) ? d_min(MAX_TX_DEPTH, Max_Tx_Depth[bs]) : 0;
     if (maxDepth == 0) {
-      code_plane(fp, src, rec, coefs, mats, L, 0, Max_Tx_Size_Rect[bs], maxLW, maxLH, useBd, markOnly, lane);
+      code_plane(fp, src, rec, coefs, mats, 0, Max_Tx_Size_Rect[bs], maxLW, maxLH, useBd, markOnly, lane);
       continue;
     }
     /* tx_depth RD search (intra only, so never markOnly) */
@@ -607,7 +624,7 @@ DEVI void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ src
       int t = Max_Tx_Size_Rect[bs];
       for (int k = 0; k < depth; k++) t = Split_Tx_Size[t];
       const long long c =
-          code_plane(fp, src, rec, coefs, mats, L, 0, t, maxLW, maxLH, useBd, false, lane) + (long long)fp.lambda * 8 * depth;
+          code_plane(fp, src, rec, coefs, mats, 0, t, maxLW, maxLH, useBd, false, lane) + (long long)fp.lambda * 8 * depth;
       if (d <= maxDepth && (best < 0 || c < best)) {
         best = c;
         bestD = depth;
@@ -625,14 +642,14 @@ __global__ __launch_bounds__(64) void k_recon_inter(const Mav1FrameParams fp, Ma
                                                     uint8_t *rec, const uint8_t *__restrict__ ref,
                                                     const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
                                                     const int32_t *__restrict__ mats) {
-  __shared__ RLds L;
+  RLds &L = gL;
   const int lane = threadIdx.x, sb = blockIdx.x >> 6, k = blockIdx.x & 63;
   if (k >= sbs[sb].num_blocks) return;
   Mav1BlockInfo *g = blocks + sbs[sb].first_block + k;
   if (!g->is_inter) return;
   if (lane < 27) ((int32_t *)&L.blk)[lane] = ((const int32_t *)g)[lane];
   __syncthreads();
-  recon_block(fp, src, rec, ref, gold, coefs, mats, L, false, false, lane);
+  recon_block(fp, src, rec, ref, gold, coefs, mats, false, false, lane);
   if (lane < 27) ((int32_t *)g)[lane] = ((const int32_t *)&L.blk)[lane];
 }
 
@@ -643,7 +660,7 @@ __global__ __launch_bounds__(64) void k_recon_intra(const Mav1FrameParams fp, Ma
                                                     uint8_t *rec, const uint8_t *__restrict__ ref,
                                                     const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
                                                     const int32_t *__restrict__ mats, int t) {
-  __shared__ RLds L;
+  RLds &L = gL;
   const int lane = threadIdx.x;
   const int tile = blockIdx.x % ((fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb), r = blockIdx.x / ((fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb);
   const int c0 = tile * fp.tile_w_sb, cl = t - 2 * r;
@@ -669,7 +686,7 @@ __global__ __launch_bounds__(64) void k_recon_intra(const Mav1FrameParams fp, Ma
     if (lane < 27) ((int32_t *)&L.blk)[lane] = ((const int32_t *)g)[lane];
     __syncthreads();
     const bool inter = L.blk.is_inter;
-    recon_block(fp, src, rec, ref, gold, coefs, mats, L, true, inter, lane);
+    recon_block(fp, src, rec, ref, gold, coefs, mats, true, inter, lane);
     if (!inter && lane < 27) ((int32_t *)g)[lane] = ((const int32_t *)&L.blk)[lane];
     __syncthreads();
   }
