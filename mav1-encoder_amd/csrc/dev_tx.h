@@ -316,8 +316,13 @@ DEVI int dq_denom(int tx) {
   return 1;
 }
 
+/* Row strides of the LDS blocks, padded by one word so that lanes walking
+ * down a column hit distinct banks. */
+#define RS 65 /* residual / inverse-transform output, h x w */
+#define QS 33 /* dequantised coefficients, Min(32,h) x Min(32,w) */
+
 /* 2-D inverse transform (7.13.3): dq holds Min(32,h) x Min(32,w) values with
- * row stride 32; res receives h x w residuals with row stride 64. One lane
+ * row stride QS; res receives h x w residuals with row stride RS. One lane
  * per row, then one lane per column; each (kind, length) pass is its own
  * non-inlined function so the unrolled butterflies exist once in the code
  * object. */
@@ -338,14 +343,21 @@ __device__ __attribute__((noinline)) void itx_rows(const int32_t *dq, int32_t *r
   constexpr int W = 1 << N;
   for (int i = lane; i < h; i += 64) {
     int32_t T[W];
+    int nz = 0;
 #pragma unroll
     for (int j = 0; j < W; j++) {
-      int32_t v = (i < 32 && j < 32) ? dq[i * 32 + j] : 0;
+      int32_t v = (i < 32 && j < 32) ? dq[i * QS + j] : 0;
+      nz |= v;
       T[j] = rect ? (int32_t)(((long long)v * 2896 + 2048) >> 12) : v;
     }
-    t_1d<KIND, N>(T, 16);
+    if (nz) { /* every 1-D kernel maps an all-zero row to zeros */
+      t_1d<KIND, N>(T, 16);
 #pragma unroll
-    for (int j = 0; j < W; j++) res[i * 64 + j] = d_clip(-(1 << 15), (1 << 15) - 1, d_round2(T[j], rowShift));
+      for (int j = 0; j < W; j++) res[i * RS + j] = d_clip(-(1 << 15), (1 << 15) - 1, d_round2(T[j], rowShift));
+    } else {
+#pragma unroll
+      for (int j = 0; j < W; j++) res[i * RS + j] = 0;
+    }
   }
 }
 template <int KIND, int N>
@@ -354,10 +366,10 @@ __device__ __attribute__((noinline)) void itx_cols(int32_t *res, int w, int lane
   for (int j = lane; j < w; j += 64) {
     int32_t T[H];
 #pragma unroll
-    for (int i = 0; i < H; i++) T[i] = res[i * 64 + j];
+    for (int i = 0; i < H; i++) T[i] = res[i * RS + j];
     t_1d<KIND, N>(T, 16);
 #pragma unroll
-    for (int i = 0; i < H; i++) res[i * 64 + j] = d_round2(T[i], 4);
+    for (int i = 0; i < H; i++) res[i * RS + j] = d_round2(T[i], 4);
   }
 }
 
@@ -398,20 +410,23 @@ __device__ __attribute__((noinline)) void wave_itx2d(const int32_t *dq, int32_t 
 }
 
 /* Forward 2-D transform through the Q20 matrices (k_fwd.h:35-59): resid
- * h x w (stride 64) -> coef Min(32,h) x Min(32,w) (stride 32); tmp holds
- * h x Min(32,w) (stride 32). */
+ * h x w (stride RS) -> coef Min(32,h) x Min(32,w) (stride 32); tmp holds
+ * h x Min(32,w) (stride 32). Row pass: one lane per residual row, the
+ * matrix row read uniformly by the wave; column pass: one lane per output. */
 __device__ __attribute__((noinline)) void wave_fwd2d(const int32_t *resid, int32_t *tmp, int32_t *coef, int txSz, int txType,
                      const int32_t *__restrict__ mats, int lane) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
   const int tw = d_min(32, w), th = d_min(32, h);
   const int32_t *Mh = mats + (row_kind(txType) * 5 + lw - 2) * 4096;
   const int32_t *Mv = mats + (col_kind(txType) * 5 + lh - 2) * 4096;
-  for (int k = lane; k < h * tw; k += 64) {
-    int i = k / tw, kk = k - i * tw;
-    const int32_t *m = Mh + kk * w, *rr = resid + i * 64;
-    long long s = 0;
-    for (int j = 0; j < w; j++) s += (long long)rr[j] * m[j];
-    tmp[i * 32 + kk] = (int32_t)((s + 32768) >> 16);
+  for (int i = lane; i < h; i += 64) {
+    const int32_t *rr = resid + i * RS;
+    for (int kk = 0; kk < tw; kk++) {
+      const int32_t *m = Mh + kk * w;
+      long long s = 0;
+      for (int j = 0; j < w; j++) s += (long long)rr[j] * m[j];
+      tmp[i * 32 + kk] = (int32_t)((s + 32768) >> 16);
+    }
   }
   __syncthreads();
   const int sh = Transform_Row_Shift[txSz] + 4;
@@ -446,14 +461,14 @@ DEVI void wave_quantize(const int32_t *coef, int32_t *q, int txSz, int dcQ, int 
   __syncthreads();
 }
 
-/* Dequantisation (spec 7.12.3), output stride 32. */
+/* Dequantisation (spec 7.12.3), output stride QS. */
 DEVI void wave_dequant(const int32_t *q, int32_t *dq, int txSz, int dcQ, int acQ, int lane) {
   const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
   for (int k = lane; k < th * tw; k += 64) {
     int i = k / tw, j = k - i * tw;
     long long d = (long long)q[k] * (k == 0 ? dcQ : acQ), ad = d < 0 ? -d : d;
     long long d2 = (ad & 0xFFFFFF) >> (denom >> 1);
-    dq[i * 32 + j] = d_clip(-(1 << 15), (1 << 15) - 1, (int)(d < 0 ? -d2 : d2));
+    dq[i * QS + j] = d_clip(-(1 << 15), (1 << 15) - 1, (int)(d < 0 ? -d2 : d2));
   }
   __syncthreads();
 }

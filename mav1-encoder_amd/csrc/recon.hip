@@ -28,14 +28,14 @@ struct RLds {
       int32_t p0[64 * 64];
     } pr;
     struct {
-      int32_t resid[64 * 64]; /* residual, then the inverse transform's output */
+      int32_t resid[64 * RS]; /* residual, then the inverse transform's output */
       int32_t tmp[64 * 32];
     } fw;
   } u;
   int32_t coef[32 * 32];
   int32_t qtry[32 * 32];
   int32_t qbest[32 * 32];
-  int32_t dq[32 * 32];
+  int32_t dq[32 * QS];
   int32_t cfl[32 * 32];
   int edgeA[EDGE_NEG + 2 * 64 + 32], edgeL[EDGE_NEG + 2 * 64 + 32], ecopy[2 * 64 + 40];
   uint8_t bd[3 * BDD * BDD];
@@ -126,7 +126,7 @@ DEVI int wave_coef_bits(const int32_t *q, const int16_t *scan, int eob, int lane
 DEVI void wht_fwd(const int32_t *r, int32_t *out) {
   int t[16];
   for (int j = 0; j < 4; j++) {
-    int a = r[j], b = r[64 + j], c = r[128 + j], d = r[192 + j];
+    int a = r[j], b = r[RS + j], c = r[2 * RS + j], d = r[3 * RS + j];
     a += b;
     d -= c;
     int e = (a - d) >> 1;
@@ -182,7 +182,7 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
     int eob = 0;
     if (lane == 0) {
       for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) resid[i * 64 + j] = (int)spx[(y + i) * S + x + j] - (int)px[(y + i) * S + x + j];
+        for (int j = 0; j < 4; j++) resid[i * RS + j] = (int)spx[(y + i) * S + x + j] - (int)px[(y + i) * S + x + j];
       int32_t q16[16];
       wht_fwd(resid, q16);
       for (int c = 15; c >= 0; c--)
@@ -228,7 +228,7 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
     for (int k = lane; k < w * h; k += 64) {
       int i = k / w, j = k - i * w;
       int o = (y + (fud ? h - 1 - i : i)) * S + x + (flr ? w - 1 - j : j);
-      resid[i * 64 + j] = (int)spx[o] - (int)px[o];
+      resid[i * RS + j] = (int)spx[o] - (int)px[o];
     }
     __syncthreads();
     wave_fwd2d(resid, L.u.fw.tmp, L.coef, txSz, t, mats, lane);
@@ -254,14 +254,14 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
       for (int k = lane; k < w * h; k += 64) {
         int i = k / w, j = k - i * w;
         int o = (y + (fud ? h - 1 - i : i)) * S + x + (flr ? w - 1 - j : j);
-        int d = (int)spx[o] - d_clip1((int)px[o] + resid[i * 64 + j]);
+        int d = (int)spx[o] - d_clip1((int)px[o] + resid[i * RS + j]);
         e += d * d;
       }
       sse = wave_sum64(e);
     } else {
       long long e = 0;
       for (int k = lane; k < w * h; k += 64) {
-        int v = resid[(k / w) * 64 + (k % w)];
+        int v = resid[(k / w) * RS + (k % w)];
         e += v * v;
       }
       sse = wave_sum64(e);
@@ -285,7 +285,7 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
     for (int k = lane; k < w * h; k += 64) {
       int i = k / w, j = k - i * w;
       int o = (y + (fud ? h - 1 - i : i)) * S + x + (flr ? w - 1 - j : j);
-      px[o] = (uint8_t)d_clip1((int)px[o] + resid[i * 64 + j]);
+      px[o] = (uint8_t)d_clip1((int)px[o] + resid[i * RS + j]);
     }
   }
   __syncthreads();
@@ -643,14 +643,18 @@ __global__ __launch_bounds__(64) void k_recon_inter(const Mav1FrameParams fp, Ma
                                                     const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
                                                     const int32_t *__restrict__ mats) {
   RLds &L = gL;
-  const int lane = threadIdx.x, sb = blockIdx.x >> 6, k = blockIdx.x & 63;
-  if (k >= sbs[sb].num_blocks) return;
-  Mav1BlockInfo *g = blocks + sbs[sb].first_block + k;
-  if (!g->is_inter) return;
-  if (lane < 27) ((int32_t *)&L.blk)[lane] = ((const int32_t *)g)[lane];
-  __syncthreads();
-  recon_block(fp, src, rec, ref, gold, coefs, mats, false, false, lane);
-  if (lane < 27) ((int32_t *)g)[lane] = ((const int32_t *)&L.blk)[lane];
+  const int lane = threadIdx.x, nslot = fp.sb_cols * fp.sb_rows * 64;
+  for (int slot = blockIdx.x; slot < nslot; slot += gridDim.x) {
+    const int sb = slot >> 6, k = slot & 63;
+    if (k >= sbs[sb].num_blocks) continue;
+    Mav1BlockInfo *g = blocks + sbs[sb].first_block + k;
+    if (!g->is_inter) continue;
+    if (lane < 27) ((int32_t *)&L.blk)[lane] = ((const int32_t *)g)[lane];
+    __syncthreads();
+    recon_block(fp, src, rec, ref, gold, coefs, mats, false, false, lane);
+    if (lane < 27) ((int32_t *)g)[lane] = ((const int32_t *)&L.blk)[lane];
+    __syncthreads();
+  }
 }
 
 /* Phase B: superblocks of anti-diagonal t, one wave each; grid = tile
@@ -720,6 +724,8 @@ __global__ __launch_bounds__(64) void k_edge_types(const Mav1FrameParams fp, Mav
   b->interp_filter = (b->interp_filter & 15) | fy << 4 | fuv << 5;
 }
 
+static inline int d_min_host(int a, int b) { return a < b ? a : b; }
+
 hipError_t launch_edge_types(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
                              uint8_t *ys, uint8_t *uvs) {
   const int nsb = fp.sb_cols * fp.sb_rows;
@@ -732,7 +738,9 @@ hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
                               const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold,
                               int32_t *coefs, const int32_t *mats) {
   const int nsb = fp.sb_cols * fp.sb_rows;
-  hipLaunchKernelGGL(k_recon_inter, dim3(nsb * 64), dim3(64), 0, st, fp, blocks, sbs, src, rec, ref, gold, coefs, mats);
+  /* a fixed grid (two resident workgroups per CU) striding over the block slots */
+  hipLaunchKernelGGL(k_recon_inter, dim3(d_min_host(nsb * 64, 2048)), dim3(64), 0, st, fp, blocks, sbs, src, rec, ref,
+                     gold, coefs, mats);
   return hipGetLastError();
 }
 
