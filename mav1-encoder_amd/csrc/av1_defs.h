@@ -55,6 +55,7 @@ enum { RESTORE_NONE = 0, RESTORE_WIENER = 1, RESTORE_SGRPROJ = 2, RESTORE_SWITCH
 #define SGRPROJ_MTABLE_BITS 20
 #define SGRPROJ_RECIP_BITS 12
 #define SGRPROJ_SGR_BITS 8
+#define SGRPROJ_PRJ_SUBEXP_K 4
 #define RESTORATION_TILESIZE_MAX 256
 #define MAX_TILE_WIDTH 4096
 #define MAX_TILE_COLS 64

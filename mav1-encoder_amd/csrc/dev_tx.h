@@ -411,22 +411,20 @@ __device__ __attribute__((noinline)) void wave_itx2d(const int32_t *dq, int32_t 
 
 /* Forward 2-D transform through the Q20 matrices (k_fwd.h:35-59): resid
  * h x w (stride RS) -> coef Min(32,h) x Min(32,w) (stride 32); tmp holds
- * h x Min(32,w) (stride 32). Row pass: one lane per residual row, the
- * matrix row read uniformly by the wave; column pass: one lane per output. */
+ * h x Min(32,w) (stride 32). One lane per output in both passes. */
 __device__ __attribute__((noinline)) void wave_fwd2d(const int32_t *resid, int32_t *tmp, int32_t *coef, int txSz, int txType,
                      const int32_t *__restrict__ mats, int lane) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
   const int tw = d_min(32, w), th = d_min(32, h);
   const int32_t *Mh = mats + (row_kind(txType) * 5 + lw - 2) * 4096;
   const int32_t *Mv = mats + (col_kind(txType) * 5 + lh - 2) * 4096;
-  for (int i = lane; i < h; i += 64) {
-    const int32_t *rr = resid + i * RS;
-    for (int kk = 0; kk < tw; kk++) {
-      const int32_t *m = Mh + kk * w;
-      long long s = 0;
-      for (int j = 0; j < w; j++) s += (long long)rr[j] * m[j];
-      tmp[i * 32 + kk] = (int32_t)((s + 32768) >> 16);
-    }
+  const int ltw = d_min(5, lw);
+  for (int k = lane; k < h * tw; k += 64) {
+    const int i = k >> ltw, kk = k & (tw - 1);
+    const int32_t *m = Mh + kk * w, *rr = resid + i * RS;
+    long long s = 0;
+    for (int j = 0; j < w; j++) s += (long long)rr[j] * m[j];
+    tmp[i * 32 + kk] = (int32_t)((s + 32768) >> 16);
   }
   __syncthreads();
   const int sh = Transform_Row_Shift[txSz] + 4;

@@ -643,18 +643,14 @@ __global__ __launch_bounds__(64) void k_recon_inter(const Mav1FrameParams fp, Ma
                                                     const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
                                                     const int32_t *__restrict__ mats) {
   RLds &L = gL;
-  const int lane = threadIdx.x, nslot = fp.sb_cols * fp.sb_rows * 64;
-  for (int slot = blockIdx.x; slot < nslot; slot += gridDim.x) {
-    const int sb = slot >> 6, k = slot & 63;
-    if (k >= sbs[sb].num_blocks) continue;
-    Mav1BlockInfo *g = blocks + sbs[sb].first_block + k;
-    if (!g->is_inter) continue;
-    if (lane < 27) ((int32_t *)&L.blk)[lane] = ((const int32_t *)g)[lane];
-    __syncthreads();
-    recon_block(fp, src, rec, ref, gold, coefs, mats, false, false, lane);
-    if (lane < 27) ((int32_t *)g)[lane] = ((const int32_t *)&L.blk)[lane];
-    __syncthreads();
-  }
+  const int lane = threadIdx.x, sb = blockIdx.x >> 6, k = blockIdx.x & 63;
+  if (k >= sbs[sb].num_blocks) return;
+  Mav1BlockInfo *g = blocks + sbs[sb].first_block + k;
+  if (!g->is_inter) return;
+  if (lane < 27) ((int32_t *)&L.blk)[lane] = ((const int32_t *)g)[lane];
+  __syncthreads();
+  recon_block(fp, src, rec, ref, gold, coefs, mats, false, false, lane);
+  if (lane < 27) ((int32_t *)g)[lane] = ((const int32_t *)&L.blk)[lane];
 }
 
 /* Phase B: superblocks of anti-diagonal t, one wave each; grid = tile
@@ -724,8 +720,6 @@ __global__ __launch_bounds__(64) void k_edge_types(const Mav1FrameParams fp, Mav
   b->interp_filter = (b->interp_filter & 15) | fy << 4 | fuv << 5;
 }
 
-static inline int d_min_host(int a, int b) { return a < b ? a : b; }
-
 hipError_t launch_edge_types(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
                              uint8_t *ys, uint8_t *uvs) {
   const int nsb = fp.sb_cols * fp.sb_rows;
@@ -738,8 +732,7 @@ hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
                               const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold,
                               int32_t *coefs, const int32_t *mats) {
   const int nsb = fp.sb_cols * fp.sb_rows;
-  /* a fixed grid (two resident workgroups per CU) striding over the block slots */
-  hipLaunchKernelGGL(k_recon_inter, dim3(d_min_host(nsb * 64, 2048)), dim3(64), 0, st, fp, blocks, sbs, src, rec, ref,
+  hipLaunchKernelGGL(k_recon_inter, dim3(nsb * 64), dim3(64), 0, st, fp, blocks, sbs, src, rec, ref,
                      gold, coefs, mats);
   return hipGetLastError();
 }

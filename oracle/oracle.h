@@ -64,6 +64,27 @@ void oracle_fwd_2d(const int32_t *resid, int32_t *out, int txSz, int txType, con
 void oracle_quantize(const int32_t *coef, int32_t *quant, int txSz, int dcQ, int acQ);
 void oracle_dequant(const int32_t *quant, int32_t *dq, int txSz, int dcQ, int acQ);
 
+/* --- in-loop filters --- */
+/* Per mi (mi_rows x mi_cols bytes): bsize | skip << 5 | tx_depth << 6. */
+void oracle_filter_info(const Mav1FrameParams *fp, const Mav1BlockInfo *blocks, const Mav1SbInfo *sbs, uint8_t *info);
+/* Deblocking of rec in place. */
+void oracle_loop_filter(const Mav1FrameParams *fp, uint8_t *rec, const uint8_t *info);
+/* CDEF of cur with parameter sets 0 .. n-1 into out + k * frameBytes. */
+void oracle_cdef(const Mav1FrameParams *fp, const uint8_t *cur, uint8_t *out, int64_t frameBytes, const uint8_t *info,
+                 int n);
+/* Per superblock the set of smallest SSE against src, copied into rec;
+ * choice: one byte per superblock. */
+void oracle_cdef_select(const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdefbuf, int64_t frameBytes,
+                        int n, uint8_t *rec, int8_t *choice);
+/* Loop restoration of the CDEF output (cdef) with the deblocked frame
+ * (dbk) above / below stripes into out. */
+void oracle_lr_apply(const Mav1FrameParams *fp, const uint8_t *cdef, const uint8_t *dbk, uint8_t *out,
+                     const Mav1LrUnit *units);
+/* Per-unit restoration parameter search; writes units and fp->lr_type.
+ * tmp: frame-sized scratch. */
+void oracle_lr_select(Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdef, const uint8_t *dbk, uint8_t *tmp,
+                      Mav1LrUnit *units, int lrTest);
+
 /* --- shared prediction primitives --- */
 /* Intra prediction arguments (the IntraArgs record of src/kernels/k_intra.h:79-85). */
 typedef struct {

@@ -115,3 +115,27 @@ void ref_quantize(const int32_t *coef, int32_t *quant, int txSz, int dcQ, int ac
 }
 void ref_dequant(const int32_t *quant, int32_t *dq, int txSz, int dcQ, int acQ) { dequant_block(quant, dq, txSz, dcQ, acQ); }
 void ref_forward_matrices(int32_t *mats) { fwd_build_matrices(mats); }
+
+/* The serial form of cpu_filter's deblocking (src/backend_cpu.c:158-170). */
+void ref_loop_filter(const FrameParams *fp, uint8_t *rec, const uint8_t *info) {
+  for (int pass = 0; pass < 2; pass++) {
+    int lines = pass == 0 ? fp->mi_rows : fp->mi_cols;
+    for (int i = 0; i < 3 * lines; i++) {
+      int plane = i / lines, line = i % lines;
+      if (plane > 0 && (line & 1)) continue;
+      lf_line(fp, rec, info, plane, pass, line);
+    }
+  }
+}
+
+/* The serial form of cpu_filter's CDEF (src/backend_cpu.c:171-176). */
+void ref_cdef(const FrameParams *fp, const uint8_t *rec, uint8_t *out, long frameBytes, const uint8_t *info, int n) {
+  int bc = (fp->mi_cols + 1) >> 1, br = (fp->mi_rows + 1) >> 1;
+  for (int i = 0; i < bc * br; i++) cdef_block_all(fp, rec, out, frameBytes, info, (i / bc) * 2, (i % bc) * 2, n);
+}
+
+/* The serial form of cpu_restore (src/backend_cpu.c:179-188). */
+void ref_lr_apply(const FrameParams *fp, const uint8_t *cdef, const uint8_t *dbk, uint8_t *out, const LrUnit *units) {
+  int rows4 = (fp->frame_h + 3) >> 2, cols4 = (fp->frame_w + 3) >> 2, n = rows4 * cols4;
+  for (int g = 0; g < 3 * n; g++) lr_block(fp, g / n, (g % n) / cols4, (g % n) % cols4, cdef, dbk, out, units);
+}

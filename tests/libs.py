@@ -180,3 +180,79 @@ def edge_types(impl, fp, blocks, sbs):
     b = np.ascontiguousarray(blocks, np.int32).copy()
     impl.fn("edge_types")(C.byref(fp), mav1hip.ptr(b), mav1hip.ptr(np.ascontiguousarray(sbs, np.int32)))
     return b
+
+
+def _bind_filters(lib, pre):
+    sigs = {
+        "loop_filter": (None, [FP, P, P]),
+        "cdef": (None, [FP, P, P, C.c_long, P, C.c_int]),
+        "lr_apply": (None, [FP, P, P, P, P]),
+        "lr_select": (None, [FP, P, P, P, P, P, C.c_int]),
+        "filter_info": (None, [FP, P, P, P]),
+        "cdef_select": (None, [FP, P, P, C.c_long, C.c_int, P, P]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, pre + name, None)
+        if f is not None:
+            f.restype = res
+            f.argtypes = args
+
+
+def filter_info(fp, blocks, sbs):
+    """The oracle's per-mi filter info bytes."""
+    lib = oracle()
+    _bind_filters(lib, "oracle_")
+    info = np.zeros(fp.mi_rows * fp.mi_cols, np.uint8)
+    lib.oracle_filter_info(C.byref(fp), mav1hip.ptr(np.ascontiguousarray(blocks, np.int32)),
+                           mav1hip.ptr(np.ascontiguousarray(sbs, np.int32)), mav1hip.ptr(info))
+    return info
+
+
+class Filters:
+    """In-loop filter stages of the oracle ('oracle') or the reference ('ref')."""
+
+    def __init__(self, which):
+        self.lib = oracle() if which == "oracle" else ref()
+        self.pre = "oracle_" if which == "oracle" else "ref_"
+        _bind_filters(self.lib, self.pre)
+
+    def fn(self, name):
+        return getattr(self.lib, self.pre + name)
+
+    def loop_filter(self, fp, rec, info):
+        out = np.ascontiguousarray(rec, np.uint8).copy()
+        self.fn("loop_filter")(C.byref(fp), mav1hip.ptr(out), mav1hip.ptr(info))
+        return out
+
+    def cdef(self, fp, rec, info, n):
+        fb = rec.size
+        out = np.zeros(n * fb, np.uint8)
+        self.fn("cdef")(C.byref(fp), mav1hip.ptr(rec), mav1hip.ptr(out), fb, mav1hip.ptr(info), n)
+        return out.reshape(n, fb)
+
+    def lr_apply(self, fp, cdef, dbk, units):
+        out = np.zeros_like(cdef)
+        self.fn("lr_apply")(C.byref(fp), mav1hip.ptr(cdef), mav1hip.ptr(dbk), mav1hip.ptr(out),
+                            mav1hip.ptr(np.ascontiguousarray(units, np.int32)))
+        return out
+
+    def lr_select(self, fp, src, cdef, dbk, lr_test=0):
+        """Returns (units int32[n, 12], lr_type[3])."""
+        n = sum(fp.lr_unit_rows[p] * fp.lr_unit_cols[p] for p in range(3))
+        units = np.zeros((n, 12), np.int32)
+        tmp = np.zeros_like(cdef)
+        f = mav1hip.FrameParams.from_buffer_copy(fp)
+        self.fn("lr_select")(C.byref(f), mav1hip.ptr(src), mav1hip.ptr(cdef), mav1hip.ptr(dbk), mav1hip.ptr(tmp),
+                             mav1hip.ptr(units), lr_test)
+        return units, [f.lr_type[p] for p in range(3)]
+
+
+def cdef_select(fp, src, cdefbuf, rec):
+    """The oracle's per-superblock CDEF choice: (rec, choice)."""
+    lib = oracle()
+    _bind_filters(lib, "oracle_")
+    out = np.ascontiguousarray(rec, np.uint8).copy()
+    choice = np.zeros(fp.sb_cols * fp.sb_rows, np.int8)
+    lib.oracle_cdef_select(C.byref(fp), mav1hip.ptr(src), mav1hip.ptr(cdefbuf), cdefbuf.shape[1], cdefbuf.shape[0],
+                           mav1hip.ptr(out), mav1hip.ptr(choice))
+    return out, choice
