@@ -7,10 +7,10 @@ with 12 objects moving up to +-48 px/frame, luma noise sigma 3), encoded with
 (hidden ALTREF frame predicting from LAST, then a non-reference leaf frame
 predicting from LAST and ALTREF). A step is one frame of that sequence
 through the backend stages built on the GPU so far (listed in
-config["stages"]), with every input frame already resident in HBM (a device
+config["stages"]), with every source frame already resident in HBM (a device
 frame pool filled before the timed region). Reference pictures are the
-source pictures of the referenced frames (the reconstruction stage is not
-in the step yet).
+reconstructed, filtered pictures of the referenced frames, kept in two
+device slots (closed loop, starting from a coded key frame).
 
 Prints one JSON line (rank 0) with the throughput in frames/s, the roofline
 of the dominant kernel (per-launch durations from HIP events recorded on the
@@ -78,6 +78,15 @@ def algorithmic_bytes(kernel, fp, nref):
         return nsb * 64 * 108 // max(1, fp.tile_w_sb + 2 * (fp.sb_rows - 1))
     if kernel == "edge_types":
         return nsb * 64 * 108 * 2 + 2 * fp.mi_rows * fp.mi_cols
+    mi = fp.mi_rows * fp.mi_cols
+    if kernel == "filter_info":
+        return nsb * 64 * 108 + mi
+    if kernel == "deblock":
+        return 2 * F + mi
+    if kernel == "cdef":
+        return F + 2 * F + mi  # turbo: two parameter sets written
+    if kernel == "cdef_select":
+        return 2 * F + F + F
     return 0
 
 
@@ -103,21 +112,48 @@ def main():
     frames = [synth.pad_frame(geo, gen.frame(t)) for t in range(uniq)]
 
     ctx = mav1hip.Context(cfg)
-    ctx.pool_alloc(uniq)
+    # pool: the source frames, then two reference slots (LAST / ALTREF reconstructions)
+    ctx.pool_alloc(uniq + 2)
     for i, f in enumerate(frames):
         ctx.pool_upload(i, f)
-    params = {}
-    for kind in ("arf", "leaf"):
-        params[kind] = mav1hip.setup_params(cfg, mav1hip.turbo_frame(kind, 1))[0]
+    params, plans = {}, {}
+    for kind in ("key", "arf", "leaf"):
+        fr = mav1hip.turbo_frame(kind, 0 if kind == "key" else 1)
+        fp, npre = mav1hip.setup_params(cfg, fr)
+        params[kind], plans[kind] = fp, mav1hip.filter_plan(cfg, fr, fp, npre)
+    slot_of = {}
 
-    def step(kind, f, last, alt):
-        fp = params[kind]
-        ctx.bind_frames(f % uniq, last % uniq, alt % uniq if alt >= 0 else -1)
-        ctx.set_params(fp)
-        ctx.launch_motion_search()
+    def run_stages(kind):
+        plan = plans[kind]
+        if kind != "key":
+            ctx.launch_motion_search()
         ctx.launch_analyze()
         ctx.launch_edge_types()
         ctx.launch_recon()
+        if plan.filter_info:
+            ctx.launch_filter_info()
+            if plan.deblock:
+                ctx.launch_deblock()
+            ctx.launch_cdef(plan.cdef_sets)
+            if plan.cdef_select:
+                ctx.launch_cdef_select(plan.cdef_sets)
+
+    def step(kind, f, last, alt):
+        ref, gold = slot_of[last], (slot_of[alt] if alt >= 0 else -1)
+        ctx.bind_frames(f % uniq, ref, gold)
+        ctx.set_params(params[kind])
+        run_stages(kind)
+        if kind == "arf":  # the reconstruction becomes a reference: the slot LAST does not hold
+            target = uniq if ref != uniq else uniq + 1
+            ctx.pool_store(target, mav1hip.BUF_REC)
+            slot_of[f] = target
+
+    # frame 0: the key frame (untimed), its reconstruction in the first slot
+    ctx.bind_frames(0, -1, -1)
+    ctx.set_params(params["key"])
+    run_stages("key")
+    ctx.pool_store(uniq, mav1hip.BUF_REC)
+    slot_of[0] = uniq
 
     for s in order[:args.warmup]:
         step(*s)
@@ -132,7 +168,7 @@ def main():
     ctx.timing(False)
     kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3", "analyze_level0",
                "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table", "edge_types", "recon_inter",
-               "recon_intra"]
+               "recon_intra", "filter_info", "deblock", "cdef", "cdef_select"]
     stats = {k: ctx.timing_get(k) for k in kernels}
     fps = args.steps / elapsed
 
@@ -168,9 +204,11 @@ def main():
         "data": "synthetic (seeded generator: 6 sinusoids + 12 moving rectangles + noise)",
         "config": {
             "workload": "1920x1080 8-bit 4:2:0, --turbo -q 125, turbo GOP (ARF + leaf), plain HIP path",
-            "stages": ["motion_search", "analyze", "edge_types", "recon"],
-            "frame_kinds": "alternating hidden ALTREF (1 ref) / leaf (2 refs)",
-            "references": "source pictures (reconstruction not in the step yet)",
+            "stages": ["motion_search", "analyze", "edge_types", "recon", "filter_info", "deblock", "cdef",
+                       "cdef_select"],
+            "frame_kinds": "alternating hidden ALTREF (1 ref) / leaf (2 refs); leaf frames are not filtered",
+            "references": "the reconstructed, filtered pictures of the referenced frames (closed loop from a coded key frame)",
+            "restoration": "off (the turbo default); the restoration search is not in the step yet",
             "device": ctx.device_name,
         },
         "kernels_ms_per_launch": {k: round(v[1] / max(v[0], 1), 5) for k, v in stats.items()},
@@ -203,7 +241,13 @@ def cpu_baseline(seconds, frames, params, uniq):
         path, kind, pre = os.path.join(ROOT, "oracle", "liboracle.so"), "port", "oracle_"
     lib = C.CDLL(path)
     me, ana, rec_fn = getattr(lib, pre + "motion_search"), getattr(lib, pre + "analyze"), getattr(lib, pre + "recon")
-    orc = C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))  # the driver's edge filter types
+    lf, cdef = getattr(lib, pre + "loop_filter"), getattr(lib, pre + "cdef")
+    # the driver's edge filter types, filter info and CDEF choice (the reference's are in its frame driver)
+    orc = C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+    fb = frames[0].size
+    info = np.zeros(params["arf"].mi_rows * params["arf"].mi_cols, np.uint8)
+    cbuf = np.zeros(2 * fb, np.uint8)
+    choice = np.zeros(params["arf"].sb_cols * params["arf"].sb_rows, np.int8)
     mats = mav1hip.forward_matrices()
     rec = np.zeros_like(frames[0])
     coefs = np.zeros(params["arf"].sb_cols * params["arf"].sb_rows * 6144, np.int32)
@@ -226,6 +270,12 @@ def cpu_baseline(seconds, frames, params, uniq):
         orc.oracle_edge_types(C.byref(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs))
         rec_fn(C.byref(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs), s, mav1hip.ptr(rec), r, g, mav1hip.ptr(coefs),
                mav1hip.ptr(mats))
+        if kind_f == "arf":
+            orc.oracle_filter_info(C.byref(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs), mav1hip.ptr(info))
+            lf(C.byref(fp), mav1hip.ptr(rec), mav1hip.ptr(info))
+            cdef(C.byref(fp), mav1hip.ptr(rec), mav1hip.ptr(cbuf), C.c_long(fb), mav1hip.ptr(info), 2)
+            orc.oracle_cdef_select(C.byref(fp), s, mav1hip.ptr(cbuf), C.c_int64(fb), 2, mav1hip.ptr(rec),
+                                   mav1hip.ptr(choice))
         done += 1
         if time.perf_counter() - t0 >= seconds and done % 2 == 0:
             break
@@ -235,7 +285,8 @@ def cpu_baseline(seconds, frames, params, uniq):
         "unit": "frames/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"{done} frames of the same 1080p turbo GOP (ARF/leaf alternating), motion search + analysis + edge types + reconstruction, "
+        "sample": f"{done} frames of the same 1080p turbo GOP (ARF/leaf alternating), motion search + analysis + edge types + "
+                  f"reconstruction + (ARF) filter info, deblocking, CDEF, CDEF choice; references are source pictures, "
                   f"{dt:.1f} s on one host core ({os.cpu_count()} visible)",
     }
 

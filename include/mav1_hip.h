@@ -90,6 +90,20 @@ int32_t mav1hip_frame_qindex(const Mav1HipConfig *cfg, const Mav1HipFrame *fr);
 int32_t mav1hip_setup_params(const Mav1HipConfig *cfg, const Mav1HipFrame *fr, int32_t base_q_idx,
                              Mav1FrameParams *fp);
 
+/* Which in-loop filter stages a frame runs (run_frame_core,
+ * src/encoder.c:618-640): filter info and the filter stage unless the frame
+ * has nothing to filter (turbo leaf frames), CDEF sets of the filter stage,
+ * the per-superblock CDEF choice, loop restoration. */
+typedef struct {
+  int32_t filter_info;  /* build the per-mi filter info */
+  int32_t deblock;      /* a deblocking level is non-zero */
+  int32_t cdef_sets;    /* CDEF parameter sets the filter stage produces (0: none) */
+  int32_t cdef_select;  /* choose a set per superblock into the reconstruction */
+  int32_t restoration;  /* loop-restoration search and apply */
+} Mav1HipFilterPlan;
+int32_t mav1hip_filter_plan(const Mav1HipConfig *cfg, const Mav1HipFrame *fr, const Mav1FrameParams *fp,
+                            int32_t npresets, Mav1HipFilterPlan *plan);
+
 /* The forward-transform matrices in Q20 (fwd_build_matrices,
  * src/fwd_matrices.c:57-65): 15 slots of 4096 int32, slot
  * (kind * 5 + log2(n) - 2) for kind 0 DCT, 1 ADST, 2 identity. Every context
@@ -149,6 +163,47 @@ int mav1hip_edge_types(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo 
 int mav1hip_recon(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
                   const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold, int32_t *coefs);
 
+/* The driver's build_filter_info (src/encoder.c:513-526): per mi
+ * (mi_rows * mi_cols bytes) bsize | skip << 5 | tx_depth << 6 of the coded
+ * blocks. */
+int mav1hip_filter_info(mav1hip_ctx *c, const Mav1FrameParams *fp, const Mav1BlockInfo *blocks,
+                        const Mav1SbInfo *sbs, uint8_t *info);
+
+/* Backend.filter (src/backend_cpu.c:158-177): deblocking of rec in place
+ * (lf_line, src/kernels/k_lf.h), then CDEF of the deblocked frame with
+ * parameter sets 0 .. nsets-1 into cdefbuf + k * frame_bytes
+ * (cdef_block_all, src/kernels/k_cdef.h:124-150). nsets <= 8; cdefbuf may
+ * be NULL when nsets is 0. */
+int mav1hip_filter(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *info, uint8_t *rec, int32_t nsets,
+                   uint8_t *cdefbuf);
+
+/* The driver's cdef_select (src/encoder.c:530-563): per superblock the set
+ * with the smallest SSE against src over the visible area (first minimum)
+ * into choice[sb], copied from cdefbuf into rec over the mi-aligned area. */
+int mav1hip_cdef_select(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdefbuf,
+                        int32_t nsets, uint8_t *rec, int8_t *choice);
+
+/* Backend.restore (src/backend_cpu.c:179-188, lr_block of
+ * src/kernels/k_lr.h): loop restoration of the CDEF output cdef, with the
+ * deblocked frame dbk above and below each stripe, into out; units holds
+ * the per-unit parameters (fp->lr_unit_off indexing), planes whose lr_type
+ * is RESTORE_NONE are copied. */
+int mav1hip_restore(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *cdef, const uint8_t *dbk,
+                    const Mav1LrUnit *units, uint8_t *out);
+
+/* The driver's lr_select (src/lr_search.c:235-359): per restoration unit
+ * the parameters of smallest 16 * SSE + lambda * bits among NONE, the
+ * fitted and refined Wiener filter and the 16 self-guided sets, given the
+ * source, the CDEF output cdef and the deblocked frame dbk. units receives
+ * one record per unit (fp->lr_unit_off indexing), lr_type[3] the frame's
+ * restoration type per plane (the reference writes fp->lr_type). lr_test
+ * forces Wiener on even units and the self-guided set of smallest SSE on
+ * odd ones (Mav1Config.lr_test). reruns[3] (may be NULL) receives the
+ * number of units per plane whose tap refinement was recomputed with the
+ * true references (a diagnostic of the speculative search). */
+int mav1hip_lr_select(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdef,
+                      const uint8_t *dbk, int32_t lr_test, Mav1LrUnit *units, int32_t lr_type[3], int32_t reruns[3]);
+
 /* ------------------------------------------------------------------------
  * Device-resident form
  * ------------------------------------------------------------------------ */
@@ -166,8 +221,15 @@ enum {
   MAV1HIP_BUF_BLOCKS = 8,  /* coded blocks, 64 records per superblock */
   MAV1HIP_BUF_SBS = 9,     /* superblock records */
   MAV1HIP_BUF_REC = 10,    /* reconstruction */
-  MAV1HIP_BUF_COEFS = 11   /* quantised levels, 6144 int32 per superblock */
+  MAV1HIP_BUF_COEFS = 11,  /* quantised levels, 6144 int32 per superblock */
+  MAV1HIP_BUF_LFMASK = 12, /* per-mi filter info */
+  MAV1HIP_BUF_CDEFBUF = 13, /* CDEF outputs, MAV1HIP_MAX_CDEF_SETS frames */
+  MAV1HIP_BUF_DBK = 14,    /* deblocked frame kept for loop restoration */
+  MAV1HIP_BUF_LROUT = 15,  /* loop-restoration output */
+  MAV1HIP_BUF_LRUNITS = 16, /* Mav1LrUnit per restoration unit */
+  MAV1HIP_BUF_CDEF_CHOICE = 17 /* CDEF set per superblock, int8 */
 };
+#define MAV1HIP_MAX_CDEF_SETS 8
 /* Size in bytes of a device buffer. */
 size_t mav1hip_buffer_bytes(const mav1hip_ctx *c, int32_t buf);
 size_t mav1hip_planes_entry_bytes(const mav1hip_ctx *c);
@@ -179,6 +241,20 @@ int mav1hip_launch_motion_search(mav1hip_ctx *c);
 int mav1hip_launch_analyze(mav1hip_ctx *c);
 int mav1hip_launch_edge_types(mav1hip_ctx *c);
 int mav1hip_launch_recon(mav1hip_ctx *c);
+/* Filter stages on the device buffers: BLOCKS -> LFMASK; REC deblocked in
+ * place; REC -> CDEFBUF (nsets); REC copied to DBK; CDEFBUF -> REC and
+ * CDEF_CHOICE; REC (CDEF output) + DBK + LRUNITS -> LROUT with the
+ * restoration types of the last mav1hip_launch_lr_select, or of the frame
+ * parameters when LRUNITS was uploaded (mav1hip_set_params copies them). */
+int mav1hip_launch_filter_info(mav1hip_ctx *c);
+int mav1hip_launch_deblock(mav1hip_ctx *c);
+int mav1hip_launch_cdef(mav1hip_ctx *c, int32_t nsets);
+int mav1hip_launch_save_deblocked(mav1hip_ctx *c);
+int mav1hip_launch_cdef_select(mav1hip_ctx *c, int32_t nsets);
+/* REC (CDEF output) + DBK + SRC -> LRUNITS and the per-plane restoration
+ * types that mav1hip_launch_restore applies. */
+int mav1hip_launch_lr_select(mav1hip_ctx *c, int32_t lr_test);
+int mav1hip_launch_restore(mav1hip_ctx *c);
 /* Waits for everything enqueued on the context's stream. */
 int mav1hip_sync(mav1hip_ctx *c);
 
@@ -189,6 +265,10 @@ int mav1hip_sync(mav1hip_ctx *c);
 int mav1hip_pool_alloc(mav1hip_ctx *c, int32_t n);
 int mav1hip_pool_upload(mav1hip_ctx *c, int32_t idx, const uint8_t *frame);
 int mav1hip_bind_frames(mav1hip_ctx *c, int32_t src_idx, int32_t ref_idx, int32_t gold_idx);
+/* Copies a device frame buffer (MAV1HIP_BUF_REC or MAV1HIP_BUF_LROUT) into
+ * pool frame idx on the context's stream: a reconstructed picture becomes a
+ * reference for the frames that follow. */
+int mav1hip_pool_store(mav1hip_ctx *c, int32_t idx, int32_t buf);
 
 /* Per-kernel timing with HIP events recorded on the context's stream
  * around every launch while enabled. mav1hip_timing_get returns the number

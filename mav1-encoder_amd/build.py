@@ -12,7 +12,9 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmav1hip.so")
 
-HIP_SOURCES = ["api.hip", "me.hip", "analysis.hip", "recon.hip"]
+HIP_SOURCES = ["api.hip", "me.hip", "analysis.hip", "recon.hip", "filters.hip", "lrsearch.hip"]
+# sources whose floating-point results must round exactly as written (no FMA contraction)
+STRICT_FP = {"lrsearch.hip"}
 C_SOURCES = ["params.c", "fwdmats.c"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -36,7 +38,8 @@ def build(verbose=False):
     jobs = []
     for s in HIP_SOURCES:
         o = os.path.join(BUILD, s.replace(".hip", ".o"))
-        jobs.append(([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o], o))
+        extra = ["-ffp-contract=off"] if s in STRICT_FP else []
+        jobs.append(([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, s), "-o", o], o))
     for s in C_SOURCES:
         o = os.path.join(BUILD, s.replace(".c", ".o"))
         jobs.append((["gcc", "-O2", "-fPIC", "-std=gnu11", "-Wall", "-ffp-contract=off", "-c", os.path.join(CSRC, s),

@@ -44,6 +44,13 @@ struct mav1hip_ctx {
   uint8_t *d_rec;
   int32_t *d_coefs, *d_mats;
   uint8_t *d_ys, *d_uvs;
+  uint8_t *d_lfmask, *d_cdefbuf, *d_dbk, *d_lrout;
+  int *d_dirvar;
+  Mav1LrUnit *d_lrunits;
+  int8_t *d_choice;
+  void *d_lrstats;
+  int *d_lrref, *d_lrtype, *d_lrreruns;
+  int lr_units;
   long entry_bytes;
   size_t planes_bytes;
   /* frames the stages read (own buffers or pool frames) */
@@ -178,6 +185,26 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   rc |= dmalloc(c, (void **)&c->d_mats, sizeof(int32_t) * 4096 * 15);
   rc |= dmalloc(c, (void **)&c->d_ys, (size_t)c->mi_rows * c->mi_cols);
   rc |= dmalloc(c, (void **)&c->d_uvs, (size_t)c->mi_rows * c->mi_cols);
+  rc |= dmalloc(c, (void **)&c->d_lfmask, (size_t)c->mi_rows * c->mi_cols);
+  rc |= dmalloc(c, (void **)&c->d_cdefbuf, c->frame_bytes * MAV1HIP_MAX_CDEF_SETS);
+  rc |= dmalloc(c, (void **)&c->d_dbk, c->frame_bytes);
+  rc |= dmalloc(c, (void **)&c->d_lrout, c->frame_bytes);
+  rc |= dmalloc(c, (void **)&c->d_dirvar, sizeof(int) * (size_t)(c->mi_rows >> 1) * (c->mi_cols >> 1));
+  rc |= dmalloc(c, (void **)&c->d_choice, (size_t)c->num_sbs);
+  {
+    /* restoration units of 64x64 per plane (count_units_in_frame) */
+    c->lr_units = 0;
+    for (int p = 0; p < 3; p++) {
+      int sub = p > 0, pw = (cfg->width + sub) >> sub, ph = (cfg->height + sub) >> sub;
+      int rows = (ph + 32) / 64, cols = (pw + 32) / 64;
+      c->lr_units += (rows < 1 ? 1 : rows) * (cols < 1 ? 1 : cols);
+    }
+    rc |= dmalloc(c, (void **)&c->d_lrunits, sizeof(Mav1LrUnit) * (size_t)c->lr_units);
+    rc |= dmalloc(c, &c->d_lrstats, lr_stat_bytes() * (size_t)c->lr_units);
+    rc |= dmalloc(c, (void **)&c->d_lrref, sizeof(int) * 6 * (size_t)c->lr_units);
+    rc |= dmalloc(c, (void **)&c->d_lrtype, sizeof(int) * 3);
+    rc |= dmalloc(c, (void **)&c->d_lrreruns, sizeof(int) * 3);
+  }
   if (rc != MAV1HIP_OK) {
     if (err && errlen) snprintf(err, errlen, "%s", c->err);
     mav1hip_destroy(c);
@@ -196,6 +223,13 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   (void)hipMemsetAsync(c->d_coefs, 0, sizeof(int32_t) * MAV1_COEFS_PER_SB * (size_t)c->num_sbs, c->stream);
   (void)hipMemsetAsync(c->d_ys, 0, (size_t)c->mi_rows * c->mi_cols, c->stream);
   (void)hipMemsetAsync(c->d_uvs, 0, (size_t)c->mi_rows * c->mi_cols, c->stream);
+  (void)hipMemsetAsync(c->d_lfmask, 0, (size_t)c->mi_rows * c->mi_cols, c->stream);
+  (void)hipMemsetAsync(c->d_cdefbuf, 0, c->frame_bytes * MAV1HIP_MAX_CDEF_SETS, c->stream);
+  (void)hipMemsetAsync(c->d_dbk, 0, c->frame_bytes, c->stream);
+  (void)hipMemsetAsync(c->d_lrout, 0, c->frame_bytes, c->stream);
+  (void)hipMemsetAsync(c->d_choice, 0, (size_t)c->num_sbs, c->stream);
+  (void)hipMemsetAsync(c->d_lrunits, 0, sizeof(Mav1LrUnit) * (size_t)c->lr_units, c->stream);
+  (void)hipMemsetAsync(c->d_lrtype, 0, sizeof(int) * 3, c->stream);
   {
     static int32_t mats[4096 * 15];
     static int have = 0;
@@ -235,6 +269,17 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipFree(c->d_mats);
   (void)hipFree(c->d_ys);
   (void)hipFree(c->d_uvs);
+  (void)hipFree(c->d_lfmask);
+  (void)hipFree(c->d_cdefbuf);
+  (void)hipFree(c->d_dbk);
+  (void)hipFree(c->d_lrout);
+  (void)hipFree(c->d_dirvar);
+  (void)hipFree(c->d_choice);
+  (void)hipFree(c->d_lrunits);
+  (void)hipFree(c->d_lrstats);
+  (void)hipFree(c->d_lrref);
+  (void)hipFree(c->d_lrtype);
+  (void)hipFree(c->d_lrreruns);
   (void)hipFree(c->d_pool);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -260,6 +305,12 @@ static void *buf_ptr(mav1hip_ctx *c, int buf, size_t *bytes) {
     case MAV1HIP_BUF_SBS: *bytes = sizeof(Mav1SbInfo) * (size_t)c->num_sbs; return c->d_sbs;
     case MAV1HIP_BUF_REC: *bytes = c->frame_bytes; return c->d_rec;
     case MAV1HIP_BUF_COEFS: *bytes = sizeof(int32_t) * MAV1_COEFS_PER_SB * (size_t)c->num_sbs; return c->d_coefs;
+    case MAV1HIP_BUF_LFMASK: *bytes = (size_t)c->mi_rows * c->mi_cols; return c->d_lfmask;
+    case MAV1HIP_BUF_CDEFBUF: *bytes = c->frame_bytes * MAV1HIP_MAX_CDEF_SETS; return c->d_cdefbuf;
+    case MAV1HIP_BUF_DBK: *bytes = c->frame_bytes; return c->d_dbk;
+    case MAV1HIP_BUF_LROUT: *bytes = c->frame_bytes; return c->d_lrout;
+    case MAV1HIP_BUF_LRUNITS: *bytes = sizeof(Mav1LrUnit) * (size_t)c->lr_units; return c->d_lrunits;
+    case MAV1HIP_BUF_CDEF_CHOICE: *bytes = (size_t)c->num_sbs; return c->d_choice;
     default: *bytes = 0; return nullptr;
   }
 }
@@ -310,6 +361,7 @@ int mav1hip_set_params(mav1hip_ctx *c, const Mav1FrameParams *fp) {
   if (rc) return rc;
   c->fp = *fp;
   c->fp_set = 1;
+  HIPCHK(c, hipMemcpyAsync(c->d_lrtype, fp->lr_type, sizeof(int) * 3, hipMemcpyHostToDevice, c->stream));
   return MAV1HIP_OK;
 }
 
@@ -416,6 +468,155 @@ int mav1hip_launch_recon(mav1hip_ctx *c) {
   return MAV1HIP_OK;
 }
 
+int mav1hip_launch_filter_info(mav1hip_ctx *c) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "filter info: no frame parameters set");
+  return timed(c, "filter_info", [&] { return launch_filter_info(c->stream, c->fp, c->d_blocks, c->d_sbs, c->d_lfmask); });
+}
+
+int mav1hip_launch_deblock(mav1hip_ctx *c) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "deblock: no frame parameters set");
+  return timed(c, "deblock", [&] { return launch_deblock(c->stream, c->fp, c->d_rec, c->d_lfmask); });
+}
+
+int mav1hip_launch_cdef(mav1hip_ctx *c, int32_t nsets) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "cdef: no frame parameters set");
+  if (nsets < 0 || nsets > MAV1HIP_MAX_CDEF_SETS) return fail(c, MAV1HIP_EINVAL, "cdef: %d sets (at most %d)", nsets,
+                                                           MAV1HIP_MAX_CDEF_SETS);
+  if (!nsets) return MAV1HIP_OK;
+  return timed(c, "cdef", [&] {
+    return launch_cdef(c->stream, c->fp, c->d_rec, c->d_cdefbuf, (long long)c->frame_bytes, c->d_lfmask, c->d_dirvar,
+                       nsets);
+  });
+}
+
+int mav1hip_launch_save_deblocked(mav1hip_ctx *c) {
+  if (!c) return MAV1HIP_EINVAL;
+  HIPCHK(c, hipMemcpyAsync(c->d_dbk, c->d_rec, c->frame_bytes, hipMemcpyDeviceToDevice, c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_launch_cdef_select(mav1hip_ctx *c, int32_t nsets) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "cdef select: no frame parameters set");
+  if (nsets < 1 || nsets > MAV1HIP_MAX_CDEF_SETS) return fail(c, MAV1HIP_EINVAL, "cdef select: %d sets", nsets);
+  return timed(c, "cdef_select", [&] {
+    return launch_cdef_select(c->stream, c->fp, c->b_src, c->d_cdefbuf, (long long)c->frame_bytes, nsets, c->d_rec,
+                              c->d_choice);
+  });
+}
+
+static int check_lr_params(mav1hip_ctx *c, const Mav1FrameParams &fp) {
+  int n = 0;
+  for (int p = 0; p < 3; p++) {
+    if (fp.lr_unit_size[p] != 64 || fp.lr_unit_rows[p] < 1 || fp.lr_unit_cols[p] < 1 || fp.lr_unit_off[p] != n)
+      return fail(c, MAV1HIP_EINVAL, "loop restoration: unit layout of plane %d is not the 64x64 one", p);
+    n += fp.lr_unit_rows[p] * fp.lr_unit_cols[p];
+  }
+  if (n != c->lr_units) return fail(c, MAV1HIP_EINVAL, "loop restoration: %d units, the context holds %d", n, c->lr_units);
+  return MAV1HIP_OK;
+}
+
+int mav1hip_launch_restore(mav1hip_ctx *c) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "restore: no frame parameters set");
+  int rc = check_lr_params(c, c->fp);
+  if (rc) return rc;
+  return timed(c, "lr_apply", [&] {
+    return launch_lr_apply(c->stream, c->fp, c->d_rec, c->d_dbk, c->d_lrout, c->d_lrunits, c->d_lrtype);
+  });
+}
+
+int mav1hip_launch_lr_select(mav1hip_ctx *c, int32_t lr_test) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "lr select: no frame parameters set");
+  int rc = check_lr_params(c, c->fp);
+  if (rc) return rc;
+  return timed(c, "lr_select", [&] {
+    return launch_lr_select(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrref, c->d_lrunits,
+                            c->d_lrtype, lr_test != 0, c->d_lrreruns, c->lr_units);
+  });
+}
+
+int mav1hip_lr_select(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdef,
+                      const uint8_t *dbk, int32_t lr_test, Mav1LrUnit *units, int32_t lr_type[3], int32_t reruns[3]) {
+  if (!c || !fp || !src || !cdef || !dbk || !units || !lr_type) return fail(c, MAV1HIP_EINVAL, "lr select: missing buffer");
+  int rc;
+  if ((rc = mav1hip_set_params(c, fp))) return rc;
+  if ((rc = check_lr_params(c, *fp))) return rc;
+  c->b_src = c->d_src;
+  HIPCHK(c, hipMemcpyAsync(c->d_src, src, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_rec, cdef, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_dbk, dbk, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  if ((rc = mav1hip_launch_lr_select(c, lr_test))) return rc;
+  HIPCHK(c, hipMemcpyAsync(units, c->d_lrunits, sizeof(Mav1LrUnit) * (size_t)c->lr_units, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(lr_type, c->d_lrtype, sizeof(int) * 3, hipMemcpyDeviceToHost, c->stream));
+  if (reruns) HIPCHK(c, hipMemcpyAsync(reruns, c->d_lrreruns, sizeof(int) * 3, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_filter_info(mav1hip_ctx *c, const Mav1FrameParams *fp, const Mav1BlockInfo *blocks,
+                        const Mav1SbInfo *sbs, uint8_t *info) {
+  if (!c || !fp || !blocks || !sbs || !info) return fail(c, MAV1HIP_EINVAL, "filter info: missing buffer");
+  int rc;
+  if ((rc = mav1hip_set_params(c, fp))) return rc;
+  size_t nb = sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs;
+  HIPCHK(c, hipMemcpyAsync(c->d_blocks, blocks, nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_sbs, sbs, sizeof(Mav1SbInfo) * (size_t)c->num_sbs, hipMemcpyHostToDevice, c->stream));
+  if ((rc = mav1hip_launch_filter_info(c))) return rc;
+  HIPCHK(c, hipMemcpyAsync(info, c->d_lfmask, (size_t)c->mi_rows * c->mi_cols, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_filter(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *info, uint8_t *rec, int32_t nsets,
+                   uint8_t *cdefbuf) {
+  if (!c || !fp || !info || !rec || (nsets && !cdefbuf)) return fail(c, MAV1HIP_EINVAL, "filter: missing buffer");
+  int rc;
+  if ((rc = mav1hip_set_params(c, fp))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_lfmask, info, (size_t)c->mi_rows * c->mi_cols, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_rec, rec, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  if ((rc = mav1hip_launch_deblock(c))) return rc;
+  if ((rc = mav1hip_launch_cdef(c, nsets))) return rc;
+  HIPCHK(c, hipMemcpyAsync(rec, c->d_rec, c->frame_bytes, hipMemcpyDeviceToHost, c->stream));
+  if (nsets)
+    HIPCHK(c, hipMemcpyAsync(cdefbuf, c->d_cdefbuf, c->frame_bytes * nsets, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_cdef_select(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdefbuf,
+                        int32_t nsets, uint8_t *rec, int8_t *choice) {
+  if (!c || !fp || !src || !cdefbuf || !rec || !choice) return fail(c, MAV1HIP_EINVAL, "cdef select: missing buffer");
+  if (nsets < 1 || nsets > MAV1HIP_MAX_CDEF_SETS) return fail(c, MAV1HIP_EINVAL, "cdef select: %d sets", nsets);
+  int rc;
+  if ((rc = mav1hip_set_params(c, fp))) return rc;
+  c->b_src = c->d_src;
+  HIPCHK(c, hipMemcpyAsync(c->d_src, src, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_cdefbuf, cdefbuf, c->frame_bytes * nsets, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_rec, rec, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  if ((rc = mav1hip_launch_cdef_select(c, nsets))) return rc;
+  HIPCHK(c, hipMemcpyAsync(rec, c->d_rec, c->frame_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(choice, c->d_choice, (size_t)c->num_sbs, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_restore(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *cdef, const uint8_t *dbk,
+                    const Mav1LrUnit *units, uint8_t *out) {
+  if (!c || !fp || !cdef || !dbk || !units || !out) return fail(c, MAV1HIP_EINVAL, "restore: missing buffer");
+  int rc;
+  if ((rc = mav1hip_set_params(c, fp))) return rc;
+  if ((rc = check_lr_params(c, *fp))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_rec, cdef, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_dbk, dbk, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_lrunits, units, sizeof(Mav1LrUnit) * (size_t)c->lr_units, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_lrout, out, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
+  if ((rc = mav1hip_launch_restore(c))) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, c->d_lrout, c->frame_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
 int mav1hip_edge_types(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs) {
   if (!c || !fp || !blocks || !sbs) return fail(c, MAV1HIP_EINVAL, "edge types: missing buffer");
   int rc;
@@ -513,6 +714,17 @@ int mav1hip_bind_frames(mav1hip_ctx *c, int32_t s, int32_t r, int32_t g) {
   c->b_src = s < 0 ? c->d_src : c->d_pool + (size_t)s * c->frame_bytes;
   c->b_ref = r < 0 ? c->d_ref : c->d_pool + (size_t)r * c->frame_bytes;
   c->b_gold = g < 0 ? c->d_gold : c->d_pool + (size_t)g * c->frame_bytes;
+  return MAV1HIP_OK;
+}
+
+int mav1hip_pool_store(mav1hip_ctx *c, int32_t idx, int32_t buf) {
+  if (!c || idx < 0 || idx >= c->pool_n) return fail(c, MAV1HIP_EINVAL, "pool_store: bad index %d", idx);
+  if (buf != MAV1HIP_BUF_REC && buf != MAV1HIP_BUF_LROUT) return fail(c, MAV1HIP_EINVAL, "pool_store: buffer %d", buf);
+  uint8_t *dst = c->d_pool + (size_t)idx * c->frame_bytes;
+  if (dst == c->b_src || dst == c->b_ref || dst == c->b_gold)
+    return fail(c, MAV1HIP_EINVAL, "pool_store: frame %d is bound to a stage input", idx);
+  HIPCHK(c, hipMemcpyAsync(dst, buf == MAV1HIP_BUF_REC ? c->d_rec : c->d_lrout, c->frame_bytes,
+                           hipMemcpyDeviceToDevice, c->stream));
   return MAV1HIP_OK;
 }
 

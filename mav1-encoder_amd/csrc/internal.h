@@ -40,4 +40,21 @@ hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, int t, 
                               const Mav1SbInfo *sbs, const uint8_t *src, uint8_t *rec, const uint8_t *ref,
                               const uint8_t *gold, int32_t *coefs, const int32_t *mats);
 
+/* filters.hip */
+hipError_t launch_filter_info(hipStream_t st, const Mav1FrameParams &fp, const Mav1BlockInfo *blocks,
+                              const Mav1SbInfo *sbs, uint8_t *info);
+hipError_t launch_deblock(hipStream_t st, const Mav1FrameParams &fp, uint8_t *rec, const uint8_t *info);
+hipError_t launch_cdef(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *cur, uint8_t *out,
+                       long long frameBytes, const uint8_t *info, int *dirvar, int n);
+hipError_t launch_cdef_select(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdefbuf,
+                              long long frameBytes, int n, uint8_t *rec, int8_t *choice);
+hipError_t launch_lr_apply(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *cdef, const uint8_t *dbk,
+                           uint8_t *out, const Mav1LrUnit *units, const int *lrType);
+
+/* lrsearch.hip */
+size_t lr_stat_bytes();
+hipError_t launch_lr_select(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
+                            const uint8_t *dbk, void *stats, int *refW, Mav1LrUnit *units, int *lrType, int lrTest,
+                            int *reruns, int nunits);
+
 #endif

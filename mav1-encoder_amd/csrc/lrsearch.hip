@@ -1,0 +1,678 @@
+/* Loop-restoration parameter search on gfx950 (behaviour of lr_select,
+ * src/lr_search.c:235-359): per restoration unit NONE, a separable
+ * least-squares Wiener fit refined greedily tap by tap, and the 16
+ * self-guided sets with least-squares projection weights; the smallest
+ * 16 * SSE + lambda * bits wins.
+ *
+ * The choice of a unit depends on the previous unit of its plane only
+ * through the subexp references of the coded parameters (bit counts), so:
+ *   k_lr_fit    one workgroup per unit, all units at once: SSE of NONE, the
+ *               Wiener fit, each self-guided set's weights and SSE;
+ *   k_lr_path   one workgroup per unit: the greedy tap refinement under an
+ *               assumed incoming Wiener reference;
+ *   k_lr_chain  one workgroup per plane walks its units in order with the
+ *               true references, reruns a unit's refinement where the
+ *               assumption was wrong, and writes the chosen parameters.
+ * The Wiener fit is FP64 and its rounded result decides the coded taps: it
+ * accumulates in the reference's order (block by block, sample by sample,
+ * one lane per accumulator) with the reference's operand order, and this
+ * file is compiled without floating-point contraction. Every other sum is
+ * an exact integer. */
+#include "dev_common.h"
+#include "internal.h"
+
+#pragma clang fp contract(off)
+
+#define LR_PITCH 104   /* unit width (<= 95) + 6 halo columns, rounded up */
+#define LR_ROWS 128    /* unit rows (<= 103) + 6 halo rows per stripe segment (<= 4) */
+#define LR_MAXSEG 4
+#define LR_MAXPX (104 * 96)
+#define LR_T 256       /* threads per workgroup */
+
+struct LrGeo {
+  int p, sub, x0, x1, y0, y1, endX, endY;
+  int nseg;
+  int segY0[LR_MAXSEG], segY1[LR_MAXSEG], segS0[LR_MAXSEG], segS1[LR_MAXSEG], segBase[LR_MAXSEG];
+  int bw, bh, nbr, nbc;
+};
+
+/* Working set of the workgroup searching one unit. */
+struct LrLds {
+  uint8_t img[LR_ROWS * LR_PITCH]; /* stripe-resolved source samples (get_source_sample) per segment */
+  union {
+    struct {
+      int16_t f0[LR_MAXPX], f1[LR_MAXPX]; /* self-guided filter outputs */
+    } sg;
+    struct {
+      double phi[3][LR_T + 128], tgt[LR_T + 128]; /* one block row of fit terms, in sample order */
+    } wf;
+    struct {
+      int16_t mid[2][LR_ROWS * 96]; /* Wiener horizontal pass: current taps, candidate taps */
+    } wp;
+  } u;
+  long long red[LR_T / 64 * 6];
+  double acc[12];
+  int itmp[16];
+};
+__shared__ LrLds gLr;
+
+/* Per-unit search state between the kernels. */
+struct LrUnitStat {
+  long long sseNone;
+  long long sseSgr[16];
+  int xqd[16][2];
+  int w0[6];          /* fitted Wiener taps: [pass * 3 + j] */
+  int refAssumed[6];  /* incoming Wiener reference the path was computed with */
+  int wStar[6];       /* refined taps */
+  long long sseStar;  /* SSE of the refined taps */
+  long long costStar; /* their cost under refAssumed */
+};
+
+DEVI int lr_stripe_row(int k, int sub) { return (64 * k - 8) >> sub; }
+
+/* Pixel bounds of unit u and its stripe segments (the 4x4 blocks of
+ * lr_block_setup, src/kernels/k_lr.h:28-45, whose unit is u). */
+DEVI void lr_geo(const Mav1FrameParams &fp, int u, LrGeo &g) {
+  const int p = u >= fp.lr_unit_off[2] ? 2 : u >= fp.lr_unit_off[1] ? 1 : 0;
+  const int l = u - fp.lr_unit_off[p], ucols = fp.lr_unit_cols[p], urows = fp.lr_unit_rows[p];
+  const int ur = l / ucols, uc = l - (l / ucols) * ucols, sub = p > 0;
+  g.p = p;
+  g.sub = sub;
+  g.endX = ((fp.frame_w + sub) >> sub) - 1;
+  g.endY = ((fp.frame_h + sub) >> sub) - 1;
+  g.x0 = uc * 64;
+  g.x1 = uc == ucols - 1 ? g.endX + 1 : d_min((uc + 1) * 64, g.endX + 1);
+  const int off = 8 >> sub;
+  g.y0 = d_max(0, ur * 64 - off);
+  g.y1 = ur == urows - 1 ? g.endY + 1 : d_min((ur + 1) * 64 - off, g.endY + 1);
+  g.bw = 4 >> sub;
+  g.bh = 4 >> sub;
+  g.nbr = (g.y1 - g.y0 + g.bh - 1) / g.bh;
+  g.nbc = (g.x1 - g.x0 + g.bw - 1) / g.bw;
+  int k = ((g.y0 << sub) + 8) / 64, base = 0;
+  g.nseg = 0;
+  for (int y = g.y0; y < g.y1 && g.nseg < LR_MAXSEG; k++) {
+    const int s0 = lr_stripe_row(k, sub), s1 = lr_stripe_row(k + 1, sub) - 1;
+    const int a = d_max(y, s0), b = d_min(g.y1, s1 + 1);
+    g.segY0[g.nseg] = a;
+    g.segY1[g.nseg] = b;
+    g.segS0[g.nseg] = s0;
+    g.segS1[g.nseg] = s1;
+    g.segBase[g.nseg] = base;
+    base += (b - a + 6) * LR_PITCH;
+    g.nseg++;
+    y = b;
+  }
+}
+
+DEVI int lr_seg_of(const LrGeo &g, int y) { return y < g.segY1[0] ? 0 : y < g.segY1[1] ? 1 : y < g.segY1[2] ? 2 : 3; }
+
+/* Sample (x, y) of segment s, -3 <= x - x0 < w + 3, -3 <= y - segY0 < h + 3. */
+DEVI int lr_img(const LrGeo &g, int s, int x, int y) {
+  return gLr.img[g.segBase[s] + (y - g.segY0[s] + 3) * LR_PITCH + (x - g.x0 + 3)];
+}
+
+/* get_source_sample (7.17.6) of every sample the unit's filters read. */
+DEVI void lr_stage(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t *cdef, const uint8_t *dbk, int tid) {
+  const int S = fp.stride[g.p], o = fp.offset[g.p], w = g.x1 - g.x0 + 6;
+  for (int s = 0; s < g.nseg; s++) {
+    const int h = g.segY1[s] - g.segY0[s] + 6;
+    for (int q = tid; q < w * h; q += LR_T) {
+      const int r = q / w, c = q - r * w;
+      const int x = d_clip(0, g.endX, g.x0 - 3 + c);
+      int y = d_clip(0, g.endY, g.segY0[s] - 3 + r);
+      const uint8_t *pl = cdef;
+      if (y < g.segS0[s]) {
+        y = d_max(g.segS0[s] - 2, y);
+        pl = dbk;
+      } else if (y > g.segS1[s]) {
+        y = d_min(g.segS1[s] + 2, y);
+        pl = dbk;
+      }
+      gLr.img[g.segBase[s] + r * LR_PITCH + c] = pl[o + y * S + x];
+    }
+  }
+  __syncthreads();
+}
+
+/* Block-row / sample-order helpers: block row br holds h rows of nbc
+ * blocks; sample q of the row is in block q / (h * bw) (the last block may
+ * be narrower). */
+DEVI void lr_row_sample(const LrGeo &g, int br, int q, int &x, int &y) {
+  const int y0 = g.y0 + br * g.bh, h = d_min(g.bh, g.endY - y0 + 1);
+  const int full = h * g.bw, j = q / full;
+  const int bx = g.x0 + j * g.bw, w = d_min(g.bw, g.x1 - bx);
+  const int rem = q - j * full;
+  y = y0 + rem / w;
+  x = bx + rem - (rem / w) * w;
+}
+DEVI int lr_row_count(const LrGeo &g, int br) {
+  const int y0 = g.y0 + br * g.bh;
+  return d_min(g.bh, g.endY - y0 + 1) * (g.x1 - g.x0);
+}
+
+DEVI long long block_sum64(long long v, int tid) {
+  v = wave_sum64(v);
+  if ((tid & 63) == 0) gLr.red[tid >> 6] = v;
+  __syncthreads();
+  long long s = 0;
+#pragma unroll
+  for (int i = 0; i < LR_T / 64; i++) s += gLr.red[i];
+  __syncthreads();
+  return s;
+}
+
+/* ---- subexp bit counts (lr_search.c:18-79) ---- */
+DEVI int lr_recenter(int r, int v) {
+  if (v > 2 * r) return v;
+  return v >= r ? (v - r) << 1 : ((r - v) << 1) - 1;
+}
+DEVI int lr_subexp_bits(int low, int high, int k, int r, int v) {
+  const int mx = high - low;
+  r -= low;
+  v -= low;
+  const int x = 2 * r <= mx ? lr_recenter(r, v) : lr_recenter(mx - 1 - r, mx - 1 - v);
+  int bits = 0, mk = 0;
+  for (int i = 0;; i++) {
+    const int b2 = i ? k + i - 1 : k, a = 1 << b2;
+    if (mx <= mk + 3 * a) {
+      const int n = mx - mk, w = d_floorlog2((unsigned)n) + 1, m = (1 << w) - n;
+      return bits + (x - mk < m ? w - 1 : w);
+    }
+    bits++;
+    if (x < mk + a) return bits + b2;
+    mk += a;
+  }
+}
+DEVI int lr_wiener_bits(int p, const int *w, const int *ref) {
+  int bits = 0;
+  for (int pass = 0; pass < 2; pass++)
+    for (int j = p ? 1 : 0; j < 3; j++)
+      bits += lr_subexp_bits(Wiener_Taps_Min[j], Wiener_Taps_Max[j] + 1, Wiener_Taps_K[j], ref[pass * 3 + j],
+                             w[pass * 3 + j]);
+  return bits;
+}
+DEVI int lr_sgr_bits(int set, const int *xqd, const int *refX) {
+  int bits = SGRPROJ_PARAMS_BITS;
+  for (int i = 0; i < 2; i++)
+    if (SGR_PARAM(set, i * 2))
+      bits += lr_subexp_bits(Sgrproj_Xqd_Min[i], Sgrproj_Xqd_Max[i] + 1, SGRPROJ_PRJ_SUBEXP_K, refX[i], xqd[i]);
+  return bits;
+}
+DEVI long long lr_rd(const Mav1FrameParams &fp, long long sse, int bits) { return sse * 16 + (long long)fp.lambda * bits; }
+
+/* ---- self-guided filter of one 4x4 block (7.17.3) from the staged samples ---- */
+DEVI void lr_box_block(const LrGeo &g, int s, int bx, int by, int w, int h, int set, int pass, int *F) {
+  const int r = SGR_PARAM(set, pass * 2);
+  const int eps = SGR_PARAM(set, pass * 2 + 1), n = (2 * r + 1) * (2 * r + 1), n2e = n * n * eps;
+  const int sc = ((1 << SGRPROJ_MTABLE_BITS) + n2e / 2) / n2e;
+  const int one = ((1 << SGRPROJ_RECIP_BITS) + n / 2) / n;
+  int A[6][6], B[6][6];
+  for (int i = -1; i <= h; i++)
+    for (int j = -1; j <= w; j++) {
+      int sq = 0, sum = 0;
+      for (int dy = -r; dy <= r; dy++)
+        for (int dx = -r; dx <= r; dx++) {
+          const int c = lr_img(g, s, bx + j + dx, by + i + dy);
+          sq += c * c;
+          sum += c;
+        }
+      long long pv = (long long)sq * n - (long long)sum * sum;
+      if (pv < 0) pv = 0;
+      const long long z = (pv * sc + (1 << (SGRPROJ_MTABLE_BITS - 1))) >> SGRPROJ_MTABLE_BITS;
+      const int a2 = z >= 255 ? 256 : z == 0 ? 1 : (int)(((z << SGRPROJ_SGR_BITS) + z / 2) / (z + 1));
+      const long long b2 = (long long)((1 << SGRPROJ_SGR_BITS) - a2) * sum * one;
+      A[i + 1][j + 1] = a2;
+      B[i + 1][j + 1] = (int)((b2 + (1 << (SGRPROJ_RECIP_BITS - 1))) >> SGRPROJ_RECIP_BITS);
+    }
+  for (int i = 0; i < h; i++) {
+    const int sh = SGRPROJ_SGR_BITS + ((pass == 0 && (i & 1)) ? 4 : 5) - SGRPROJ_RST_BITS;
+    for (int j = 0; j < w; j++) {
+      long long a = 0, bb = 0;
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+          const int wgt = pass == 0 ? (((i + dy) & 1) ? (dx == 0 ? 6 : 5) : 0) : (dx == 0 || dy == 0) ? 4 : 3;
+          a += wgt * A[i + dy + 1][j + dx + 1];
+          bb += wgt * B[i + dy + 1][j + dx + 1];
+        }
+      const long long v = a * lr_img(g, s, bx + j, by + i) + bb;
+      F[i * 4 + j] = (int)((v + (1ll << (sh - 1))) >> sh);
+    }
+  }
+}
+
+/* ---- Wiener SSE of a unit for taps w (pass-major) ---- */
+/* Horizontal pass of taps hf into mid buffer m: every staged row, the
+ * unit's columns. */
+DEVI void lr_mid(const LrGeo &g, const int *hw, int m, int tid) {
+  int hf[7];
+  hf[3] = 128;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    hf[k] = hf[6 - k] = hw[k];
+    hf[3] -= 2 * hw[k];
+  }
+  const int uw = g.x1 - g.x0;
+  int rowBase = 0;
+  for (int s = 0; s < g.nseg; s++) {
+    const int rows = g.segY1[s] - g.segY0[s] + 6;
+    for (int q = tid; q < rows * uw; q += LR_T) {
+      const int r = q / uw, c = q - r * uw;
+      const uint8_t *ip = gLr.img + g.segBase[s] + r * LR_PITCH + c;
+      int acc = 0;
+#pragma unroll
+      for (int t = 0; t < 7; t++) acc += hf[t] * ip[t];
+      gLr.u.wp.mid[m][(rowBase + r) * 96 + c] = (int16_t)d_clip(-2048, 8191 - 2048, d_round2(acc, 3));
+    }
+    rowBase += rows;
+  }
+  __syncthreads();
+}
+/* Vertical pass of taps vw over mid buffer m and SSE against the source. */
+DEVI long long lr_wiener_sse(const Mav1FrameParams &fp, const LrGeo &g, const int *vw, int m, const uint8_t *src,
+                             int tid) {
+  int vf[7];
+  vf[3] = 128;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    vf[k] = vf[6 - k] = vw[k];
+    vf[3] -= 2 * vw[k];
+  }
+  const int uw = g.x1 - g.x0, S = fp.stride[g.p], o = fp.offset[g.p];
+  long long e = 0;
+  int rowBase = 0;
+  for (int s = 0; s < g.nseg; s++) {
+    const int h = g.segY1[s] - g.segY0[s];
+    for (int q = tid; q < h * uw; q += LR_T) {
+      const int r = q / uw, c = q - r * uw;
+      const int16_t *mp = gLr.u.wp.mid[m] + (rowBase + r) * 96 + c;
+      int acc = 0;
+#pragma unroll
+      for (int t = 0; t < 7; t++) acc += vf[t] * mp[t * 96];
+      const int d = d_clip1(d_round2(acc, 11)) - (int)src[o + (g.segY0[s] + r) * S + g.x0 + c];
+      e += d * d;
+    }
+    rowBase += h + 6;
+  }
+  return block_sum64(e, tid);
+}
+
+/* Greedy +-1 refinement of each coded tap (lr_search.c:283-301) from the
+ * taps in w (updated in place) with incoming reference ref; returns the
+ * final cost, *sseOut its SSE. Needs the staged samples. */
+DEVI long long lr_refine(const Mav1FrameParams &fp, const LrGeo &g, int *w, const int *ref, const uint8_t *src,
+                         long long *sseOut, int tid) {
+  int cur = 0;
+  lr_mid(g, w + 3, cur, tid);
+  long long sse = lr_wiener_sse(fp, g, w, cur, src, tid);
+  long long wc = lr_rd(fp, sse, 2 + lr_wiener_bits(g.p, w, ref));
+  for (int pass = 0; pass < 2; pass++)
+    for (int j = g.p ? 1 : 0; j < 3; j++)
+      for (int d = -1; d <= 1; d += 2) {
+        int t[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) t[k] = w[k];
+        t[pass * 3 + j] += d;
+        if (t[pass * 3 + j] < Wiener_Taps_Min[j] || t[pass * 3 + j] > Wiener_Taps_Max[j]) continue;
+        long long ts;
+        if (pass == 0) {
+          ts = lr_wiener_sse(fp, g, t, cur, src, tid);
+        } else {
+          lr_mid(g, t + 3, cur ^ 1, tid);
+          ts = lr_wiener_sse(fp, g, t, cur ^ 1, src, tid);
+        }
+        const long long tc = lr_rd(fp, ts, 2 + lr_wiener_bits(g.p, t, ref));
+        if (tc < wc) {
+          wc = tc;
+          sse = ts;
+#pragma unroll
+          for (int k = 0; k < 6; k++) w[k] = t[k];
+          if (pass == 1) cur ^= 1;
+        }
+      }
+  *sseOut = sse;
+  return wc;
+}
+
+/* Gaussian elimination with partial pivoting (lr_search.c:98-129). */
+DEVI int lr_solve(int n, double M[3][3], double v[3], double *x) {
+  for (int i = 0; i < n; i++) {
+    int piv = i;
+    for (int r = i + 1; r < n; r++)
+      if (fabs(M[r][i]) > fabs(M[piv][i])) piv = r;
+    if (fabs(M[piv][i]) < 1e-9) return 0;
+    if (piv != i) {
+      for (int k = 0; k < n; k++) {
+        const double t = M[i][k];
+        M[i][k] = M[piv][k];
+        M[piv][k] = t;
+      }
+      const double t = v[i];
+      v[i] = v[piv];
+      v[piv] = t;
+    }
+    for (int r = i + 1; r < n; r++) {
+      const double f = M[r][i] / M[i][i];
+      for (int k = i; k < n; k++) M[r][k] -= f * M[i][k];
+      v[r] -= f * v[i];
+    }
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = v[i];
+    for (int k = i + 1; k < n; k++) s -= M[i][k] * x[k];
+    x[i] = s / M[i][i];
+  }
+  return 1;
+}
+
+/* Separable least-squares Wiener fit (lr_search.c:132-180): four
+ * alternating solves; the sums run over the unit's samples in the
+ * reference's order, one lane per accumulator. */
+DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t *src, int *out, int tid) {
+  const int first = g.p ? 1 : 0, nc = 3 - first, nacc = nc * nc + nc;
+  const int S = fp.stride[g.p], o = fp.offset[g.p];
+  double fv[7] = {0, 0, 0, 1, 0, 0, 0}, fh[7] = {0, 0, 0, 1, 0, 0, 0};
+  for (int it = 0; it < 4; it++) {
+    const int vert = !(it & 1);
+    double acc = 0; /* lane tid < nacc: M[a][b] (a * nc + b), then V[a] */
+    for (int br = 0; br < g.nbr; br++) {
+      const int cnt = lr_row_count(g, br);
+      for (int c0 = 0; c0 < cnt; c0 += LR_T) {
+        const int q = c0 + tid;
+        if (q < cnt) {
+          int x, y;
+          lr_row_sample(g, br, q, x, y);
+          const int s = lr_seg_of(g, y);
+          double Z[7];
+          for (int a = 0; a < 7; a++) {
+            double sum = 0;
+            for (int t = 0; t < 7; t++)
+              sum += vert ? fh[t] * (double)lr_img(g, s, x + t - 3, y + a - 3) : fv[t] * (double)lr_img(g, s, x + a - 3, y + t - 3);
+            Z[a] = sum;
+          }
+          for (int a = first; a < 3; a++) gLr.u.wf.phi[a - first][q - c0] = Z[a] + Z[6 - a] - 2 * Z[3];
+          gLr.u.wf.tgt[q - c0] = (double)src[o + y * S + x] - Z[3];
+        }
+        __syncthreads();
+        if (tid < nacc) {
+          const int n = d_min(LR_T, cnt - c0);
+          const int a = tid < nc * nc ? tid / nc : tid - nc * nc, b = tid < nc * nc ? tid - (tid / nc) * nc : -1;
+          const double *pa = gLr.u.wf.phi[a], *pb = b >= 0 ? gLr.u.wf.phi[b] : gLr.u.wf.tgt;
+          for (int k = 0; k < n; k++) acc += pa[k] * pb[k];
+        }
+        __syncthreads();
+      }
+    }
+    if (tid < nacc) gLr.acc[tid] = acc;
+    __syncthreads();
+    if (tid == 0) {
+      double M[3][3], V[3], x[3] = {0, 0, 0};
+      for (int a = 0; a < nc; a++) {
+        V[a] = gLr.acc[nc * nc + a];
+        for (int b = 0; b < nc; b++) M[a][b] = gLr.acc[a * nc + b];
+      }
+      for (int a = 0; a < nc; a++) M[a][a] += 1e-3;
+      const int ok = lr_solve(nc, M, V, x);
+      gLr.itmp[0] = ok;
+      for (int a = 0; a < 3; a++) gLr.acc[a] = a < first ? 0 : x[a - first];
+    }
+    __syncthreads();
+    if (!gLr.itmp[0]) break;
+    double *dst = vert ? fv : fh, sum = 0;
+    for (int a = 0; a < 3; a++) {
+      const double v = gLr.acc[a];
+      dst[a] = dst[6 - a] = v;
+      sum += v;
+    }
+    dst[3] = 1 - 2 * sum;
+    __syncthreads();
+  }
+  for (int pass = 0; pass < 2; pass++)
+    for (int j = 0; j < 3; j++)
+      out[pass * 3 + j] =
+          j < first ? 0 : d_clip(Wiener_Taps_Min[j], Wiener_Taps_Max[j], (int)lround((pass ? fh : fv)[j] * 128));
+}
+
+/* Per unit: SSE of NONE, the Wiener fit, and each self-guided set's
+ * weights (sgr_fit, lr_search.c:183-231) and SSE. */
+__global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+                                                 const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
+                                                 LrUnitStat *__restrict__ stats) {
+  const int u = blockIdx.x, tid = threadIdx.x;
+  LrGeo g;
+  lr_geo(fp, u, g);
+  lr_stage(fp, g, cdef, dbk, tid);
+  LrUnitStat &st = stats[u];
+  const int S = fp.stride[g.p], o = fp.offset[g.p], uw = g.x1 - g.x0, uh = g.y1 - g.y0;
+  long long e = 0;
+  for (int q = tid; q < uw * uh; q += LR_T) {
+    const int a = o + (g.y0 + q / uw) * S + g.x0 + q % uw;
+    const int d = (int)cdef[a] - (int)src[a];
+    e += d * d;
+  }
+  e = block_sum64(e, tid);
+  if (tid == 0) st.sseNone = e;
+  int w0[6];
+  lr_wiener_fit(fp, g, src, w0, tid);
+  if (tid < 6) st.w0[tid] = w0[tid];
+  const int nblk = g.nbr * g.nbc;
+  for (int set = 0; set < 16; set++) {
+    const int r0 = SGR_PARAM(set, 0), r1 = SGR_PARAM(set, 2);
+    long long h00 = 0, h01 = 0, h11 = 0, c0 = 0, c1 = 0;
+    for (int bi = tid; bi < nblk; bi += LR_T) {
+      const int br = bi / g.nbc, bc = bi - br * g.nbc;
+      const int by = g.y0 + br * g.bh, bx = g.x0 + bc * g.bw;
+      const int h = d_min(g.bh, g.endY - by + 1), w = d_min(g.bw, g.x1 - bx), s = lr_seg_of(g, by);
+      int F0[16], F1[16];
+      if (r0) lr_box_block(g, s, bx, by, w, h, set, 0, F0);
+      if (r1) lr_box_block(g, s, bx, by, w, h, set, 1, F1);
+      for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++) {
+          const int uu = lr_img(g, s, bx + j, by + i) << SGRPROJ_RST_BITS;
+          const int sv = ((int)src[o + (by + i) * S + bx + j] << SGRPROJ_RST_BITS) - uu;
+          const int e0 = r0 ? F0[i * 4 + j] - uu : 0, e1 = r1 ? F1[i * 4 + j] - uu : 0;
+          h00 += (long long)e0 * e0;
+          h01 += (long long)e0 * e1;
+          h11 += (long long)e1 * e1;
+          c0 += (long long)e0 * sv;
+          c1 += (long long)e1 * sv;
+          const int pi = (by + i - g.y0) * uw + bx + j - g.x0;
+          gLr.u.sg.f0[pi] = (int16_t)(r0 ? F0[i * 4 + j] : 0);
+          gLr.u.sg.f1[pi] = (int16_t)(r1 ? F1[i * 4 + j] : 0);
+        }
+    }
+    h00 = block_sum64(h00, tid);
+    h01 = block_sum64(h01, tid);
+    h11 = block_sum64(h11, tid);
+    c0 = block_sum64(c0, tid);
+    c1 = block_sum64(c1, tid);
+    if (tid == 0) {
+      /* the reference's double sums of integer terms are exact: the same
+       * values as these integer sums */
+      const double H00 = (double)h00, H01 = (double)h01, H11 = (double)h11, C0 = (double)c0, C1 = (double)c1;
+      double a = 0, bb = 0;
+      if (r0 && r1) {
+        const double det = H00 * H11 - H01 * H01;
+        if (fabs(det) > 1e-9) {
+          a = (C0 * H11 - C1 * H01) / det;
+          bb = (C1 * H00 - C0 * H01) / det;
+        }
+      } else if (r0) {
+        if (H00 > 0) a = C0 / H00;
+      } else if (r1) {
+        if (H11 > 0) bb = C1 / H11;
+      }
+      const int w0q = r0 ? (int)lround(a * (1 << SGRPROJ_PRJ_BITS)) : 0, w2q = (int)lround(bb * (1 << SGRPROJ_PRJ_BITS));
+      const int x0 = d_clip(Sgrproj_Xqd_Min[0], Sgrproj_Xqd_Max[0], w0q);
+      const int x1 = d_clip(Sgrproj_Xqd_Min[1], Sgrproj_Xqd_Max[1], (1 << SGRPROJ_PRJ_BITS) - x0 - (r1 ? w2q : 0));
+      gLr.itmp[0] = x0;
+      gLr.itmp[1] = x1;
+      st.xqd[set][0] = x0;
+      st.xqd[set][1] = x1;
+    }
+    __syncthreads();
+    const int xw0 = gLr.itmp[0], xw1 = gLr.itmp[1], xw2 = (1 << SGRPROJ_PRJ_BITS) - xw0 - xw1;
+    long long es = 0;
+    for (int q = tid; q < uw * uh; q += LR_T) {
+      const int y = g.y0 + q / uw, x = g.x0 + q % uw;
+      const int uu = lr_img(g, lr_seg_of(g, y), x, y) << SGRPROJ_RST_BITS;
+      const int v = xw1 * uu + (r0 ? xw0 * gLr.u.sg.f0[q] : xw0 * uu) + (r1 ? xw2 * gLr.u.sg.f1[q] : xw2 * uu);
+      const int d = d_clip1(d_round2(v, SGRPROJ_RST_BITS + SGRPROJ_PRJ_BITS)) - (int)src[o + y * S + x];
+      es += d * d;
+    }
+    es = block_sum64(es, tid);
+    if (tid == 0) st.sseSgr[set] = es;
+    __syncthreads();
+  }
+}
+
+/* Per unit: the greedy refinement of the fitted taps under the incoming
+ * reference refW (one per unit, pass-major). */
+__global__ __launch_bounds__(LR_T) void k_lr_path(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+                                                  const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
+                                                  LrUnitStat *__restrict__ stats, const int *__restrict__ refW) {
+  const int u = blockIdx.x, tid = threadIdx.x;
+  LrGeo g;
+  lr_geo(fp, u, g);
+  lr_stage(fp, g, cdef, dbk, tid);
+  LrUnitStat &st = stats[u];
+  int w[6], ref[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    w[k] = st.w0[k];
+    ref[k] = refW[u * 6 + k];
+  }
+  long long sse;
+  const long long cost = lr_refine(fp, g, w, ref, src, &sse, tid);
+  if (tid == 0) {
+    for (int k = 0; k < 6; k++) {
+      st.wStar[k] = w[k];
+      st.refAssumed[k] = ref[k];
+    }
+    st.sseStar = sse;
+    st.costStar = cost;
+  }
+}
+
+/* Per plane, units in raster order with the true running references: the
+ * choice of lr_select (lr_search.c:262-349), rerunning a unit's refinement
+ * where k_lr_path assumed another reference. units[] and lrType[plane]
+ * receive the result, reruns[plane] the number of refinements rerun. */
+__global__ __launch_bounds__(LR_T) void k_lr_chain(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+                                                   const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
+                                                   const LrUnitStat *__restrict__ stats, Mav1LrUnit *__restrict__ units,
+                                                   int *__restrict__ lrType, int lrTest, int *__restrict__ reruns) {
+  const int p = blockIdx.x, tid = threadIdx.x;
+  __shared__ int sRef[6], sRefX[2], sW[6];
+  __shared__ long long sCost, sSse;
+  __shared__ int sRerun;
+  if (tid < 6) sRef[tid] = Wiener_Taps_Mid[tid % 3];
+  if (tid < 2) sRefX[tid] = Sgrproj_Xqd_Mid[tid];
+  __syncthreads();
+  const int nu = fp.lr_unit_rows[p] * fp.lr_unit_cols[p];
+  int used = 0, nrerun = 0;
+  for (int l = 0; l < nu; l++) {
+    const int u = fp.lr_unit_off[p] + l;
+    const LrUnitStat &st = stats[u];
+    if (tid == 0) {
+      int same = 1;
+      for (int k = 0; k < 6; k++) same &= st.refAssumed[k] == sRef[k];
+      sRerun = !same;
+      for (int k = 0; k < 6; k++) sW[k] = st.wStar[k];
+      sCost = st.costStar;
+      sSse = st.sseStar;
+    }
+    __syncthreads();
+    if (sRerun) {
+      LrGeo g;
+      lr_geo(fp, u, g);
+      lr_stage(fp, g, cdef, dbk, tid);
+      int w[6], ref[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        w[k] = st.w0[k];
+        ref[k] = sRef[k];
+      }
+      long long sse;
+      const long long cost = lr_refine(fp, g, w, ref, src, &sse, tid);
+      if (tid == 0) {
+        for (int k = 0; k < 6; k++) sW[k] = w[k];
+        sCost = cost;
+        sSse = sse;
+      }
+      nrerun++;
+      __syncthreads();
+    }
+    if (tid == 0) {
+      Mav1LrUnit best = {};
+      best.type = RESTORE_NONE;
+      long long bestCost = lr_rd(fp, st.sseNone, 1);
+      Mav1LrUnit wien = {};
+      wien.type = RESTORE_WIENER;
+      for (int k = 0; k < 6; k++) wien.wiener[k / 3][k % 3] = sW[k];
+      if (sCost < bestCost) {
+        bestCost = sCost;
+        best = wien;
+      }
+      for (int set = 0; set < 16; set++) {
+        const long long sc = lr_rd(fp, st.sseSgr[set], 2 + lr_sgr_bits(set, st.xqd[set], sRefX));
+        if (sc < bestCost) {
+          bestCost = sc;
+          best = Mav1LrUnit{};
+          best.type = RESTORE_SGRPROJ;
+          best.set = set;
+          best.xqd[0] = st.xqd[set][0];
+          best.xqd[1] = st.xqd[set][1];
+        }
+      }
+      if (lrTest) {
+        if (!(l & 1)) best = wien;
+        else {
+          long long bs = -1;
+          for (int set = 0; set < 16; set++)
+            if (bs < 0 || st.sseSgr[set] < bs) {
+              bs = st.sseSgr[set];
+              best = Mav1LrUnit{};
+              best.type = RESTORE_SGRPROJ;
+              best.set = set;
+              best.xqd[0] = st.xqd[set][0];
+              best.xqd[1] = st.xqd[set][1];
+            }
+        }
+      }
+      if (best.type == RESTORE_WIENER)
+        for (int k = 0; k < 6; k++)
+          if (p == 0 || k % 3) sRef[k] = best.wiener[k / 3][k % 3];
+      if (best.type == RESTORE_SGRPROJ)
+        for (int i = 0; i < 2; i++) sRefX[i] = best.xqd[i];
+      units[u] = best;
+      used |= best.type != RESTORE_NONE;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    lrType[p] = used ? RESTORE_SWITCHABLE : RESTORE_NONE;
+    reruns[p] = nrerun;
+  }
+}
+
+__global__ void k_lr_ref_init(int *__restrict__ refW, int nunits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nunits * 6) refW[i] = Wiener_Taps_Mid[i % 3];
+}
+
+size_t lr_stat_bytes() { return sizeof(LrUnitStat); }
+
+hipError_t launch_lr_select(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
+                            const uint8_t *dbk, void *stats, int *refW, Mav1LrUnit *units, int *lrType, int lrTest,
+                            int *reruns, int nunits) {
+  LrUnitStat *s = (LrUnitStat *)stats;
+  hipLaunchKernelGGL(k_lr_fit, dim3(nunits), dim3(LR_T), 0, st, fp, src, cdef, dbk, s);
+  hipLaunchKernelGGL(k_lr_ref_init, dim3((nunits * 6 + 255) / 256), dim3(256), 0, st, refW, nunits);
+  hipLaunchKernelGGL(k_lr_path, dim3(nunits), dim3(LR_T), 0, st, fp, src, cdef, dbk, s, (const int *)refW);
+  hipLaunchKernelGGL(k_lr_chain, dim3(3), dim3(LR_T), 0, st, fp, src, cdef, dbk, (const LrUnitStat *)s, units, lrType,
+                     lrTest, reruns);
+  return hipGetLastError();
+}

@@ -192,3 +192,20 @@ int32_t mav1hip_setup_params(const Mav1HipConfig *cfg, const Mav1HipFrame *fr, i
   if (leafopt & 4) presetsOn = 0;
   return presetsOn;
 }
+
+int32_t mav1hip_filter_plan(const Mav1HipConfig *cfg, const Mav1HipFrame *fr, const Mav1FrameParams *fp,
+                            int32_t npresets, Mav1HipFilterPlan *plan) {
+  if (!cfg || !fr || !fp || !plan) return MAV1HIP_EINVAL;
+  /* run_frame_core's filter decisions (src/encoder.c:618-640) */
+  int leafoptCfg = cfg->knob_leafopt >= 0 ? cfg->knob_leafopt : 0x3F;
+  int leafopt = (cfg->altref && fr->nonref) ? leafoptCfg : 0;
+  int cdefOff = (leafopt & 4) != 0;
+  int lfOn = fp->lf_level[0] || fp->lf_level[1] || fp->lf_level[2] || fp->lf_level[3];
+  int filt = lfOn || (npresets && !cdefOff) || !cfg->turbo;
+  plan->filter_info = filt;
+  plan->deblock = filt && lfOn;
+  plan->cdef_sets = filt && !cdefOff ? npresets : 0;
+  plan->cdef_select = cfg->cdef && fp->base_q_idx > 0 && cfg->qindex > 0 && !cdefOff && npresets > 0;
+  plan->restoration = cfg->restoration && cfg->qindex > 0 && !(leafopt & 8);
+  return MAV1HIP_OK;
+}

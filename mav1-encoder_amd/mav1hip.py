@@ -16,7 +16,8 @@ NUM_NODES = 85
 A_SLOTS = 13
 TURBO_DEFAULT = 0x47CBF1
 (BUF_SRC, BUF_REF, BUF_GOLD, BUF_MVFIELD, BUF_MVGOLD, BUF_PLANES, BUF_ACOSTS, BUF_AINFOS, BUF_BLOCKS,
- BUF_SBS, BUF_REC, BUF_COEFS) = range(12)
+ BUF_SBS, BUF_REC, BUF_COEFS, BUF_LFMASK, BUF_CDEFBUF, BUF_DBK, BUF_LROUT, BUF_LRUNITS, BUF_CDEF_CHOICE) = range(18)
+MAX_CDEF_SETS = 8
 
 
 class FrameParams(C.Structure):
@@ -73,6 +74,10 @@ class Frame(C.Structure):
         "is_key", "frame_kind", "nonref", "order_hint", "reference_select", "gold_differs")]
 
 
+class FilterPlan(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("filter_info", "deblock", "cdef_sets", "cdef_select", "restoration")]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
@@ -107,9 +112,24 @@ def _load():
         "mav1hip_pool_alloc": (C.c_int, [P, i32]),
         "mav1hip_pool_upload": (C.c_int, [P, i32, P]),
         "mav1hip_bind_frames": (C.c_int, [P, i32, i32, i32]),
+        "mav1hip_pool_store": (C.c_int, [P, i32, i32]),
         "mav1hip_timing_enable": (C.c_int, [P, i32]),
         "mav1hip_timing_reset": (C.c_int, [P]),
         "mav1hip_timing_get": (C.c_int, [P, C.c_char_p, C.POINTER(i32), C.POINTER(C.c_double)]),
+        "mav1hip_filter_plan": (i32, [C.POINTER(Config), C.POINTER(Frame), C.POINTER(FrameParams), i32,
+                                      C.POINTER(FilterPlan)]),
+        "mav1hip_filter_info": (C.c_int, [P, C.POINTER(FrameParams), P, P, P]),
+        "mav1hip_filter": (C.c_int, [P, C.POINTER(FrameParams), P, P, i32, P]),
+        "mav1hip_cdef_select": (C.c_int, [P, C.POINTER(FrameParams), P, P, i32, P, P]),
+        "mav1hip_restore": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, P]),
+        "mav1hip_launch_filter_info": (C.c_int, [P]),
+        "mav1hip_launch_deblock": (C.c_int, [P]),
+        "mav1hip_launch_cdef": (C.c_int, [P, i32]),
+        "mav1hip_launch_save_deblocked": (C.c_int, [P]),
+        "mav1hip_launch_cdef_select": (C.c_int, [P, i32]),
+        "mav1hip_launch_restore": (C.c_int, [P]),
+        "mav1hip_launch_lr_select": (C.c_int, [P, i32]),
+        "mav1hip_lr_select": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, i32, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -164,6 +184,12 @@ def setup_params(cfg, fr, q=None):
     fp = FrameParams()
     n = lib().mav1hip_setup_params(C.byref(cfg), C.byref(fr), q, C.byref(fp))
     return fp, n
+
+
+def filter_plan(cfg, fr, fp, npresets):
+    plan = FilterPlan()
+    lib().mav1hip_filter_plan(C.byref(cfg), C.byref(fr), C.byref(fp), npresets, C.byref(plan))
+    return plan
 
 
 def forward_matrices():
@@ -244,6 +270,70 @@ class Context:
         self._chk(rc, "recon")
         return rec, coefs, b
 
+    def filter_info(self, fp, blocks, sbs):
+        info = np.zeros(fp.mi_rows * fp.mi_cols, np.uint8)
+        rc = lib().mav1hip_filter_info(self.h, C.byref(fp), ptr(np.ascontiguousarray(blocks, np.int32)),
+                                       ptr(np.ascontiguousarray(sbs, np.int32)), ptr(info))
+        self._chk(rc, "filter_info")
+        return info
+
+    def filter(self, fp, info, rec, nsets):
+        """Deblocked copy of rec and the nsets CDEF outputs (nsets x frame_bytes)."""
+        out = np.ascontiguousarray(rec, np.uint8).copy()
+        cdef = np.zeros((max(nsets, 1), out.size), np.uint8)
+        rc = lib().mav1hip_filter(self.h, C.byref(fp), ptr(np.ascontiguousarray(info, np.uint8)), ptr(out), nsets,
+                                  ptr(cdef))
+        self._chk(rc, "filter")
+        return out, cdef[:nsets]
+
+    def cdef_select(self, fp, src, cdefbuf, rec):
+        out = np.ascontiguousarray(rec, np.uint8).copy()
+        choice = np.zeros(fp.sb_cols * fp.sb_rows, np.int8)
+        cdefbuf = np.ascontiguousarray(cdefbuf, np.uint8)
+        rc = lib().mav1hip_cdef_select(self.h, C.byref(fp), ptr(src), ptr(cdefbuf), cdefbuf.shape[0], ptr(out),
+                                       ptr(choice))
+        self._chk(rc, "cdef_select")
+        return out, choice
+
+    def restore(self, fp, cdef, dbk, units, out=None):
+        out = np.zeros_like(cdef) if out is None else np.ascontiguousarray(out, np.uint8).copy()
+        rc = lib().mav1hip_restore(self.h, C.byref(fp), ptr(cdef), ptr(dbk), ptr(np.ascontiguousarray(units, np.int32)),
+                                   ptr(out))
+        self._chk(rc, "restore")
+        return out
+
+    def launch_filter_info(self):
+        self._chk(lib().mav1hip_launch_filter_info(self.h), "launch_filter_info")
+
+    def launch_deblock(self):
+        self._chk(lib().mav1hip_launch_deblock(self.h), "launch_deblock")
+
+    def launch_cdef(self, nsets):
+        self._chk(lib().mav1hip_launch_cdef(self.h, nsets), "launch_cdef")
+
+    def launch_save_deblocked(self):
+        self._chk(lib().mav1hip_launch_save_deblocked(self.h), "launch_save_deblocked")
+
+    def launch_cdef_select(self, nsets):
+        self._chk(lib().mav1hip_launch_cdef_select(self.h, nsets), "launch_cdef_select")
+
+    def lr_select(self, fp, src, cdef, dbk, lr_test=0):
+        """Returns (units int32[n, 12], lr_type[3], reruns[3])."""
+        n = sum(fp.lr_unit_rows[p] * fp.lr_unit_cols[p] for p in range(3))
+        units = np.zeros((n, 12), np.int32)
+        lt = np.zeros(3, np.int32)
+        rr = np.zeros(3, np.int32)
+        rc = lib().mav1hip_lr_select(self.h, C.byref(fp), ptr(src), ptr(cdef), ptr(dbk), lr_test, ptr(units), ptr(lt),
+                                     ptr(rr))
+        self._chk(rc, "lr_select")
+        return units, [int(v) for v in lt], [int(v) for v in rr]
+
+    def launch_lr_select(self, lr_test=0):
+        self._chk(lib().mav1hip_launch_lr_select(self.h, lr_test), "launch_lr_select")
+
+    def launch_restore(self):
+        self._chk(lib().mav1hip_launch_restore(self.h), "launch_restore")
+
     def launch_edge_types(self):
         self._chk(lib().mav1hip_launch_edge_types(self.h), "launch_edge_types")
 
@@ -282,6 +372,9 @@ class Context:
 
     def bind_frames(self, s, r, g):
         self._chk(lib().mav1hip_bind_frames(self.h, s, r, g), "bind_frames")
+
+    def pool_store(self, idx, buf):
+        self._chk(lib().mav1hip_pool_store(self.h, idx, buf), "pool_store")
 
     def timing(self, on=True):
         self._chk(lib().mav1hip_timing_enable(self.h, 1 if on else 0), "timing_enable")

@@ -243,3 +243,30 @@ def test_lr_select_equals_reference(name, lr_test):
     assert np.array_equal(ua, ub)
     if lr_test or name.startswith("q40"):
         assert (ua[:, 0] == 1).any() and (ua[:, 0] == 2).any(), "both filters occur"
+
+
+def test_filter_plan_turbo_gop():
+    """run_frame_core's decisions for the turbo mini-GOP (src/encoder.c:
+    618-640): the ARF is deblocked, CDEF'd with two sets and selected, and
+    restored only when restoration is on; leaf frames filter nothing; a key
+    frame with CDEF off still deblocks."""
+    for lr in (0, 1):
+        cfg = mav1hip.default_config(1920, 1080, turbo=1, qindex=125, restoration=lr)
+        for kind in ("key", "arf", "leaf"):
+            fr = mav1hip.turbo_frame(kind, 1)
+            fp, n = mav1hip.setup_params(cfg, fr)
+            plan = mav1hip.filter_plan(cfg, fr, fp, n)
+            if kind == "leaf":
+                assert (plan.filter_info, plan.deblock, plan.cdef_sets, plan.cdef_select, plan.restoration) == (0, 0, 0, 0, 0)
+            else:
+                assert plan.filter_info and plan.deblock and plan.cdef_sets == 2 and plan.cdef_select
+                assert plan.restoration == lr
+    cfg = mav1hip.default_config(352, 288, turbo=0, qindex=125, cdef=0)
+    fr = mav1hip.turbo_frame("key", 0)
+    fp, n = mav1hip.setup_params(cfg, fr)
+    plan = mav1hip.filter_plan(cfg, fr, fp, n)
+    assert n == 0 and plan.filter_info and plan.deblock and plan.cdef_sets == 0 and not plan.cdef_select
+    cfg = mav1hip.default_config(352, 288, turbo=0, qindex=0)
+    fp, n = mav1hip.setup_params(cfg, fr)
+    plan = mav1hip.filter_plan(cfg, fr, fp, n)
+    assert not plan.deblock and not plan.cdef_select and not plan.restoration
