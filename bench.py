@@ -87,6 +87,10 @@ def algorithmic_bytes(kernel, fp, nref):
         return F + 2 * F + mi  # turbo: two parameter sets written
     if kernel == "cdef_select":
         return 2 * F + F + F
+    if kernel == "lr_select":
+        return 3 * F + 48 * sum(fp.lr_unit_rows[p] * fp.lr_unit_cols[p] for p in range(3))
+    if kernel == "lr_apply":
+        return 3 * F
     return 0
 
 
@@ -103,7 +107,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    cfg = mav1hip.default_config(W, H, turbo=1, qindex=Q)
+    # deblocking + CDEF + loop restoration on (turbo turns restoration off unless asked for)
+    cfg = mav1hip.default_config(W, H, turbo=1, qindex=Q, restoration=1)
     geo = synth.Geometry(W, H, cfg.tile_cols_log2)
     order = coding_order(args.warmup + args.steps)
     nframes = max(max(f, l, a) for _, f, l, a in order) + 1
@@ -135,24 +140,29 @@ def main():
             if plan.deblock:
                 ctx.launch_deblock()
             ctx.launch_cdef(plan.cdef_sets)
+            if plan.restoration:
+                ctx.launch_save_deblocked()
             if plan.cdef_select:
                 ctx.launch_cdef_select(plan.cdef_sets)
+        if plan.restoration:
+            ctx.launch_lr_select(0)
+            ctx.launch_restore()
+        return mav1hip.BUF_LROUT if plan.restoration else mav1hip.BUF_REC
 
     def step(kind, f, last, alt):
         ref, gold = slot_of[last], (slot_of[alt] if alt >= 0 else -1)
         ctx.bind_frames(f % uniq, ref, gold)
         ctx.set_params(params[kind])
-        run_stages(kind)
-        if kind == "arf":  # the reconstruction becomes a reference: the slot LAST does not hold
+        final = run_stages(kind)
+        if kind == "arf":  # the final picture becomes a reference: the slot LAST does not hold
             target = uniq if ref != uniq else uniq + 1
-            ctx.pool_store(target, mav1hip.BUF_REC)
+            ctx.pool_store(target, final)
             slot_of[f] = target
 
     # frame 0: the key frame (untimed), its reconstruction in the first slot
     ctx.bind_frames(0, -1, -1)
     ctx.set_params(params["key"])
-    run_stages("key")
-    ctx.pool_store(uniq, mav1hip.BUF_REC)
+    ctx.pool_store(uniq, run_stages("key"))
     slot_of[0] = uniq
 
     for s in order[:args.warmup]:
@@ -168,7 +178,7 @@ def main():
     ctx.timing(False)
     kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3", "analyze_level0",
                "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table", "edge_types", "recon_inter",
-               "recon_intra", "filter_info", "deblock", "cdef", "cdef_select"]
+               "recon_intra", "filter_info", "deblock", "cdef", "cdef_select", "lr_select", "lr_apply"]
     stats = {k: ctx.timing_get(k) for k in kernels}
     fps = args.steps / elapsed
 
@@ -205,10 +215,10 @@ def main():
         "config": {
             "workload": "1920x1080 8-bit 4:2:0, --turbo -q 125, turbo GOP (ARF + leaf), plain HIP path",
             "stages": ["motion_search", "analyze", "edge_types", "recon", "filter_info", "deblock", "cdef",
-                       "cdef_select"],
+                       "cdef_select", "lr_select", "lr_apply"],
             "frame_kinds": "alternating hidden ALTREF (1 ref) / leaf (2 refs); leaf frames are not filtered",
             "references": "the reconstructed, filtered pictures of the referenced frames (closed loop from a coded key frame)",
-            "restoration": "off (the turbo default); the restoration search is not in the step yet",
+            "filters": "deblocking + CDEF + loop restoration on hidden ALTREF frames (leaf frames skip them)",
             "device": ctx.device_name,
         },
         "kernels_ms_per_launch": {k: round(v[1] / max(v[0], 1), 5) for k, v in stats.items()},
@@ -232,9 +242,10 @@ def main():
 
 def cpu_baseline(seconds, frames, params, uniq):
     """The reference's own kernels (oracle/_ref: motion search, analysis,
-    reconstruction; the edge filter types from the oracle, as the reference's
-    are in its driver), serial on one host core, over the same frame
-    sequence until `seconds` have passed."""
+    reconstruction, deblocking, CDEF, restoration search and apply; the edge
+    filter types, filter info and CDEF choice from the oracle, as the
+    reference's are in its frame driver), serial on one host core, over the
+    same frame sequence until `seconds` have passed (whole mini-GOPs)."""
     path = os.path.join(ROOT, "oracle", "_ref", "libmav1ref.so")
     kind, pre = "reference", "ref_"
     if not os.path.exists(path):
@@ -242,6 +253,10 @@ def cpu_baseline(seconds, frames, params, uniq):
     lib = C.CDLL(path)
     me, ana, rec_fn = getattr(lib, pre + "motion_search"), getattr(lib, pre + "analyze"), getattr(lib, pre + "recon")
     lf, cdef = getattr(lib, pre + "loop_filter"), getattr(lib, pre + "cdef")
+    lrsel, lrapp = getattr(lib, pre + "lr_select"), getattr(lib, pre + "lr_apply")
+    nunits = sum(params["arf"].lr_unit_rows[p] * params["arf"].lr_unit_cols[p] for p in range(3))
+    units = np.zeros(nunits * 12, np.int32)
+    dbk, tmp, lrout = np.zeros_like(frames[0]), np.zeros_like(frames[0]), np.zeros_like(frames[0])
     # the driver's edge filter types, filter info and CDEF choice (the reference's are in its frame driver)
     orc = C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
     fb = frames[0].size
@@ -274,8 +289,12 @@ def cpu_baseline(seconds, frames, params, uniq):
             orc.oracle_filter_info(C.byref(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs), mav1hip.ptr(info))
             lf(C.byref(fp), mav1hip.ptr(rec), mav1hip.ptr(info))
             cdef(C.byref(fp), mav1hip.ptr(rec), mav1hip.ptr(cbuf), C.c_long(fb), mav1hip.ptr(info), 2)
+            dbk[:] = rec
             orc.oracle_cdef_select(C.byref(fp), s, mav1hip.ptr(cbuf), C.c_int64(fb), 2, mav1hip.ptr(rec),
                                    mav1hip.ptr(choice))
+            f2 = mav1hip.FrameParams.from_buffer_copy(fp)
+            lrsel(C.byref(f2), s, mav1hip.ptr(rec), mav1hip.ptr(dbk), mav1hip.ptr(tmp), mav1hip.ptr(units), 0)
+            lrapp(C.byref(f2), mav1hip.ptr(rec), mav1hip.ptr(dbk), mav1hip.ptr(lrout), mav1hip.ptr(units))
         done += 1
         if time.perf_counter() - t0 >= seconds and done % 2 == 0:
             break
@@ -286,7 +305,9 @@ def cpu_baseline(seconds, frames, params, uniq):
         "cores": 1,
         "kind": kind,
         "sample": f"{done} frames of the same 1080p turbo GOP (ARF/leaf alternating), motion search + analysis + edge types + "
-                  f"reconstruction + (ARF) filter info, deblocking, CDEF, CDEF choice; references are source pictures, "
+                  f"reconstruction + (ARF) filter info, deblocking, CDEF, CDEF choice, restoration search and apply "
+                  f"(the search alone takes ~45 s per 1080p frame, so the sample is whole mini-GOPs past the time "
+                  f"bound); references are source pictures, "
                   f"{dt:.1f} s on one host core ({os.cpu_count()} visible)",
     }
 

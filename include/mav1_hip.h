@@ -119,7 +119,7 @@ typedef struct mav1hip_ctx mav1hip_ctx;
  * convention of backend_metal_create, src/backend.h:37: NULL and a message
  * in err when there is no usable device). */
 mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen);
-/* Backend.destroy (src/backend.h:22). */
+/* Backend.destroy (src/backend.h:20). */
 void mav1hip_destroy(mav1hip_ctx *c);
 const char *mav1hip_error(const mav1hip_ctx *c);
 /* Name of the device the context runs on. */
