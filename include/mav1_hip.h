@@ -204,6 +204,20 @@ int mav1hip_restore(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *cd
 int mav1hip_lr_select(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdef,
                       const uint8_t *dbk, int32_t lr_test, Mav1LrUnit *units, int32_t lr_type[3], int32_t reruns[3]);
 
+/* The per-frame decision record of the driver's dump_decisions
+ * (src/encoder.c:659-704, MAV1_DUMP_DEC): 11 int32 (magic 'MDEC',
+ * order_hint, last_hint, arf_hint, frame_type 0 key / 1 inter, frame_kind,
+ * nonref, base_q_idx, rows8, cols8, tile_bytes) then rows8 * cols8 cells of
+ * 8 bytes (bsize, mode 0 intra / 1 single / 2 compound, ref_frame, 0,
+ * int16 mv_row, int16 mv_col) of the last block covering each 8x8 cell.
+ * hdr7 = {order_hint, last_hint, arf_hint, frame_type, frame_kind, nonref,
+ * tile_bytes} (driver state); the cells come from the coded blocks. out
+ * holds mav1hip_decision_record_bytes(fp) bytes; the caller appends it to
+ * its dump file. */
+size_t mav1hip_decision_record_bytes(const Mav1FrameParams *fp);
+int mav1hip_decision_record(mav1hip_ctx *c, const Mav1FrameParams *fp, const Mav1BlockInfo *blocks,
+                            const Mav1SbInfo *sbs, const int32_t hdr7[7], uint8_t *out);
+
 /* ------------------------------------------------------------------------
  * Device-resident form
  * ------------------------------------------------------------------------ */
@@ -227,7 +241,8 @@ enum {
   MAV1HIP_BUF_DBK = 14,    /* deblocked frame kept for loop restoration */
   MAV1HIP_BUF_LROUT = 15,  /* loop-restoration output */
   MAV1HIP_BUF_LRUNITS = 16, /* Mav1LrUnit per restoration unit */
-  MAV1HIP_BUF_CDEF_CHOICE = 17 /* CDEF set per superblock, int8 */
+  MAV1HIP_BUF_CDEF_CHOICE = 17, /* CDEF set per superblock, int8 */
+  MAV1HIP_BUF_DECGRID = 18 /* decision cells, 8 bytes per 8x8 */
 };
 #define MAV1HIP_MAX_CDEF_SETS 8
 /* Size in bytes of a device buffer. */
@@ -255,6 +270,8 @@ int mav1hip_launch_cdef_select(mav1hip_ctx *c, int32_t nsets);
  * types that mav1hip_launch_restore applies. */
 int mav1hip_launch_lr_select(mav1hip_ctx *c, int32_t lr_test);
 int mav1hip_launch_restore(mav1hip_ctx *c);
+/* BLOCKS -> DECGRID (the cells of the decision record). */
+int mav1hip_launch_decision_grid(mav1hip_ctx *c);
 /* Waits for everything enqueued on the context's stream. */
 int mav1hip_sync(mav1hip_ctx *c);
 

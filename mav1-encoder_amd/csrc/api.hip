@@ -50,6 +50,8 @@ struct mav1hip_ctx {
   int8_t *d_choice;
   void *d_lrstats;
   int *d_lrref, *d_lrtype, *d_lrreruns;
+  uint8_t *d_decgrid;
+  size_t decgrid_bytes;
   int lr_units;
   long entry_bytes;
   size_t planes_bytes;
@@ -191,6 +193,8 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   rc |= dmalloc(c, (void **)&c->d_lrout, c->frame_bytes);
   rc |= dmalloc(c, (void **)&c->d_dirvar, sizeof(int) * (size_t)(c->mi_rows >> 1) * (c->mi_cols >> 1));
   rc |= dmalloc(c, (void **)&c->d_choice, (size_t)c->num_sbs);
+  c->decgrid_bytes = (size_t)((c->mi_rows + 1) >> 1) * ((c->mi_cols + 1) >> 1) * 8;
+  rc |= dmalloc(c, (void **)&c->d_decgrid, c->decgrid_bytes);
   {
     /* restoration units of 64x64 per plane (count_units_in_frame) */
     c->lr_units = 0;
@@ -275,6 +279,7 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipFree(c->d_lrout);
   (void)hipFree(c->d_dirvar);
   (void)hipFree(c->d_choice);
+  (void)hipFree(c->d_decgrid);
   (void)hipFree(c->d_lrunits);
   (void)hipFree(c->d_lrstats);
   (void)hipFree(c->d_lrref);
@@ -311,6 +316,7 @@ static void *buf_ptr(mav1hip_ctx *c, int buf, size_t *bytes) {
     case MAV1HIP_BUF_LROUT: *bytes = c->frame_bytes; return c->d_lrout;
     case MAV1HIP_BUF_LRUNITS: *bytes = sizeof(Mav1LrUnit) * (size_t)c->lr_units; return c->d_lrunits;
     case MAV1HIP_BUF_CDEF_CHOICE: *bytes = (size_t)c->num_sbs; return c->d_choice;
+    case MAV1HIP_BUF_DECGRID: *bytes = c->decgrid_bytes; return c->d_decgrid;
     default: *bytes = 0; return nullptr;
   }
 }
@@ -613,6 +619,34 @@ int mav1hip_restore(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *cd
   HIPCHK(c, hipMemcpyAsync(c->d_lrout, out, c->frame_bytes, hipMemcpyHostToDevice, c->stream));
   if ((rc = mav1hip_launch_restore(c))) return rc;
   HIPCHK(c, hipMemcpyAsync(out, c->d_lrout, c->frame_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_launch_decision_grid(mav1hip_ctx *c) {
+  if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "decision grid: no frame parameters set");
+  return timed(c, "decision_grid",
+               [&] { return launch_decision_grid(c->stream, c->fp, c->d_blocks, c->d_sbs, c->d_decgrid); });
+}
+
+size_t mav1hip_decision_record_bytes(const Mav1FrameParams *fp) {
+  if (!fp) return 0;
+  return 44 + (size_t)((fp->mi_rows + 1) >> 1) * ((fp->mi_cols + 1) >> 1) * 8;
+}
+
+int mav1hip_decision_record(mav1hip_ctx *c, const Mav1FrameParams *fp, const Mav1BlockInfo *blocks,
+                            const Mav1SbInfo *sbs, const int32_t hdr7[7], uint8_t *out) {
+  if (!c || !fp || !blocks || !sbs || !hdr7 || !out) return fail(c, MAV1HIP_EINVAL, "decision record: missing buffer");
+  int rc;
+  if ((rc = mav1hip_set_params(c, fp))) return rc;
+  size_t nb = sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)c->num_sbs;
+  HIPCHK(c, hipMemcpyAsync(c->d_blocks, blocks, nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_sbs, sbs, sizeof(Mav1SbInfo) * (size_t)c->num_sbs, hipMemcpyHostToDevice, c->stream));
+  if ((rc = mav1hip_launch_decision_grid(c))) return rc;
+  const int32_t hdr[11] = {0x4345444D, hdr7[0], hdr7[1], hdr7[2], hdr7[3], hdr7[4], hdr7[5], fp->base_q_idx,
+                           (c->mi_rows + 1) >> 1, (c->mi_cols + 1) >> 1, hdr7[6]};
+  memcpy(out, hdr, sizeof hdr);
+  HIPCHK(c, hipMemcpyAsync(out + sizeof hdr, c->d_decgrid, c->decgrid_bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return MAV1HIP_OK;
 }

@@ -316,14 +316,50 @@ DEVI int dq_denom(int tx) {
   return 1;
 }
 
-/* Row strides of the LDS blocks, padded by one word so that lanes walking
- * down a column hit distinct banks. */
-#define RS 65 /* residual / inverse-transform output, h x w */
-#define QS 33 /* dequantised coefficients, Min(32,h) x Min(32,w) */
+/* The threads that code one transform block: a whole workgroup (wg) or a
+ * single wave. A wave's lanes run in lockstep, so ordering its LDS accesses
+ * needs only a compiler fence; a workgroup needs a barrier. red: LDS space
+ * for one partial per wave of a workgroup reduction. */
+struct TxCtx {
+  int tid, nt;
+  bool wg;
+  long long *red;
+};
+
+DEVI void tx_sync(const TxCtx &c) {
+  if (c.wg) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+DEVI long long tx_sum64(const TxCtx &c, long long v) {
+  v = wave_sum64(v);
+  if (!c.wg) return v;
+  if ((c.tid & 63) == 0) c.red[c.tid >> 6] = v;
+  __syncthreads();
+  long long s = 0;
+  for (int i = 0; i < (c.nt >> 6); i++) s += c.red[i];
+  __syncthreads();
+  return s;
+}
+DEVI int tx_max(const TxCtx &c, int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = d_max(v, __shfl_xor(v, o, 64));
+  if (!c.wg) return v;
+  if ((c.tid & 63) == 0) c.red[c.tid >> 6] = v;
+  __syncthreads();
+  int m = 0;
+  for (int i = 0; i < (c.nt >> 6); i++) m = d_max(m, (int)c.red[i]);
+  __syncthreads();
+  return m;
+}
 
 /* 2-D inverse transform (7.13.3): dq holds Min(32,h) x Min(32,w) values with
- * row stride QS; res receives h x w residuals with row stride RS. One lane
- * per row, then one lane per column; each (kind, length) pass is its own
+ * row stride qs; res receives h x w residuals with row stride rs. One
+ * thread per row, then one per column; each (kind, length) pass is its own
  * non-inlined function so the unrolled butterflies exist once in the code
  * object. */
 template <int KIND, int N>
@@ -338,38 +374,38 @@ DEVI void t_1d(int32_t *T, int r) {
 }
 
 template <int KIND, int N>
-__device__ __attribute__((noinline)) void itx_rows(const int32_t *dq, int32_t *res, int h, int rect, int rowShift,
-                                                    int lane) {
+__device__ __attribute__((noinline)) void itx_rows(const int32_t *dq, int qs, int32_t *res, int rs, int h, int rect,
+                                                    int rowShift, int tid, int nt) {
   constexpr int W = 1 << N;
-  for (int i = lane; i < h; i += 64) {
+  for (int i = tid; i < h; i += nt) {
     int32_t T[W];
     int nz = 0;
 #pragma unroll
     for (int j = 0; j < W; j++) {
-      int32_t v = (i < 32 && j < 32) ? dq[i * QS + j] : 0;
+      int32_t v = (i < 32 && j < 32) ? dq[i * qs + j] : 0;
       nz |= v;
       T[j] = rect ? (int32_t)(((long long)v * 2896 + 2048) >> 12) : v;
     }
     if (nz) { /* every 1-D kernel maps an all-zero row to zeros */
       t_1d<KIND, N>(T, 16);
 #pragma unroll
-      for (int j = 0; j < W; j++) res[i * RS + j] = d_clip(-(1 << 15), (1 << 15) - 1, d_round2(T[j], rowShift));
+      for (int j = 0; j < W; j++) res[i * rs + j] = d_clip(-(1 << 15), (1 << 15) - 1, d_round2(T[j], rowShift));
     } else {
 #pragma unroll
-      for (int j = 0; j < W; j++) res[i * RS + j] = 0;
+      for (int j = 0; j < W; j++) res[i * rs + j] = 0;
     }
   }
 }
 template <int KIND, int N>
-__device__ __attribute__((noinline)) void itx_cols(int32_t *res, int w, int lane) {
+__device__ __attribute__((noinline)) void itx_cols(int32_t *res, int rs, int w, int tid, int nt) {
   constexpr int H = 1 << N;
-  for (int j = lane; j < w; j += 64) {
+  for (int j = tid; j < w; j += nt) {
     int32_t T[H];
 #pragma unroll
-    for (int i = 0; i < H; i++) T[i] = res[i * RS + j];
+    for (int i = 0; i < H; i++) T[i] = res[i * rs + j];
     t_1d<KIND, N>(T, 16);
 #pragma unroll
-    for (int i = 0; i < H; i++) res[i * RS + j] = d_round2(T[i], 4);
+    for (int i = 0; i < H; i++) res[i * rs + j] = d_round2(T[i], 4);
   }
 }
 
@@ -399,76 +435,105 @@ __device__ __attribute__((noinline)) void itx_cols(int32_t *res, int w, int lane
     }                                                                       \
   } while (0)
 
-__device__ __attribute__((noinline)) void wave_itx2d(const int32_t *dq, int32_t *res, int txSz, int txType, int lane) {
+__device__ __attribute__((noinline)) void tx_inverse(const TxCtx &c, const int32_t *dq, int qs, int32_t *res, int rs,
+                                                      int txSz, int txType) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
   const int rect = (lw - lh == 1) || (lh - lw == 1);
   const int rk = row_kind(txType), ck = col_kind(txType);
-  ITX_DISPATCH(itx_rows, rk, lw, dq, res, 1 << lh, rect, (int)Transform_Row_Shift[txSz], lane);
-  __syncthreads();
-  ITX_DISPATCH(itx_cols, ck, lh, res, 1 << lw, lane);
-  __syncthreads();
+  ITX_DISPATCH(itx_rows, rk, lw, dq, qs, res, rs, 1 << lh, rect, (int)Transform_Row_Shift[txSz], c.tid, c.nt);
+  tx_sync(c);
+  ITX_DISPATCH(itx_cols, ck, lh, res, rs, 1 << lw, c.tid, c.nt);
+  tx_sync(c);
 }
 
 /* Forward 2-D transform through the Q20 matrices (k_fwd.h:35-59): resid
- * h x w (stride RS) -> coef Min(32,h) x Min(32,w) (stride 32); tmp holds
- * h x Min(32,w) (stride 32). One lane per output in both passes. */
-__device__ __attribute__((noinline)) void wave_fwd2d(const int32_t *resid, int32_t *tmp, int32_t *coef, int txSz, int txType,
-                     const int32_t *__restrict__ mats, int lane) {
-  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
-  const int tw = d_min(32, w), th = d_min(32, h);
-  const int32_t *Mh = mats + (row_kind(txType) * 5 + lw - 2) * 4096;
-  const int32_t *Mv = mats + (col_kind(txType) * 5 + lh - 2) * 4096;
-  const int ltw = d_min(5, lw);
-  for (int k = lane; k < h * tw; k += 64) {
+ * h x w (row stride rs) -> coef Min(32,h) x Min(32,w) (row stride tw); tmp
+ * holds h x Min(32,w) (row stride tw). One thread per output in both
+ * passes; the sums of a fixed length are unrolled by up to 16 so the matrix
+ * loads issue back to back. */
+template <int W>
+DEVI void fwd_rows(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int h, int ltw,
+                   const int32_t *__restrict__ Mh) {
+  const int tw = 1 << ltw;
+  for (int k = c.tid; k < h * tw; k += c.nt) {
     const int i = k >> ltw, kk = k & (tw - 1);
-    const int32_t *m = Mh + kk * w, *rr = resid + i * RS;
+    const int32_t *rr = resid + i * rs, *m = Mh + kk * W;
     long long s = 0;
-    for (int j = 0; j < w; j++) s += (long long)rr[j] * m[j];
-    tmp[i * 32 + kk] = (int32_t)((s + 32768) >> 16);
+#pragma unroll 16
+    for (int j = 0; j < W; j++) s += (long long)rr[j] * m[j];
+    tmp[k] = (int32_t)((s + 32768) >> 16);
   }
-  __syncthreads();
-  const int sh = Transform_Row_Shift[txSz] + 4;
-  const bool rect = (lw - lh == 1) || (lh - lw == 1);
-  for (int k = lane; k < th * tw; k += 64) {
-    int k2 = k / tw, kk = k - k2 * tw;
-    const int32_t *m = Mv + k2 * h;
+}
+template <int H>
+DEVI void fwd_cols(const TxCtx &c, const int32_t *tmp, int32_t *coef, int th, int ltw, const int32_t *__restrict__ Mv,
+                   int sh, bool rect) {
+  const int tw = 1 << ltw;
+  for (int k = c.tid; k < th * tw; k += c.nt) {
+    const int k2 = k >> ltw, kk = k & (tw - 1);
+    const int32_t *m = Mv + k2 * H;
     long long s = 0;
-    for (int i = 0; i < h; i++) s += (long long)m[i] * tmp[i * 32 + kk];
+#pragma unroll 16
+    for (int i = 0; i < H; i++) s += (long long)m[i] * tmp[i * tw + kk];
     s *= (1ll << sh);
     if (rect) s = ((s + 2048) >> 12) * 5793;
-    coef[k2 * 32 + kk] = (int32_t)(s >= 0 ? (s + (1ll << 19)) >> 20 : -((-s + (1ll << 19)) >> 20));
+    coef[k] = (int32_t)(s >= 0 ? (s + (1ll << 19)) >> 20 : -((-s + (1ll << 19)) >> 20));
   }
-  __syncthreads();
+}
+
+__device__ __attribute__((noinline)) void tx_forward(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp,
+                                                      int32_t *coef, int txSz, int txType,
+                                                      const int32_t *__restrict__ mats) {
+  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], h = 1 << lh;
+  const int ltw = d_min(5, lw), th = d_min(32, h);
+  const int32_t *Mh = mats + (row_kind(txType) * 5 + lw - 2) * 4096;
+  const int32_t *Mv = mats + (col_kind(txType) * 5 + lh - 2) * 4096;
+  switch (lw) {
+    case 2: fwd_rows<4>(c, resid, rs, tmp, h, ltw, Mh); break;
+    case 3: fwd_rows<8>(c, resid, rs, tmp, h, ltw, Mh); break;
+    case 4: fwd_rows<16>(c, resid, rs, tmp, h, ltw, Mh); break;
+    case 5: fwd_rows<32>(c, resid, rs, tmp, h, ltw, Mh); break;
+    default: fwd_rows<64>(c, resid, rs, tmp, h, ltw, Mh); break;
+  }
+  tx_sync(c);
+  const int sh = Transform_Row_Shift[txSz] + 4;
+  const bool rect = (lw - lh == 1) || (lh - lw == 1);
+  switch (lh) {
+    case 2: fwd_cols<4>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
+    case 3: fwd_cols<8>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
+    case 4: fwd_cols<16>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
+    case 5: fwd_cols<32>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
+    default: fwd_cols<64>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
+  }
+  tx_sync(c);
 }
 
 /* Dead-zone quantiser (k_fwd.h:66-90) into the decoder layout (stride tw). */
-DEVI void wave_quantize(const int32_t *coef, int32_t *q, int txSz, int dcQ, int acQ, int lane) {
+DEVI void tx_quantize(const TxCtx &c, const int32_t *coef, int32_t *q, int txSz, int dcQ, int acQ) {
   const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
   const unsigned long long invDc = (1ull << 44) / (unsigned)dcQ + 1, invAc = (1ull << 44) / (unsigned)acQ + 1;
-  for (int k = lane; k < th * tw; k += 64) {
-    int i = k / tw, j = k - i * tw;
+  for (int k = c.tid; k < th * tw; k += c.nt) {
     const bool dc = k == 0;
     const int qv = dc ? dcQ : acQ, rnd = dc ? 26 : 22;
-    const int v = coef[i * 32 + j];
+    const int v = coef[k];
     long long a = (long long)d_abs(v) * denom;
     long long t = (a + (((long long)qv * rnd) >> 2)) >> 4;
     int level = t < (1ll << 19) ? (int)(((unsigned long long)t * (dc ? invDc : invAc)) >> 44) : (int)(t / qv);
     if (level > 0xFFFF) level = 0xFFFF;
     q[k] = v < 0 ? -level : level;
   }
-  __syncthreads();
+  tx_sync(c);
 }
 
-/* Dequantisation (spec 7.12.3), output stride QS. */
-DEVI void wave_dequant(const int32_t *q, int32_t *dq, int txSz, int dcQ, int acQ, int lane) {
+/* Dequantisation (spec 7.12.3), output row stride qs. */
+DEVI void tx_dequant(const TxCtx &c, const int32_t *q, int32_t *dq, int qs, int txSz, int dcQ, int acQ) {
   const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
-  for (int k = lane; k < th * tw; k += 64) {
+  for (int k = c.tid; k < th * tw; k += c.nt) {
     int i = k / tw, j = k - i * tw;
     long long d = (long long)q[k] * (k == 0 ? dcQ : acQ), ad = d < 0 ? -d : d;
     long long d2 = (ad & 0xFFFFFF) >> (denom >> 1);
-    dq[i * QS + j] = d_clip(-(1 << 15), (1 << 15) - 1, (int)(d < 0 ? -d2 : d2));
+    dq[i * qs + j] = d_clip(-(1 << 15), (1 << 15) - 1, (int)(d < 0 ? -d2 : d2));
   }
-  __syncthreads();
+  tx_sync(c);
 }
 
 #endif

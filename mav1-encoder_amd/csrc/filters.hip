@@ -508,3 +508,48 @@ hipError_t launch_lr_apply(hipStream_t st, const Mav1FrameParams &fp, const uint
                      fp, cdef, dbk, out, units, lrType);
   return hipGetLastError();
 }
+
+/* ---- decision record (src/encoder.c:659-704) ---- */
+
+/* Per superblock, lane per block: the 8-byte cells of the 8x8 grid the
+ * block covers, where no later block of the superblock (in decode order)
+ * covers them too, so each cell holds the last block as the serial dump
+ * leaves it. Cells outside every block are cleared first. */
+__global__ __launch_bounds__(64) void k_decision_grid(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
+                                                      const Mav1SbInfo *__restrict__ sbs, uint8_t *__restrict__ grid) {
+  const int sb = blockIdx.x, lane = threadIdx.x;
+  const int rows8 = (fp.mi_rows + 1) >> 1, cols8 = (fp.mi_cols + 1) >> 1;
+  const int sr = sb / fp.sb_cols, sc = sb % fp.sb_cols;
+  /* the superblock's 8x8 cells */
+  for (int q = lane; q < 64; q += 64) {
+    const int r = sr * 8 + (q >> 3), c = sc * 8 + (q & 7);
+    if (r < rows8 && c < cols8) *(uint2 *)(grid + ((size_t)r * cols8 + c) * 8) = make_uint2(0, 0);
+  }
+  __syncthreads();
+  const int n = sbs[sb].num_blocks;
+  if (lane >= n) return;
+  const Mav1BlockInfo *base = blocks + sbs[sb].first_block;
+  const Mav1BlockInfo &b = base[lane];
+  const int w8 = (Num_4x4_Blocks_Wide[b.bsize] + 1) >> 1, h8 = (Num_4x4_Blocks_High[b.bsize] + 1) >> 1;
+  const int r0 = b.mi_row >> 1, c0 = b.mi_col >> 1;
+  const uint32_t lo = (uint32_t)(uint8_t)b.bsize | (uint32_t)(!b.is_inter ? 0 : b.ref_frame1 ? 2 : 1) << 8 |
+                      (uint32_t)(uint8_t)b.ref_frame << 16;
+  const uint32_t hi = (uint32_t)(uint16_t)(int16_t)b.mv_row | (uint32_t)(uint16_t)(int16_t)b.mv_col << 16;
+  for (int r = r0; r < d_min(rows8, r0 + h8); r++)
+    for (int c = c0; c < d_min(cols8, c0 + w8); c++) {
+      bool later = false;
+      for (int j = lane + 1; j < n && !later; j++) {
+        const Mav1BlockInfo &o = base[j];
+        const int or0 = o.mi_row >> 1, oc0 = o.mi_col >> 1;
+        later = r >= or0 && r < or0 + ((Num_4x4_Blocks_High[o.bsize] + 1) >> 1) && c >= oc0 &&
+                c < oc0 + ((Num_4x4_Blocks_Wide[o.bsize] + 1) >> 1);
+      }
+      if (!later) *(uint2 *)(grid + ((size_t)r * cols8 + c) * 8) = make_uint2(lo, hi);
+    }
+}
+
+hipError_t launch_decision_grid(hipStream_t st, const Mav1FrameParams &fp, const Mav1BlockInfo *blocks,
+                                const Mav1SbInfo *sbs, uint8_t *grid) {
+  hipLaunchKernelGGL(k_decision_grid, dim3(fp.sb_cols * fp.sb_rows), dim3(64), 0, st, fp, blocks, sbs, grid);
+  return hipGetLastError();
+}

@@ -51,6 +51,9 @@ hipError_t launch_cdef_select(hipStream_t st, const Mav1FrameParams &fp, const u
 hipError_t launch_lr_apply(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *cdef, const uint8_t *dbk,
                            uint8_t *out, const Mav1LrUnit *units, const int *lrType);
 
+hipError_t launch_decision_grid(hipStream_t st, const Mav1FrameParams &fp, const Mav1BlockInfo *blocks,
+                                const Mav1SbInfo *sbs, uint8_t *grid);
+
 /* lrsearch.hip */
 size_t lr_stat_bytes();
 hipError_t launch_lr_select(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,

@@ -3,52 +3,109 @@
  * the dependencies of src/backend_cpu.c:81-86) and the intra edge filter
  * types of the frame driver (src/encoder.c:319-346).
  *
- * Inter blocks never read reconstructed neighbours: phase A codes every
- * inter block of the frame at once (one wave per block). Intra blocks read
- * the reconstruction of earlier blocks: phase B runs one launch per
- * anti-diagonal t = column-in-tile + 2 * row of the superblock grid (left,
- * above and above-right superblocks are on earlier diagonals, tile columns
- * are independent), one wave per superblock walking its blocks in decode
- * order with the BlockDecoded flags in LDS. Within a transform block the
- * lanes share every step: residual, forward transform (matrix form, int64
- * sums), quantiser, eob / rate / SSE reductions, inverse transform (one
- * lane per row, then per column, in registers), reconstruction. */
+ * Inter blocks never read reconstructed neighbours: the inter phase codes
+ * every inter block of the frame at once, one 512-thread workgroup per
+ * block. Intra blocks read the reconstruction of earlier blocks: the intra
+ * phase runs one launch per anti-diagonal t = column-in-tile + 2 * row of
+ * the superblock grid (left, above and above-right superblocks are on
+ * earlier diagonals, tile columns are independent), one workgroup per
+ * superblock walking its blocks in decode order with the BlockDecoded flags
+ * in LDS.
+ *
+ * Inside a workgroup the candidates of every search run side by side: the
+ * transform types of a block of at most 16x16 one per wave (each wave codes
+ * its candidate alone: residual, forward transform, quantiser, eob and
+ * rate, inverse transform and distortion), the nine dual-filter pairs over
+ * three shared horizontal passes, the 33 CfL alphas over the waves. Larger
+ * transforms (32- and 64-point) use the whole workgroup per candidate. */
 #include "dev_pred.h"
 #include "dev_tx.h"
 #include "internal.h"
 
+#define RT 512       /* threads of a reconstruction workgroup */
+#define RW (RT / 64) /* its waves */
 #define BDD 18
 #define BDI(p, y, x) ((p) * BDD * BDD + ((y) + 1) * BDD + ((x) + 1))
 
-struct RLds {
-  union {
-    struct {
-      uint8_t win[71 * 72];
-      int16_t mid[71 * 64];
-      int32_t p0[64 * 64];
-    } pr;
-    struct {
-      int32_t resid[64 * RS]; /* residual, then the inverse transform's output */
-      int32_t tmp[64 * 32];
-    } fw;
-  } u;
+#define SRS 17 /* row strides of a small (<= 16x16) block: residual, dequantised */
+#define SQS 17
+#define BRS 65 /* and of a large one */
+#define BQS 33
+
+/* One wave's working set for a transform block of at most 16x16. */
+struct SmallTx {
+  int32_t resid[16 * SRS]; /* residual, then the inverse transform's output */
+  int32_t tmp[256];
+  int32_t coef[256];
+  int32_t q[256];
+  int32_t dq[16 * SQS];
+};
+/* The workgroup's working set for a larger transform block. */
+struct BigTx {
+  int32_t resid[64 * BRS];
+  int32_t tmp[64 * 32];
   int32_t coef[32 * 32];
   int32_t qtry[32 * 32];
   int32_t qbest[32 * 32];
-  int32_t dq[32 * QS];
-  int32_t cfl[32 * 32];
+  int32_t dq[32 * BQS];
+};
+/* Inter prediction: the reference window and the horizontal pass of up to
+ * three filters; compound: the first prediction and a band of the second. */
+struct PredLds {
+  uint8_t win[71 * 72];
+  int16_t mid[3][71 * 64];
+};
+struct CompLds {
+  uint8_t win[71 * 72];
+  int16_t mid[71 * 64];
+  int32_t p0[64 * 64];
+  int32_t p1[16 * 64];
+};
+
+struct RLds {
+  union {
+    SmallTx sm[RW];
+    BigTx big;
+    PredLds pr;
+    CompLds cp;
+    int32_t cfl[32 * 32];
+  } u;
+  int32_t qbest[256];
+  long long candCost[16];
+  int candEob[16];
+  long long aSse[33];
+  long long fcost[9];
+  long long red[RW];
   int edgeA[EDGE_NEG + 2 * 64 + 32], edgeL[EDGE_NEG + 2 * 64 + 32], ecopy[2 * 64 + 40];
   uint8_t bd[3 * BDD * BDD];
   uint8_t bdSave[BDD * BDD];
   uint8_t save[64 * 64];
+  int ival[4];
   Mav1BlockInfo blk;
 };
 
-/* The working set of the wave that reconstructs a block / superblock
- * (one per workgroup). File scope so that the non-inlined stage functions
- * address it as LDS. */
+/* The working set of the workgroup that reconstructs a block / superblock.
+ * File scope so that the non-inlined stage functions address it as LDS. */
 __shared__ RLds gL;
 #define NOINL __device__ __attribute__((noinline))
+
+DEVI TxCtx wg_ctx() {
+  TxCtx c;
+  c.tid = threadIdx.x;
+  c.nt = RT;
+  c.wg = true;
+  c.red = gL.red;
+  return c;
+}
+DEVI TxCtx wave_ctx() {
+  TxCtx c;
+  c.tid = threadIdx.x & 63;
+  c.nt = 64;
+  c.wg = false;
+  c.red = nullptr;
+  return c;
+}
+DEVI long long wg_sum64(long long v) { return tx_sum64(wg_ctx(), v); }
 
 /* ---- scan / tx-set helpers (spec 5.11.47 / 5.11.48) ---- */
 DEVI const int16_t *scan_of(int txSz, int kind) {
@@ -97,36 +154,92 @@ DEVI int uv_tx_size(int bs) {
 }
 DEVI int tile_start(const Mav1FrameParams &fp, int miCol) { return ((miCol >> 4) / fp.tile_w_sb) * fp.tile_w_sb * 16; }
 
-DEVI int wave_max(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = d_max(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
 /* find_eob (k_recon.h:50-54): 1 + last scan position with a non-zero level. */
-DEVI int wave_eob(const int32_t *q, const int16_t *scan, int n, int lane) {
+DEVI int tx_eob(const TxCtx &c, const int32_t *q, const int16_t *scan, int n) {
   int m = 0;
-  for (int c = lane; c < n; c += 64)
-    if (q[scan[c]]) m = c + 1;
-  return wave_max(m);
+  for (int k = c.tid; k < n; k += c.nt)
+    if (q[scan[k]]) m = k + 1;
+  return tx_max(c, m);
 }
 /* est_coef_bits (k_recon.h:39-48) */
-DEVI int wave_coef_bits(const int32_t *q, const int16_t *scan, int eob, int lane) {
+DEVI int tx_bits(const TxCtx &c, const int32_t *q, const int16_t *scan, int eob) {
   if (!eob) return 8;
-  int b = 0;
-  for (int c = lane; c < eob; c += 64) {
-    int l = d_abs(q[scan[c]]);
+  long long b = 0;
+  for (int k = c.tid; k < eob; k += c.nt) {
+    const int l = d_abs(q[scan[k]]);
     b += l ? 8 * (2 + 2 * d_floorlog2((unsigned)l)) : 6;
   }
-  return 48 + 16 * d_floorlog2((unsigned)eob + 1) + wave_sum(b);
+  return 48 + 16 * d_floorlog2((unsigned)eob + 1) + (int)tx_sum64(c, b);
+}
+
+/* Residual of transform block (x, y), stored in the type's flipped order. */
+DEVI void tx_residual(const TxCtx &c, const uint8_t *spx, const uint8_t *px, int S, int x, int y, int lw, int lh, int t,
+                      int32_t *resid, int rs) {
+  const int w = 1 << lw, h = 1 << lh, fud = flip_ud(t), flr = flip_lr(t);
+  for (int k = c.tid; k < w * h; k += c.nt) {
+    const int i = k >> lw, j = k & (w - 1);
+    const int o = (y + (fud ? h - 1 - i : i)) * S + x + (flr ? w - 1 - j : j);
+    resid[i * rs + j] = (int)spx[o] - (int)px[o];
+  }
+  tx_sync(c);
+}
+/* px += residual (inverse transform output in flipped order), clipped. */
+DEVI void tx_add(const TxCtx &c, uint8_t *px, int S, int x, int y, int lw, int lh, int t, const int32_t *res, int rs) {
+  const int w = 1 << lw, h = 1 << lh, fud = flip_ud(t), flr = flip_lr(t);
+  for (int k = c.tid; k < w * h; k += c.nt) {
+    const int i = k >> lw, j = k & (w - 1);
+    const int o = (y + (fud ? h - 1 - i : i)) * S + x + (flr ? w - 1 - j : j);
+    px[o] = (uint8_t)d_clip1((int)px[o] + res[i * rs + j]);
+  }
+  tx_sync(c);
+}
+
+/* One candidate type of one transform block (k_recon.h:60-173 loop body):
+ * levels into q, returns the eob; the RD cost 16 * SSE + lambda * bits / 8
+ * in *costOut (0 when `direct`: a single type and no depth search). */
+NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__restrict__ spx, const uint8_t *px, int S,
+                  int x, int y, int txSz, int plane, int t, bool direct, int32_t *resid, int rs, int32_t *tmp,
+                  int32_t *coef, int32_t *q, int32_t *dq, int qs, const int32_t *__restrict__ mats,
+                  long long *costOut) {
+  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
+  const int dcQ = fp.dc_q[plane], acQ = fp.ac_q[plane];
+  tx_residual(c, spx, px, S, x, y, lw, lh, t, resid, rs);
+  tx_forward(c, resid, rs, tmp, coef, txSz, t, mats);
+  tx_quantize(c, coef, q, txSz, dcQ, acQ);
+  const int16_t *scan = get_scan(txSz, t);
+  const int eob = tx_eob(c, q, scan, seg_eob(txSz));
+  if (direct) {
+    *costOut = 0;
+    return eob;
+  }
+  long long e = 0;
+  if (eob > 0) {
+    tx_dequant(c, q, dq, qs, txSz, dcQ, acQ);
+    tx_inverse(c, dq, qs, resid, rs, txSz, t);
+    const int fud = flip_ud(t), flr = flip_lr(t);
+    for (int k = c.tid; k < w * h; k += c.nt) {
+      const int i = k >> lw, j = k & (w - 1);
+      const int o = (y + (fud ? h - 1 - i : i)) * S + x + (flr ? w - 1 - j : j);
+      const int d = (int)spx[o] - d_clip1((int)px[o] + resid[i * rs + j]);
+      e += d * d;
+    }
+  } else {
+    for (int k = c.tid; k < w * h; k += c.nt) {
+      const int v = resid[(k >> lw) * rs + (k & (w - 1))];
+      e += v * v;
+    }
+  }
+  const long long sse = tx_sum64(c, e);
+  *costOut = sse * 16 + (long long)fp.lambda * tx_bits(c, q, scan, eob) / 8;
+  return eob;
 }
 
 /* Lossless 4x4: forward WHT (k_fwd.h:112-136) and inverse WHT (7.13.2.10),
- * one lane. */
+ * one thread. */
 DEVI void wht_fwd(const int32_t *r, int32_t *out) {
   int t[16];
   for (int j = 0; j < 4; j++) {
-    int a = r[j], b = r[RS + j], c = r[2 * RS + j], d = r[3 * RS + j];
+    int a = r[j], b = r[4 + j], c = r[8 + j], d = r[12 + j];
     a += b;
     d -= c;
     int e = (a - d) >> 1;
@@ -169,22 +282,25 @@ DEVI void wht_inv1(int32_t *T, int shift) {
   T[3] = d;
 }
 
-/* One transform block whose prediction is in px (k_recon.h:60-173).
- * Returns the eob; the chosen type and RD cost in *outType / *outCost. */
+/* One transform block whose prediction is in px (k_recon.h:60-173): the
+ * cheapest of the candidate types (first minimum), its levels into coefOut,
+ * its reconstruction into px. Returns the eob; the type and cost in
+ * *outType / *outCost. */
 NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ spx, uint8_t *px, int S, int x, int y,
-                       int txSz, int plane, const int *cands, int nc, int32_t *coefOut,
-                       const int32_t *__restrict__ mats, int lane, int *outType, long long *outCost) {
+                        int txSz, int plane, const int *cands, int nc, int32_t *coefOut,
+                        const int32_t *__restrict__ mats, int *outType, long long *outCost) {
   RLds &L = gL;
-  const int w = Tx_Width[txSz], h = Tx_Height[txSz], tw = d_min(32, w), th = d_min(32, h);
+  const int tid = threadIdx.x;
+  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
+  const int tw = d_min(32, w), th = d_min(32, h);
   const int dcQ = fp.dc_q[plane], acQ = fp.ac_q[plane];
-  int32_t *resid = L.u.fw.resid;
   if (fp.lossless) {
-    int eob = 0;
-    if (lane == 0) {
+    if (tid == 0) {
+      int32_t r16[16], q16[16];
       for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) resid[i * RS + j] = (int)spx[(y + i) * S + x + j] - (int)px[(y + i) * S + x + j];
-      int32_t q16[16];
-      wht_fwd(resid, q16);
+        for (int j = 0; j < 4; j++) r16[i * 4 + j] = (int)spx[(y + i) * S + x + j] - (int)px[(y + i) * S + x + j];
+      wht_fwd(r16, q16);
+      int eob = 0;
       for (int c = 15; c >= 0; c--)
         if (q16[Default_Scan_4x4[c]]) {
           eob = c + 1;
@@ -210,85 +326,87 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
         }
         for (int i = 0; i < 4; i++)
           for (int j = 0; j < 4; j++) {
-            int o = (y + i) * S + x + j;
+            const int o = (y + i) * S + x + j;
             px[o] = (uint8_t)d_clip1((int)px[o] + r4[i * 4 + j]);
           }
       }
+      L.ival[0] = eob;
     }
+    __syncthreads();
+    const int eob = L.ival[0];
     __syncthreads();
     *outType = DCT_DCT;
     *outCost = 0;
-    return __shfl(eob, 0, 64);
+    return eob;
   }
   const bool direct = nc == 1 && !fp.tx_mode_select;
   long long bestCost = -1;
   int bestType = DCT_DCT, bestEob = 0;
-  for (int ci = 0; ci < nc; ci++) {
-    const int t = cands[ci], fud = flip_ud(t), flr = flip_lr(t);
-    for (int k = lane; k < w * h; k += 64) {
-      int i = k / w, j = k - i * w;
-      int o = (y + (fud ? h - 1 - i : i)) * S + x + (flr ? w - 1 - j : j);
-      resid[i * RS + j] = (int)spx[o] - (int)px[o];
+  if (w <= 16 && h <= 16) {
+    /* one candidate per wave */
+    const int wave = tid >> 6;
+    const TxCtx c = wave_ctx();
+    for (int r0 = 0; r0 < nc; r0 += RW) {
+      const int ci = r0 + wave;
+      if (ci < nc) {
+        SmallTx &W = L.u.sm[wave];
+        long long cost;
+        const int eob = tx_eval(c, fp, spx, px, S, x, y, txSz, plane, cands[ci], direct, W.resid, SRS, W.tmp, W.coef,
+                                W.q, W.dq, SQS, mats, &cost);
+        if (c.tid == 0) {
+          L.candCost[ci] = cost;
+          L.candEob[ci] = eob;
+        }
+      }
+      __syncthreads();
+      int pick = -1;
+      for (int k = r0; k < d_min(nc, r0 + RW); k++)
+        if (bestCost < 0 || L.candCost[k] < bestCost) {
+          bestCost = L.candCost[k];
+          pick = k;
+        }
+      if (pick >= 0) {
+        bestEob = L.candEob[pick];
+        bestType = bestEob > 0 ? cands[pick] : DCT_DCT;
+        const int32_t *src = L.u.sm[pick - r0].q;
+        for (int k = tid; k < tw * th; k += RT) L.qbest[k] = src[k];
+      }
+      __syncthreads();
+    }
+    for (int k = tid; k < tw * th; k += RT) coefOut[k] = L.qbest[k];
+    if (bestEob > 0 && wave == 0) {
+      SmallTx &W = L.u.sm[0];
+      tx_dequant(c, L.qbest, W.dq, SQS, txSz, dcQ, acQ);
+      tx_inverse(c, W.dq, SQS, W.resid, SRS, txSz, bestType);
+      tx_add(c, px, S, x, y, lw, lh, bestType, W.resid, SRS);
     }
     __syncthreads();
-    wave_fwd2d(resid, L.u.fw.tmp, L.coef, txSz, t, mats, lane);
-    int32_t *qo = direct ? L.qbest : L.qtry;
-    wave_quantize(L.coef, qo, txSz, dcQ, acQ, lane);
-    const int16_t *scan = get_scan(txSz, t);
-    const int eob = wave_eob(qo, scan, seg_eob(txSz), lane);
-    if (direct) {
-      if (eob > 0) {
-        wave_dequant(qo, L.dq, txSz, dcQ, acQ, lane);
-        wave_itx2d(L.dq, resid, txSz, t, lane);
+  } else {
+    /* the whole workgroup per candidate */
+    const TxCtx c = wg_ctx();
+    BigTx &B = L.u.big;
+    for (int ci = 0; ci < nc; ci++) {
+      long long cost;
+      int32_t *qo = direct ? B.qbest : B.qtry;
+      const int eob = tx_eval(c, fp, spx, px, S, x, y, txSz, plane, cands[ci], direct, B.resid, BRS, B.tmp, B.coef, qo,
+                              B.dq, BQS, mats, &cost);
+      if (bestCost < 0 || cost < bestCost) {
+        bestCost = cost;
+        bestType = eob > 0 ? cands[ci] : DCT_DCT;
+        bestEob = eob;
+        if (!direct)
+          for (int k = tid; k < tw * th; k += RT) B.qbest[k] = B.qtry[k];
       }
-      bestCost = 0;
-      bestType = eob > 0 ? t : DCT_DCT;
-      bestEob = eob;
-      continue;
+      __syncthreads();
     }
-    long long sse = 0;
-    if (eob > 0) {
-      wave_dequant(L.qtry, L.dq, txSz, dcQ, acQ, lane);
-      wave_itx2d(L.dq, resid, txSz, t, lane);
-      long long e = 0;
-      for (int k = lane; k < w * h; k += 64) {
-        int i = k / w, j = k - i * w;
-        int o = (y + (fud ? h - 1 - i : i)) * S + x + (flr ? w - 1 - j : j);
-        int d = (int)spx[o] - d_clip1((int)px[o] + resid[i * RS + j]);
-        e += d * d;
-      }
-      sse = wave_sum64(e);
-    } else {
-      long long e = 0;
-      for (int k = lane; k < w * h; k += 64) {
-        int v = resid[(k / w) * RS + (k % w)];
-        e += v * v;
-      }
-      sse = wave_sum64(e);
-    }
-    const long long cost = sse * 16 + (long long)fp.lambda * wave_coef_bits(L.qtry, scan, eob, lane) / 8;
-    if (bestCost < 0 || cost < bestCost) {
-      bestCost = cost;
-      bestType = eob > 0 ? t : DCT_DCT;
-      bestEob = eob;
-      for (int k = lane; k < tw * th; k += 64) L.qbest[k] = L.qtry[k];
+    for (int k = tid; k < tw * th; k += RT) coefOut[k] = B.qbest[k];
+    if (bestEob > 0) {
+      tx_dequant(c, B.qbest, B.dq, BQS, txSz, dcQ, acQ);
+      tx_inverse(c, B.dq, BQS, B.resid, BRS, txSz, bestType);
+      tx_add(c, px, S, x, y, lw, lh, bestType, B.resid, BRS);
     }
     __syncthreads();
   }
-  for (int k = lane; k < tw * th; k += 64) coefOut[k] = L.qbest[k];
-  if (bestEob > 0) {
-    const int fud = flip_ud(bestType), flr = flip_lr(bestType);
-    if (nc > 1) {
-      wave_dequant(L.qbest, L.dq, txSz, dcQ, acQ, lane);
-      wave_itx2d(L.dq, resid, txSz, bestType, lane);
-    }
-    for (int k = lane; k < w * h; k += 64) {
-      int i = k / w, j = k - i * w;
-      int o = (y + (fud ? h - 1 - i : i)) * S + x + (flr ? w - 1 - j : j);
-      px[o] = (uint8_t)d_clip1((int)px[o] + resid[i * RS + j]);
-    }
-  }
-  __syncthreads();
   *outType = bestType;
   *outCost = bestCost;
   return bestEob;
@@ -316,11 +434,11 @@ DEVI int use_upsample(int w, int h, int filterType, int delta) {
   return filterType == 0 ? (w + h <= 16) : (w + h <= 8);
 }
 /* 7.11.2.12 on e[-1 .. sz-2] */
-DEVI void wave_edge_filter(int *e, int *cp, int sz, int strength, int lane) {
+DEVI void wg_edge_filter(int *e, int *cp, int sz, int strength, int tid) {
   if (!strength) return;
-  for (int i = lane; i < sz; i += 64) cp[i] = e[i - 1];
+  for (int i = tid; i < sz; i += RT) cp[i] = e[i - 1];
   __syncthreads();
-  for (int i = 1 + lane; i < sz; i += 64) {
+  for (int i = 1 + tid; i < sz; i += RT) {
     int s = 0;
 #pragma unroll
     for (int j = 0; j < 5; j++) s += INTRA_EDGE_KERNEL(strength - 1, j) * cp[d_clip(0, sz - 1, i - 2 + j)];
@@ -329,15 +447,15 @@ DEVI void wave_edge_filter(int *e, int *cp, int sz, int strength, int lane) {
   __syncthreads();
 }
 /* 7.11.2.11 */
-DEVI void wave_edge_upsample(int *e, int *dup, int numPx, int lane) {
-  for (int i = lane - 1; i < numPx; i += 64) dup[i + 2] = e[i];
-  if (lane == 0) {
+DEVI void wg_edge_upsample(int *e, int *dup, int numPx, int tid) {
+  for (int i = tid - 1; i < numPx; i += RT) dup[i + 2] = e[i];
+  if (tid == 0) {
     dup[0] = e[-1];
     dup[numPx + 2] = e[numPx - 1];
   }
   __syncthreads();
-  if (lane == 0) e[-2] = dup[0];
-  for (int i = lane; i < numPx; i += 64) {
+  if (tid == 0) e[-2] = dup[0];
+  for (int i = tid; i < numPx; i += RT) {
     int s = -dup[i] + 9 * dup[i + 1] + 9 * dup[i + 2] - dup[i + 3];
     e[2 * i - 1] = d_clip1(d_round2(s, 4));
     e[2 * i] = dup[i + 2];
@@ -345,13 +463,24 @@ DEVI void wave_edge_upsample(int *e, int *dup, int numPx, int lane) {
   __syncthreads();
 }
 
-NOINL void intra_tx_predict(const Mav1FrameParams &fp, uint8_t *px, int S, int sx, int sy, int lw, int lh, int mode,
-                            int delta, int hl, int ha, int har, int hbl, int maxX, int maxY, int edgeFilter,
-                            int filterType, int lane) {
+NOINL void intra_tx_predict(uint8_t *px, int S, int sx, int sy, int lw, int lh, int mode, int delta, int hl, int ha,
+                            int har, int hbl, int maxX, int maxY, int edgeFilter, int filterType) {
   RLds &L = gL;
-  const int w = 1 << lw, h = 1 << lh;
+  const int tid = threadIdx.x, w = 1 << lw, h = 1 << lh;
   int *A = L.edgeA + EDGE_NEG, *Lc = L.edgeL + EDGE_NEG;
-  wave_intra_edges(px, S, sx, sy, w, h, hl, ha, har, hbl, maxX, maxY, A, Lc, lane);
+  /* 7.11.2 steps 1-3: the neighbouring samples */
+  {
+    const int aLim = d_min(maxX, sx + (har ? 2 * w : w) - 1), lLim = d_min(maxY, sy + (hbl ? 2 * h : h) - 1);
+    for (int i = tid; i < w + h; i += RT) {
+      A[i] = ha ? px[(sy - 1) * S + d_min(aLim, sx + i)] : hl ? px[sy * S + sx - 1] : 127;
+      Lc[i] = hl ? px[d_min(lLim, sy + i) * S + sx - 1] : ha ? px[(sy - 1) * S + sx] : 129;
+    }
+    if (tid == 0) {
+      const int corner = (ha && hl) ? px[(sy - 1) * S + sx - 1] : ha ? px[(sy - 1) * S + sx] : hl ? px[sy * S + sx - 1] : 128;
+      A[-1] = Lc[-1] = corner;
+    }
+    __syncthreads();
+  }
   IntraMode im;
   intra_mode_setup(im, mode, delta, lw, lh);
   im.corner = A[-1];
@@ -359,68 +488,87 @@ NOINL void intra_tx_predict(const Mav1FrameParams &fp, uint8_t *px, int S, int s
     const int pa = im.pAngle;
     if (pa != 90 && pa != 180) {
       if (pa > 90 && pa < 180 && w + h >= 24) {
-        if (lane == 0) A[-1] = Lc[-1] = d_round2(Lc[0] * 5 + A[-1] * 6 + A[0] * 5, 4);
+        const int v = d_round2(Lc[0] * 5 + A[-1] * 6 + A[0] * 5, 4);
+        __syncthreads();
+        if (tid == 0) A[-1] = Lc[-1] = v;
         __syncthreads();
       }
       if (ha)
-        wave_edge_filter(A, L.ecopy, d_min(w, maxX - sx + 1) + (pa < 90 ? h : 0) + 1,
-                         edge_strength(w, h, filterType, pa - 90), lane);
+        wg_edge_filter(A, L.ecopy, d_min(w, maxX - sx + 1) + (pa < 90 ? h : 0) + 1, edge_strength(w, h, filterType, pa - 90),
+                       tid);
       if (hl)
-        wave_edge_filter(Lc, L.ecopy, d_min(h, maxY - sy + 1) + (pa > 180 ? w : 0) + 1,
-                         edge_strength(w, h, filterType, pa - 180), lane);
+        wg_edge_filter(Lc, L.ecopy, d_min(h, maxY - sy + 1) + (pa > 180 ? w : 0) + 1,
+                       edge_strength(w, h, filterType, pa - 180), tid);
     }
     im.upA = use_upsample(w, h, filterType, pa - 90);
-    if (im.upA) wave_edge_upsample(A, L.ecopy, w + (pa < 90 ? h : 0), lane);
+    if (im.upA) wg_edge_upsample(A, L.ecopy, w + (pa < 90 ? h : 0), tid);
     im.upL = use_upsample(w, h, filterType, pa - 180);
-    if (im.upL) wave_edge_upsample(Lc, L.ecopy, h + (pa > 180 ? w : 0), lane);
+    if (im.upL) wg_edge_upsample(Lc, L.ecopy, h + (pa > 180 ? w : 0), tid);
   }
-  im.dcv = mode == DC_PRED ? wave_intra_dc(A, Lc, w, h, lw, lh, hl, ha, lane) : 0;
-  for (int k = lane; k < w * h; k += 64) {
-    int i = k >> lw, j = k & (w - 1);
+  if (mode == DC_PRED) {
+    long long s = 0;
+    for (int i = tid; i < w; i += RT) s += ha ? A[i] : 0;
+    for (int i = tid; i < h; i += RT) s += hl ? Lc[i] : 0;
+    const int sum = (int)wg_sum64(s);
+    im.dcv = (ha && hl) ? (sum + ((w + h) >> 1)) / (w + h) : hl ? d_clip1((sum + (h >> 1)) >> lh)
+             : ha ? d_clip1((sum + (w >> 1)) >> lw) : 128;
+  } else {
+    im.dcv = 0;
+  }
+  for (int k = tid; k < w * h; k += RT) {
+    const int i = k >> lw, j = k & (w - 1);
     px[(sy + i) * S + sx + j] = (uint8_t)intra_px(im, A, Lc, i, j);
   }
   __syncthreads();
 }
 
 /* CfL (7.11.5) over the DC prediction in px: the alpha with the smallest
- * SSE against the source (first minimum over -16..16) unless given. */
+ * SSE against the source (first minimum over -16..16) unless given; the
+ * alphas are scored one per wave. */
 NOINL int cfl_tx(const uint8_t *luma, int LS, int sx, int sy, int maxLW, int maxLH, int lw, int lh, uint8_t *px, int S,
-                 const uint8_t *spx, int alphaIn, int search, int excludeZero, int lane) {
+                 const uint8_t *__restrict__ spx, int alphaIn, int search, int excludeZero) {
   RLds &L = gL;
-  const int w = 1 << lw, h = 1 << lh, n = w * h;
+  const int tid = threadIdx.x, w = 1 << lw, h = 1 << lh, n = w * h;
   const int dcv = px[sy * S + sx]; /* DC prediction: one value over the block */
-  int s = 0;
-  for (int k = lane; k < n; k += 64) {
-    int i = k >> lw, j = k & (w - 1);
-    int ly = d_min((sy + i) << 1, maxLH - 2), lx = d_min((sx + j) << 1, maxLW - 2);
+  long long s = 0;
+  for (int k = tid; k < n; k += RT) {
+    const int i = k >> lw, j = k & (w - 1);
+    const int ly = d_min((sy + i) << 1, maxLH - 2), lx = d_min((sx + j) << 1, maxLW - 2);
     const uint8_t *l = luma + ly * LS + lx;
-    int v = ((int)l[0] + l[1] + l[LS] + l[LS + 1]) << 1;
-    L.cfl[k] = v;
+    const int v = ((int)l[0] + l[1] + l[LS] + l[LS + 1]) << 1;
+    L.u.cfl[k] = v;
     s += v;
   }
-  const int avg = d_round2(wave_sum(s), lw + lh);
-  __syncthreads();
+  const int avg = d_round2((int)wg_sum64(s), lw + lh);
   int alpha = alphaIn;
   if (search) {
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int a0 = -16; a0 <= 16; a0 += RW) {
+      const int a = a0 + wave;
+      if (a <= 16) {
+        long long e = 0;
+        for (int k = lane; k < n; k += 64) {
+          const int i = k >> lw, j = k & (w - 1);
+          const int d = d_clip1(dcv + d_round2signed(a * (L.u.cfl[k] - avg), 6)) - spx[(sy + i) * S + sx + j];
+          e += d * d;
+        }
+        e = wave_sum64(e);
+        if (lane == 0) L.aSse[a + 16] = e;
+      }
+    }
+    __syncthreads();
     long long best = -1;
     for (int a = -16; a <= 16; a++) {
-      int e = 0;
-      for (int k = lane; k < n; k += 64) {
-        int i = k >> lw, j = k & (w - 1);
-        int d = d_clip1(dcv + d_round2signed(a * (L.cfl[k] - avg), 6)) - spx[(sy + i) * S + sx + j];
-        e += d * d;
-      }
-      long long sse = wave_sum64(e);
       if (a == 0 && excludeZero) continue;
-      if (best < 0 || sse < best) {
-        best = sse;
+      if (best < 0 || L.aSse[a + 16] < best) {
+        best = L.aSse[a + 16];
         alpha = a;
       }
     }
   }
-  for (int k = lane; k < n; k += 64) {
-    int i = k >> lw, j = k & (w - 1);
-    px[(sy + i) * S + sx + j] = (uint8_t)d_clip1(dcv + d_round2signed(alpha * (L.cfl[k] - avg), 6));
+  for (int k = tid; k < n; k += RT) {
+    const int i = k >> lw, j = k & (w - 1);
+    px[(sy + i) * S + sx + j] = (uint8_t)d_clip1(dcv + d_round2signed(alpha * (L.u.cfl[k] - avg), 6));
   }
   __syncthreads();
   return alpha;
@@ -430,9 +578,10 @@ NOINL int cfl_tx(const uint8_t *luma, int LS, int sx, int sy, int maxLW, int max
  * BlockDecoded flags (the block was coded by the inter phase). */
 NOINL long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, uint8_t *rec,
                            int32_t *__restrict__ coefs, const int32_t *__restrict__ mats, int plane, int txSz,
-                           int &maxLW, int &maxLH, bool useBd, bool markOnly, int lane) {
+                           int &maxLW, int &maxLH, bool useBd, bool markOnly) {
   RLds &L = gL;
   Mav1BlockInfo &b = L.blk;
+  const int tid = threadIdx.x;
   const int bs = b.bsize, sub = plane > 0, S = fp.stride[plane];
   const int availU = b.mi_row > 0, availL = b.mi_col > tile_start(fp, b.mi_col);
   const int psz = SUBSAMPLED_SIZE(bs, sub, sub);
@@ -465,21 +614,21 @@ NOINL long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict_
         if (!b.is_inter) {
           const int isCfl = plane > 0 && b.uv_mode == UV_CFL_PRED;
           const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
-          intra_tx_predict(fp, px, S, sx, sy, lw, lh, plane == 0 ? b.y_mode : (isCfl ? DC_PRED : b.uv_mode),
+          intra_tx_predict(px, S, sx, sy, lw, lh, plane == 0 ? b.y_mode : (isCfl ? DC_PRED : b.uv_mode),
                            plane == 0 ? b.angle_y : b.angle_uv, availL || x > 0, availU || y > 0,
                            L.bd[BDI(plane, (sbr >> sub) - 1, (sbc >> sub) + stepX)],
                            L.bd[BDI(plane, (sbr >> sub) + stepY, (sbc >> sub) - 1)], ((fp.mi_cols * 4) >> sub) - 1,
                            ((fp.mi_rows * 4) >> sub) - 1, fp.enable_intra_edge_filter,
-                           (b.interp_filter >> (plane == 0 ? 4 : 5)) & 1, lane);
+                           (b.interp_filter >> (plane == 0 ? 4 : 5)) & 1);
           if (isCfl) {
             int a;
             if (txIdx > 0)
               a = cfl_tx(rec + fp.offset[0], fp.stride[0], sx, sy, maxLW, maxLH, lw, lh, px, S, spx,
-                         plane == 1 ? b.cfl_alpha_u : b.cfl_alpha_v, 0, 0, lane);
+                         plane == 1 ? b.cfl_alpha_u : b.cfl_alpha_v, 0, 0);
             else
               a = cfl_tx(rec + fp.offset[0], fp.stride[0], sx, sy, maxLW, maxLH, lw, lh, px, S, spx, 0, 1,
-                         plane == 2 && b.cfl_alpha_u == 0, lane);
-            if (txIdx == 0 && lane == 0) {
+                         plane == 2 && b.cfl_alpha_u == 0);
+            if (txIdx == 0 && tid == 0) {
               if (plane == 1) b.cfl_alpha_u = a;
               else b.cfl_alpha_v = a;
             }
@@ -493,10 +642,10 @@ NOINL long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict_
         int t = DCT_DCT;
         long long c = 0;
         const int eob = code_tx_block(fp, spx, px, S, sx, sy, txSz, plane, cands, nc,
-                                      coefs + b.coef_off[plane] + txIdx * Tx_Width[txSz] * Tx_Height[txSz], mats,
-                                      lane, &t, &c);
+                                      coefs + b.coef_off[plane] + txIdx * Tx_Width[txSz] * Tx_Height[txSz], mats, &t,
+                                      &c);
         total += c;
-        if (plane == 0 && lane == 0) {
+        if (plane == 0 && tid == 0) {
           if (txIdx < 16) {
             const int sh = 4 * (txIdx & 7);
             uint32_t *f = (uint32_t *)(txIdx < 8 ? &b.tx_types_lo : &b.tx_types_hi);
@@ -507,63 +656,148 @@ NOINL long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict_
         anyNz |= eob > 0;
       }
       txIdx++;
-      if (useBd && lane == 0)
+      if (useBd && tid == 0)
         for (int i = 0; i < stepY; i++)
           for (int j = 0; j < stepX; j++) L.bd[BDI(plane, (sbr >> sub) + i, (sbc >> sub) + j)] = 1;
       __syncthreads();
     }
-  if (!markOnly && lane == 0) b.eob[plane] = anyNz;
+  if (!markOnly && tid == 0) b.eob[plane] = anyNz;
   __syncthreads();
   return total;
 }
 
-/* recon_block (k_recon.h:293-394) for L.blk. Inter blocks are fully coded
- * by the inter phase; the intra phase only marks their decoded area. */
-NOINL void predict_inter_plane(const uint8_t *__restrict__ plane, int S, int refW, int refH, int x, int y, int w,
-                              int h, int mvr, int mvc, int sub, int filter, uint8_t *dst, int ds, int32_t *dst32,
-                              int round1, int lane) {
-  wave_predict_inter(plane, S, refW, refH, x, y, w, h, mvr, mvc, sub, filter, gL.u.pr.win, gL.u.pr.mid, dst, ds, dst32,
-                     round1, lane);
+/* ---- inter prediction by the workgroup ---- */
+DEVI void filter_taps(int w, int h, int filter, int posX, int posY, int *kh, int *kv) {
+  int fv = filter & 3, fh = (filter >> 2) & 3;
+  if (w <= 4) fh = (fh == EIGHTTAP_SMOOTH) ? 5 : (fh == EIGHTTAP || fh == EIGHTTAP_SHARP) ? 4 : fh;
+  if (h <= 4) fv = (fv == EIGHTTAP_SMOOTH) ? 5 : (fv == EIGHTTAP || fv == EIGHTTAP_SHARP) ? 4 : fv;
+  const int16_t *th = SUBPEL_FILTER(fh, (posX >> 6) & SUBPEL_MASK), *tv = SUBPEL_FILTER(fv, (posY >> 6) & SUBPEL_MASK);
+#pragma unroll
+  for (int t = 0; t < 8; t++) {
+    kh[t] = th[t];
+    kv[t] = tv[t];
+  }
+}
+/* The (h + 7) x (w + 7) reference samples of the block (row stride w + 8),
+ * clamped to the reference plane (7.11.3.3). */
+DEVI void load_window(const uint8_t *__restrict__ plane, int S, int refW, int refH, int ix, int iy, int w, int h, int sub,
+                      uint8_t *win, int tid) {
+  const int lastX = ((refW + sub) >> sub) - 1, lastY = ((refH + sub) >> sub) - 1;
+  const int WW = w + 8, nr = h + 7;
+  for (int k = tid; k < nr * (w + 7); k += RT) {
+    const int r = k / (w + 7), c = k - r * (w + 7);
+    win[r * WW + c] = plane[d_clip(0, lastY, iy - 3 + r) * S + d_clip(0, lastX, ix - 3 + c)];
+  }
+}
+DEVI void hpass(const uint8_t *win, int w, int h, const int *kh, int16_t *mid, int tid) {
+  const int WW = w + 8;
+  for (int k = tid; k < (h + 7) * w; k += RT) {
+    const int r = k / w, c = k - r * w;
+    const uint8_t *q = win + r * WW + c;
+    int acc = 0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc += kh[t] * q[t];
+    mid[k] = (int16_t)d_round2(acc, 3);
+  }
 }
 
+/* Inter prediction of a w x h block of one plane (7.11.3): round1 11 writes
+ * clipped bytes to dst (row stride ds), round1 7 (compound) the unclipped
+ * int32 values to dst32. */
+NOINL void predict_inter_plane(const uint8_t *__restrict__ plane, int S, int refW, int refH, int x, int y, int w, int h,
+                               int mvr, int mvc, int sub, int filter, uint8_t *dst, int ds, int32_t *dst32, int round1,
+                               uint8_t *win, int16_t *mid) {
+  const int tid = threadIdx.x;
+  const int posX = ((x << SUBPEL_BITS) + ((2 * mvc) >> sub)) * 64 + 32;
+  const int posY = ((y << SUBPEL_BITS) + ((2 * mvr) >> sub)) * 64 + 32;
+  int kh[8], kv[8];
+  filter_taps(w, h, filter, posX, posY, kh, kv);
+  load_window(plane, S, refW, refH, posX >> 10, posY >> 10, w, h, sub, win, tid);
+  __syncthreads();
+  hpass(win, w, h, kh, mid, tid);
+  __syncthreads();
+  for (int k = tid; k < h * w; k += RT) {
+    const int r = k / w, c = k - r * w;
+    int acc = 0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc += kv[t] * mid[(r + t) * w + c];
+    if (round1 == 11) dst[r * ds + c] = (uint8_t)d_clip1(d_round2(acc, 11));
+    else dst32[r * ds + c] = d_round2(acc, round1);
+  }
+  __syncthreads();
+}
+
+/* The dual-filter choice of a single-reference inter block (k_recon.h:
+ * 312-334): SSE of the luma prediction of each of the 9 filter pairs over
+ * the visible area, + lambda per non-regular filter, first minimum. The
+ * window is shared, the three horizontal passes are computed once, each
+ * wave scores pairs. */
+NOINL int dual_filter_search(const Mav1FrameParams &fp, const uint8_t *__restrict__ refp,
+                             const uint8_t *__restrict__ src) {
+  RLds &L = gL;
+  const Mav1BlockInfo &b = L.blk;
+  const int tid = threadIdx.x, bs = b.bsize;
+  const int w = BLOCK_WIDTH(bs), h = BLOCK_HEIGHT(bs), x0 = b.mi_col * 4, y0 = b.mi_row * 4, S = fp.stride[0];
+  const int vw = d_min(w, fp.frame_w - x0), vh = d_min(h, fp.frame_h - y0);
+  const int posX = ((x0 << SUBPEL_BITS) + 2 * b.mv_col) * 64 + 32, posY = ((y0 << SUBPEL_BITS) + 2 * b.mv_row) * 64 + 32;
+  load_window(refp + fp.offset[0], S, fp.frame_w, fp.frame_h, posX >> 10, posY >> 10, w, h, 0, L.u.pr.win, tid);
+  __syncthreads();
+  for (int f = 0; f < 3; f++) {
+    int kh[8], kv[8];
+    filter_taps(w, h, f << 2, posX, posY, kh, kv);
+    hpass(L.u.pr.win, w, h, kh, L.u.pr.mid[f], tid);
+  }
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63;
+  const uint8_t *sp = src + fp.offset[0] + y0 * S + x0;
+  for (int f = wave; f < 9; f += RW) {
+    int kh[8], kv[8];
+    filter_taps(w, h, f % 3, posX, posY, kh, kv);
+    const int16_t *mid = L.u.pr.mid[f / 3];
+    long long e = 0;
+    for (int k = lane; k < vw * vh; k += 64) {
+      const int r = k / vw, c = k - r * vw;
+      int acc = 0;
+#pragma unroll
+      for (int t = 0; t < 8; t++) acc += kv[t] * mid[(r + t) * w + c];
+      const int d = d_clip1(d_round2(acc, 11)) - (int)sp[r * S + c];
+      e += d * d;
+    }
+    e = wave_sum64(e);
+    if (lane == 0) L.fcost[f] = e * 16 + (long long)fp.lambda * ((f % 3 != 0) + (f / 3 != 0));
+  }
+  __syncthreads();
+  long long best = -1;
+  int bestF = 0;
+  for (int f = 0; f < 9; f++)
+    if (best < 0 || L.fcost[f] < best) {
+      best = L.fcost[f];
+      bestF = (f % 3) | ((f / 3) << 2);
+    }
+  __syncthreads();
+  return bestF;
+}
+
+/* recon_block (k_recon.h:293-394) for L.blk. Inter blocks are fully coded
+ * by the inter phase; the intra phase only marks their decoded area. */
 NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, uint8_t *rec,
                        const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
-                       const int32_t *__restrict__ mats, bool useBd, bool markOnly, int lane) {
+                       const int32_t *__restrict__ mats, bool useBd, bool markOnly) {
   RLds &L = gL;
   Mav1BlockInfo &b = L.blk;
+  const int tid = threadIdx.x;
   const int bs = b.bsize, comp = b.is_inter && b.ref_frame1 > INTRA_FRAME;
   const uint8_t *r0 = (b.is_inter && b.ref_frame != LAST_FRAME) ? gold : ref;
   int maxLW = 0, maxLH = 0;
   if (!markOnly) {
-    if (lane == 0) {
+    if (tid == 0) {
       b.tx_depth = 0;
       b.interp_filter &= 0x30;
     }
     __syncthreads();
     if (b.is_inter && fp.switchable_interp && !comp) {
-      const int w = BLOCK_WIDTH(bs), h = BLOCK_HEIGHT(bs), x0 = b.mi_col * 4, y0 = b.mi_row * 4, S = fp.stride[0];
-      uint8_t *px = rec + fp.offset[0] + y0 * S + x0;
-      const uint8_t *sp = src + fp.offset[0] + y0 * S + x0;
-      const int vw = d_min(w, fp.frame_w - x0), vh = d_min(h, fp.frame_h - y0);
-      long long best = -1;
-      int bestF = 0;
-      for (int f = 0; f < 9; f++) {
-        const int packed = (f % 3) | ((f / 3) << 2);
-        predict_inter_plane(r0 + fp.offset[0], S, fp.frame_w, fp.frame_h, x0, y0, w, h, b.mv_row, b.mv_col, 0, packed,
-                            px, S, nullptr, 11, lane);
-        long long e = 0;
-        for (int k = lane; k < vw * vh; k += 64) {
-          int i = k / vw, j = k - i * vw;
-          int d = (int)px[i * S + j] - (int)sp[i * S + j];
-          e += d * d;
-        }
-        const long long c = wave_sum64(e) * 16 + (long long)fp.lambda * ((f % 3 != 0) + (f / 3 != 0));
-        if (best < 0 || c < best) {
-          best = c;
-          bestF = packed;
-        }
-      }
-      if (lane == 0) b.interp_filter = bestF;
+      const int f = dual_filter_search(fp, r0, src);
+      if (tid == 0) b.interp_filter = f;
       __syncthreads();
     }
   }
@@ -573,43 +807,44 @@ NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ sr
     const int w = BLOCK_WIDTH(bs) >> sub, h = BLOCK_HEIGHT(bs) >> sub;
     if (!markOnly) {
       if (comp) {
-        predict_inter_plane(r0 + fp.offset[plane], S, fp.frame_w, fp.frame_h, baseX, baseY, w, h, b.mv_row, b.mv_col,
-                            sub, b.interp_filter, nullptr, w, L.u.pr.p0, 7, lane);
-        /* second prediction's vertical pass combined with the first one */
-        int32_t *p1 = (int32_t *)L.coef; /* free until code_plane */
+        CompLds &P = L.u.cp;
+        predict_inter_plane(r0 + fp.offset[plane], S, fp.frame_w, fp.frame_h, baseX, baseY, w, h, b.mv_row, b.mv_col, sub,
+                            b.interp_filter, nullptr, w, P.p0, 7, P.win, P.mid);
+        /* the second prediction in bands of 16 rows, combined with the first */
         for (int yy = 0; yy < h; yy += 16) {
           const int hh = d_min(16, h - yy);
           predict_inter_plane(gold + fp.offset[plane], S, fp.frame_w, fp.frame_h, baseX, baseY + yy, w, hh, b.mv1_row,
-                              b.mv1_col, sub, b.interp_filter, nullptr, w, p1, 7, lane);
+                              b.mv1_col, sub, b.interp_filter, nullptr, w, P.p1, 7, P.win, P.mid);
           uint8_t *d = rec + fp.offset[plane] + (baseY + yy) * S + baseX;
-          for (int k = lane; k < w * hh; k += 64) {
-            int i = k / w, j = k - i * w;
-            d[i * S + j] = (uint8_t)d_clip1(d_round2(L.u.pr.p0[(yy + i) * w + j] + p1[i * w + j], 5));
+          for (int k = tid; k < w * hh; k += RT) {
+            const int i = k / w, j = k - i * w;
+            d[i * S + j] = (uint8_t)d_clip1(d_round2(P.p0[(yy + i) * w + j] + P.p1[i * w + j], 5));
           }
           __syncthreads();
         }
       } else if (b.is_inter) {
         predict_inter_plane(r0 + fp.offset[plane], S, fp.frame_w, fp.frame_h, baseX, baseY, w, h, b.mv_row, b.mv_col,
-                            sub, b.interp_filter, rec + fp.offset[plane] + baseY * S + baseX, S, nullptr, 11, lane);
+                            sub, b.interp_filter, rec + fp.offset[plane] + baseY * S + baseX, S, nullptr, 11, L.u.pr.win,
+                            L.u.pr.mid[0]);
       }
     }
     if (fp.lossless) {
-      code_plane(fp, src, rec, coefs, mats, plane, TX_4X4, maxLW, maxLH, useBd, markOnly, lane);
+      code_plane(fp, src, rec, coefs, mats, plane, TX_4X4, maxLW, maxLH, useBd, markOnly);
       continue;
     }
     if (plane > 0) {
-      code_plane(fp, src, rec, coefs, mats, plane, uv_tx_size(bs), maxLW, maxLH, useBd, markOnly, lane);
+      code_plane(fp, src, rec, coefs, mats, plane, uv_tx_size(bs), maxLW, maxLH, useBd, markOnly);
       continue;
     }
     const int maxDepth = (!b.is_inter && fp.tx_mode_select) ? d_min(MAX_TX_DEPTH, Max_Tx_Depth[bs]) : 0;
     if (maxDepth == 0) {
-      code_plane(fp, src, rec, coefs, mats, 0, Max_Tx_Size_Rect[bs], maxLW, maxLH, useBd, markOnly, lane);
+      code_plane(fp, src, rec, coefs, mats, 0, Max_Tx_Size_Rect[bs], maxLW, maxLH, useBd, markOnly);
       continue;
     }
     /* tx_depth RD search (intra only, so never markOnly) */
     uint8_t *px = rec + fp.offset[0] + baseY * S + baseX;
-    for (int k = lane; k < w * h; k += 64) L.save[k] = px[(k / w) * S + (k % w)];
-    for (int k = lane; k < BDD * BDD; k += 64) L.bdSave[k] = L.bd[k];
+    for (int k = tid; k < w * h; k += RT) L.save[k] = px[(k / w) * S + (k % w)];
+    for (int k = tid; k < BDD * BDD; k += RT) L.bdSave[k] = L.bd[k];
     __syncthreads();
     long long best = -1;
     int bestD = 0;
@@ -617,63 +852,67 @@ NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ sr
       const int depth = d <= maxDepth ? d : bestD;
       if (d == maxDepth + 1 && bestD == maxDepth) break;
       if (d > 0) {
-        for (int k = lane; k < w * h; k += 64) px[(k / w) * S + (k % w)] = L.save[k];
-        for (int k = lane; k < BDD * BDD; k += 64) L.bd[k] = L.bdSave[k];
+        for (int k = tid; k < w * h; k += RT) px[(k / w) * S + (k % w)] = L.save[k];
+        for (int k = tid; k < BDD * BDD; k += RT) L.bd[k] = L.bdSave[k];
         __syncthreads();
       }
       int t = Max_Tx_Size_Rect[bs];
       for (int k = 0; k < depth; k++) t = Split_Tx_Size[t];
       const long long c =
-          code_plane(fp, src, rec, coefs, mats, 0, t, maxLW, maxLH, useBd, false, lane) + (long long)fp.lambda * 8 * depth;
+          code_plane(fp, src, rec, coefs, mats, 0, t, maxLW, maxLH, useBd, false) + (long long)fp.lambda * 8 * depth;
       if (d <= maxDepth && (best < 0 || c < best)) {
         best = c;
         bestD = depth;
       }
       if (d == maxDepth + 1) break;
     }
-    if (lane == 0) b.tx_depth = bestD;
+    if (tid == 0) b.tx_depth = bestD;
     __syncthreads();
   }
 }
 
-/* Phase A: every inter block of the frame, one wave per block slot. */
-__global__ __launch_bounds__(64) void k_recon_inter(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
+/* Inter phase: every inter block of the frame, one workgroup per block slot. */
+__global__ __launch_bounds__(RT) void k_recon_inter(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
                                                     const Mav1SbInfo *__restrict__ sbs, const uint8_t *__restrict__ src,
                                                     uint8_t *rec, const uint8_t *__restrict__ ref,
                                                     const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
                                                     const int32_t *__restrict__ mats) {
   RLds &L = gL;
-  const int lane = threadIdx.x, sb = blockIdx.x >> 6, k = blockIdx.x & 63;
+  const int tid = threadIdx.x, sb = blockIdx.x >> 6, k = blockIdx.x & 63;
   if (k >= sbs[sb].num_blocks) return;
   Mav1BlockInfo *g = blocks + sbs[sb].first_block + k;
   if (!g->is_inter) return;
-  if (lane < 27) ((int32_t *)&L.blk)[lane] = ((const int32_t *)g)[lane];
+  if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)g)[tid];
   __syncthreads();
-  recon_block(fp, src, rec, ref, gold, coefs, mats, false, false, lane);
-  if (lane < 27) ((int32_t *)g)[lane] = ((const int32_t *)&L.blk)[lane];
+  recon_block(fp, src, rec, ref, gold, coefs, mats, false, false);
+  if (tid < 27) ((int32_t *)g)[tid] = ((const int32_t *)&L.blk)[tid];
 }
 
-/* Phase B: superblocks of anti-diagonal t, one wave each; grid = tile
- * columns x superblock rows. */
-__global__ __launch_bounds__(64) void k_recon_intra(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
+/* Intra phase: superblocks of anti-diagonal t, one workgroup each; grid =
+ * tile columns x superblock rows. */
+__global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
                                                     const Mav1SbInfo *__restrict__ sbs, const uint8_t *__restrict__ src,
                                                     uint8_t *rec, const uint8_t *__restrict__ ref,
                                                     const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
                                                     const int32_t *__restrict__ mats, int t) {
   RLds &L = gL;
-  const int lane = threadIdx.x;
-  const int tile = blockIdx.x % ((fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb), r = blockIdx.x / ((fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb);
+  const int tid = threadIdx.x;
+  const int tiles = (fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb;
+  const int tile = blockIdx.x % tiles, r = blockIdx.x / tiles;
   const int c0 = tile * fp.tile_w_sb, cl = t - 2 * r;
   if (cl < 0 || cl >= fp.tile_w_sb || c0 + cl >= fp.sb_cols || r >= fp.sb_rows) return;
   const int sbIdx = r * fp.sb_cols + c0 + cl;
   const Mav1SbInfo &S = sbs[sbIdx];
-  int nIntra = 0;
-  for (int i = lane; i < S.num_blocks; i += 64) nIntra += !blocks[S.first_block + i].is_inter;
-  if (wave_sum(nIntra) == 0) return;
+  if (tid == 0) L.ival[1] = 0;
+  __syncthreads();
+  for (int i = tid; i < S.num_blocks; i += RT)
+    if (!blocks[S.first_block + i].is_inter) L.ival[1] = 1;
+  __syncthreads();
+  if (!L.ival[1]) return;
   /* clear_block_decoded_flags (k_recon.h:402-414) */
   const int mr = r * 16, mc = (c0 + cl) * 16;
   const int tileEnd = d_min(fp.mi_cols, tile_start(fp, mc) + fp.tile_w_sb * 16);
-  for (int q = lane; q < 3 * BDD * BDD; q += 64) {
+  for (int q = tid; q < 3 * BDD * BDD; q += RT) {
     const int p = q / (BDD * BDD), e = q - p * BDD * BDD, y = e / BDD - 1, x = e % BDD - 1, sub = p > 0;
     const int w4 = (tileEnd - mc) >> sub, h4 = (fp.mi_rows - mr) >> sub;
     uint8_t v = (y < 0 && x < w4) ? 1 : (x < 0 && y < h4) ? 1 : 0;
@@ -683,11 +922,11 @@ __global__ __launch_bounds__(64) void k_recon_intra(const Mav1FrameParams fp, Ma
   __syncthreads();
   for (int i = 0; i < S.num_blocks; i++) {
     Mav1BlockInfo *g = blocks + S.first_block + i;
-    if (lane < 27) ((int32_t *)&L.blk)[lane] = ((const int32_t *)g)[lane];
+    if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)g)[tid];
     __syncthreads();
     const bool inter = L.blk.is_inter;
-    recon_block(fp, src, rec, ref, gold, coefs, mats, true, inter, lane);
-    if (!inter && lane < 27) ((int32_t *)g)[lane] = ((const int32_t *)&L.blk)[lane];
+    recon_block(fp, src, rec, ref, gold, coefs, mats, true, inter);
+    if (!inter && tid < 27) ((int32_t *)g)[tid] = ((const int32_t *)&L.blk)[tid];
     __syncthreads();
   }
 }
@@ -732,7 +971,7 @@ hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
                               const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold,
                               int32_t *coefs, const int32_t *mats) {
   const int nsb = fp.sb_cols * fp.sb_rows;
-  hipLaunchKernelGGL(k_recon_inter, dim3(nsb * 64), dim3(64), 0, st, fp, blocks, sbs, src, rec, ref,
+  hipLaunchKernelGGL(k_recon_inter, dim3(nsb * 64), dim3(RT), 0, st, fp, blocks, sbs, src, rec, ref,
                      gold, coefs, mats);
   return hipGetLastError();
 }
@@ -743,7 +982,7 @@ hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, int t, 
                               const Mav1SbInfo *sbs, const uint8_t *src, uint8_t *rec, const uint8_t *ref,
                               const uint8_t *gold, int32_t *coefs, const int32_t *mats) {
   const int tiles = (fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb;
-  hipLaunchKernelGGL(k_recon_intra, dim3(tiles * fp.sb_rows), dim3(64), 0, st, fp, blocks, sbs, src, rec, ref, gold,
+  hipLaunchKernelGGL(k_recon_intra, dim3(tiles * fp.sb_rows), dim3(RT), 0, st, fp, blocks, sbs, src, rec, ref, gold,
                      coefs, mats, t);
   return hipGetLastError();
 }

@@ -16,7 +16,8 @@ NUM_NODES = 85
 A_SLOTS = 13
 TURBO_DEFAULT = 0x47CBF1
 (BUF_SRC, BUF_REF, BUF_GOLD, BUF_MVFIELD, BUF_MVGOLD, BUF_PLANES, BUF_ACOSTS, BUF_AINFOS, BUF_BLOCKS,
- BUF_SBS, BUF_REC, BUF_COEFS, BUF_LFMASK, BUF_CDEFBUF, BUF_DBK, BUF_LROUT, BUF_LRUNITS, BUF_CDEF_CHOICE) = range(18)
+ BUF_SBS, BUF_REC, BUF_COEFS, BUF_LFMASK, BUF_CDEFBUF, BUF_DBK, BUF_LROUT, BUF_LRUNITS, BUF_CDEF_CHOICE,
+ BUF_DECGRID) = range(19)
 MAX_CDEF_SETS = 8
 
 
@@ -129,6 +130,9 @@ def _load():
         "mav1hip_launch_cdef_select": (C.c_int, [P, i32]),
         "mav1hip_launch_restore": (C.c_int, [P]),
         "mav1hip_launch_lr_select": (C.c_int, [P, i32]),
+        "mav1hip_decision_record_bytes": (C.c_size_t, [C.POINTER(FrameParams)]),
+        "mav1hip_decision_record": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, P]),
+        "mav1hip_launch_decision_grid": (C.c_int, [P]),
         "mav1hip_lr_select": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, i32, P, P, P]),
     }
     for name, (res, args) in sig.items():
@@ -330,6 +334,18 @@ class Context:
 
     def launch_lr_select(self, lr_test=0):
         self._chk(lib().mav1hip_launch_lr_select(self.h, lr_test), "launch_lr_select")
+
+    def decision_record(self, fp, blocks, sbs, hdr7):
+        """The dump_decisions record of the coded blocks (bytes)."""
+        out = np.zeros(lib().mav1hip_decision_record_bytes(C.byref(fp)), np.uint8)
+        h = np.asarray(hdr7, np.int32)
+        rc = lib().mav1hip_decision_record(self.h, C.byref(fp), ptr(np.ascontiguousarray(blocks, np.int32)),
+                                           ptr(np.ascontiguousarray(sbs, np.int32)), ptr(h), ptr(out))
+        self._chk(rc, "decision_record")
+        return out
+
+    def launch_decision_grid(self):
+        self._chk(lib().mav1hip_launch_decision_grid(self.h), "launch_decision_grid")
 
     def launch_restore(self):
         self._chk(lib().mav1hip_launch_restore(self.h), "launch_restore")

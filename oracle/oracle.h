@@ -85,6 +85,10 @@ void oracle_lr_apply(const Mav1FrameParams *fp, const uint8_t *cdef, const uint8
 void oracle_lr_select(Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdef, const uint8_t *dbk, uint8_t *tmp,
                       Mav1LrUnit *units, int lrTest);
 
+/* --- decision record --- */
+/* Per-8x8 grid ((mi_rows+1)/2 x (mi_cols+1)/2 cells of 8 bytes). */
+void oracle_decision_grid(const Mav1FrameParams *fp, const Mav1BlockInfo *blocks, const Mav1SbInfo *sbs, uint8_t *grid);
+
 /* --- shared prediction primitives --- */
 /* Intra prediction arguments (the IntraArgs record of src/kernels/k_intra.h:79-85). */
 typedef struct {

@@ -207,3 +207,27 @@ def test_lr_select_equals_oracle(gpu, name):
     finally:
         ctx.close()
     print(name, "units", len(uo), "types", np.bincount(uo[:, 0], minlength=3), "reruns", reruns)
+
+
+@pytest.mark.parametrize("name", ["turbo_arf_352x288", "key_full_204x116", "full_rd_136x104"])
+def test_decision_record_equals_oracle(gpu, name):
+    """The dump_decisions record: header fields and the per-8x8 cells of the
+    coded blocks equal the oracle's grid."""
+    import ctypes as C
+    w, h, kind, seed, kw = CASES[name]
+    cfg, fr, fp, npresets, frames, rec, blocks, sbs = coded(w, h, kind, seed, **dict(kw))
+    hdr7 = [5, 4, 6, 1, 2, 1, 12345]
+    ctx = mav1hip.Context(cfg)
+    try:
+        rec_bytes = ctx.decision_record(fp, blocks, sbs, hdr7)
+    finally:
+        ctx.close()
+    rows8, cols8 = (fp.mi_rows + 1) >> 1, (fp.mi_cols + 1) >> 1
+    hdr = np.frombuffer(rec_bytes[:44].tobytes(), np.int32)
+    assert list(hdr) == [0x4345444D, 5, 4, 6, 1, 2, 1, fp.base_q_idx, rows8, cols8, 12345]
+    lib = libs.oracle()
+    lib.oracle_decision_grid.argtypes = [C.c_void_p] * 4
+    grid = np.zeros(rows8 * cols8 * 8, np.uint8)
+    lib.oracle_decision_grid(C.addressof(fp), mav1hip.ptr(np.ascontiguousarray(blocks, np.int32)),
+                             mav1hip.ptr(np.ascontiguousarray(sbs, np.int32)), mav1hip.ptr(grid))
+    assert np.array_equal(rec_bytes[44:], grid)

@@ -270,3 +270,41 @@ def test_filter_plan_turbo_gop():
     fp, n = mav1hip.setup_params(cfg, fr)
     plan = mav1hip.filter_plan(cfg, fr, fp, n)
     assert not plan.deblock and not plan.cdef_select and not plan.restoration
+
+
+def test_decision_grid_known_layout():
+    """The dump_decisions cells of a hand-built superblock: a 32x32 intra
+    block, a compound 16x16 block, four 4x4 blocks sharing one 8x8 cell (the
+    last in decode order wins), a cell no block covers stays zero."""
+    import ctypes as C
+    cfg = mav1hip.default_config(64, 64, turbo=0, qindex=125)
+    fp, _ = mav1hip.setup_params(cfg, mav1hip.turbo_frame("key", 0))
+    blocks = np.zeros((64, 27), np.int32)
+    sbs = np.zeros((1, 89), np.int32)
+    rows = [
+        (0, 0, 9, 0, 0, 0, 0, 0),        # mi_row, mi_col, bsize, is_inter, ref_frame, ref_frame1, mv_row, mv_col
+        (0, 8, 6, 1, 1, 7, -12, 34),
+        (8, 8, 0, 1, 1, 0, 5, -6),
+        (8, 9, 0, 1, 2, 0, 7, -8),
+        (9, 8, 0, 0, 0, 0, 0, 0),
+        (9, 9, 0, 1, 1, 0, -3, 300),
+    ]
+    for k, (r, c, bs, inter, rf, rf1, mr, mc) in enumerate(rows):
+        blocks[k, 0:4] = (r, c, bs, inter)
+        blocks[k, 8:10] = (mr, mc)
+        blocks[k, 23:25] = (rf, rf1)
+    sbs[0, 1] = len(rows)
+    lib = libs.oracle()
+    lib.oracle_decision_grid.argtypes = [C.c_void_p] * 4
+    grid = np.full(8 * 8 * 8, 0xAB, np.uint8)
+    lib.oracle_decision_grid(C.addressof(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs), mav1hip.ptr(grid))
+    cells = grid.reshape(8, 8, 8)
+
+    def cell(bs, mode, rf, mr, mc):
+        return np.frombuffer(np.array([bs, mode, rf, 0], np.uint8).tobytes() + np.array([mr, mc], np.int16).tobytes(),
+                             np.uint8)
+
+    assert all(np.array_equal(cells[r, c], cell(9, 0, 0, 0, 0)) for r in range(4) for c in range(4))
+    assert all(np.array_equal(cells[r, c], cell(6, 2, 1, -12, 34)) for r in range(2) for c in range(4, 6))
+    assert np.array_equal(cells[4, 4], cell(0, 1, 1, -3, 300))
+    assert not cells[4, 5].any() and not cells[7, 7].any() and not cells[0, 6].any()
