@@ -374,21 +374,26 @@ int mav1hip_set_params(mav1hip_ctx *c, const Mav1FrameParams *fp) {
 int mav1hip_launch_motion_search(mav1hip_ctx *c) {
   if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "motion search: no frame parameters set");
   const Mav1FrameParams &fp = c->fp;
-  if (!(fp.turbo & MAV1_T_ME_HPEL))
-    return fail(c, MAV1HIP_EUNSUP, "motion search without quarter-pel planes (turbo bit 16) is not built yet");
+  const bool hp = (fp.turbo & MAV1_T_ME_HPEL) != 0;
+  if (!hp && (fp.turbo & MAV1_T_ME_DEC))
+    return fail(c, MAV1HIP_EUNSUP,
+                "motion search: decimated-plane stages (turbo bit 32) without the quarter-pel planes (bit 16) read "
+                "planes no stage builds");
   int nref = fp.golden ? 2 : 1;
   uint8_t *srcDec = c->d_planes + 2 * c->entry_bytes;
   size_t n16 = (size_t)16 * fp.frame_w * fp.frame_h;
   int rc;
-  for (int r = 0; r < nref; r++) {
-    const uint8_t *refp = r ? c->b_gold : c->b_ref;
-    uint8_t *ent = c->d_planes + r * c->entry_bytes;
-    if ((rc = timed(c, "qpel_build", [&] { return launch_qpel_build(c->stream, fp, refp, ent); }))) return rc;
+  if (hp) {
+    for (int r = 0; r < nref; r++) {
+      const uint8_t *refp = r ? c->b_gold : c->b_ref;
+      uint8_t *ent = c->d_planes + r * c->entry_bytes;
+      if ((rc = timed(c, "qpel_build", [&] { return launch_qpel_build(c->stream, fp, refp, ent); }))) return rc;
+      if (fp.turbo & MAV1_T_ME_DEC)
+        if ((rc = timed(c, "dec_build", [&] { return launch_dec_build(c->stream, fp, refp, ent + n16); }))) return rc;
+    }
     if (fp.turbo & MAV1_T_ME_DEC)
-      if ((rc = timed(c, "dec_build", [&] { return launch_dec_build(c->stream, fp, refp, ent + n16); }))) return rc;
+      if ((rc = timed(c, "dec_build", [&] { return launch_dec_build(c->stream, fp, c->b_src, srcDec); }))) return rc;
   }
-  if (fp.turbo & MAV1_T_ME_DEC)
-    if ((rc = timed(c, "dec_build", [&] { return launch_dec_build(c->stream, fp, c->b_src, srcDec); }))) return rc;
   static const char *names[4] = {"me_level0", "me_level1", "me_level2", "me_level3"};
   for (int level = 0; level < 4; level++)
     if ((rc = timed(c, names[level], [&] {

@@ -212,7 +212,53 @@ DEVI int subpel_cost_hp(const Mav1FrameParams &fp, const uint8_t *srcL, const ui
   return wave_sum(s) * st;
 }
 
-template <int LEVEL>
+/* Sub-pel cost at an even 1/8-pel vector without the quarter-pel planes
+ * (k_me.h:247-258): the normative EIGHTTAP prediction of the block
+ * (7.11.3, staged through LDS) and the SATD of all its 4x4 units. */
+template <int BW>
+DEVI int subpel_cost_pred(const Mav1FrameParams &fp, const uint8_t *srcL, const uint8_t *refY, int x, int y, int mvr,
+                          int mvc, uint8_t *pwin, int16_t *pmid, int lane) {
+  constexpr int WW = BW + 8, NR = BW / 4;
+  const int S = fp.stride[0], lastX = fp.frame_w - 1, lastY = fp.frame_h - 1;
+  const int posX = ((x << SUBPEL_BITS) + 2 * mvc) * 64 + 32, posY = ((y << SUBPEL_BITS) + 2 * mvr) * 64 + 32;
+  const int ix = posX >> 10, iy = posY >> 10;
+  const int16_t *kh = SUBPEL_FILTER(EIGHTTAP, (posX >> 6) & SUBPEL_MASK), *kv = SUBPEL_FILTER(EIGHTTAP, (posY >> 6) & SUBPEL_MASK);
+  __syncthreads();
+  for (int k = lane; k < (BW + 7) * (BW + 7); k += 64) {
+    const int r = k / (BW + 7), c = k - r * (BW + 7);
+    pwin[r * WW + c] = refY[d_clip(0, lastY, iy - 3 + r) * S + d_clip(0, lastX, ix - 3 + c)];
+  }
+  __syncthreads();
+  for (int k = lane; k < (BW + 7) * BW; k += 64) {
+    const int r = k / BW, c = k - r * BW;
+    const uint8_t *q = pwin + r * WW + c;
+    int acc = 0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc += kh[t] * q[t];
+    pmid[k] = (int16_t)d_round2(acc, 3);
+  }
+  __syncthreads();
+  int s = 0;
+  for (int u = lane; u < NR * NR; u += 64) {
+    const int ucol = u % NR, urow = u / NR;
+    int d[16];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int r = urow * 4 + i, c = ucol * 4 + j;
+        int acc = 0;
+#pragma unroll
+        for (int t = 0; t < 8; t++) acc += kv[t] * pmid[(r + t) * BW + c];
+        d[4 * i + j] = (int)srcL[r * BW + c] - d_clip1(d_round2(acc, 11));
+      }
+    s += satd4x4_arr(d);
+  }
+  return wave_sum(s);
+}
+
+/* HP: sub-pel costs from the quarter-pel planes, else from predictions. */
+template <int LEVEL, bool HP>
 __global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
                                                  const uint8_t *__restrict__ ref0, const uint8_t *__restrict__ ref1,
                                                  const uint8_t *__restrict__ planes, long entryBytes,
@@ -223,6 +269,8 @@ __global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const
   constexpr int WW = BW + 2 * WIN_R;
   __shared__ uint8_t srcL[BW * BW];
   __shared__ uint8_t win[WW * WW];
+  __shared__ uint8_t pwin[HP ? 1 : (BW + 7) * (BW + 8)];
+  __shared__ int16_t pmid[HP ? 1 : (BW + 7) * BW];
   const int lane = threadIdx.x;
   const int r = blockIdx.x / nodesPerRef, node = blockIdx.x - r * nodesPerRef;
   const int sb = node >> (2 * LEVEL), i = node & (N * N - 1);
@@ -327,7 +375,11 @@ __global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const
    * T_ME_HALF01 stops nodes >= 32 wide at half-pel */
   int mvr = m.nstage ? by * 8 : m.pr, mvc = m.nstage ? bx * 8 : m.pc;
   const int st = (TB(fp, MAV1_T_ME_USUB) && BW >= 32) ? 2 : 1;
-  int bestS = subpel_cost_hp<BW>(fp, srcL, pl, m.x, m.y, mvr, mvc, st, lane);
+  auto subpel = [&](int r_, int c_) {
+    if constexpr (HP) return subpel_cost_hp<BW>(fp, srcL, pl, m.x, m.y, r_, c_, st, lane);
+    else return subpel_cost_pred<BW>(fp, srcL, refY, m.x, m.y, r_, c_, pwin, pmid, lane);
+  };
+  int bestS = subpel(mvr, mvc);
   const int minStep = (BW >= 32 && TB(fp, MAV1_T_ME_HALF01)) ? 4 : 2;
   const bool cross = TB(fp, MAV1_T_ME_CROSS);
   for (int step = 4; step >= minStep; step >>= 1) {
@@ -336,7 +388,7 @@ __global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const
       int k = cross ? (j == 0 ? 1 : j == 1 ? 3 : j == 2 ? 4 : 6) : j;
       int dr = (k < 3 ? -1 : k < 5 ? 0 : 1) * step;
       int dc = ((k == 0 || k == 3 || k == 5) ? -1 : (k == 1 || k == 6) ? 0 : 1) * step;
-      int c = subpel_cost_hp<BW>(fp, srcL, pl, m.x, m.y, cr + dr, cc + dc, st, lane);
+      int c = subpel(cr + dr, cc + dc);
       if (c < bestS) {
         bestS = c;
         mvr = cr + dr;
@@ -350,28 +402,29 @@ __global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const
   }
 }
 
+template <int LEVEL>
+static void launch_level(hipStream_t st, bool hp, dim3 grid, const Mav1FrameParams &fp, const uint8_t *src,
+                         const uint8_t *ref0, const uint8_t *ref1, const uint8_t *planes, long entryBytes,
+                         const uint8_t *srcDec, int32_t *field0, int32_t *field1, int nodesPerRef) {
+  if (hp)
+    hipLaunchKernelGGL((k_me_level<LEVEL, true>), grid, dim3(64), 0, st, fp, src, ref0, ref1, planes, entryBytes, srcDec,
+                       field0, field1, nodesPerRef);
+  else
+    hipLaunchKernelGGL((k_me_level<LEVEL, false>), grid, dim3(64), 0, st, fp, src, ref0, ref1, planes, entryBytes,
+                       srcDec, field0, field1, nodesPerRef);
+}
+
 hipError_t launch_me_level(hipStream_t st, int level, const Mav1FrameParams &fp, const uint8_t *src,
                            const uint8_t *ref0, const uint8_t *ref1, const uint8_t *planes, long entryBytes,
                            const uint8_t *srcDec, int32_t *field0, int32_t *field1, int nref) {
   int nodesPerRef = fp.sb_cols * fp.sb_rows << (2 * level);
-  dim3 grid(nodesPerRef * nref), block(64);
+  dim3 grid(nodesPerRef * nref);
+  const bool hp = (fp.turbo & MAV1_T_ME_HPEL) != 0;
   switch (level) {
-    case 0:
-      hipLaunchKernelGGL(k_me_level<0>, grid, block, 0, st, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0,
-                         field1, nodesPerRef);
-      break;
-    case 1:
-      hipLaunchKernelGGL(k_me_level<1>, grid, block, 0, st, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0,
-                         field1, nodesPerRef);
-      break;
-    case 2:
-      hipLaunchKernelGGL(k_me_level<2>, grid, block, 0, st, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0,
-                         field1, nodesPerRef);
-      break;
-    default:
-      hipLaunchKernelGGL(k_me_level<3>, grid, block, 0, st, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0,
-                         field1, nodesPerRef);
-      break;
+    case 0: launch_level<0>(st, hp, grid, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0, field1, nodesPerRef); break;
+    case 1: launch_level<1>(st, hp, grid, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0, field1, nodesPerRef); break;
+    case 2: launch_level<2>(st, hp, grid, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0, field1, nodesPerRef); break;
+    default: launch_level<3>(st, hp, grid, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0, field1, nodesPerRef); break;
   }
   return hipGetLastError();
 }

@@ -85,3 +85,30 @@ def test_motion_search_device_resident_pool(gpu):
         if fp.golden:
             assert np.array_equal(ctx.download(mav1hip.BUF_MVGOLD, np.int32), go)
     ctx.close()
+
+
+@pytest.mark.parametrize("variant", ["full_rd", "full_rd_352", "mask_no_hpel"])
+def test_motion_search_without_planes(gpu, variant):
+    """The non-turbo search and a turbo mask without the plane bits: sub-pel
+    costs from EIGHTTAP block predictions."""
+    if variant == "full_rd":
+        w, h, seed, kind, kw = 136, 104, 17, "inter", {"turbo": 0, "multi_ref": 1}
+    elif variant == "full_rd_352":
+        w, h, seed, kind, kw = 352, 288, 4, "inter", {"turbo": 0, "multi_ref": 1}
+    else:
+        w, h, seed, kind, kw = 200, 136, 9, "leaf", {"knob_turbo_mask": 0x47CBF1 & ~0x30}
+    print(f"\n[me] no planes {variant}", flush=True)
+    turbo = kw.pop("turbo", 1)
+    cfg = mav1hip.default_config(w, h, turbo=turbo, qindex=125, **kw)
+    fr = mav1hip.turbo_frame(kind, 1) if kind != "inter" else mav1hip.frame_info(order_hint=1, gold_differs=1)
+    fp, _ = mav1hip.setup_params(cfg, fr)
+    geo, frames = libs.make_sequence(w, h, 3, seed, n_objects=5, max_vel=10)
+    ctx = mav1hip.Context(cfg)
+    try:
+        mv, mvg = ctx.motion_search(fp, frames[1], frames[0], frames[2])
+    finally:
+        ctx.close()
+    mo, go = libs.Impl("oracle").motion_search(fp, frames[1], frames[0], frames[2])
+    assert np.array_equal(mv, mo)
+    if fp.golden:
+        assert np.array_equal(mvg, go)

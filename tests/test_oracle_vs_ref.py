@@ -128,3 +128,28 @@ def test_predict_intra_all_modes():
                     assert np.array_equal(a, b), f"mode {mode} delta {d} size {lw},{lh} args {args}"
                     n += 1
     assert n > 1000
+
+
+@pytest.mark.parametrize("variant", ["full_rd", "full_rd_352", "mask_no_hpel"])
+def test_motion_search_without_quarter_pel_planes(variant):
+    """The non-turbo search (level 0 on a step-4 grid over +-32, +-3
+    refinement, +-4 below, sub-pel from the EIGHTTAP prediction of the
+    block) and a turbo mask without the plane bits (16, 32)."""
+    if variant == "full_rd":
+        w, h, seed = 136, 104, 17
+        fp = _frame_params(w, h, "inter", turbo=0, multi_ref=1)
+    elif variant == "full_rd_352":
+        w, h, seed = 352, 288, 4
+        fp = _frame_params(w, h, "inter", turbo=0, multi_ref=1)
+    else:
+        w, h, seed = 200, 136, 9
+        fp = _frame_params(w, h, "leaf", knob_turbo_mask=0x47CBF1 & ~0x30)
+    geo, frames = libs.make_sequence(w, h, 3, seed, n_objects=5, max_vel=10)
+    assert not (fp.turbo & 0x10)
+    o, r = libs.Impl("oracle"), libs.Impl("ref")
+    mo, go = o.motion_search(fp, frames[1], frames[0], frames[2])
+    mr, gr = r.motion_search(fp, frames[1], frames[0], frames[2])
+    assert np.array_equal(mo, mr)
+    if fp.golden:
+        assert np.array_equal(go, gr)
+    assert np.any(mo % 4 != 0), "no quarter-pel vector: the sub-pel path was not exercised"
