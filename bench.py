@@ -87,8 +87,10 @@ def algorithmic_bytes(kernel, fp, nref):
         return F + 2 * F + mi  # turbo: two parameter sets written
     if kernel == "cdef_select":
         return 2 * F + F + F
-    if kernel == "lr_select":
-        return 3 * F + 48 * sum(fp.lr_unit_rows[p] * fp.lr_unit_cols[p] for p in range(3))
+    if kernel in ("lr_fit", "lr_path"):
+        return 3 * F
+    if kernel == "lr_chain":
+        return 48 * sum(fp.lr_unit_rows[p] * fp.lr_unit_cols[p] for p in range(3))
     if kernel == "lr_apply":
         return 3 * F
     return 0
@@ -178,7 +180,7 @@ def main():
     ctx.timing(False)
     kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3", "analyze_level0",
                "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table", "edge_types", "recon_inter",
-               "recon_intra", "filter_info", "deblock", "cdef", "cdef_select", "lr_select", "lr_apply"]
+               "recon_intra", "filter_info", "deblock", "cdef", "cdef_select", "lr_fit", "lr_path", "lr_chain", "lr_apply"]
     stats = {k: ctx.timing_get(k) for k in kernels}
     fps = args.steps / elapsed
 

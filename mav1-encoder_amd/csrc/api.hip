@@ -51,6 +51,8 @@ struct mav1hip_ctx {
   void *d_lrstats;
   int *d_lrref, *d_lrtype, *d_lrreruns;
   uint8_t *d_decgrid;
+  int *d_rwork;
+  int num_cu;
   size_t decgrid_bytes;
   int lr_units;
   long entry_bytes;
@@ -165,8 +167,11 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
     return nullptr;
   }
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess)
+  c->num_cu = 256;
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
     snprintf(c->devname, sizeof c->devname, "%s (%s)", prop.name, prop.gcnArchName);
+    if (prop.multiProcessorCount > 0) c->num_cu = prop.multiProcessorCount;
+  }
   long W = c->cfg.width, H = c->cfg.height;
   /* entries start 256-byte aligned (the plane build stores 32-bit words) */
   c->entry_bytes = (16 * W * H + ((W + 3) >> 2) * ((H + 3) >> 2) + 255) & ~255L;
@@ -194,6 +199,7 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   rc |= dmalloc(c, (void **)&c->d_dirvar, sizeof(int) * (size_t)(c->mi_rows >> 1) * (c->mi_cols >> 1));
   rc |= dmalloc(c, (void **)&c->d_choice, (size_t)c->num_sbs);
   c->decgrid_bytes = (size_t)((c->mi_rows + 1) >> 1) * ((c->mi_cols + 1) >> 1) * 8;
+  rc |= dmalloc(c, (void **)&c->d_rwork, sizeof(int) * (2 + (size_t)c->num_sbs * MAV1_MAX_BLOCKS_PER_SB));
   rc |= dmalloc(c, (void **)&c->d_decgrid, c->decgrid_bytes);
   {
     /* restoration units of 64x64 per plane (count_units_in_frame) */
@@ -280,6 +286,7 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipFree(c->d_dirvar);
   (void)hipFree(c->d_choice);
   (void)hipFree(c->d_decgrid);
+  (void)hipFree(c->d_rwork);
   (void)hipFree(c->d_lrunits);
   (void)hipFree(c->d_lrstats);
   (void)hipFree(c->d_lrref);
@@ -466,7 +473,7 @@ int mav1hip_launch_recon(mav1hip_ctx *c) {
   const Mav1FrameParams &fp = c->fp;
   int rc = timed(c, "recon_inter", [&] {
     return launch_recon_inter(c->stream, fp, c->d_blocks, c->d_sbs, c->b_src, c->d_rec, c->b_ref, c->b_gold,
-                              c->d_coefs, c->d_mats);
+                              c->d_coefs, c->d_mats, c->d_rwork, 2 * c->num_cu);
   });
   if (rc) return rc;
   const int steps = recon_intra_steps(fp);
@@ -539,9 +546,17 @@ int mav1hip_launch_lr_select(mav1hip_ctx *c, int32_t lr_test) {
   if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "lr select: no frame parameters set");
   int rc = check_lr_params(c, c->fp);
   if (rc) return rc;
-  return timed(c, "lr_select", [&] {
-    return launch_lr_select(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrref, c->d_lrunits,
-                            c->d_lrtype, lr_test != 0, c->d_lrreruns, c->lr_units);
+  if ((rc = timed(c, "lr_fit", [&] {
+         return launch_lr_fit(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrref, c->lr_units);
+       })))
+    return rc;
+  if ((rc = timed(c, "lr_path", [&] {
+         return launch_lr_path(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrref, c->lr_units);
+       })))
+    return rc;
+  return timed(c, "lr_chain", [&] {
+    return launch_lr_chain(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrunits, c->d_lrtype,
+                           lr_test != 0, c->d_lrreruns);
   });
 }
 

@@ -316,6 +316,12 @@ DEVI int dq_denom(int tx) {
   return 1;
 }
 
+/* The transform working arrays live in LDS; the stage functions receive
+ * them as generic pointers and address them through 32-bit LDS pointers. */
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+DEVI lds_i32 *lds(int32_t *p) { return (lds_i32 *)p; }
+DEVI const lds_i32 *lds(const int32_t *p) { return (const lds_i32 *)p; }
+
 /* The threads that code one transform block: a whole workgroup (wg) or a
  * single wave. A wave's lanes run in lockstep, so ordering its LDS accesses
  * needs only a compiler fence; a workgroup needs a barrier. red: LDS space
@@ -374,9 +380,11 @@ DEVI void t_1d(int32_t *T, int r) {
 }
 
 template <int KIND, int N>
-__device__ __attribute__((noinline)) void itx_rows(const int32_t *dq, int qs, int32_t *res, int rs, int h, int rect,
+__device__ __attribute__((noinline)) void itx_rows(const int32_t *dq_, int qs, int32_t *res_, int rs, int h, int rect,
                                                     int rowShift, int tid, int nt) {
   constexpr int W = 1 << N;
+  const lds_i32 *dq = lds(dq_);
+  lds_i32 *res = lds(res_);
   for (int i = tid; i < h; i += nt) {
     int32_t T[W];
     int nz = 0;
@@ -397,8 +405,9 @@ __device__ __attribute__((noinline)) void itx_rows(const int32_t *dq, int qs, in
   }
 }
 template <int KIND, int N>
-__device__ __attribute__((noinline)) void itx_cols(int32_t *res, int rs, int w, int tid, int nt) {
+__device__ __attribute__((noinline)) void itx_cols(int32_t *res_, int rs, int w, int tid, int nt) {
   constexpr int H = 1 << N;
+  lds_i32 *res = lds(res_);
   for (int j = tid; j < w; j += nt) {
     int32_t T[H];
 #pragma unroll
@@ -450,14 +459,18 @@ __device__ __attribute__((noinline)) void tx_inverse(const TxCtx &c, const int32
  * h x w (row stride rs) -> coef Min(32,h) x Min(32,w) (row stride tw); tmp
  * holds h x Min(32,w) (row stride tw). One thread per output in both
  * passes; the sums of a fixed length are unrolled by up to 16 so the matrix
- * loads issue back to back. */
+ * loads issue back to back. smats (may be NULL): the packed short matrices
+ * in LDS, used for lengths up to 16. */
 template <int W>
-DEVI void fwd_rows(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int h, int ltw,
+DEVI void fwd_rows(const TxCtx &c, const int32_t *resid_, int rs, int32_t *tmp_, int h, int ltw,
                    const int32_t *__restrict__ Mh) {
   const int tw = 1 << ltw;
+  const lds_i32 *resid = lds(resid_);
+  lds_i32 *tmp = lds(tmp_);
   for (int k = c.tid; k < h * tw; k += c.nt) {
     const int i = k >> ltw, kk = k & (tw - 1);
-    const int32_t *rr = resid + i * rs, *m = Mh + kk * W;
+    const lds_i32 *rr = resid + i * rs;
+    const int32_t *m = Mh + kk * W;
     long long s = 0;
 #pragma unroll 16
     for (int j = 0; j < W; j++) s += (long long)rr[j] * m[j];
@@ -465,9 +478,11 @@ DEVI void fwd_rows(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, i
   }
 }
 template <int H>
-DEVI void fwd_cols(const TxCtx &c, const int32_t *tmp, int32_t *coef, int th, int ltw, const int32_t *__restrict__ Mv,
-                   int sh, bool rect) {
+DEVI void fwd_cols(const TxCtx &c, const int32_t *tmp_, int32_t *coef_, int th, int ltw,
+                   const int32_t *__restrict__ Mv, int sh, bool rect) {
   const int tw = 1 << ltw;
+  const lds_i32 *tmp = lds(tmp_);
+  lds_i32 *coef = lds(coef_);
   for (int k = c.tid; k < th * tw; k += c.nt) {
     const int k2 = k >> ltw, kk = k & (tw - 1);
     const int32_t *m = Mv + k2 * H;
@@ -480,13 +495,19 @@ DEVI void fwd_cols(const TxCtx &c, const int32_t *tmp, int32_t *coef, int th, in
   }
 }
 
+/* The 4-, 8- and 16-point matrices of the three kinds packed back to back
+ * (TX_SMATS ints): kind k, length 1 << n at k * 336 + {0, 16, 80}[n - 2]. */
+#define TX_SMATS (3 * 336)
+DEVI int smat_off(int kind, int n) { return kind * 336 + (n == 2 ? 0 : n == 3 ? 16 : 80); }
+
 __device__ __attribute__((noinline)) void tx_forward(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp,
                                                       int32_t *coef, int txSz, int txType,
-                                                      const int32_t *__restrict__ mats) {
+                                                      const int32_t *__restrict__ mats, const int32_t *smats) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], h = 1 << lh;
   const int ltw = d_min(5, lw), th = d_min(32, h);
-  const int32_t *Mh = mats + (row_kind(txType) * 5 + lw - 2) * 4096;
-  const int32_t *Mv = mats + (col_kind(txType) * 5 + lh - 2) * 4096;
+  const int rk = row_kind(txType), ck = col_kind(txType);
+  const int32_t *Mh = (smats && lw <= 4) ? smats + smat_off(rk, lw) : mats + (rk * 5 + lw - 2) * 4096;
+  const int32_t *Mv = (smats && lh <= 4) ? smats + smat_off(ck, lh) : mats + (ck * 5 + lh - 2) * 4096;
   switch (lw) {
     case 2: fwd_rows<4>(c, resid, rs, tmp, h, ltw, Mh); break;
     case 3: fwd_rows<8>(c, resid, rs, tmp, h, ltw, Mh); break;
@@ -508,7 +529,9 @@ __device__ __attribute__((noinline)) void tx_forward(const TxCtx &c, const int32
 }
 
 /* Dead-zone quantiser (k_fwd.h:66-90) into the decoder layout (stride tw). */
-DEVI void tx_quantize(const TxCtx &c, const int32_t *coef, int32_t *q, int txSz, int dcQ, int acQ) {
+DEVI void tx_quantize(const TxCtx &c, const int32_t *coef_, int32_t *q_, int txSz, int dcQ, int acQ) {
+  const lds_i32 *coef = lds(coef_);
+  lds_i32 *q = lds(q_);
   const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
   const unsigned long long invDc = (1ull << 44) / (unsigned)dcQ + 1, invAc = (1ull << 44) / (unsigned)acQ + 1;
   for (int k = c.tid; k < th * tw; k += c.nt) {
@@ -525,7 +548,9 @@ DEVI void tx_quantize(const TxCtx &c, const int32_t *coef, int32_t *q, int txSz,
 }
 
 /* Dequantisation (spec 7.12.3), output row stride qs. */
-DEVI void tx_dequant(const TxCtx &c, const int32_t *q, int32_t *dq, int qs, int txSz, int dcQ, int acQ) {
+DEVI void tx_dequant(const TxCtx &c, const int32_t *q_, int32_t *dq_, int qs, int txSz, int dcQ, int acQ) {
+  const lds_i32 *q = lds(q_);
+  lds_i32 *dq = lds(dq_);
   const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
   for (int k = c.tid; k < th * tw; k += c.nt) {
     int i = k / tw, j = k - i * tw;

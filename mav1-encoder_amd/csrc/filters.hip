@@ -412,7 +412,7 @@ DEVI int lr_box(const Mav1FrameParams &fp, int p, const LrBlk &b, const uint8_t 
       long long pv = (long long)sq * n - (long long)sum * sum;
       if (pv < 0) pv = 0;
       const long long z = (pv * s + (1 << (SGRPROJ_MTABLE_BITS - 1))) >> SGRPROJ_MTABLE_BITS;
-      const int a2 = z >= 255 ? 256 : z == 0 ? 1 : (int)(((z << SGRPROJ_SGR_BITS) + z / 2) / (z + 1));
+      const int zi = (int)d_min(255, (int)z), a2 = z >= 255 ? 256 : z == 0 ? 1 : ((zi << SGRPROJ_SGR_BITS) + zi / 2) / (zi + 1);
       const long long b2 = (long long)((1 << SGRPROJ_SGR_BITS) - a2) * sum * one;
       A[i + 1][j + 1] = a2;
       B[i + 1][j + 1] = (int)((b2 + (1 << (SGRPROJ_RECIP_BITS - 1))) >> SGRPROJ_RECIP_BITS);

@@ -32,9 +32,11 @@ hipError_t launch_analyze_table(hipStream_t st, const Mav1FrameParams &fp, const
 /* Reconstruction: inter blocks (all at once), then the intra wavefront. */
 hipError_t launch_edge_types(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
                              uint8_t *ys, uint8_t *uvs);
+/* work: 2 + num_sbs * 64 ints of device scratch (the inter block list);
+ * workgroups: the persistent grid. */
 hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
                               const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold,
-                              int32_t *coefs, const int32_t *mats);
+                              int32_t *coefs, const int32_t *mats, int *work, int workgroups);
 int recon_intra_steps(const Mav1FrameParams &fp);
 hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, int t, Mav1BlockInfo *blocks,
                               const Mav1SbInfo *sbs, const uint8_t *src, uint8_t *rec, const uint8_t *ref,
@@ -56,8 +58,12 @@ hipError_t launch_decision_grid(hipStream_t st, const Mav1FrameParams &fp, const
 
 /* lrsearch.hip */
 size_t lr_stat_bytes();
-hipError_t launch_lr_select(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
-                            const uint8_t *dbk, void *stats, int *refW, Mav1LrUnit *units, int *lrType, int lrTest,
-                            int *reruns, int nunits);
+hipError_t launch_lr_fit(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
+                         const uint8_t *dbk, void *stats, int *refW, int nunits);
+hipError_t launch_lr_path(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
+                          const uint8_t *dbk, void *stats, const int *refW, int nunits);
+hipError_t launch_lr_chain(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
+                           const uint8_t *dbk, const void *stats, Mav1LrUnit *units, int *lrType, int lrTest,
+                           int *reruns);
 
 #endif

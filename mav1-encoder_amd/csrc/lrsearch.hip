@@ -220,7 +220,7 @@ DEVI void lr_box_block(const LrGeo &g, int s, int bx, int by, int w, int h, int 
       long long pv = (long long)sq * n - (long long)sum * sum;
       if (pv < 0) pv = 0;
       const long long z = (pv * sc + (1 << (SGRPROJ_MTABLE_BITS - 1))) >> SGRPROJ_MTABLE_BITS;
-      const int a2 = z >= 255 ? 256 : z == 0 ? 1 : (int)(((z << SGRPROJ_SGR_BITS) + z / 2) / (z + 1));
+      const int zi = (int)d_min(255, (int)z), a2 = z >= 255 ? 256 : z == 0 ? 1 : ((zi << SGRPROJ_SGR_BITS) + zi / 2) / (zi + 1);
       const long long b2 = (long long)((1 << SGRPROJ_SGR_BITS) - a2) * sum * one;
       A[i + 1][j + 1] = a2;
       B[i + 1][j + 1] = (int)((b2 + (1 << (SGRPROJ_RECIP_BITS - 1))) >> SGRPROJ_RECIP_BITS);
@@ -665,14 +665,23 @@ __global__ void k_lr_ref_init(int *__restrict__ refW, int nunits) {
 
 size_t lr_stat_bytes() { return sizeof(LrUnitStat); }
 
-hipError_t launch_lr_select(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
-                            const uint8_t *dbk, void *stats, int *refW, Mav1LrUnit *units, int *lrType, int lrTest,
-                            int *reruns, int nunits) {
-  LrUnitStat *s = (LrUnitStat *)stats;
-  hipLaunchKernelGGL(k_lr_fit, dim3(nunits), dim3(LR_T), 0, st, fp, src, cdef, dbk, s);
+hipError_t launch_lr_fit(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
+                         const uint8_t *dbk, void *stats, int *refW, int nunits) {
+  hipLaunchKernelGGL(k_lr_fit, dim3(nunits), dim3(LR_T), 0, st, fp, src, cdef, dbk, (LrUnitStat *)stats);
   hipLaunchKernelGGL(k_lr_ref_init, dim3((nunits * 6 + 255) / 256), dim3(256), 0, st, refW, nunits);
-  hipLaunchKernelGGL(k_lr_path, dim3(nunits), dim3(LR_T), 0, st, fp, src, cdef, dbk, s, (const int *)refW);
-  hipLaunchKernelGGL(k_lr_chain, dim3(3), dim3(LR_T), 0, st, fp, src, cdef, dbk, (const LrUnitStat *)s, units, lrType,
-                     lrTest, reruns);
+  return hipGetLastError();
+}
+
+hipError_t launch_lr_path(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
+                          const uint8_t *dbk, void *stats, const int *refW, int nunits) {
+  hipLaunchKernelGGL(k_lr_path, dim3(nunits), dim3(LR_T), 0, st, fp, src, cdef, dbk, (LrUnitStat *)stats, refW);
+  return hipGetLastError();
+}
+
+hipError_t launch_lr_chain(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
+                           const uint8_t *dbk, const void *stats, Mav1LrUnit *units, int *lrType, int lrTest,
+                           int *reruns) {
+  hipLaunchKernelGGL(k_lr_chain, dim3(3), dim3(LR_T), 0, st, fp, src, cdef, dbk, (const LrUnitStat *)stats, units,
+                     lrType, lrTest, reruns);
   return hipGetLastError();
 }

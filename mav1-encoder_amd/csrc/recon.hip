@@ -80,6 +80,8 @@ struct RLds {
   uint8_t bd[3 * BDD * BDD];
   uint8_t bdSave[BDD * BDD];
   uint8_t save[64 * 64];
+  uint8_t tsrc[64 * 64], tpred[64 * 64]; /* the transform block's source and prediction */
+  int32_t smats[TX_SMATS];
   int ival[4];
   Mav1BlockInfo blk;
 };
@@ -155,15 +157,17 @@ DEVI int uv_tx_size(int bs) {
 DEVI int tile_start(const Mav1FrameParams &fp, int miCol) { return ((miCol >> 4) / fp.tile_w_sb) * fp.tile_w_sb * 16; }
 
 /* find_eob (k_recon.h:50-54): 1 + last scan position with a non-zero level. */
-DEVI int tx_eob(const TxCtx &c, const int32_t *q, const int16_t *scan, int n) {
+DEVI int tx_eob(const TxCtx &c, const int32_t *q_, const int16_t *scan, int n) {
+  const lds_i32 *q = lds(q_);
   int m = 0;
   for (int k = c.tid; k < n; k += c.nt)
     if (q[scan[k]]) m = k + 1;
   return tx_max(c, m);
 }
 /* est_coef_bits (k_recon.h:39-48) */
-DEVI int tx_bits(const TxCtx &c, const int32_t *q, const int16_t *scan, int eob) {
+DEVI int tx_bits(const TxCtx &c, const int32_t *q_, const int16_t *scan, int eob) {
   if (!eob) return 8;
+  const lds_i32 *q = lds(q_);
   long long b = 0;
   for (int k = c.tid; k < eob; k += c.nt) {
     const int l = d_abs(q[scan[k]]);
@@ -174,7 +178,8 @@ DEVI int tx_bits(const TxCtx &c, const int32_t *q, const int16_t *scan, int eob)
 
 /* Residual of transform block (x, y), stored in the type's flipped order. */
 DEVI void tx_residual(const TxCtx &c, const uint8_t *spx, const uint8_t *px, int S, int x, int y, int lw, int lh, int t,
-                      int32_t *resid, int rs) {
+                      int32_t *resid_, int rs) {
+  lds_i32 *resid = lds(resid_);
   const int w = 1 << lw, h = 1 << lh, fud = flip_ud(t), flr = flip_lr(t);
   for (int k = c.tid; k < w * h; k += c.nt) {
     const int i = k >> lw, j = k & (w - 1);
@@ -184,7 +189,8 @@ DEVI void tx_residual(const TxCtx &c, const uint8_t *spx, const uint8_t *px, int
   tx_sync(c);
 }
 /* px += residual (inverse transform output in flipped order), clipped. */
-DEVI void tx_add(const TxCtx &c, uint8_t *px, int S, int x, int y, int lw, int lh, int t, const int32_t *res, int rs) {
+DEVI void tx_add(const TxCtx &c, uint8_t *px, int S, int x, int y, int lw, int lh, int t, const int32_t *res_, int rs) {
+  const lds_i32 *res = lds(res_);
   const int w = 1 << lw, h = 1 << lh, fud = flip_ud(t), flr = flip_lr(t);
   for (int k = c.tid; k < w * h; k += c.nt) {
     const int i = k >> lw, j = k & (w - 1);
@@ -204,7 +210,7 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
   const int dcQ = fp.dc_q[plane], acQ = fp.ac_q[plane];
   tx_residual(c, spx, px, S, x, y, lw, lh, t, resid, rs);
-  tx_forward(c, resid, rs, tmp, coef, txSz, t, mats);
+  tx_forward(c, resid, rs, tmp, coef, txSz, t, mats, gL.smats);
   tx_quantize(c, coef, q, txSz, dcQ, acQ);
   const int16_t *scan = get_scan(txSz, t);
   const int eob = tx_eob(c, q, scan, seg_eob(txSz));
@@ -213,6 +219,7 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
     return eob;
   }
   long long e = 0;
+  const lds_i32 *rl = lds(resid);
   if (eob > 0) {
     tx_dequant(c, q, dq, qs, txSz, dcQ, acQ);
     tx_inverse(c, dq, qs, resid, rs, txSz, t);
@@ -220,12 +227,12 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
     for (int k = c.tid; k < w * h; k += c.nt) {
       const int i = k >> lw, j = k & (w - 1);
       const int o = (y + (fud ? h - 1 - i : i)) * S + x + (flr ? w - 1 - j : j);
-      const int d = (int)spx[o] - d_clip1((int)px[o] + resid[i * rs + j]);
+      const int d = (int)spx[o] - d_clip1((int)px[o] + rl[i * rs + j]);
       e += d * d;
     }
   } else {
     for (int k = c.tid; k < w * h; k += c.nt) {
-      const int v = resid[(k >> lw) * rs + (k & (w - 1))];
+      const int v = rl[(k >> lw) * rs + (k & (w - 1))];
       e += v * v;
     }
   }
@@ -342,6 +349,13 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
   const bool direct = nc == 1 && !fp.tx_mode_select;
   long long bestCost = -1;
   int bestType = DCT_DCT, bestEob = 0;
+  /* the candidates read the block's source and prediction from LDS */
+  for (int k = tid; k < w * h; k += RT) {
+    const int o = (y + (k >> lw)) * S + x + (k & (w - 1));
+    L.tsrc[k] = spx[o];
+    L.tpred[k] = px[o];
+  }
+  __syncthreads();
   if (w <= 16 && h <= 16) {
     /* one candidate per wave */
     const int wave = tid >> 6;
@@ -351,8 +365,8 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
       if (ci < nc) {
         SmallTx &W = L.u.sm[wave];
         long long cost;
-        const int eob = tx_eval(c, fp, spx, px, S, x, y, txSz, plane, cands[ci], direct, W.resid, SRS, W.tmp, W.coef,
-                                W.q, W.dq, SQS, mats, &cost);
+        const int eob = tx_eval(c, fp, L.tsrc, L.tpred, w, 0, 0, txSz, plane, cands[ci], direct, W.resid, SRS, W.tmp,
+                                W.coef, W.q, W.dq, SQS, mats, &cost);
         if (c.tid == 0) {
           L.candCost[ci] = cost;
           L.candEob[ci] = eob;
@@ -388,8 +402,8 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
     for (int ci = 0; ci < nc; ci++) {
       long long cost;
       int32_t *qo = direct ? B.qbest : B.qtry;
-      const int eob = tx_eval(c, fp, spx, px, S, x, y, txSz, plane, cands[ci], direct, B.resid, BRS, B.tmp, B.coef, qo,
-                              B.dq, BQS, mats, &cost);
+      const int eob = tx_eval(c, fp, L.tsrc, L.tpred, w, 0, 0, txSz, plane, cands[ci], direct, B.resid, BRS, B.tmp,
+                              B.coef, qo, B.dq, BQS, mats, &cost);
       if (bestCost < 0 || cost < bestCost) {
         bestCost = cost;
         bestType = eob > 0 ? cands[ci] : DCT_DCT;
@@ -871,21 +885,51 @@ NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ sr
   }
 }
 
-/* Inter phase: every inter block of the frame, one workgroup per block slot. */
-__global__ __launch_bounds__(RT) void k_recon_inter(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
-                                                    const Mav1SbInfo *__restrict__ sbs, const uint8_t *__restrict__ src,
-                                                    uint8_t *rec, const uint8_t *__restrict__ ref,
-                                                    const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
-                                                    const int32_t *__restrict__ mats) {
-  RLds &L = gL;
-  const int tid = threadIdx.x, sb = blockIdx.x >> 6, k = blockIdx.x & 63;
+DEVI void stage_smats(const int32_t *__restrict__ mats) {
+  for (int q = threadIdx.x; q < TX_SMATS; q += RT) {
+    const int kind = q / 336, r = q - kind * 336;
+    const int n = r < 16 ? 2 : r < 80 ? 3 : 4, idx = r - (n == 2 ? 0 : n == 3 ? 16 : 80);
+    gL.smats[q] = mats[(kind * 5 + n - 2) * 4096 + idx];
+  }
+}
+
+/* The inter blocks of the frame as a work list: work[0] = count, work[1] =
+ * the next entry to take, work[2 + i] = block index. */
+__global__ void k_recon_list_reset(int *work) {
+  if (threadIdx.x < 2) work[threadIdx.x] = 0;
+}
+__global__ __launch_bounds__(64) void k_recon_list(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
+                                                   const Mav1SbInfo *__restrict__ sbs, int *work) {
+  const int sb = blockIdx.x, k = threadIdx.x;
   if (k >= sbs[sb].num_blocks) return;
-  Mav1BlockInfo *g = blocks + sbs[sb].first_block + k;
-  if (!g->is_inter) return;
-  if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)g)[tid];
-  __syncthreads();
-  recon_block(fp, src, rec, ref, gold, coefs, mats, false, false);
-  if (tid < 27) ((int32_t *)g)[tid] = ((const int32_t *)&L.blk)[tid];
+  const int bi = sbs[sb].first_block + k;
+  if (blocks[bi].is_inter) work[2 + atomicAdd(&work[0], 1)] = bi;
+}
+
+/* Inter phase: persistent workgroups take the inter blocks off the list
+ * one at a time until it is empty. */
+__global__ __launch_bounds__(RT) void k_recon_inter(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
+                                                    const uint8_t *__restrict__ src, uint8_t *rec,
+                                                    const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold,
+                                                    int32_t *__restrict__ coefs, const int32_t *__restrict__ mats,
+                                                    int *work) {
+  RLds &L = gL;
+  const int tid = threadIdx.x;
+  stage_smats(mats);
+  const int count = work[0];
+  for (;;) {
+    if (tid == 0) L.ival[2] = atomicAdd(&work[1], 1);
+    __syncthreads();
+    const int idx = L.ival[2];
+    __syncthreads();
+    if (idx >= count) break;
+    Mav1BlockInfo *g = blocks + work[2 + idx];
+    if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)g)[tid];
+    __syncthreads();
+    recon_block(fp, src, rec, ref, gold, coefs, mats, false, false);
+    if (tid < 27) ((int32_t *)g)[tid] = ((const int32_t *)&L.blk)[tid];
+    __syncthreads();
+  }
 }
 
 /* Intra phase: superblocks of anti-diagonal t, one workgroup each; grid =
@@ -903,6 +947,7 @@ __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Ma
   if (cl < 0 || cl >= fp.tile_w_sb || c0 + cl >= fp.sb_cols || r >= fp.sb_rows) return;
   const int sbIdx = r * fp.sb_cols + c0 + cl;
   const Mav1SbInfo &S = sbs[sbIdx];
+  stage_smats(mats);
   if (tid == 0) L.ival[1] = 0;
   __syncthreads();
   for (int i = tid; i < S.num_blocks; i += RT)
@@ -969,10 +1014,12 @@ hipError_t launch_edge_types(hipStream_t st, const Mav1FrameParams &fp, Mav1Bloc
 
 hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
                               const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold,
-                              int32_t *coefs, const int32_t *mats) {
+                              int32_t *coefs, const int32_t *mats, int *work, int workgroups) {
   const int nsb = fp.sb_cols * fp.sb_rows;
-  hipLaunchKernelGGL(k_recon_inter, dim3(nsb * 64), dim3(RT), 0, st, fp, blocks, sbs, src, rec, ref,
-                     gold, coefs, mats);
+  hipLaunchKernelGGL(k_recon_list_reset, dim3(1), dim3(64), 0, st, work);
+  hipLaunchKernelGGL(k_recon_list, dim3(nsb), dim3(64), 0, st, fp, (const Mav1BlockInfo *)blocks, sbs, work);
+  hipLaunchKernelGGL(k_recon_inter, dim3(workgroups), dim3(RT), 0, st, fp, blocks, src, rec, ref, gold, coefs, mats,
+                     work);
   return hipGetLastError();
 }
 
