@@ -380,7 +380,7 @@ DEVI void t_1d(int32_t *T, int r) {
 }
 
 template <int KIND, int N>
-__device__ __attribute__((noinline)) void itx_rows(const int32_t *dq_, int qs, int32_t *res_, int rs, int h, int rect,
+DEVI void itx_rows(const int32_t *dq_, int qs, int32_t *res_, int rs, int h, int rect,
                                                     int rowShift, int tid, int nt) {
   constexpr int W = 1 << N;
   const lds_i32 *dq = lds(dq_);
@@ -405,7 +405,7 @@ __device__ __attribute__((noinline)) void itx_rows(const int32_t *dq_, int qs, i
   }
 }
 template <int KIND, int N>
-__device__ __attribute__((noinline)) void itx_cols(int32_t *res_, int rs, int w, int tid, int nt) {
+DEVI void itx_cols(int32_t *res_, int rs, int w, int tid, int nt) {
   constexpr int H = 1 << N;
   lds_i32 *res = lds(res_);
   for (int j = tid; j < w; j += nt) {
@@ -444,8 +444,7 @@ __device__ __attribute__((noinline)) void itx_cols(int32_t *res_, int rs, int w,
     }                                                                       \
   } while (0)
 
-__device__ __attribute__((noinline)) void tx_inverse(const TxCtx &c, const int32_t *dq, int qs, int32_t *res, int rs,
-                                                      int txSz, int txType) {
+DEVI void tx_inverse(const TxCtx &c, const int32_t *dq, int qs, int32_t *res, int rs, int txSz, int txType) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
   const int rect = (lw - lh == 1) || (lh - lw == 1);
   const int rk = row_kind(txType), ck = col_kind(txType);
@@ -461,31 +460,29 @@ __device__ __attribute__((noinline)) void tx_inverse(const TxCtx &c, const int32
  * passes; the sums of a fixed length are unrolled by up to 16 so the matrix
  * loads issue back to back. smats (may be NULL): the packed short matrices
  * in LDS, used for lengths up to 16. */
-template <int W>
-DEVI void fwd_rows(const TxCtx &c, const int32_t *resid_, int rs, int32_t *tmp_, int h, int ltw,
-                   const int32_t *__restrict__ Mh) {
+template <int W, class MP>
+DEVI void fwd_rows(const TxCtx &c, const int32_t *resid_, int rs, int32_t *tmp_, int h, int ltw, MP Mh) {
   const int tw = 1 << ltw;
   const lds_i32 *resid = lds(resid_);
   lds_i32 *tmp = lds(tmp_);
   for (int k = c.tid; k < h * tw; k += c.nt) {
     const int i = k >> ltw, kk = k & (tw - 1);
     const lds_i32 *rr = resid + i * rs;
-    const int32_t *m = Mh + kk * W;
+    const MP m = Mh + kk * W;
     long long s = 0;
 #pragma unroll 16
     for (int j = 0; j < W; j++) s += (long long)rr[j] * m[j];
     tmp[k] = (int32_t)((s + 32768) >> 16);
   }
 }
-template <int H>
-DEVI void fwd_cols(const TxCtx &c, const int32_t *tmp_, int32_t *coef_, int th, int ltw,
-                   const int32_t *__restrict__ Mv, int sh, bool rect) {
+template <int H, class MP>
+DEVI void fwd_cols(const TxCtx &c, const int32_t *tmp_, int32_t *coef_, int th, int ltw, MP Mv, int sh, bool rect) {
   const int tw = 1 << ltw;
   const lds_i32 *tmp = lds(tmp_);
   lds_i32 *coef = lds(coef_);
   for (int k = c.tid; k < th * tw; k += c.nt) {
     const int k2 = k >> ltw, kk = k & (tw - 1);
-    const int32_t *m = Mv + k2 * H;
+    const MP m = Mv + k2 * H;
     long long s = 0;
 #pragma unroll 16
     for (int i = 0; i < H; i++) s += (long long)m[i] * tmp[i * tw + kk];
@@ -500,14 +497,11 @@ DEVI void fwd_cols(const TxCtx &c, const int32_t *tmp_, int32_t *coef_, int th, 
 #define TX_SMATS (3 * 336)
 DEVI int smat_off(int kind, int n) { return kind * 336 + (n == 2 ? 0 : n == 3 ? 16 : 80); }
 
-__device__ __attribute__((noinline)) void tx_forward(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp,
-                                                      int32_t *coef, int txSz, int txType,
-                                                      const int32_t *__restrict__ mats, const int32_t *smats) {
+template <class MP>
+DEVI void tx_forward_m(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int32_t *coef, int txSz, MP Mh,
+                       MP Mv) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], h = 1 << lh;
   const int ltw = d_min(5, lw), th = d_min(32, h);
-  const int rk = row_kind(txType), ck = col_kind(txType);
-  const int32_t *Mh = (smats && lw <= 4) ? smats + smat_off(rk, lw) : mats + (rk * 5 + lw - 2) * 4096;
-  const int32_t *Mv = (smats && lh <= 4) ? smats + smat_off(ck, lh) : mats + (ck * 5 + lh - 2) * 4096;
   switch (lw) {
     case 2: fwd_rows<4>(c, resid, rs, tmp, h, ltw, Mh); break;
     case 3: fwd_rows<8>(c, resid, rs, tmp, h, ltw, Mh); break;
@@ -526,6 +520,18 @@ __device__ __attribute__((noinline)) void tx_forward(const TxCtx &c, const int32
     default: fwd_cols<64>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
   }
   tx_sync(c);
+}
+
+/* smats (may be NULL): the packed short matrices in LDS, used when both
+ * lengths are at most 16; otherwise the full matrices in global memory. */
+DEVI void tx_forward(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int32_t *coef, int txSz, int txType,
+                     const int32_t *__restrict__ mats, const int32_t *smats) {
+  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
+  const int rk = row_kind(txType), ck = col_kind(txType);
+  if (smats && lw <= 4 && lh <= 4)
+    tx_forward_m(c, resid, rs, tmp, coef, txSz, lds(smats) + smat_off(rk, lw), lds(smats) + smat_off(ck, lh));
+  else
+    tx_forward_m(c, resid, rs, tmp, coef, txSz, mats + (rk * 5 + lw - 2) * 4096, mats + (ck * 5 + lh - 2) * 4096);
 }
 
 /* Dead-zone quantiser (k_fwd.h:66-90) into the decoder layout (stride tw). */

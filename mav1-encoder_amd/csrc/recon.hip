@@ -89,7 +89,7 @@ struct RLds {
 /* The working set of the workgroup that reconstructs a block / superblock.
  * File scope so that the non-inlined stage functions address it as LDS. */
 __shared__ RLds gL;
-#define NOINL __device__ __attribute__((noinline))
+#define NOINL __device__ __forceinline__
 
 DEVI TxCtx wg_ctx() {
   TxCtx c;
@@ -347,6 +347,21 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
     return eob;
   }
   const bool direct = nc == 1 && !fp.tx_mode_select;
+  /* up to 16x16: one candidate per wave; larger: the whole workgroup per
+   * candidate. One code path for both keeps a single inlined copy of the
+   * transform stages. */
+  const bool small = w <= 16 && h <= 16;
+  const int wave = tid >> 6, per = small ? RW : 1;
+  const TxCtx c = small ? wave_ctx() : wg_ctx();
+  int32_t *resid, *tmp, *coef, *qtry, *dq, *qbest;
+  int rs, qs;
+  if (small) {
+    SmallTx &W = L.u.sm[wave];
+    resid = W.resid, tmp = W.tmp, coef = W.coef, qtry = W.q, dq = W.dq, qbest = L.qbest, rs = SRS, qs = SQS;
+  } else {
+    BigTx &B = L.u.big;
+    resid = B.resid, tmp = B.tmp, coef = B.coef, qtry = B.qtry, dq = B.dq, qbest = B.qbest, rs = BRS, qs = BQS;
+  }
   long long bestCost = -1;
   int bestType = DCT_DCT, bestEob = 0;
   /* the candidates read the block's source and prediction from LDS */
@@ -356,71 +371,40 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
     L.tpred[k] = px[o];
   }
   __syncthreads();
-  if (w <= 16 && h <= 16) {
-    /* one candidate per wave */
-    const int wave = tid >> 6;
-    const TxCtx c = wave_ctx();
-    for (int r0 = 0; r0 < nc; r0 += RW) {
-      const int ci = r0 + wave;
-      if (ci < nc) {
-        SmallTx &W = L.u.sm[wave];
-        long long cost;
-        const int eob = tx_eval(c, fp, L.tsrc, L.tpred, w, 0, 0, txSz, plane, cands[ci], direct, W.resid, SRS, W.tmp,
-                                W.coef, W.q, W.dq, SQS, mats, &cost);
-        if (c.tid == 0) {
-          L.candCost[ci] = cost;
-          L.candEob[ci] = eob;
-        }
+  for (int r0 = 0; r0 < nc; r0 += per) {
+    const int ci = r0 + (small ? wave : 0);
+    if (ci < nc) {
+      long long cost;
+      const int eob = tx_eval(c, fp, L.tsrc, L.tpred, w, 0, 0, txSz, plane, cands[ci], direct, resid, rs, tmp, coef,
+                              qtry, dq, qs, mats, &cost);
+      if (c.tid == 0) {
+        L.candCost[ci] = cost;
+        L.candEob[ci] = eob;
       }
-      __syncthreads();
-      int pick = -1;
-      for (int k = r0; k < d_min(nc, r0 + RW); k++)
-        if (bestCost < 0 || L.candCost[k] < bestCost) {
-          bestCost = L.candCost[k];
-          pick = k;
-        }
-      if (pick >= 0) {
-        bestEob = L.candEob[pick];
-        bestType = bestEob > 0 ? cands[pick] : DCT_DCT;
-        const int32_t *src = L.u.sm[pick - r0].q;
-        for (int k = tid; k < tw * th; k += RT) L.qbest[k] = src[k];
-      }
-      __syncthreads();
-    }
-    for (int k = tid; k < tw * th; k += RT) coefOut[k] = L.qbest[k];
-    if (bestEob > 0 && wave == 0) {
-      SmallTx &W = L.u.sm[0];
-      tx_dequant(c, L.qbest, W.dq, SQS, txSz, dcQ, acQ);
-      tx_inverse(c, W.dq, SQS, W.resid, SRS, txSz, bestType);
-      tx_add(c, px, S, x, y, lw, lh, bestType, W.resid, SRS);
     }
     __syncthreads();
-  } else {
-    /* the whole workgroup per candidate */
-    const TxCtx c = wg_ctx();
-    BigTx &B = L.u.big;
-    for (int ci = 0; ci < nc; ci++) {
-      long long cost;
-      int32_t *qo = direct ? B.qbest : B.qtry;
-      const int eob = tx_eval(c, fp, L.tsrc, L.tpred, w, 0, 0, txSz, plane, cands[ci], direct, B.resid, BRS, B.tmp,
-                              B.coef, qo, B.dq, BQS, mats, &cost);
-      if (bestCost < 0 || cost < bestCost) {
-        bestCost = cost;
-        bestType = eob > 0 ? cands[ci] : DCT_DCT;
-        bestEob = eob;
-        if (!direct)
-          for (int k = tid; k < tw * th; k += RT) B.qbest[k] = B.qtry[k];
+    /* first minimum in candidate order */
+    int pick = -1;
+    for (int k = r0; k < d_min(nc, r0 + per); k++)
+      if (bestCost < 0 || L.candCost[k] < bestCost) {
+        bestCost = L.candCost[k];
+        pick = k;
       }
-      __syncthreads();
-    }
-    for (int k = tid; k < tw * th; k += RT) coefOut[k] = B.qbest[k];
-    if (bestEob > 0) {
-      tx_dequant(c, B.qbest, B.dq, BQS, txSz, dcQ, acQ);
-      tx_inverse(c, B.dq, BQS, B.resid, BRS, txSz, bestType);
-      tx_add(c, px, S, x, y, lw, lh, bestType, B.resid, BRS);
+    if (pick >= 0) {
+      bestEob = L.candEob[pick];
+      bestType = bestEob > 0 ? cands[pick] : DCT_DCT;
+      const int32_t *from = small ? L.u.sm[pick - r0].q : L.u.big.qtry;
+      for (int k = tid; k < tw * th; k += RT) qbest[k] = from[k];
     }
     __syncthreads();
   }
+  for (int k = tid; k < tw * th; k += RT) coefOut[k] = qbest[k];
+  if (bestEob > 0 && (!small || wave == 0)) {
+    tx_dequant(c, qbest, dq, qs, txSz, dcQ, acQ);
+    tx_inverse(c, dq, qs, resid, rs, txSz, bestType);
+    tx_add(c, px, S, x, y, lw, lh, bestType, resid, rs);
+  }
+  __syncthreads();
   *outType = bestType;
   *outCost = bestCost;
   return bestEob;
@@ -635,13 +619,11 @@ NOINL long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict_
                            ((fp.mi_rows * 4) >> sub) - 1, fp.enable_intra_edge_filter,
                            (b.interp_filter >> (plane == 0 ? 4 : 5)) & 1);
           if (isCfl) {
-            int a;
-            if (txIdx > 0)
-              a = cfl_tx(rec + fp.offset[0], fp.stride[0], sx, sy, maxLW, maxLH, lw, lh, px, S, spx,
-                         plane == 1 ? b.cfl_alpha_u : b.cfl_alpha_v, 0, 0);
-            else
-              a = cfl_tx(rec + fp.offset[0], fp.stride[0], sx, sy, maxLW, maxLH, lw, lh, px, S, spx, 0, 1,
-                         plane == 2 && b.cfl_alpha_u == 0);
+            /* the first transform block searches the alpha, the others reuse it */
+            const int search = txIdx == 0;
+            const int a = cfl_tx(rec + fp.offset[0], fp.stride[0], sx, sy, maxLW, maxLH, lw, lh, px, S, spx,
+                                 search ? 0 : plane == 1 ? b.cfl_alpha_u : b.cfl_alpha_v, search,
+                                 search && plane == 2 && b.cfl_alpha_u == 0);
             if (txIdx == 0 && tid == 0) {
               if (plane == 1) b.cfl_alpha_u = a;
               else b.cfl_alpha_v = a;
@@ -842,24 +824,17 @@ NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ sr
                             L.u.pr.mid[0]);
       }
     }
-    if (fp.lossless) {
-      code_plane(fp, src, rec, coefs, mats, plane, TX_4X4, maxLW, maxLH, useBd, markOnly);
-      continue;
-    }
-    if (plane > 0) {
-      code_plane(fp, src, rec, coefs, mats, plane, uv_tx_size(bs), maxLW, maxLH, useBd, markOnly);
-      continue;
-    }
-    const int maxDepth = (!b.is_inter && fp.tx_mode_select) ? d_min(MAX_TX_DEPTH, Max_Tx_Depth[bs]) : 0;
-    if (maxDepth == 0) {
-      code_plane(fp, src, rec, coefs, mats, 0, Max_Tx_Size_Rect[bs], maxLW, maxLH, useBd, markOnly);
-      continue;
-    }
-    /* tx_depth RD search (intra only, so never markOnly) */
+    /* transform size: 4x4 when lossless, the chroma size, or the luma size
+     * of the best tx_depth (RD search of intra luma only, never markOnly) */
+    const int baseTx = fp.lossless ? TX_4X4 : plane > 0 ? uv_tx_size(bs) : Max_Tx_Size_Rect[bs];
+    const int maxDepth =
+        (plane == 0 && !fp.lossless && !b.is_inter && fp.tx_mode_select) ? d_min(MAX_TX_DEPTH, Max_Tx_Depth[bs]) : 0;
     uint8_t *px = rec + fp.offset[0] + baseY * S + baseX;
-    for (int k = tid; k < w * h; k += RT) L.save[k] = px[(k / w) * S + (k % w)];
-    for (int k = tid; k < BDD * BDD; k += RT) L.bdSave[k] = L.bd[k];
-    __syncthreads();
+    if (maxDepth > 0) {
+      for (int k = tid; k < w * h; k += RT) L.save[k] = px[(k / w) * S + (k % w)];
+      for (int k = tid; k < BDD * BDD; k += RT) L.bdSave[k] = L.bd[k];
+      __syncthreads();
+    }
     long long best = -1;
     int bestD = 0;
     for (int d = 0; d <= maxDepth + 1; d++) {
@@ -870,17 +845,17 @@ NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ sr
         for (int k = tid; k < BDD * BDD; k += RT) L.bd[k] = L.bdSave[k];
         __syncthreads();
       }
-      int t = Max_Tx_Size_Rect[bs];
+      int t = baseTx;
       for (int k = 0; k < depth; k++) t = Split_Tx_Size[t];
-      const long long c =
-          code_plane(fp, src, rec, coefs, mats, 0, t, maxLW, maxLH, useBd, false) + (long long)fp.lambda * 8 * depth;
+      const long long c = code_plane(fp, src, rec, coefs, mats, plane, t, maxLW, maxLH, useBd, markOnly) +
+                          (long long)fp.lambda * 8 * depth;
       if (d <= maxDepth && (best < 0 || c < best)) {
         best = c;
         bestD = depth;
       }
       if (d == maxDepth + 1) break;
     }
-    if (tid == 0) b.tx_depth = bestD;
+    if (plane == 0 && tid == 0 && !markOnly) b.tx_depth = bestD;
     __syncthreads();
   }
 }
