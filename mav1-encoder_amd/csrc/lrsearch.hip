@@ -53,6 +53,7 @@ struct LrLds {
   long long red[LR_T / 64 * 6];
   double acc[12];
   int itmp[16];
+  int xtab[256]; /* a of the self-guided filter by z (7.17.3) */
 };
 __shared__ LrLds gLr;
 
@@ -202,44 +203,109 @@ DEVI int lr_sgr_bits(int set, const int *xqd, const int *refX) {
 DEVI long long lr_rd(const Mav1FrameParams &fp, long long sse, int bits) { return sse * 16 + (long long)fp.lambda * bits; }
 
 /* ---- self-guided filter of one 4x4 block (7.17.3) from the staged samples ---- */
-DEVI void lr_box_block(const LrGeo &g, int s, int bx, int by, int w, int h, int set, int pass, int *F) {
-  const int r = SGR_PARAM(set, pass * 2);
-  const int eps = SGR_PARAM(set, pass * 2 + 1), n = (2 * r + 1) * (2 * r + 1), n2e = n * n * eps;
-  const int sc = ((1 << SGRPROJ_MTABLE_BITS) + n2e / 2) / n2e;
-  const int one = ((1 << SGRPROJ_RECIP_BITS) + n / 2) / n;
-  int A[6][6], B[6][6];
-  for (int i = -1; i <= h; i++)
-    for (int j = -1; j <= w; j++) {
-      int sq = 0, sum = 0;
-      for (int dy = -r; dy <= r; dy++)
-        for (int dx = -r; dx <= r; dx++) {
-          const int c = lr_img(g, s, bx + j + dx, by + i + dy);
-          sq += c * c;
-          sum += c;
-        }
-      long long pv = (long long)sq * n - (long long)sum * sum;
-      if (pv < 0) pv = 0;
-      const long long z = (pv * sc + (1 << (SGRPROJ_MTABLE_BITS - 1))) >> SGRPROJ_MTABLE_BITS;
-      const int zi = (int)d_min(255, (int)z), a2 = z >= 255 ? 256 : z == 0 ? 1 : ((zi << SGRPROJ_SGR_BITS) + zi / 2) / (zi + 1);
-      const long long b2 = (long long)((1 << SGRPROJ_SGR_BITS) - a2) * sum * one;
-      A[i + 1][j + 1] = a2;
-      B[i + 1][j + 1] = (int)((b2 + (1 << (SGRPROJ_RECIP_BITS - 1))) >> SGRPROJ_RECIP_BITS);
+/* The filter's a by z: 1 at z = 0, 256 from z = 255, else round(256 z / (z + 1)). */
+DEVI void lr_xtab(int tid) {
+  for (int z = tid; z < 256; z += LR_T)
+    gLr.xtab[z] = z == 0 ? 1 : z == 255 ? 256 : ((z << SGRPROJ_SGR_BITS) + z / 2) / (z + 1);
+  __syncthreads();
+}
+/* A and B at the (BS+2)^2 positions (rows by-1 .. by+BS, columns bx-1 ..
+ * bx+BS) around the BS x BS block at (bx, by) of segment s, box radius R
+ * (blocks: 4x4 luma, 2x2 chroma), by sliding
+ * window sums (integer, so any summation order gives the same values).
+ * Positions outside a partial block read clamped samples; they are never
+ * used. */
+template <int R, int BS>
+DEVI void lr_ab(const LrGeo &g, int s, int bx, int by, int sc, int one, int A[6][6], int B[6][6]) {
+  constexpr int P = BS + 2, N = P + 2 * R, D = 2 * R + 1, n = D * D;
+  const int xmax = g.x1 + 2, ymax = g.segY1[s] + 2;
+  const uint8_t *img = gLr.img + g.segBase[s] + (3 - g.segY0[s]) * LR_PITCH + (3 - g.x0); /* img[y * P + x] */
+  int xs[N], cs[N], cq[N];
+#pragma unroll
+  for (int c = 0; c < N; c++) {
+    xs[c] = d_min(bx - 1 - R + c, xmax);
+    cs[c] = cq[c] = 0;
+  }
+#pragma unroll
+  for (int t = 0; t < D; t++) {
+    const uint8_t *row = img + d_min(by - 1 - R + t, ymax) * LR_PITCH;
+#pragma unroll
+    for (int c = 0; c < N; c++) {
+      const int v = row[xs[c]];
+      cs[c] += v;
+      cq[c] += v * v;
     }
-  for (int i = 0; i < h; i++) {
-    const int sh = SGRPROJ_SGR_BITS + ((pass == 0 && (i & 1)) ? 4 : 5) - SGRPROJ_RST_BITS;
-    for (int j = 0; j < w; j++) {
-      long long a = 0, bb = 0;
+  }
+#pragma unroll
+  for (int i = 0; i < P; i++) {
+    if (i > 0) {
+      const uint8_t *ro = img + d_min(by - 2 - R + i, ymax) * LR_PITCH;
+      const uint8_t *rn = img + d_min(by - 2 + R + i + 1, ymax) * LR_PITCH;
+#pragma unroll
+      for (int c = 0; c < N; c++) {
+        const int vo = ro[xs[c]], vn = rn[xs[c]];
+        cs[c] += vn - vo;
+        cq[c] += vn * vn - vo * vo;
+      }
+    }
+    int sum = 0, sq = 0;
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      sum += cs[c];
+      sq += cq[c];
+    }
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+      if (j > 0) {
+        sum += cs[j + D - 1] - cs[j - 1];
+        sq += cq[j + D - 1] - cq[j - 1];
+      }
+      const int pv = d_max(0, sq * n - sum * sum);
+      const long long z = ((long long)pv * sc + (1 << (SGRPROJ_MTABLE_BITS - 1))) >> SGRPROJ_MTABLE_BITS;
+      const int a2 = gLr.xtab[(int)d_min(255ll, z)];
+      const int b2 = ((1 << SGRPROJ_SGR_BITS) - a2) * sum * one;
+      A[i][j] = a2;
+      B[i][j] = (b2 + (1 << (SGRPROJ_RECIP_BITS - 1))) >> SGRPROJ_RECIP_BITS;
+    }
+  }
+}
+/* The filtered BS x BS block (pass 0: radius 2, every other row of A / B;
+ * pass 1: radius 1) into F (row stride 4). */
+template <int PASS, int BS>
+DEVI void lr_f(const LrGeo &g, int s, int bx, int by, const int A[6][6], const int B[6][6], int *F) {
+  const uint8_t *img = gLr.img + g.segBase[s] + (3 - g.segY0[s]) * LR_PITCH + (3 - g.x0);
+  const int xmax = g.x1 + 2, ymax = g.segY1[s] + 2;
+#pragma unroll
+  for (int i = 0; i < BS; i++) {
+    const int sh = SGRPROJ_SGR_BITS + ((PASS == 0 && (i & 1)) ? 4 : 5) - SGRPROJ_RST_BITS;
+#pragma unroll
+    for (int j = 0; j < BS; j++) {
+      int a = 0, bb = 0;
 #pragma unroll
       for (int dy = -1; dy <= 1; dy++)
 #pragma unroll
         for (int dx = -1; dx <= 1; dx++) {
-          const int wgt = pass == 0 ? (((i + dy) & 1) ? (dx == 0 ? 6 : 5) : 0) : (dx == 0 || dy == 0) ? 4 : 3;
+          const int wgt = PASS == 0 ? (((i + dy) & 1) ? (dx == 0 ? 6 : 5) : 0) : (dx == 0 || dy == 0) ? 4 : 3;
           a += wgt * A[i + dy + 1][j + dx + 1];
           bb += wgt * B[i + dy + 1][j + dx + 1];
         }
-      const long long v = a * lr_img(g, s, bx + j, by + i) + bb;
-      F[i * 4 + j] = (int)((v + (1ll << (sh - 1))) >> sh);
+      const int v = a * img[d_min(by + i, ymax) * LR_PITCH + d_min(bx + j, xmax)] + bb;
+      F[i * 4 + j] = (v + (1 << (sh - 1))) >> sh;
     }
+  }
+}
+/* Both passes of one block; a pass with radius 0 is off. */
+template <int BS>
+DEVI void lr_sgr_block(const LrGeo &g, int s, int bx, int by, int r0, int r1, const int *sc, const int *one, int *F0,
+                       int *F1) {
+  int A[6][6], B[6][6];
+  if (r0) {
+    lr_ab<2, BS>(g, s, bx, by, sc[0], one[0], A, B);
+    lr_f<0, BS>(g, s, bx, by, A, B, F0);
+  }
+  if (r1) {
+    lr_ab<1, BS>(g, s, bx, by, sc[1], one[1], A, B);
+    lr_f<1, BS>(g, s, bx, by, A, B, F1);
   }
 }
 
@@ -399,7 +465,9 @@ DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t
         if (tid < nacc) {
           const int n = d_min(LR_T, cnt - c0);
           const int a = tid < nc * nc ? tid / nc : tid - nc * nc, b = tid < nc * nc ? tid - (tid / nc) * nc : -1;
-          const double *pa = gLr.u.wf.phi[a], *pb = b >= 0 ? gLr.u.wf.phi[b] : gLr.u.wf.tgt;
+          typedef __attribute__((address_space(3))) const double lds_f64;
+          const lds_f64 *pa = (lds_f64 *)gLr.u.wf.phi[a], *pb = (lds_f64 *)(b >= 0 ? gLr.u.wf.phi[b] : gLr.u.wf.tgt);
+#pragma unroll 8
           for (int k = 0; k < n; k++) acc += pa[k] * pb[k];
         }
         __syncthreads();
@@ -444,6 +512,7 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
   LrGeo g;
   lr_geo(fp, u, g);
   lr_stage(fp, g, cdef, dbk, tid);
+  lr_xtab(tid);
   LrUnitStat &st = stats[u];
   const int S = fp.stride[g.p], o = fp.offset[g.p], uw = g.x1 - g.x0, uh = g.y1 - g.y0;
   long long e = 0;
@@ -459,17 +528,28 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
   if (tid < 6) st.w0[tid] = w0[tid];
   const int nblk = g.nbr * g.nbc;
   for (int set = 0; set < 16; set++) {
+    /* the radii are 2 (pass 0) and 1 (pass 1), or 0: pass off */
     const int r0 = SGR_PARAM(set, 0), r1 = SGR_PARAM(set, 2);
+    int sc[2], one[2];
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+      const int n = pass ? 9 : 25, n2e = n * n * SGR_PARAM(set, pass * 2 + 1);
+      sc[pass] = n2e ? ((1 << SGRPROJ_MTABLE_BITS) + n2e / 2) / n2e : 0;
+      one[pass] = ((1 << SGRPROJ_RECIP_BITS) + n / 2) / n;
+    }
     long long h00 = 0, h01 = 0, h11 = 0, c0 = 0, c1 = 0;
     for (int bi = tid; bi < nblk; bi += LR_T) {
       const int br = bi / g.nbc, bc = bi - br * g.nbc;
       const int by = g.y0 + br * g.bh, bx = g.x0 + bc * g.bw;
       const int h = d_min(g.bh, g.endY - by + 1), w = d_min(g.bw, g.x1 - bx), s = lr_seg_of(g, by);
       int F0[16], F1[16];
-      if (r0) lr_box_block(g, s, bx, by, w, h, set, 0, F0);
-      if (r1) lr_box_block(g, s, bx, by, w, h, set, 1, F1);
-      for (int i = 0; i < h; i++)
-        for (int j = 0; j < w; j++) {
+      if (g.sub) lr_sgr_block<2>(g, s, bx, by, r0, r1, sc, one, F0, F1);
+      else lr_sgr_block<4>(g, s, bx, by, r0, r1, sc, one, F0, F1);
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (i >= h || j >= w) continue;
           const int uu = lr_img(g, s, bx + j, by + i) << SGRPROJ_RST_BITS;
           const int sv = ((int)src[o + (by + i) * S + bx + j] << SGRPROJ_RST_BITS) - uu;
           const int e0 = r0 ? F0[i * 4 + j] - uu : 0, e1 = r1 ? F1[i * 4 + j] - uu : 0;

@@ -62,6 +62,12 @@ struct CompLds {
   int32_t p1[16 * 64];
 };
 
+#define TS_Y 132 /* tile row strides: 4 left columns + 128 (luma) / 64 (chroma) */
+#define TS_C 68
+#define TILE_Y (65 * TS_Y)
+#define TILE_C (33 * TS_C)
+#define TILE_BYTES (TILE_Y + 2 * TILE_C)
+
 struct RLds {
   union {
     SmallTx sm[RW];
@@ -84,12 +90,27 @@ struct RLds {
   int32_t smats[TX_SMATS];
   int ival[4];
   Mav1BlockInfo blk;
+  /* intra phase: the superblock's reconstruction with the row above it and
+   * four columns left of it (luma row above: 128 samples wide, for the
+   * above-right edge), and its source in the same layout */
+  alignas(16) uint8_t tile[2][TILE_BYTES];
 };
 
 /* The working set of the workgroup that reconstructs a block / superblock.
  * File scope so that the non-inlined stage functions address it as LDS. */
 __shared__ RLds gL;
 #define NOINL __device__ __forceinline__
+
+/* Pointer p into tile k (0: reconstruction, 1: source) of `plane` with
+ * p[y * S + x] = sample (x, y) of the plane, for the superblock holding mi
+ * (miRow, miCol); S = the tile's row stride. */
+DEVI uint8_t *tile_ptr(int k, int plane, int miRow, int miCol, int &S) {
+  const int sub = plane > 0;
+  S = sub ? TS_C : TS_Y;
+  uint8_t *base = gL.tile[k] + (plane == 0 ? 0 : TILE_Y + (plane - 1) * TILE_C);
+  const int oy = (((miRow >> 4) * 64) >> sub) - 1, ox = (((miCol >> 4) * 64) >> sub) - 4;
+  return (uint8_t *)((uintptr_t)base - (uintptr_t)((long long)oy * S + ox));
+}
 
 DEVI TxCtx wg_ctx() {
   TxCtx c;
@@ -574,19 +595,30 @@ NOINL int cfl_tx(const uint8_t *luma, int LS, int sx, int sy, int maxLW, int max
 
 /* code_plane (k_recon.h:190-291) for the block in L.blk. markOnly: only the
  * BlockDecoded flags (the block was coded by the inter phase). */
-NOINL long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, uint8_t *rec,
+NOINL long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, uint8_t *rec, bool tiled,
                            int32_t *__restrict__ coefs, const int32_t *__restrict__ mats, int plane, int txSz,
                            int &maxLW, int &maxLH, bool useBd, bool markOnly) {
   RLds &L = gL;
   Mav1BlockInfo &b = L.blk;
   const int tid = threadIdx.x;
-  const int bs = b.bsize, sub = plane > 0, S = fp.stride[plane];
+  const int bs = b.bsize, sub = plane > 0;
   const int availU = b.mi_row > 0, availL = b.mi_col > tile_start(fp, b.mi_col);
   const int psz = SUBSAMPLED_SIZE(bs, sub, sub);
   const int n4w = Num_4x4_Blocks_Wide[psz], n4h = Num_4x4_Blocks_High[psz];
   const int baseX = (b.mi_col >> sub) * 4, baseY = (b.mi_row >> sub) * 4;
-  uint8_t *px = rec + fp.offset[plane];
-  const uint8_t *spx = src + fp.offset[plane];
+  /* the reconstruction, source and luma (CfL) samples: the superblock's
+   * LDS tiles in the intra phase, else the frame */
+  int S, LS;
+  uint8_t *px, *luma;
+  const uint8_t *spx;
+  if (tiled) {
+    px = tile_ptr(0, plane, b.mi_row, b.mi_col, S);
+    spx = tile_ptr(1, plane, b.mi_row, b.mi_col, S);
+    luma = tile_ptr(0, 0, b.mi_row, b.mi_col, LS);
+  } else {
+    S = fp.stride[plane], LS = fp.stride[0];
+    px = rec + fp.offset[plane], spx = src + fp.offset[plane], luma = rec + fp.offset[0];
+  }
   const int stepX = Tx_Width[txSz] >> 2, stepY = Tx_Height[txSz] >> 2;
   int cands[16], nc = 0;
   if (plane == 0) {
@@ -621,7 +653,7 @@ NOINL long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict_
           if (isCfl) {
             /* the first transform block searches the alpha, the others reuse it */
             const int search = txIdx == 0;
-            const int a = cfl_tx(rec + fp.offset[0], fp.stride[0], sx, sy, maxLW, maxLH, lw, lh, px, S, spx,
+            const int a = cfl_tx(luma, LS, sx, sy, maxLW, maxLH, lw, lh, px, S, spx,
                                  search ? 0 : plane == 1 ? b.cfl_alpha_u : b.cfl_alpha_v, search,
                                  search && plane == 2 && b.cfl_alpha_u == 0);
             if (txIdx == 0 && tid == 0) {
@@ -776,7 +808,7 @@ NOINL int dual_filter_search(const Mav1FrameParams &fp, const uint8_t *__restric
 
 /* recon_block (k_recon.h:293-394) for L.blk. Inter blocks are fully coded
  * by the inter phase; the intra phase only marks their decoded area. */
-NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, uint8_t *rec,
+NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, uint8_t *rec, bool tiled,
                        const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
                        const int32_t *__restrict__ mats, bool useBd, bool markOnly) {
   RLds &L = gL;
@@ -829,9 +861,10 @@ NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ sr
     const int baseTx = fp.lossless ? TX_4X4 : plane > 0 ? uv_tx_size(bs) : Max_Tx_Size_Rect[bs];
     const int maxDepth =
         (plane == 0 && !fp.lossless && !b.is_inter && fp.tx_mode_select) ? d_min(MAX_TX_DEPTH, Max_Tx_Depth[bs]) : 0;
-    uint8_t *px = rec + fp.offset[0] + baseY * S + baseX;
+    int TS = S;
+    uint8_t *px = (tiled ? tile_ptr(0, 0, b.mi_row, b.mi_col, TS) : rec + fp.offset[0]) + baseY * TS + baseX;
     if (maxDepth > 0) {
-      for (int k = tid; k < w * h; k += RT) L.save[k] = px[(k / w) * S + (k % w)];
+      for (int k = tid; k < w * h; k += RT) L.save[k] = px[(k / w) * TS + (k % w)];
       for (int k = tid; k < BDD * BDD; k += RT) L.bdSave[k] = L.bd[k];
       __syncthreads();
     }
@@ -841,13 +874,13 @@ NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ sr
       const int depth = d <= maxDepth ? d : bestD;
       if (d == maxDepth + 1 && bestD == maxDepth) break;
       if (d > 0) {
-        for (int k = tid; k < w * h; k += RT) px[(k / w) * S + (k % w)] = L.save[k];
+        for (int k = tid; k < w * h; k += RT) px[(k / w) * TS + (k % w)] = L.save[k];
         for (int k = tid; k < BDD * BDD; k += RT) L.bd[k] = L.bdSave[k];
         __syncthreads();
       }
       int t = baseTx;
       for (int k = 0; k < depth; k++) t = Split_Tx_Size[t];
-      const long long c = code_plane(fp, src, rec, coefs, mats, plane, t, maxLW, maxLH, useBd, markOnly) +
+      const long long c = code_plane(fp, src, rec, tiled, coefs, mats, plane, t, maxLW, maxLH, useBd, markOnly) +
                           (long long)fp.lambda * 8 * depth;
       if (d <= maxDepth && (best < 0 || c < best)) {
         best = c;
@@ -901,7 +934,7 @@ __global__ __launch_bounds__(RT) void k_recon_inter(const Mav1FrameParams fp, Ma
     Mav1BlockInfo *g = blocks + work[2 + idx];
     if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)g)[tid];
     __syncthreads();
-    recon_block(fp, src, rec, ref, gold, coefs, mats, false, false);
+    recon_block(fp, src, rec, false, ref, gold, coefs, mats, false, false);
     if (tid < 27) ((int32_t *)g)[tid] = ((const int32_t *)&L.blk)[tid];
     __syncthreads();
   }
@@ -939,15 +972,40 @@ __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Ma
     if (y == (16 >> sub) && x == -1) v = 0;
     if (y <= (16 >> sub) && x <= (16 >> sub)) L.bd[q] = v;
   }
+  /* the superblock's tiles: reconstruction (with the row above and the
+   * columns left) and source, as dwords */
+  for (int p = 0; p < 3; p++) {
+    const int sub = p > 0, TS = sub ? TS_C : TS_Y, n = 64 >> sub, wpr = TS / 4;
+    const int oy = ((r * 64) >> sub) - 1, ox = (((c0 + cl) * 64) >> sub) - 4, stride = fp.stride[p];
+    uint32_t *tr = (uint32_t *)(L.tile[0] + (p == 0 ? 0 : TILE_Y + (p - 1) * TILE_C));
+    uint32_t *ts = (uint32_t *)(L.tile[1] + (p == 0 ? 0 : TILE_Y + (p - 1) * TILE_C));
+    for (int q = tid; q < (n + 1) * wpr; q += RT) {
+      const int ty = q / wpr, tx = (q - ty * wpr) * 4, Y = oy + ty, X = ox + tx;
+      if ((ty > 0 && tx >= 4 + n) || Y < 0 || X < 0 || X >= stride) continue;
+      const int o = fp.offset[p] + Y * stride + X;
+      tr[q] = *(const uint32_t *)(rec + o);
+      if (ty > 0 && tx >= 4) ts[q] = *(const uint32_t *)(src + o);
+    }
+  }
   __syncthreads();
   for (int i = 0; i < S.num_blocks; i++) {
     Mav1BlockInfo *g = blocks + S.first_block + i;
     if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)g)[tid];
     __syncthreads();
     const bool inter = L.blk.is_inter;
-    recon_block(fp, src, rec, ref, gold, coefs, mats, true, inter);
+    recon_block(fp, src, rec, true, ref, gold, coefs, mats, true, inter);
     if (!inter && tid < 27) ((int32_t *)g)[tid] = ((const int32_t *)&L.blk)[tid];
     __syncthreads();
+  }
+  /* the superblock back into the frame */
+  for (int p = 0; p < 3; p++) {
+    const int sub = p > 0, TS = sub ? TS_C : TS_Y, n = 64 >> sub, wpr = n / 4;
+    const int y0 = (r * 64) >> sub, x0 = ((c0 + cl) * 64) >> sub, stride = fp.stride[p];
+    const uint32_t *tr = (const uint32_t *)(L.tile[0] + (p == 0 ? 0 : TILE_Y + (p - 1) * TILE_C));
+    for (int q = tid; q < n * wpr; q += RT) {
+      const int ty = q / wpr, tx = (q - ty * wpr) * 4;
+      *(uint32_t *)(rec + fp.offset[p] + (y0 + ty) * stride + x0 + tx) = tr[(ty + 1) * (TS / 4) + 1 + tx / 4];
+    }
   }
 }
 
