@@ -49,7 +49,8 @@ struct mav1hip_ctx {
   Mav1LrUnit *d_lrunits;
   int8_t *d_choice;
   void *d_lrstats;
-  int *d_lrref, *d_lrtype, *d_lrreruns;
+  void *d_lrchain;
+  int *d_lrtype, *d_lrreruns;
   uint8_t *d_decgrid;
   int *d_rwork;
   int num_cu;
@@ -211,7 +212,7 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
     }
     rc |= dmalloc(c, (void **)&c->d_lrunits, sizeof(Mav1LrUnit) * (size_t)c->lr_units);
     rc |= dmalloc(c, &c->d_lrstats, lr_stat_bytes() * (size_t)c->lr_units);
-    rc |= dmalloc(c, (void **)&c->d_lrref, sizeof(int) * 6 * (size_t)c->lr_units);
+    rc |= dmalloc(c, &c->d_lrchain, lr_chain_bytes());
     rc |= dmalloc(c, (void **)&c->d_lrtype, sizeof(int) * 3);
     rc |= dmalloc(c, (void **)&c->d_lrreruns, sizeof(int) * 3);
   }
@@ -289,7 +290,7 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipFree(c->d_rwork);
   (void)hipFree(c->d_lrunits);
   (void)hipFree(c->d_lrstats);
-  (void)hipFree(c->d_lrref);
+  (void)hipFree(c->d_lrchain);
   (void)hipFree(c->d_lrtype);
   (void)hipFree(c->d_lrreruns);
   (void)hipFree(c->d_pool);
@@ -547,17 +548,27 @@ int mav1hip_launch_lr_select(mav1hip_ctx *c, int32_t lr_test) {
   int rc = check_lr_params(c, c->fp);
   if (rc) return rc;
   if ((rc = timed(c, "lr_fit", [&] {
-         return launch_lr_fit(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrref, c->lr_units);
+         return launch_lr_fit(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrchain, c->lr_units);
        })))
     return rc;
-  if ((rc = timed(c, "lr_path", [&] {
-         return launch_lr_path(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrref, c->lr_units);
-       })))
-    return rc;
-  return timed(c, "lr_chain", [&] {
-    return launch_lr_chain(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrunits, c->d_lrtype,
-                           lr_test != 0, c->d_lrreruns);
-  });
+  /* refinement under the default reference, then up to LR_ROUNDS - 1
+   * chain launches that stop at a reference change, each followed by the
+   * refinement of the remaining units under the new reference; the last
+   * chain launch refines any further mismatching unit itself */
+  const int LR_ROUNDS = 3;
+  for (int r = 0; r < LR_ROUNDS; r++) {
+    if ((rc = timed(c, "lr_path", [&] {
+           return launch_lr_path(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrchain,
+                                 c->lr_units);
+         })))
+      return rc;
+    if ((rc = timed(c, "lr_chain", [&] {
+           return launch_lr_chain(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrunits,
+                                  c->d_lrtype, lr_test != 0, c->d_lrreruns, c->d_lrchain, r < LR_ROUNDS - 1);
+         })))
+      return rc;
+  }
+  return MAV1HIP_OK;
 }
 
 int mav1hip_lr_select(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdef,

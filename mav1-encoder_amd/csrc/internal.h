@@ -58,12 +58,17 @@ hipError_t launch_decision_grid(hipStream_t st, const Mav1FrameParams &fp, const
 
 /* lrsearch.hip */
 size_t lr_stat_bytes();
+size_t lr_chain_bytes();
+/* The fit of every unit, and the chain state reset. */
 hipError_t launch_lr_fit(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
-                         const uint8_t *dbk, void *stats, int *refW, int nunits);
+                         const uint8_t *dbk, void *stats, void *chain, int nunits);
+/* Refinement of the units from each plane's chain restart point on. */
 hipError_t launch_lr_path(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
-                          const uint8_t *dbk, void *stats, const int *refW, int nunits);
+                          const uint8_t *dbk, void *stats, const void *chain, int nunits);
+/* The unit chains; stop: end at the first reference mismatch instead of
+ * refining the unit again. */
 hipError_t launch_lr_chain(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
                            const uint8_t *dbk, const void *stats, Mav1LrUnit *units, int *lrType, int lrTest,
-                           int *reruns);
+                           int *reruns, void *chain, int stop);
 
 #endif

@@ -21,6 +21,7 @@
 #include "dev_common.h"
 #include "internal.h"
 
+#include <climits>
 #pragma clang fp contract(off)
 
 #define LR_PITCH 104   /* unit width (<= 95) + 6 halo columns, rounded up */
@@ -39,6 +40,7 @@ struct LrGeo {
 /* Working set of the workgroup searching one unit. */
 struct LrLds {
   uint8_t img[LR_ROWS * LR_PITCH]; /* stripe-resolved source samples (get_source_sample) per segment */
+  uint8_t srcu[LR_MAXPX];          /* the unit's samples of the source frame, row stride = unit width */
   union {
     struct {
       int16_t f0[LR_MAXPX], f1[LR_MAXPX]; /* self-guided filter outputs */
@@ -67,6 +69,15 @@ struct LrUnitStat {
   int wStar[6];       /* refined taps */
   long long sseStar;  /* SSE of the refined taps */
   long long costStar; /* their cost under refAssumed */
+};
+
+/* Per plane: where the unit chain stands between its launches. */
+struct LrChain {
+  int start;     /* next unit (plane index) to decide */
+  int done;      /* all units decided */
+  int used, nrerun;
+  int ref[6];    /* running Wiener reference */
+  int refX[2];   /* running self-guided reference */
 };
 
 DEVI int lr_stripe_row(int k, int sub) { return (64 * k - 8) >> sub; }
@@ -113,9 +124,15 @@ DEVI int lr_img(const LrGeo &g, int s, int x, int y) {
   return gLr.img[g.segBase[s] + (y - g.segY0[s] + 3) * LR_PITCH + (x - g.x0 + 3)];
 }
 
-/* get_source_sample (7.17.6) of every sample the unit's filters read. */
-DEVI void lr_stage(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t *cdef, const uint8_t *dbk, int tid) {
-  const int S = fp.stride[g.p], o = fp.offset[g.p], w = g.x1 - g.x0 + 6;
+/* get_source_sample (7.17.6) of every sample the unit's filters read, and
+ * the unit's source samples. */
+DEVI void lr_stage(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t *cdef, const uint8_t *dbk,
+                   const uint8_t *src, int tid) {
+  const int S = fp.stride[g.p], o = fp.offset[g.p], w = g.x1 - g.x0 + 6, uw = g.x1 - g.x0;
+  for (int q = tid; q < uw * (g.y1 - g.y0); q += LR_T) {
+    const int r = q / uw;
+    gLr.srcu[q] = src[o + (g.y0 + r) * S + g.x0 + q - r * uw];
+  }
   for (int s = 0; s < g.nseg; s++) {
     const int h = g.segY1[s] - g.segY0[s] + 6;
     for (int q = tid; q < w * h; q += LR_T) {
@@ -357,7 +374,7 @@ DEVI long long lr_wiener_sse(const Mav1FrameParams &fp, const LrGeo &g, const in
       int acc = 0;
 #pragma unroll
       for (int t = 0; t < 7; t++) acc += vf[t] * mp[t * 96];
-      const int d = d_clip1(d_round2(acc, 11)) - (int)src[o + (g.segY0[s] + r) * S + g.x0 + c];
+      const int d = d_clip1(d_round2(acc, 11)) - (int)gLr.srcu[(g.segY0[s] + r - g.y0) * uw + c];
       e += d * d;
     }
     rowBase += h + 6;
@@ -459,7 +476,7 @@ DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t
             Z[a] = sum;
           }
           for (int a = first; a < 3; a++) gLr.u.wf.phi[a - first][q - c0] = Z[a] + Z[6 - a] - 2 * Z[3];
-          gLr.u.wf.tgt[q - c0] = (double)src[o + y * S + x] - Z[3];
+          gLr.u.wf.tgt[q - c0] = (double)gLr.srcu[(y - g.y0) * (g.x1 - g.x0) + x - g.x0] - Z[3];
         }
         __syncthreads();
         if (tid < nacc) {
@@ -511,14 +528,14 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
   const int u = blockIdx.x, tid = threadIdx.x;
   LrGeo g;
   lr_geo(fp, u, g);
-  lr_stage(fp, g, cdef, dbk, tid);
+  lr_stage(fp, g, cdef, dbk, src, tid);
   lr_xtab(tid);
   LrUnitStat &st = stats[u];
   const int S = fp.stride[g.p], o = fp.offset[g.p], uw = g.x1 - g.x0, uh = g.y1 - g.y0;
   long long e = 0;
   for (int q = tid; q < uw * uh; q += LR_T) {
     const int a = o + (g.y0 + q / uw) * S + g.x0 + q % uw;
-    const int d = (int)cdef[a] - (int)src[a];
+    const int d = (int)cdef[a] - (int)gLr.srcu[q];
     e += d * d;
   }
   e = block_sum64(e, tid);
@@ -551,7 +568,7 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
         for (int j = 0; j < 4; j++) {
           if (i >= h || j >= w) continue;
           const int uu = lr_img(g, s, bx + j, by + i) << SGRPROJ_RST_BITS;
-          const int sv = ((int)src[o + (by + i) * S + bx + j] << SGRPROJ_RST_BITS) - uu;
+          const int sv = ((int)gLr.srcu[(by + i - g.y0) * uw + bx + j - g.x0] << SGRPROJ_RST_BITS) - uu;
           const int e0 = r0 ? F0[i * 4 + j] - uu : 0, e1 = r1 ? F1[i * 4 + j] - uu : 0;
           h00 += (long long)e0 * e0;
           h01 += (long long)e0 * e1;
@@ -599,7 +616,7 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
       const int y = g.y0 + q / uw, x = g.x0 + q % uw;
       const int uu = lr_img(g, lr_seg_of(g, y), x, y) << SGRPROJ_RST_BITS;
       const int v = xw1 * uu + (r0 ? xw0 * gLr.u.sg.f0[q] : xw0 * uu) + (r1 ? xw2 * gLr.u.sg.f1[q] : xw2 * uu);
-      const int d = d_clip1(d_round2(v, SGRPROJ_RST_BITS + SGRPROJ_PRJ_BITS)) - (int)src[o + y * S + x];
+      const int d = d_clip1(d_round2(v, SGRPROJ_RST_BITS + SGRPROJ_PRJ_BITS)) - (int)gLr.srcu[q];
       es += d * d;
     }
     es = block_sum64(es, tid);
@@ -608,21 +625,25 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
   }
 }
 
-/* Per unit: the greedy refinement of the fitted taps under the incoming
- * reference refW (one per unit, pass-major). */
+/* Per unit: the greedy refinement of the fitted taps under the plane's
+ * running reference, for the units from the chain's restart point on (all
+ * of them in the first pass, where the reference is the default). */
 __global__ __launch_bounds__(LR_T) void k_lr_path(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
                                                   const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
-                                                  LrUnitStat *__restrict__ stats, const int *__restrict__ refW) {
+                                                  LrUnitStat *__restrict__ stats, const LrChain *__restrict__ chain) {
   const int u = blockIdx.x, tid = threadIdx.x;
+  const int p = u < fp.lr_unit_off[1] ? 0 : u < fp.lr_unit_off[2] ? 1 : 2;
+  const LrChain &cs = chain[p];
+  if (cs.done || u - fp.lr_unit_off[p] < cs.start) return;
   LrGeo g;
   lr_geo(fp, u, g);
-  lr_stage(fp, g, cdef, dbk, tid);
+  lr_stage(fp, g, cdef, dbk, src, tid);
   LrUnitStat &st = stats[u];
   int w[6], ref[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     w[k] = st.w0[k];
-    ref[k] = refW[u * 6 + k];
+    ref[k] = cs.ref[k];
   }
   long long sse;
   const long long cost = lr_refine(fp, g, w, ref, src, &sse, tid);
@@ -637,131 +658,200 @@ __global__ __launch_bounds__(LR_T) void k_lr_path(const Mav1FrameParams fp, cons
 }
 
 /* Per plane, units in raster order with the true running references: the
- * choice of lr_select (lr_search.c:262-349), rerunning a unit's refinement
- * where k_lr_path assumed another reference. units[] and lrType[plane]
- * receive the result, reruns[plane] the number of refinements rerun. */
+ * choice of lr_select (lr_search.c:262-349). A unit whose refinement
+ * k_lr_path ran under another reference than the running one either ends
+ * the launch (stop != 0: the chain records where it stands, and the next
+ * k_lr_path pass refines the remaining units under the running reference
+ * in parallel) or is refined again here (stop == 0, the last launch).
+ * units[] and lrType[plane] receive the result, reruns[plane] the number of
+ * refinements rerun here. The unit statistics are staged into LDS LR_CB
+ * units at a time; the sixteen self-guided costs of a unit are scored one
+ * per lane. */
+#define LR_CB 32
 __global__ __launch_bounds__(LR_T) void k_lr_chain(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
                                                    const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
                                                    const LrUnitStat *__restrict__ stats, Mav1LrUnit *__restrict__ units,
-                                                   int *__restrict__ lrType, int lrTest, int *__restrict__ reruns) {
+                                                   int *__restrict__ lrType, int lrTest, int *__restrict__ reruns,
+                                                   LrChain *__restrict__ chain, int stop) {
   const int p = blockIdx.x, tid = threadIdx.x;
+  LrChain &cs = chain[p];
+  if (cs.done) return;
+  __shared__ LrUnitStat sSt[LR_CB];
+  __shared__ Mav1LrUnit sUn[LR_CB];
   __shared__ int sRef[6], sRefX[2], sW[6];
   __shared__ long long sCost, sSse;
-  __shared__ int sRerun;
-  if (tid < 6) sRef[tid] = Wiener_Taps_Mid[tid % 3];
-  if (tid < 2) sRefX[tid] = Sgrproj_Xqd_Mid[tid];
-  __syncthreads();
-  const int nu = fp.lr_unit_rows[p] * fp.lr_unit_cols[p];
-  int used = 0, nrerun = 0;
-  for (int l = 0; l < nu; l++) {
-    const int u = fp.lr_unit_off[p] + l;
-    const LrUnitStat &st = stats[u];
-    if (tid == 0) {
-      int same = 1;
-      for (int k = 0; k < 6; k++) same &= st.refAssumed[k] == sRef[k];
-      sRerun = !same;
-      for (int k = 0; k < 6; k++) sW[k] = st.wStar[k];
-      sCost = st.costStar;
-      sSse = st.sseStar;
-    }
+  __shared__ int sRerun, sStop;
+  if (tid < 6) sRef[tid] = cs.ref[tid];
+  if (tid < 2) sRefX[tid] = cs.refX[tid];
+  if (tid == 0) sStop = -1;
+  const int nu = fp.lr_unit_rows[p] * fp.lr_unit_cols[p], off = fp.lr_unit_off[p];
+  int used = cs.used, nrerun = cs.nrerun;
+  /* lane `set` (< 16): which of its passes are on; the projection ranges */
+  const int has0 = tid < 16 && SGR_PARAM(tid, 0) != 0, has1 = tid < 16 && SGR_PARAM(tid, 2) != 0;
+  const int xlo0 = Sgrproj_Xqd_Min[0], xhi0 = Sgrproj_Xqd_Max[0] + 1;
+  const int xlo1 = Sgrproj_Xqd_Min[1], xhi1 = Sgrproj_Xqd_Max[1] + 1;
+  for (int b0 = cs.start; b0 < nu; b0 += LR_CB) {
+    int nb = d_min(LR_CB, nu - b0);
+    const int words = nb * (int)(sizeof(LrUnitStat) / 4);
     __syncthreads();
-    if (sRerun) {
-      LrGeo g;
-      lr_geo(fp, u, g);
-      lr_stage(fp, g, cdef, dbk, tid);
-      int w[6], ref[6];
-#pragma unroll
-      for (int k = 0; k < 6; k++) {
-        w[k] = st.w0[k];
-        ref[k] = sRef[k];
-      }
-      long long sse;
-      const long long cost = lr_refine(fp, g, w, ref, src, &sse, tid);
+    for (int q = tid; q < words; q += LR_T) ((int *)sSt)[q] = ((const int *)(stats + off + b0))[q];
+    __syncthreads();
+    for (int li = 0; li < nb; li++) {
+      const int l = b0 + li, u = off + l;
+      const LrUnitStat &st = sSt[li];
       if (tid == 0) {
-        for (int k = 0; k < 6; k++) sW[k] = w[k];
-        sCost = cost;
-        sSse = sse;
+        int same = 1;
+        for (int k = 0; k < 6; k++) same &= st.refAssumed[k] == sRef[k];
+        sRerun = !same;
+        if (!same && stop) sStop = li;
+        for (int k = 0; k < 6; k++) sW[k] = st.wStar[k];
+        sCost = st.costStar;
+        sSse = st.sseStar;
       }
-      nrerun++;
+      __syncthreads();
+      if (sStop >= 0) {
+        nb = li;
+        break;
+      }
+      if (sRerun) {
+        LrGeo g;
+        lr_geo(fp, u, g);
+        lr_stage(fp, g, cdef, dbk, src, tid);
+        int w[6], ref[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          w[k] = st.w0[k];
+          ref[k] = sRef[k];
+        }
+        long long sse;
+        const long long cost = lr_refine(fp, g, w, ref, src, &sse, tid);
+        if (tid == 0) {
+          for (int k = 0; k < 6; k++) sW[k] = w[k];
+          sCost = cost;
+          sSse = sse;
+        }
+        nrerun++;
+        __syncthreads();
+      }
+      if (tid < 64) {
+        /* NONE, then WIENER, then the sets in order: first strict minimum */
+        long long bestCost = lr_rd(fp, st.sseNone, 1);
+        int bestKind = RESTORE_NONE, bestSet = 0;
+        if (sCost < bestCost) {
+          bestCost = sCost;
+          bestKind = RESTORE_WIENER;
+        }
+        long long sc = LLONG_MAX;
+        if (tid < 16) {
+          int bits = 2 + SGRPROJ_PARAMS_BITS;
+          if (has0) bits += lr_subexp_bits(xlo0, xhi0, SGRPROJ_PRJ_SUBEXP_K, sRefX[0], st.xqd[tid][0]);
+          if (has1) bits += lr_subexp_bits(xlo1, xhi1, SGRPROJ_PRJ_SUBEXP_K, sRefX[1], st.xqd[tid][1]);
+          sc = lr_rd(fp, st.sseSgr[tid], bits);
+        }
+        int si = tid;
+#pragma unroll
+        for (int m = 8; m >= 1; m >>= 1) {
+          const long long oc = __shfl_xor(sc, m);
+          const int oi = __shfl_xor(si, m);
+          if (oc < sc || (oc == sc && oi < si)) {
+            sc = oc;
+            si = oi;
+          }
+        }
+        if (sc < bestCost) {
+          bestKind = RESTORE_SGRPROJ;
+          bestSet = si;
+        }
+        if (tid == 0) {
+          Mav1LrUnit wien = {};
+          wien.type = RESTORE_WIENER;
+          for (int k = 0; k < 6; k++) wien.wiener[k / 3][k % 3] = sW[k];
+          Mav1LrUnit best = {};
+          best.type = RESTORE_NONE;
+          if (bestKind == RESTORE_WIENER) best = wien;
+          if (bestKind == RESTORE_SGRPROJ) {
+            best.type = RESTORE_SGRPROJ;
+            best.set = bestSet;
+            best.xqd[0] = st.xqd[bestSet][0];
+            best.xqd[1] = st.xqd[bestSet][1];
+          }
+          if (lrTest) {
+            if (!(l & 1)) best = wien;
+            else {
+              long long bs = -1;
+              for (int set = 0; set < 16; set++)
+                if (bs < 0 || st.sseSgr[set] < bs) {
+                  bs = st.sseSgr[set];
+                  best = Mav1LrUnit{};
+                  best.type = RESTORE_SGRPROJ;
+                  best.set = set;
+                  best.xqd[0] = st.xqd[set][0];
+                  best.xqd[1] = st.xqd[set][1];
+                }
+            }
+          }
+          if (best.type == RESTORE_WIENER)
+            for (int k = 0; k < 6; k++)
+              if (p == 0 || k % 3) sRef[k] = best.wiener[k / 3][k % 3];
+          if (best.type == RESTORE_SGRPROJ)
+            for (int i = 0; i < 2; i++) sRefX[i] = best.xqd[i];
+          sUn[li] = best;
+          used |= best.type != RESTORE_NONE;
+        }
+      }
       __syncthreads();
     }
-    if (tid == 0) {
-      Mav1LrUnit best = {};
-      best.type = RESTORE_NONE;
-      long long bestCost = lr_rd(fp, st.sseNone, 1);
-      Mav1LrUnit wien = {};
-      wien.type = RESTORE_WIENER;
-      for (int k = 0; k < 6; k++) wien.wiener[k / 3][k % 3] = sW[k];
-      if (sCost < bestCost) {
-        bestCost = sCost;
-        best = wien;
+    for (int q = tid; q < nb * (int)(sizeof(Mav1LrUnit) / 4); q += LR_T)
+      ((int *)(units + off + b0))[q] = ((const int *)sUn)[q];
+    if (sStop >= 0) {
+      /* the state for the refinement pass and the next launch */
+      if (tid == 0) {
+        cs.start = b0 + sStop;
+        cs.used = used;
+        cs.nrerun = nrerun;
+        for (int k = 0; k < 6; k++) cs.ref[k] = sRef[k];
+        for (int k = 0; k < 2; k++) cs.refX[k] = sRefX[k];
       }
-      for (int set = 0; set < 16; set++) {
-        const long long sc = lr_rd(fp, st.sseSgr[set], 2 + lr_sgr_bits(set, st.xqd[set], sRefX));
-        if (sc < bestCost) {
-          bestCost = sc;
-          best = Mav1LrUnit{};
-          best.type = RESTORE_SGRPROJ;
-          best.set = set;
-          best.xqd[0] = st.xqd[set][0];
-          best.xqd[1] = st.xqd[set][1];
-        }
-      }
-      if (lrTest) {
-        if (!(l & 1)) best = wien;
-        else {
-          long long bs = -1;
-          for (int set = 0; set < 16; set++)
-            if (bs < 0 || st.sseSgr[set] < bs) {
-              bs = st.sseSgr[set];
-              best = Mav1LrUnit{};
-              best.type = RESTORE_SGRPROJ;
-              best.set = set;
-              best.xqd[0] = st.xqd[set][0];
-              best.xqd[1] = st.xqd[set][1];
-            }
-        }
-      }
-      if (best.type == RESTORE_WIENER)
-        for (int k = 0; k < 6; k++)
-          if (p == 0 || k % 3) sRef[k] = best.wiener[k / 3][k % 3];
-      if (best.type == RESTORE_SGRPROJ)
-        for (int i = 0; i < 2; i++) sRefX[i] = best.xqd[i];
-      units[u] = best;
-      used |= best.type != RESTORE_NONE;
+      return;
     }
-    __syncthreads();
   }
   if (tid == 0) {
+    cs.done = 1;
     lrType[p] = used ? RESTORE_SWITCHABLE : RESTORE_NONE;
     reruns[p] = nrerun;
   }
 }
 
-__global__ void k_lr_ref_init(int *__restrict__ refW, int nunits) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nunits * 6) refW[i] = Wiener_Taps_Mid[i % 3];
+__global__ void k_lr_chain_init(LrChain *__restrict__ chain) {
+  const int p = threadIdx.x;
+  if (p >= 3) return;
+  LrChain c = {};
+  for (int k = 0; k < 6; k++) c.ref[k] = Wiener_Taps_Mid[k % 3];
+  for (int k = 0; k < 2; k++) c.refX[k] = Sgrproj_Xqd_Mid[k];
+  chain[p] = c;
 }
 
 size_t lr_stat_bytes() { return sizeof(LrUnitStat); }
+size_t lr_chain_bytes() { return 3 * sizeof(LrChain); }
 
 hipError_t launch_lr_fit(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
-                         const uint8_t *dbk, void *stats, int *refW, int nunits) {
+                         const uint8_t *dbk, void *stats, void *chain, int nunits) {
   hipLaunchKernelGGL(k_lr_fit, dim3(nunits), dim3(LR_T), 0, st, fp, src, cdef, dbk, (LrUnitStat *)stats);
-  hipLaunchKernelGGL(k_lr_ref_init, dim3((nunits * 6 + 255) / 256), dim3(256), 0, st, refW, nunits);
+  hipLaunchKernelGGL(k_lr_chain_init, dim3(1), dim3(64), 0, st, (LrChain *)chain);
   return hipGetLastError();
 }
 
 hipError_t launch_lr_path(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
-                          const uint8_t *dbk, void *stats, const int *refW, int nunits) {
-  hipLaunchKernelGGL(k_lr_path, dim3(nunits), dim3(LR_T), 0, st, fp, src, cdef, dbk, (LrUnitStat *)stats, refW);
+                          const uint8_t *dbk, void *stats, const void *chain, int nunits) {
+  hipLaunchKernelGGL(k_lr_path, dim3(nunits), dim3(LR_T), 0, st, fp, src, cdef, dbk, (LrUnitStat *)stats,
+                     (const LrChain *)chain);
   return hipGetLastError();
 }
 
 hipError_t launch_lr_chain(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
                            const uint8_t *dbk, const void *stats, Mav1LrUnit *units, int *lrType, int lrTest,
-                           int *reruns) {
+                           int *reruns, void *chain, int stop) {
   hipLaunchKernelGGL(k_lr_chain, dim3(3), dim3(LR_T), 0, st, fp, src, cdef, dbk, (const LrUnitStat *)stats, units,
-                     lrType, lrTest, reruns);
+                     lrType, lrTest, reruns, (LrChain *)chain, stop);
   return hipGetLastError();
 }
