@@ -18,6 +18,8 @@
  * rate, inverse transform and distortion), the nine dual-filter pairs over
  * three shared horizontal passes, the 33 CfL alphas over the waves. Larger
  * transforms (32- and 64-point) use the whole workgroup per candidate. */
+#include <climits>
+
 #include "dev_pred.h"
 #include "dev_tx.h"
 #include "internal.h"
@@ -68,6 +70,14 @@ struct CompLds {
 #define TILE_C (33 * TS_C)
 #define TILE_BYTES (TILE_Y + 2 * TILE_C)
 
+/* Per-wave LDS of the small intra blocks' prediction (edges, CfL). */
+#define SI_EDGE (EDGE_NEG + 2 * 32 + 16)
+struct SiWave {
+  int eA[SI_EDGE], eL[SI_EDGE], ecp[2 * 32 + 40];
+  int cfl[64], cflAvg, cflDc;
+  long long aSse[33];
+};
+
 struct RLds {
   union {
     SmallTx sm[RW];
@@ -94,6 +104,8 @@ struct RLds {
    * four columns left of it (luma row above: 128 samples wide, for the
    * above-right edge), and its source in the same layout */
   alignas(16) uint8_t tile[2][TILE_BYTES];
+  SiWave si[3];             /* the small-block path: luma wave, U wave, V wave */
+  Mav1BlockInfo slot[2];    /* its blocks in flight */
 };
 
 /* The working set of the workgroup that reconstructs a block / superblock.
@@ -452,45 +464,48 @@ DEVI int use_upsample(int w, int h, int filterType, int delta) {
   if (d <= 0 || d >= 40) return 0;
   return filterType == 0 ? (w + h <= 16) : (w + h <= 8);
 }
-/* 7.11.2.12 on e[-1 .. sz-2] */
-DEVI void wg_edge_filter(int *e, int *cp, int sz, int strength, int tid) {
+/* 7.11.2.12 on e[-1 .. sz-2], by the team c with copy space cp */
+DEVI void edge_filter(const TxCtx &c, int *e, int *cp, int sz, int strength) {
   if (!strength) return;
-  for (int i = tid; i < sz; i += RT) cp[i] = e[i - 1];
-  __syncthreads();
-  for (int i = 1 + tid; i < sz; i += RT) {
+  for (int i = c.tid; i < sz; i += c.nt) cp[i] = e[i - 1];
+  tx_sync(c);
+  for (int i = 1 + c.tid; i < sz; i += c.nt) {
     int s = 0;
 #pragma unroll
     for (int j = 0; j < 5; j++) s += INTRA_EDGE_KERNEL(strength - 1, j) * cp[d_clip(0, sz - 1, i - 2 + j)];
     e[i - 1] = (s + 8) >> 4;
   }
-  __syncthreads();
+  tx_sync(c);
 }
 /* 7.11.2.11 */
-DEVI void wg_edge_upsample(int *e, int *dup, int numPx, int tid) {
-  for (int i = tid - 1; i < numPx; i += RT) dup[i + 2] = e[i];
-  if (tid == 0) {
+DEVI void edge_upsample(const TxCtx &c, int *e, int *dup, int numPx) {
+  for (int i = c.tid - 1; i < numPx; i += c.nt) dup[i + 2] = e[i];
+  if (c.tid == 0) {
     dup[0] = e[-1];
     dup[numPx + 2] = e[numPx - 1];
   }
-  __syncthreads();
-  if (tid == 0) e[-2] = dup[0];
-  for (int i = tid; i < numPx; i += RT) {
+  tx_sync(c);
+  if (c.tid == 0) e[-2] = dup[0];
+  for (int i = c.tid; i < numPx; i += c.nt) {
     int s = -dup[i] + 9 * dup[i + 1] + 9 * dup[i + 2] - dup[i + 3];
     e[2 * i - 1] = d_clip1(d_round2(s, 4));
     e[2 * i] = dup[i + 2];
   }
-  __syncthreads();
+  tx_sync(c);
 }
 
-NOINL void intra_tx_predict(uint8_t *px, int S, int sx, int sy, int lw, int lh, int mode, int delta, int hl, int ha,
-                            int har, int hbl, int maxX, int maxY, int edgeFilter, int filterType) {
-  RLds &L = gL;
-  const int tid = threadIdx.x, w = 1 << lw, h = 1 << lh;
-  int *A = L.edgeA + EDGE_NEG, *Lc = L.edgeL + EDGE_NEG;
+/* Intra prediction of one transform block into px by the team c (the
+ * workgroup or one wave), edge arrays eA / eL (addressed from -EDGE_NEG)
+ * and copy space ecp. */
+DEVI void intra_tx_predict(const TxCtx &c, uint8_t *px, int S, int sx, int sy, int lw, int lh, int mode, int delta,
+                           int hl, int ha, int har, int hbl, int maxX, int maxY, int edgeFilter, int filterType,
+                           int *eA, int *eL, int *ecp) {
+  const int tid = c.tid, w = 1 << lw, h = 1 << lh;
+  int *A = eA + EDGE_NEG, *Lc = eL + EDGE_NEG;
   /* 7.11.2 steps 1-3: the neighbouring samples */
   {
     const int aLim = d_min(maxX, sx + (har ? 2 * w : w) - 1), lLim = d_min(maxY, sy + (hbl ? 2 * h : h) - 1);
-    for (int i = tid; i < w + h; i += RT) {
+    for (int i = tid; i < w + h; i += c.nt) {
       A[i] = ha ? px[(sy - 1) * S + d_min(aLim, sx + i)] : hl ? px[sy * S + sx - 1] : 127;
       Lc[i] = hl ? px[d_min(lLim, sy + i) * S + sx - 1] : ha ? px[(sy - 1) * S + sx] : 129;
     }
@@ -498,7 +513,7 @@ NOINL void intra_tx_predict(uint8_t *px, int S, int sx, int sy, int lw, int lh, 
       const int corner = (ha && hl) ? px[(sy - 1) * S + sx - 1] : ha ? px[(sy - 1) * S + sx] : hl ? px[sy * S + sx - 1] : 128;
       A[-1] = Lc[-1] = corner;
     }
-    __syncthreads();
+    tx_sync(c);
   }
   IntraMode im;
   intra_mode_setup(im, mode, delta, lw, lh);
@@ -508,37 +523,36 @@ NOINL void intra_tx_predict(uint8_t *px, int S, int sx, int sy, int lw, int lh, 
     if (pa != 90 && pa != 180) {
       if (pa > 90 && pa < 180 && w + h >= 24) {
         const int v = d_round2(Lc[0] * 5 + A[-1] * 6 + A[0] * 5, 4);
-        __syncthreads();
+        tx_sync(c);
         if (tid == 0) A[-1] = Lc[-1] = v;
-        __syncthreads();
+        tx_sync(c);
       }
       if (ha)
-        wg_edge_filter(A, L.ecopy, d_min(w, maxX - sx + 1) + (pa < 90 ? h : 0) + 1, edge_strength(w, h, filterType, pa - 90),
-                       tid);
+        edge_filter(c, A, ecp, d_min(w, maxX - sx + 1) + (pa < 90 ? h : 0) + 1, edge_strength(w, h, filterType, pa - 90));
       if (hl)
-        wg_edge_filter(Lc, L.ecopy, d_min(h, maxY - sy + 1) + (pa > 180 ? w : 0) + 1,
-                       edge_strength(w, h, filterType, pa - 180), tid);
+        edge_filter(c, Lc, ecp, d_min(h, maxY - sy + 1) + (pa > 180 ? w : 0) + 1,
+                    edge_strength(w, h, filterType, pa - 180));
     }
     im.upA = use_upsample(w, h, filterType, pa - 90);
-    if (im.upA) wg_edge_upsample(A, L.ecopy, w + (pa < 90 ? h : 0), tid);
+    if (im.upA) edge_upsample(c, A, ecp, w + (pa < 90 ? h : 0));
     im.upL = use_upsample(w, h, filterType, pa - 180);
-    if (im.upL) wg_edge_upsample(Lc, L.ecopy, h + (pa > 180 ? w : 0), tid);
+    if (im.upL) edge_upsample(c, Lc, ecp, h + (pa > 180 ? w : 0));
   }
   if (mode == DC_PRED) {
     long long s = 0;
-    for (int i = tid; i < w; i += RT) s += ha ? A[i] : 0;
-    for (int i = tid; i < h; i += RT) s += hl ? Lc[i] : 0;
-    const int sum = (int)wg_sum64(s);
+    for (int i = tid; i < w; i += c.nt) s += ha ? A[i] : 0;
+    for (int i = tid; i < h; i += c.nt) s += hl ? Lc[i] : 0;
+    const int sum = (int)tx_sum64(c, s);
     im.dcv = (ha && hl) ? (sum + ((w + h) >> 1)) / (w + h) : hl ? d_clip1((sum + (h >> 1)) >> lh)
              : ha ? d_clip1((sum + (w >> 1)) >> lw) : 128;
   } else {
     im.dcv = 0;
   }
-  for (int k = tid; k < w * h; k += RT) {
+  for (int k = tid; k < w * h; k += c.nt) {
     const int i = k >> lw, j = k & (w - 1);
     px[(sy + i) * S + sx + j] = (uint8_t)intra_px(im, A, Lc, i, j);
   }
-  __syncthreads();
+  tx_sync(c);
 }
 
 /* CfL (7.11.5) over the DC prediction in px: the alpha with the smallest
@@ -644,12 +658,12 @@ NOINL long long code_plane(const Mav1FrameParams &fp, const uint8_t *__restrict_
         if (!b.is_inter) {
           const int isCfl = plane > 0 && b.uv_mode == UV_CFL_PRED;
           const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
-          intra_tx_predict(px, S, sx, sy, lw, lh, plane == 0 ? b.y_mode : (isCfl ? DC_PRED : b.uv_mode),
+          intra_tx_predict(wg_ctx(), px, S, sx, sy, lw, lh, plane == 0 ? b.y_mode : (isCfl ? DC_PRED : b.uv_mode),
                            plane == 0 ? b.angle_y : b.angle_uv, availL || x > 0, availU || y > 0,
                            L.bd[BDI(plane, (sbr >> sub) - 1, (sbc >> sub) + stepX)],
                            L.bd[BDI(plane, (sbr >> sub) + stepY, (sbc >> sub) - 1)], ((fp.mi_cols * 4) >> sub) - 1,
                            ((fp.mi_rows * 4) >> sub) - 1, fp.enable_intra_edge_filter,
-                           (b.interp_filter >> (plane == 0 ? 4 : 5)) & 1);
+                           (b.interp_filter >> (plane == 0 ? 4 : 5)) & 1, L.edgeA, L.edgeL, L.ecopy);
           if (isCfl) {
             /* the first transform block searches the alpha, the others reuse it */
             const int search = txIdx == 0;
@@ -893,6 +907,270 @@ NOINL void recon_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ sr
   }
 }
 
+/* ---- small intra blocks ----
+ * Intra blocks of 8x8 .. 16x16 without a tx_depth search code one transform
+ * block per plane. The intra phase pipelines them: while the workgroup codes
+ * the luma of block i (prediction on wave 0, the transform types one per
+ * wave, the chosen one on wave 0), waves SI_WU and SI_WV code the U and V
+ * planes of block i-1. Each plane's BlockDecoded flags are set in decode
+ * order by its own stream, and CfL reads the luma of its block, which the
+ * previous step finished. */
+#define SI_WU 6
+#define SI_WV 7
+
+DEVI bool si_eligible(const Mav1FrameParams &fp, const Mav1BlockInfo &b) {
+  const int bs = b.bsize;
+  return !b.is_inter && !fp.lossless && !fp.tx_mode_select &&
+         (bs == BLOCK_8X8 || bs == BLOCK_8X16 || bs == BLOCK_16X8 || bs == BLOCK_16X16);
+}
+
+/* The BlockDecoded marks of one plane of block b, transform block by
+ * transform block as code_plane sets them (one thread). */
+DEVI void si_mark(const Mav1FrameParams &fp, const Mav1BlockInfo &b, int plane) {
+  const int sub = plane > 0, bs = b.bsize;
+  const int txSz = fp.lossless ? TX_4X4 : plane ? uv_tx_size(bs) : Max_Tx_Size_Rect[bs];
+  const int psz = SUBSAMPLED_SIZE(bs, sub, sub), n4w = Num_4x4_Blocks_Wide[psz], n4h = Num_4x4_Blocks_High[psz];
+  const int baseX = (b.mi_col >> sub) * 4, baseY = (b.mi_row >> sub) * 4;
+  const int stepX = Tx_Width[txSz] >> 2, stepY = Tx_Height[txSz] >> 2;
+  for (int y = 0; y < n4h; y += stepY)
+    for (int x = 0; x < n4w; x += stepX) {
+      const int sx = baseX + 4 * x, sy = baseY + 4 * y;
+      if (sx >= ((fp.mi_cols * 4) >> sub) || sy >= ((fp.mi_rows * 4) >> sub)) continue;
+      const int sbr = ((sy << sub) >> 2) & 15, sbc = ((sx << sub) >> 2) & 15;
+      for (int i = 0; i < stepY; i++)
+        for (int j = 0; j < stepX; j++) gL.bd[BDI(plane, (sbr >> sub) + i, (sbc >> sub) + j)] = 1;
+    }
+}
+
+/* Geometry of plane `plane` of small block b. */
+struct SiGeo {
+  int sub, txSz, lw, lh, sx, sy, S;
+  uint8_t *px;
+  const uint8_t *spx;
+};
+DEVI SiGeo si_geo(const Mav1BlockInfo &b, int plane) {
+  SiGeo g;
+  g.sub = plane > 0;
+  g.txSz = plane ? uv_tx_size(b.bsize) : Max_Tx_Size_Rect[b.bsize];
+  g.lw = Tx_Width_Log2[g.txSz];
+  g.lh = Tx_Height_Log2[g.txSz];
+  g.sx = (b.mi_col >> g.sub) * 4;
+  g.sy = (b.mi_row >> g.sub) * 4;
+  g.px = tile_ptr(0, plane, b.mi_row, b.mi_col, g.S);
+  g.spx = tile_ptr(1, plane, b.mi_row, b.mi_col, g.S);
+  return g;
+}
+
+/* Prediction of one plane of small block b by one wave (code_plane's
+ * intra_tx_predict call); for CfL also the alphas' SSEs (cfl_tx), and for U
+ * the chosen alpha applied. */
+DEVI void si_predict(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane, SiWave &W) {
+  const TxCtx c = wave_ctx();
+  const int lane = c.tid;
+  const SiGeo g = si_geo(b, plane);
+  const int sub = g.sub, stepX = Tx_Width[g.txSz] >> 2, stepY = Tx_Height[g.txSz] >> 2;
+  const int sbr = ((g.sy << sub) >> 2) & 15, sbc = ((g.sx << sub) >> 2) & 15;
+  const int availU = b.mi_row > 0, availL = b.mi_col > tile_start(fp, b.mi_col);
+  const int isCfl = plane > 0 && b.uv_mode == UV_CFL_PRED;
+  intra_tx_predict(c, g.px, g.S, g.sx, g.sy, g.lw, g.lh, plane == 0 ? b.y_mode : (isCfl ? DC_PRED : b.uv_mode),
+                   plane == 0 ? b.angle_y : b.angle_uv, availL, availU,
+                   gL.bd[BDI(plane, (sbr >> sub) - 1, (sbc >> sub) + stepX)],
+                   gL.bd[BDI(plane, (sbr >> sub) + stepY, (sbc >> sub) - 1)], ((fp.mi_cols * 4) >> sub) - 1,
+                   ((fp.mi_rows * 4) >> sub) - 1, fp.enable_intra_edge_filter,
+                   (b.interp_filter >> (plane == 0 ? 4 : 5)) & 1, W.eA, W.eL, W.ecp);
+  if (!isCfl) return;
+  /* CfL (7.11.5) over the DC prediction: the luma average of each sample,
+   * then one alpha per lane */
+  const int w = 1 << g.lw, h = 1 << g.lh, n = w * h;
+  int LS;
+  const uint8_t *luma = tile_ptr(0, 0, b.mi_row, b.mi_col, LS);
+  const int lumaTx = Max_Tx_Size_Rect[b.bsize];
+  const int maxLW = b.mi_col * 4 + Tx_Width[lumaTx], maxLH = b.mi_row * 4 + Tx_Height[lumaTx];
+  const int dcv = g.px[g.sy * g.S + g.sx];
+  int sum = 0;
+  for (int k = lane; k < n; k += 64) {
+    const int i = k >> g.lw, j = k & (w - 1);
+    const int ly = d_min((g.sy + i) << 1, maxLH - 2), lx = d_min((g.sx + j) << 1, maxLW - 2);
+    const uint8_t *l = luma + ly * LS + lx;
+    const int v = ((int)l[0] + l[1] + l[LS] + l[LS + 1]) << 1;
+    W.cfl[k] = v;
+    sum += v;
+  }
+  const int avg = d_round2(wave_sum(sum), g.lw + g.lh);
+  tx_sync(c);
+  if (lane < 33) {
+    const int a = lane - 16;
+    long long e = 0;
+    for (int k = 0; k < n; k++) {
+      const int i = k >> g.lw, j = k & (w - 1);
+      const int d = d_clip1(dcv + d_round2signed(a * (W.cfl[k] - avg), 6)) - g.spx[(g.sy + i) * g.S + g.sx + j];
+      e += d * d;
+    }
+    W.aSse[lane] = e;
+  }
+  if (lane == 0) {
+    W.cflAvg = avg;
+    W.cflDc = dcv;
+  }
+  tx_sync(c);
+}
+
+/* CfL: the alpha of smallest SSE (first in -16..16, 0 excluded when asked),
+ * applied to the DC prediction. */
+DEVI int si_cfl_apply(const Mav1BlockInfo &b, int plane, SiWave &W, int excludeZero) {
+  const TxCtx c = wave_ctx();
+  const int lane = c.tid;
+  const SiGeo g = si_geo(b, plane);
+  const int w = 1 << g.lw, n = w << g.lh;
+  long long v = (lane < 33 && !(excludeZero && lane == 16)) ? W.aSse[lane] : LLONG_MAX;
+  int idx = lane;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const long long ov = __shfl_xor(v, m);
+    const int oi = __shfl_xor(idx, m);
+    if (ov < v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
+  }
+  const int alpha = idx - 16, avg = W.cflAvg, dcv = W.cflDc;
+  tx_sync(c);
+  for (int k = lane; k < n; k += 64) {
+    const int i = k >> g.lw, j = k & (w - 1);
+    g.px[(g.sy + i) * g.S + g.sx + j] = (uint8_t)d_clip1(dcv + d_round2signed(alpha * (W.cfl[k] - avg), 6));
+  }
+  tx_sync(c);
+  return alpha;
+}
+
+/* The chroma plane of small block b by one wave: its single transform type
+ * (code_plane), levels, reconstruction, flags. */
+DEVI void si_code_chroma(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane, int32_t *__restrict__ coefs,
+                         const int32_t *__restrict__ mats, SmallTx &T) {
+  const TxCtx c = wave_ctx();
+  const SiGeo g = si_geo(b, plane);
+  const int set = get_tx_set(g.txSz, 0, fp.reduced_tx_set);
+  int t = Mode_To_Txfm[b.uv_mode];
+  if (Tx_Size_Sqr_Up[g.txSz] > TX_32X32 || !in_set(set, t, 0)) t = DCT_DCT;
+  long long cost;
+  const int eob = tx_eval(c, fp, g.spx, g.px, g.S, g.sx, g.sy, g.txSz, plane, t, true, T.resid, SRS, T.tmp, T.coef,
+                          T.q, T.dq, SQS, mats, &cost);
+  const int type = eob > 0 ? t : DCT_DCT;
+  const int tw = d_min(32, 1 << g.lw), th = d_min(32, 1 << g.lh);
+  int32_t *co = coefs + b.coef_off[plane];
+  for (int k = c.tid; k < tw * th; k += 64) co[k] = T.q[k];
+  if (eob > 0) {
+    tx_dequant(c, T.q, T.dq, SQS, g.txSz, fp.dc_q[plane], fp.ac_q[plane]);
+    tx_inverse(c, T.dq, SQS, T.resid, SRS, g.txSz, type);
+    tx_add(c, g.px, g.S, g.sx, g.sy, g.lw, g.lh, type, T.resid, SRS);
+  }
+  if (c.tid == 0) {
+    b.eob[plane] = eob > 0;
+    si_mark(fp, b, plane);
+  }
+}
+
+/* One pipeline step: luma of the block in slot ls (lk: 1 code, 2 mark
+ * only, 0 none) and chroma of the block in slot cs (ck likewise). */
+DEVI void si_step(const Mav1FrameParams &fp, int ls, int lk, int cs, int ck, int32_t *__restrict__ coefs,
+                  const int32_t *__restrict__ mats) {
+  RLds &L = gL;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  /* P1: predictions (luma on wave 0, U and V on their waves) */
+  const int cplane = wave == SI_WU ? 1 : wave == SI_WV ? 2 : 0;
+  if ((lk == 1 && wave == 0) || (ck == 1 && cplane)) {
+    Mav1BlockInfo &b = L.slot[cplane ? cs : ls];
+    if (tid == 0) {
+      b.tx_depth = 0;
+      b.interp_filter &= 0x30;
+    }
+    si_predict(fp, b, cplane, L.si[cplane]);
+    if (cplane == 1 && b.uv_mode == UV_CFL_PRED) {
+      const int a = si_cfl_apply(b, 1, L.si[1], 0);
+      if ((tid & 63) == 0) b.cfl_alpha_u = a;
+    }
+  }
+  __syncthreads();
+  /* P2: the luma transform types, one per wave */
+  int bestType = DCT_DCT, bestEob = 0;
+  const SiGeo gy = lk == 1 ? si_geo(L.slot[ls], 0) : SiGeo{};
+  if (lk == 1) {
+    const Mav1BlockInfo &b = L.slot[ls];
+    const int set = get_tx_set(gy.txSz, 0, fp.reduced_tx_set);
+    int cands[16], nc = 0;
+    if (!fp.tx_search || set == TX_SET_DCTONLY) cands[nc++] = DCT_DCT;
+    else
+      for (int t = 0; t < 16; t++)
+        if (in_set(set, t, 0)) cands[nc++] = t;
+    (void)b;
+    const bool direct = nc == 1;
+    const int tw = d_min(32, 1 << gy.lw), th = d_min(32, 1 << gy.lh);
+    const TxCtx c = wave_ctx();
+    long long bestCost = -1;
+    for (int r0 = 0; r0 < nc; r0 += RW) {
+      const int ci = r0 + wave;
+      if (ci < nc) {
+        SmallTx &W = L.u.sm[wave];
+        long long cost;
+        const int eob = tx_eval(c, fp, gy.spx, gy.px, gy.S, gy.sx, gy.sy, gy.txSz, 0, cands[ci], direct, W.resid, SRS,
+                                W.tmp, W.coef, W.q, W.dq, SQS, mats, &cost);
+        if (c.tid == 0) {
+          L.candCost[ci] = cost;
+          L.candEob[ci] = eob;
+        }
+      }
+      __syncthreads();
+      int pick = -1;
+      for (int k = r0; k < d_min(nc, r0 + RW); k++)
+        if (bestCost < 0 || L.candCost[k] < bestCost) {
+          bestCost = L.candCost[k];
+          pick = k;
+        }
+      if (pick >= 0) {
+        bestEob = L.candEob[pick];
+        bestType = bestEob > 0 ? cands[pick] : DCT_DCT;
+        const int32_t *from = L.u.sm[pick - r0].q;
+        for (int k = tid; k < tw * th; k += RT) L.qbest[k] = from[k];
+      }
+      __syncthreads();
+    }
+  }
+  /* P3: the chosen luma type on wave 0; chroma coding; marks */
+  if (lk == 1 && wave == 0) {
+    Mav1BlockInfo &b = L.slot[ls];
+    const TxCtx c = wave_ctx();
+    SmallTx &W = L.u.sm[0];
+    const int tw = d_min(32, 1 << gy.lw), th = d_min(32, 1 << gy.lh);
+    int32_t *co = coefs + b.coef_off[0];
+    for (int k = c.tid; k < tw * th; k += 64) co[k] = L.qbest[k];
+    if (bestEob > 0) {
+      tx_dequant(c, L.qbest, W.dq, SQS, gy.txSz, fp.dc_q[0], fp.ac_q[0]);
+      tx_inverse(c, W.dq, SQS, W.resid, SRS, gy.txSz, bestType);
+      tx_add(c, gy.px, gy.S, gy.sx, gy.sy, gy.lw, gy.lh, bestType, W.resid, SRS);
+    }
+    if (c.tid == 0) {
+      b.tx_types_lo = (int32_t)(((uint32_t)b.tx_types_lo & ~15u) | (uint32_t)bestType);
+      b.tx_type = bestType;
+      b.eob[0] = bestEob > 0;
+      si_mark(fp, b, 0);
+    }
+  }
+  if (lk == 2 && tid == 0) si_mark(fp, L.slot[ls], 0);
+  if (ck == 1 && cplane) {
+    Mav1BlockInfo &b = L.slot[cs];
+    if (cplane == 2 && b.uv_mode == UV_CFL_PRED) {
+      const int a = si_cfl_apply(b, 2, L.si[2], b.cfl_alpha_u == 0);
+      if ((tid & 63) == 0) b.cfl_alpha_v = a;
+    }
+    si_code_chroma(fp, b, cplane, coefs, mats, L.u.sm[wave]);
+  }
+  if (ck == 2 && tid == 0) {
+    si_mark(fp, L.slot[cs], 1);
+    si_mark(fp, L.slot[cs], 2);
+  }
+  __syncthreads();
+}
+
 DEVI void stage_smats(const int32_t *__restrict__ mats) {
   for (int q = threadIdx.x; q < TX_SMATS; q += RT) {
     const int kind = q / 336, r = q - kind * 336;
@@ -988,13 +1266,35 @@ __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Ma
     }
   }
   __syncthreads();
-  for (int i = 0; i < S.num_blocks; i++) {
-    Mav1BlockInfo *g = blocks + S.first_block + i;
-    if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)g)[tid];
+  /* blocks in decode order: small intra blocks through the two-stage
+   * pipeline (chroma one step behind luma), inter blocks' flags through
+   * the same streams, any other intra block by the whole workgroup once the
+   * pipeline has drained */
+  int pend = -1, pendKind = 0, pendIdx = 0;
+  for (int i = 0; i <= S.num_blocks; i++) {
+    const int sl = pend == 0 ? 1 : 0;
+    if (i < S.num_blocks && tid < 27)
+      ((int32_t *)&L.slot[sl])[tid] = ((const int32_t *)(blocks + S.first_block + i))[tid];
     __syncthreads();
-    const bool inter = L.blk.is_inter;
-    recon_block(fp, src, rec, true, ref, gold, coefs, mats, true, inter);
-    if (!inter && tid < 27) ((int32_t *)g)[tid] = ((const int32_t *)&L.blk)[tid];
+    const int kind = i == S.num_blocks ? 0 : L.slot[sl].is_inter ? 2 : si_eligible(fp, L.slot[sl]) ? 1 : 3;
+    if (kind == 1 || kind == 2) {
+      si_step(fp, sl, kind, pend < 0 ? 0 : pend, pend < 0 ? 0 : pendKind, coefs, mats);
+    } else if (pend >= 0) {
+      si_step(fp, 0, 0, pend, pendKind, coefs, mats);
+    }
+    if (pend >= 0 && pendKind == 1 && tid < 27)
+      ((int32_t *)(blocks + S.first_block + pendIdx))[tid] = ((const int32_t *)&L.slot[pend])[tid];
+    pend = -1;
+    if (kind == 1 || kind == 2) {
+      pend = sl;
+      pendKind = kind;
+      pendIdx = i;
+    } else if (kind == 3) {
+      if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)&L.slot[sl])[tid];
+      __syncthreads();
+      recon_block(fp, src, rec, true, ref, gold, coefs, mats, true, false);
+      if (tid < 27) ((int32_t *)(blocks + S.first_block + i))[tid] = ((const int32_t *)&L.blk)[tid];
+    }
     __syncthreads();
   }
   /* the superblock back into the frame */

@@ -256,3 +256,23 @@ def cdef_select(fp, src, cdefbuf, rec):
     lib.oracle_cdef_select(C.byref(fp), mav1hip.ptr(src), mav1hip.ptr(cdefbuf), cdefbuf.shape[1], cdefbuf.shape[0],
                            mav1hip.ptr(out), mav1hip.ptr(choice))
     return out, choice
+
+
+def force_intra_modes(fp, blocks, sbs, seed):
+    """Random luma / chroma intra modes and angles on every intra block (CfL
+    on blocks up to 32x32); returns (blocks, CfL count, directional count)."""
+    rng = np.random.default_rng(seed)
+    blocks = blocks.copy()
+    n_cfl = n_dir = 0
+    for s in range(fp.sb_cols * fp.sb_rows):
+        for k in range(sbs[s, 0], sbs[s, 0] + sbs[s, 1]):
+            b = blocks[k]
+            if b[3]:
+                continue
+            ym = int(rng.integers(0, 13))
+            b[4], b[6] = ym, int(rng.integers(-3, 4)) if 1 <= ym <= 8 else 0
+            um = int(rng.integers(0, 14 if b[2] <= 9 else 13))
+            b[5], b[7] = um, int(rng.integers(-3, 4)) if 1 <= um <= 8 else 0
+            n_cfl += um == 13
+            n_dir += 1 <= ym <= 8
+    return blocks, n_cfl, n_dir

@@ -70,3 +70,17 @@ def test_recon_other_presets(gpu, variant):
     else:
         args = _setup(w, h, "key", 7, turbo=0, intra_edge_filter=0)
     _check(*args)
+
+
+@pytest.mark.parametrize("kind,w,h,seed", [("key", 352, 288, 11), ("key", 200, 136, 12), ("arf", 352, 288, 13)])
+def test_recon_forced_intra_modes(gpu, kind, w, h, seed):
+    """Every intra block gets a random luma mode and angle, and a random
+    chroma mode (CfL on blocks up to 32x32) and angle: all prediction paths,
+    the edge filter and upsampling cases, CfL's alpha search (U, then V
+    with alpha 0 excluded when U took 0) against the oracle."""
+    print(f"\n[recon] forced modes {kind} {w}x{h} seed {seed}", flush=True)
+    cfg, fp, frames, blocks, sbs = _setup(w, h, kind, seed)
+    blocks, n_cfl, n_dir = libs.force_intra_modes(fp, blocks, sbs, seed)
+    print(f"  intra blocks with CfL {n_cfl}, directional luma {n_dir}", flush=True)
+    assert n_cfl > 0 and n_dir > 0
+    _check(cfg, fp, frames, blocks, sbs)

@@ -208,3 +208,19 @@ def test_forward_matrices_equal_reference():
     mine = mav1hip.forward_matrices()
     assert np.array_equal(mine, libs.ref_mats())
     assert np.count_nonzero(mine) > 3500
+
+
+@needs_ref
+@pytest.mark.parametrize("kind,w,h,seed", [("key", 352, 288, 11), ("arf", 352, 288, 13)])
+def test_recon_forced_intra_modes(kind, w, h, seed):
+    """Every intra mode, angle delta and CfL on the reference's own recon."""
+    fp, frames, blocks, sbs = _pipeline(w, h, kind, seed)
+    blocks, n_cfl, n_dir = libs.force_intra_modes(fp, blocks, sbs, seed)
+    assert n_cfl > 0 and n_dir > 0
+    blocks = libs.edge_types(libs.Impl("oracle"), fp, blocks, sbs)
+    mats = libs.ref_mats()
+    a = libs.Impl("oracle").recon(fp, blocks, sbs, frames[1], frames[0], frames[2], mats)
+    b = libs.Impl("ref").recon(fp, blocks, sbs, frames[1], frames[0], frames[2], mats)
+    assert np.array_equal(a[0], b[0]), "reconstruction differs"
+    assert np.array_equal(a[1], b[1]), "coefficients differ"
+    assert np.array_equal(a[2], b[2]), "block outputs differ"
