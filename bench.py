@@ -74,8 +74,6 @@ def algorithmic_bytes(kernel, fp, nref):
     if kernel == "recon_inter":
         # prediction reads of the references, source read, reconstruction and levels written, blocks in/out
         return F * (nref + 1) + F + 4 * F + nsb * 64 * 108 * 2
-    if kernel == "recon_intra":
-        return nsb * 64 * 108 // max(1, fp.tile_w_sb + 2 * (fp.sb_rows - 1))
     if kernel == "edge_types":
         return nsb * 64 * 108 * 2 + 2 * fp.mi_rows * fp.mi_cols
     mi = fp.mi_rows * fp.mi_cols
@@ -94,6 +92,39 @@ def algorithmic_bytes(kernel, fp, nref):
     if kernel == "lr_apply":
         return 3 * F
     return 0
+
+
+# Block widths / heights by bsize (BLOCK_4X4 .. BLOCK_64X16).
+BW = [4, 4, 8, 8, 8, 16, 16, 16, 32, 32, 32, 64, 64, 64, 128, 128, 4, 16, 8, 32, 16, 64]
+BH = [4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 64, 32, 64, 128, 64, 128, 16, 4, 32, 8, 64, 16]
+
+# HBM traffic per recon_intra launch over the ALTREF / leaf mix, from the
+# FETCH_SIZE and WRITE_SIZE passes of profiles/recon_intra_1080p_turbo_pmc.md,
+# scaled by the frame copy's factors there (fetch x1.985, write x1.0).
+RECON_INTRA_TRAFFIC = 277976
+
+
+def intra_phase_bytes(fp, blocks, sbs):
+    """Bytes the intra phase of one frame must move: every superblock's
+    decode-order scan of its block records (is_inter, 4 B each); for each
+    superblock holding intra blocks its reconstruction tile with the row
+    above and the columns left (luma 132 + 64 x 68, chroma 68 + 32 x 36
+    each), its source and the tile written back (6144 B each), the short
+    transform matrices (4032 B), its block records (108 B) and the intra
+    ones written back, and the intra blocks' levels (4 B per coefficient:
+    luma Min(32, w) x Min(32, h), chroma likewise at half size)."""
+    total = 0
+    for s in range(fp.sb_cols * fp.sb_rows):
+        first, n = int(sbs[s, 0]), int(sbs[s, 1])
+        b = blocks[first:first + n]
+        intra = b[b[:, 3] == 0]
+        total += 4 * n
+        if len(intra):
+            total += (132 + 64 * 68) + 2 * (68 + 32 * 36) + 6144 + 6144 + 4032 + 108 * (n + len(intra))
+            for bs in intra[:, 2]:
+                w, h = BW[bs], BH[bs]
+                total += 4 * (min(32, w) * min(32, h) + 2 * min(32, w // 2) * min(32, h // 2))
+    return total
 
 
 def main():
@@ -178,6 +209,16 @@ def main():
     ctx.sync()
     elapsed = time.perf_counter() - t0
     ctx.timing(False)
+    # the intra phase's algorithmic bytes: one more ALTREF + leaf pair,
+    # untimed, its block records read back after each frame
+    intra_bytes = []
+    for kind, f, last, alt in coding_order(args.warmup + args.steps + 2)[-2:]:
+        step(kind, f, last, alt)
+        ctx.sync()
+        nsb = params[kind].sb_cols * params[kind].sb_rows
+        blk = ctx.download(mav1hip.BUF_BLOCKS, np.int32, nsb * 64 * 27).reshape(-1, 27)
+        sbi = ctx.download(mav1hip.BUF_SBS, np.int32, nsb * 89).reshape(-1, 89)
+        intra_bytes.append(intra_phase_bytes(params[kind], blk, sbi))
     kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3", "analyze_level0",
                "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table", "edge_types", "recon_inter",
                "recon_intra", "filter_info", "deblock", "cdef", "cdef_select", "lr_fit", "lr_path", "lr_chain", "lr_apply"]
@@ -192,6 +233,9 @@ def main():
     nref_avg = np.mean([2 if k == "leaf" else 1 for k, *_ in order[args.warmup:]])
     if dom in ("qpel_build", "dec_build"):
         per_launch = algorithmic_bytes(dom, params["arf"], 1)
+    elif dom == "recon_intra":
+        steps_per_frame = params["arf"].tile_w_sb + 2 * (params["arf"].sb_rows - 1)
+        per_launch = np.mean(intra_bytes) / steps_per_frame
     else:
         per_launch = np.mean([algorithmic_bytes(dom, params[k], 2 if k == "leaf" else 1)
                               for k, *_ in order[args.warmup:]])
@@ -231,10 +275,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
-            "traffic": None,
+            "traffic": RECON_INTRA_TRAFFIC if dom == "recon_intra" else None,
             "algorithmic_bytes_per_launch": int(per_launch),
             "avg_launch_ms": round(avg_s * 1e3, 5),
-            "note": "integer-VALU bound (SAD/SATD), no MFMA: the HBM fraction is small by construction",
+            "note": ("latency-bound dependency chain (superblock anti-diagonals, blocks in decode order), "
+                     "integer VALU, no MFMA: the HBM fraction is small by construction" if dom == "recon_intra" else
+                     "integer-VALU bound, no MFMA: the HBM fraction is small by construction"),
         },
         "cpu_baseline": cpu,
     }
