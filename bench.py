@@ -234,8 +234,11 @@ def main():
     if dom in ("qpel_build", "dec_build"):
         per_launch = algorithmic_bytes(dom, params["arf"], 1)
     elif dom == "recon_intra":
-        steps_per_frame = params["arf"].tile_w_sb + 2 * (params["arf"].sb_rows - 1)
-        per_launch = np.mean(intra_bytes) / steps_per_frame
+        # one launch per anti-diagonal of each tile column
+        fpa = params["arf"]
+        launches = sum(min(fpa.tile_w_sb, fpa.sb_cols - c0) + 2 * (fpa.sb_rows - 1)
+                       for c0 in range(0, fpa.sb_cols, fpa.tile_w_sb))
+        per_launch = np.mean(intra_bytes) / launches
     else:
         per_launch = np.mean([algorithmic_bytes(dom, params[k], 2 if k == "leaf" else 1)
                               for k, *_ in order[args.warmup:]])

@@ -23,6 +23,8 @@ struct TimedLaunch {
 
 }  // namespace
 
+#define MAX_TILE_STREAMS 4
+
 struct mav1hip_ctx {
   Mav1HipConfig cfg;
   int mi_cols, mi_rows, sb_cols, sb_rows, num_sbs, tile_w_sb;
@@ -30,6 +32,11 @@ struct mav1hip_ctx {
   size_t frame_bytes;
   int device;
   hipStream_t stream;
+  /* the intra phase's tile columns run on their own streams, forked from
+   * and joined back into `stream` by events */
+  hipStream_t tstream[MAX_TILE_STREAMS];
+  hipEvent_t tfork, tjoin[MAX_TILE_STREAMS];
+  int ntstream;
   char err[512];
   char devname[256];
   Mav1FrameParams fp;
@@ -97,22 +104,27 @@ static hipEvent_t take_event(mav1hip_ctx *c) {
   return e;
 }
 
-/* Records the launch of `name` between two events when timing is on. */
+/* Records the launch of `name` on stream st between two events when timing
+ * is on. */
 template <class F>
-static int timed(mav1hip_ctx *c, const char *name, F &&launch) {
+static int timed_on(mav1hip_ctx *c, hipStream_t st, const char *name, F &&launch) {
   hipEvent_t a = nullptr, b = nullptr;
   if (c->timing) {
     a = take_event(c);
     b = take_event(c);
-    if (a && b) HIPCHK(c, hipEventRecord(a, c->stream));
+    if (a && b) HIPCHK(c, hipEventRecord(a, st));
   }
   hipError_t e = launch();
   if (e != hipSuccess) return fail(c, MAV1HIP_EDEVICE, "launch of %s: %s", name, hipGetErrorString(e));
   if (c->timing && a && b) {
-    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipEventRecord(b, st));
     c->pending.push_back({name, a, b});
   }
   return MAV1HIP_OK;
+}
+template <class F>
+static int timed(mav1hip_ctx *c, const char *name, F &&launch) {
+  return timed_on(c, c->stream, name, launch);
 }
 
 static int collect_timing(mav1hip_ctx *c) {
@@ -166,6 +178,19 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
     if (err && errlen) snprintf(err, errlen, "cannot create a HIP stream");
     delete c;
     return nullptr;
+  }
+  {
+    const int tiles = (c->sb_cols + c->tile_w_sb - 1) / c->tile_w_sb;
+    c->ntstream = tiles < MAX_TILE_STREAMS ? tiles : MAX_TILE_STREAMS;
+    bool ok = hipEventCreateWithFlags(&c->tfork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < c->ntstream && ok; i++)
+      ok = hipStreamCreate(&c->tstream[i]) == hipSuccess &&
+           hipEventCreateWithFlags(&c->tjoin[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      if (err && errlen) snprintf(err, errlen, "cannot create the tile streams");
+      mav1hip_destroy(c);
+      return nullptr;
+    }
   }
   hipDeviceProp_t prop;
   c->num_cu = 256;
@@ -294,6 +319,11 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipFree(c->d_lrtype);
   (void)hipFree(c->d_lrreruns);
   (void)hipFree(c->d_pool);
+  for (int i = 0; i < MAX_TILE_STREAMS; i++) {
+    if (c->tstream[i]) (void)hipStreamDestroy(c->tstream[i]);
+    if (c->tjoin[i]) (void)hipEventDestroy(c->tjoin[i]);
+  }
+  if (c->tfork) (void)hipEventDestroy(c->tfork);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -477,13 +507,25 @@ int mav1hip_launch_recon(mav1hip_ctx *c) {
                               c->d_coefs, c->d_mats, c->d_rwork, 2 * c->num_cu);
   });
   if (rc) return rc;
-  const int steps = recon_intra_steps(fp);
+  /* intra phase: tile columns are independent, so each runs its
+   * anti-diagonals on its own stream (tile k on stream k mod ntstream) */
+  const int tiles = (fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb, steps = recon_intra_steps(fp);
+  HIPCHK(c, hipEventRecord(c->tfork, c->stream));
+  for (int i = 0; i < c->ntstream; i++) HIPCHK(c, hipStreamWaitEvent(c->tstream[i], c->tfork, 0));
   for (int t = 0; t < steps; t++)
-    if ((rc = timed(c, "recon_intra", [&] {
-           return launch_recon_intra(c->stream, fp, t, c->d_blocks, c->d_sbs, c->b_src, c->d_rec, c->b_ref, c->b_gold,
-                                     c->d_coefs, c->d_mats);
-         })))
-      return rc;
+    for (int k = 0; k < tiles; k++) {
+      if (t >= recon_intra_tile_steps(fp, k)) continue;
+      hipStream_t st = c->tstream[k % c->ntstream];
+      if ((rc = timed_on(c, st, "recon_intra", [&] {
+             return launch_recon_intra(st, fp, t, k, c->d_blocks, c->d_sbs, c->b_src, c->d_rec, c->b_ref, c->b_gold,
+                                       c->d_coefs, c->d_mats);
+           })))
+        return rc;
+    }
+  for (int i = 0; i < c->ntstream; i++) {
+    HIPCHK(c, hipEventRecord(c->tjoin[i], c->tstream[i]));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->tjoin[i], 0));
+  }
   return MAV1HIP_OK;
 }
 

@@ -38,7 +38,10 @@ hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
                               const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold,
                               int32_t *coefs, const int32_t *mats, int *work, int workgroups);
 int recon_intra_steps(const Mav1FrameParams &fp);
-hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, int t, Mav1BlockInfo *blocks,
+/* Anti-diagonals of one tile column: its width in superblocks + 2 (rows - 1). */
+int recon_intra_tile_steps(const Mav1FrameParams &fp, int tile);
+/* Anti-diagonal t of tile column `tile`. */
+hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, int t, int tile, Mav1BlockInfo *blocks,
                               const Mav1SbInfo *sbs, const uint8_t *src, uint8_t *rec, const uint8_t *ref,
                               const uint8_t *gold, int32_t *coefs, const int32_t *mats);
 

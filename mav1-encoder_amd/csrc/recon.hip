@@ -1218,17 +1218,16 @@ __global__ __launch_bounds__(RT) void k_recon_inter(const Mav1FrameParams fp, Ma
   }
 }
 
-/* Intra phase: superblocks of anti-diagonal t, one workgroup each; grid =
- * tile columns x superblock rows. */
+/* Intra phase: the superblocks of anti-diagonal t of one tile column, one
+ * workgroup each; grid = superblock rows. */
 __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
                                                     const Mav1SbInfo *__restrict__ sbs, const uint8_t *__restrict__ src,
                                                     uint8_t *rec, const uint8_t *__restrict__ ref,
                                                     const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
-                                                    const int32_t *__restrict__ mats, int t) {
+                                                    const int32_t *__restrict__ mats, int t, int tile) {
   RLds &L = gL;
   const int tid = threadIdx.x;
-  const int tiles = (fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb;
-  const int tile = blockIdx.x % tiles, r = blockIdx.x / tiles;
+  const int r = blockIdx.x;
   const int c0 = tile * fp.tile_w_sb, cl = t - 2 * r;
   if (cl < 0 || cl >= fp.tile_w_sb || c0 + cl >= fp.sb_cols || r >= fp.sb_rows) return;
   const int sbIdx = r * fp.sb_cols + c0 + cl;
@@ -1358,11 +1357,15 @@ hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
 
 int recon_intra_steps(const Mav1FrameParams &fp) { return fp.tile_w_sb + 2 * (fp.sb_rows - 1); }
 
-hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, int t, Mav1BlockInfo *blocks,
+int recon_intra_tile_steps(const Mav1FrameParams &fp, int tile) {
+  const int rest = fp.sb_cols - tile * fp.tile_w_sb, w = rest < fp.tile_w_sb ? rest : fp.tile_w_sb;
+  return w + 2 * (fp.sb_rows - 1);
+}
+
+hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, int t, int tile, Mav1BlockInfo *blocks,
                               const Mav1SbInfo *sbs, const uint8_t *src, uint8_t *rec, const uint8_t *ref,
                               const uint8_t *gold, int32_t *coefs, const int32_t *mats) {
-  const int tiles = (fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb;
-  hipLaunchKernelGGL(k_recon_intra, dim3(tiles * fp.sb_rows), dim3(RT), 0, st, fp, blocks, sbs, src, rec, ref, gold,
-                     coefs, mats, t);
+  hipLaunchKernelGGL(k_recon_intra, dim3(fp.sb_rows), dim3(RT), 0, st, fp, blocks, sbs, src, rec, ref, gold, coefs,
+                     mats, t, tile);
   return hipGetLastError();
 }
