@@ -101,7 +101,7 @@ BH = [4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 64, 32, 64, 128, 64, 128, 16, 4, 32, 8,
 # HBM traffic per recon_intra launch over the ALTREF / leaf mix, from the
 # FETCH_SIZE and WRITE_SIZE passes of profiles/recon_intra_1080p_turbo_pmc.md,
 # scaled by the frame copy's factors there (fetch x1.985, write x1.0).
-RECON_INTRA_TRAFFIC = 277976
+RECON_INTRA_TRAFFIC = 5798915
 
 
 def intra_phase_bytes(fp, blocks, sbs):
@@ -221,7 +221,7 @@ def main():
         intra_bytes.append(intra_phase_bytes(params[kind], blk, sbi))
     kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3", "analyze_level0",
                "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table", "edge_types", "recon_inter",
-               "recon_intra", "filter_info", "deblock", "cdef", "cdef_select", "lr_fit", "lr_path", "lr_chain", "lr_apply"]
+               "intra_sched", "recon_intra", "filter_info", "deblock", "cdef", "cdef_select", "lr_fit", "lr_path", "lr_chain", "lr_apply"]
     stats = {k: ctx.timing_get(k) for k in kernels}
     fps = args.steps / elapsed
 
@@ -234,11 +234,7 @@ def main():
     if dom in ("qpel_build", "dec_build"):
         per_launch = algorithmic_bytes(dom, params["arf"], 1)
     elif dom == "recon_intra":
-        # one launch per anti-diagonal of each tile column
-        fpa = params["arf"]
-        launches = sum(min(fpa.tile_w_sb, fpa.sb_cols - c0) + 2 * (fpa.sb_rows - 1)
-                       for c0 in range(0, fpa.sb_cols, fpa.tile_w_sb))
-        per_launch = np.mean(intra_bytes) / launches
+        per_launch = np.mean(intra_bytes)  # one launch per frame
     else:
         per_launch = np.mean([algorithmic_bytes(dom, params[k], 2 if k == "leaf" else 1)
                               for k, *_ in order[args.warmup:]])
@@ -281,7 +277,7 @@ def main():
             "traffic": RECON_INTRA_TRAFFIC if dom == "recon_intra" else None,
             "algorithmic_bytes_per_launch": int(per_launch),
             "avg_launch_ms": round(avg_s * 1e3, 5),
-            "note": ("latency-bound dependency chain (superblock anti-diagonals, blocks in decode order), "
+            "note": ("latency-bound dependency chain (superblocks in dataflow order, blocks in decode order), "
                      "integer VALU, no MFMA: the HBM fraction is small by construction" if dom == "recon_intra" else
                      "integer-VALU bound, no MFMA: the HBM fraction is small by construction"),
         },

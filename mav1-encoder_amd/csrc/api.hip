@@ -23,8 +23,6 @@ struct TimedLaunch {
 
 }  // namespace
 
-#define MAX_TILE_STREAMS 4
-
 struct mav1hip_ctx {
   Mav1HipConfig cfg;
   int mi_cols, mi_rows, sb_cols, sb_rows, num_sbs, tile_w_sb;
@@ -32,11 +30,7 @@ struct mav1hip_ctx {
   size_t frame_bytes;
   int device;
   hipStream_t stream;
-  /* the intra phase's tile columns run on their own streams, forked from
-   * and joined back into `stream` by events */
-  hipStream_t tstream[MAX_TILE_STREAMS];
-  hipEvent_t tfork, tjoin[MAX_TILE_STREAMS];
-  int ntstream;
+
   char err[512];
   char devname[256];
   Mav1FrameParams fp;
@@ -60,6 +54,7 @@ struct mav1hip_ctx {
   int *d_lrtype, *d_lrreruns;
   uint8_t *d_decgrid;
   int *d_rwork;
+  int *d_iwork; /* intra phase schedule and completion flags */
   int num_cu;
   size_t decgrid_bytes;
   int lr_units;
@@ -179,19 +174,7 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
     delete c;
     return nullptr;
   }
-  {
-    const int tiles = (c->sb_cols + c->tile_w_sb - 1) / c->tile_w_sb;
-    c->ntstream = tiles < MAX_TILE_STREAMS ? tiles : MAX_TILE_STREAMS;
-    bool ok = hipEventCreateWithFlags(&c->tfork, hipEventDisableTiming) == hipSuccess;
-    for (int i = 0; i < c->ntstream && ok; i++)
-      ok = hipStreamCreate(&c->tstream[i]) == hipSuccess &&
-           hipEventCreateWithFlags(&c->tjoin[i], hipEventDisableTiming) == hipSuccess;
-    if (!ok) {
-      if (err && errlen) snprintf(err, errlen, "cannot create the tile streams");
-      mav1hip_destroy(c);
-      return nullptr;
-    }
-  }
+
   hipDeviceProp_t prop;
   c->num_cu = 256;
   if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
@@ -226,6 +209,18 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   rc |= dmalloc(c, (void **)&c->d_choice, (size_t)c->num_sbs);
   c->decgrid_bytes = (size_t)((c->mi_rows + 1) >> 1) * ((c->mi_cols + 1) >> 1) * 8;
   rc |= dmalloc(c, (void **)&c->d_rwork, sizeof(int) * (2 + (size_t)c->num_sbs * MAV1_MAX_BLOCKS_PER_SB));
+  {
+    Mav1FrameParams g = {};
+    g.sb_cols = c->sb_cols;
+    g.sb_rows = c->sb_rows;
+    if (!intra_sched_fits(g)) {
+      if (err && errlen) snprintf(err, errlen, "frame of %d x %d superblocks is too large for the intra scheduler",
+                                  c->sb_cols, c->sb_rows);
+      mav1hip_destroy(c);
+      return nullptr;
+    }
+    rc |= dmalloc(c, (void **)&c->d_iwork, sizeof(int) * intra_work_ints(g));
+  }
   rc |= dmalloc(c, (void **)&c->d_decgrid, c->decgrid_bytes);
   {
     /* restoration units of 64x64 per plane (count_units_in_frame) */
@@ -250,6 +245,7 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
    * are zero-filled pages) */
   (void)hipMemsetAsync(c->d_planes, 0, c->planes_bytes, c->stream);
   (void)hipMemsetAsync(c->d_mvfield, 0, mvb, c->stream);
+  if (c->d_iwork) (void)hipMemsetAsync(c->d_iwork, 0, sizeof(int) * 4, c->stream);
   (void)hipMemsetAsync(c->d_mvgold, 0, mvb, c->stream);
   (void)hipMemsetAsync(c->d_acosts, 0, sizeof(long long) * MAV1_A_ITEMS * (size_t)c->num_sbs, c->stream);
   (void)hipMemsetAsync(c->d_ainfos, 0, sizeof(Mav1BlockInfo) * MAV1_A_ITEMS * (size_t)c->num_sbs, c->stream);
@@ -313,17 +309,13 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipFree(c->d_choice);
   (void)hipFree(c->d_decgrid);
   (void)hipFree(c->d_rwork);
+  (void)hipFree(c->d_iwork);
   (void)hipFree(c->d_lrunits);
   (void)hipFree(c->d_lrstats);
   (void)hipFree(c->d_lrchain);
   (void)hipFree(c->d_lrtype);
   (void)hipFree(c->d_lrreruns);
   (void)hipFree(c->d_pool);
-  for (int i = 0; i < MAX_TILE_STREAMS; i++) {
-    if (c->tstream[i]) (void)hipStreamDestroy(c->tstream[i]);
-    if (c->tjoin[i]) (void)hipEventDestroy(c->tjoin[i]);
-  }
-  if (c->tfork) (void)hipEventDestroy(c->tfork);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -507,26 +499,12 @@ int mav1hip_launch_recon(mav1hip_ctx *c) {
                               c->d_coefs, c->d_mats, c->d_rwork, 2 * c->num_cu);
   });
   if (rc) return rc;
-  /* intra phase: tile columns are independent, so each runs its
-   * anti-diagonals on its own stream (tile k on stream k mod ntstream) */
-  const int tiles = (fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb, steps = recon_intra_steps(fp);
-  HIPCHK(c, hipEventRecord(c->tfork, c->stream));
-  for (int i = 0; i < c->ntstream; i++) HIPCHK(c, hipStreamWaitEvent(c->tstream[i], c->tfork, 0));
-  for (int t = 0; t < steps; t++)
-    for (int k = 0; k < tiles; k++) {
-      if (t >= recon_intra_tile_steps(fp, k)) continue;
-      hipStream_t st = c->tstream[k % c->ntstream];
-      if ((rc = timed_on(c, st, "recon_intra", [&] {
-             return launch_recon_intra(st, fp, t, k, c->d_blocks, c->d_sbs, c->b_src, c->d_rec, c->b_ref, c->b_gold,
-                                       c->d_coefs, c->d_mats);
-           })))
-        return rc;
-    }
-  for (int i = 0; i < c->ntstream; i++) {
-    HIPCHK(c, hipEventRecord(c->tjoin[i], c->tstream[i]));
-    HIPCHK(c, hipStreamWaitEvent(c->stream, c->tjoin[i], 0));
-  }
-  return MAV1HIP_OK;
+  rc = timed(c, "intra_sched", [&] { return launch_intra_sched(c->stream, fp, c->d_blocks, c->d_sbs, c->d_iwork); });
+  if (rc) return rc;
+  return timed(c, "recon_intra", [&] {
+    return launch_recon_intra(c->stream, fp, c->d_blocks, c->d_sbs, c->b_src, c->d_rec, c->b_ref, c->b_gold,
+                              c->d_coefs, c->d_mats, c->d_iwork, c->num_cu);
+  });
 }
 
 int mav1hip_launch_filter_info(mav1hip_ctx *c) {
@@ -737,6 +715,15 @@ int mav1hip_edge_types(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo 
   return MAV1HIP_OK;
 }
 
+/* After a synchronisation: the intra phase's error word (a dependency wait
+ * that ran out, which would leave the reconstruction wrong). */
+static int check_intra(mav1hip_ctx *c) {
+  int e = 0;
+  HIPCHK(c, hipMemcpy(&e, c->d_iwork + 2, sizeof e, hipMemcpyDeviceToHost));
+  if (e) return fail(c, MAV1HIP_EDEVICE, "intra phase: a superblock dependency wait ran out");
+  return MAV1HIP_OK;
+}
+
 int mav1hip_recon(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
                   const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold, int32_t *coefs) {
   if (!c || !fp || !blocks || !sbs || !src || !rec || !coefs) return fail(c, MAV1HIP_EINVAL, "recon: missing buffer");
@@ -760,13 +747,13 @@ int mav1hip_recon(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo *bloc
   HIPCHK(c, hipMemcpyAsync(rec, c->d_rec, c->frame_bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(coefs, c->d_coefs, ncoef, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  return MAV1HIP_OK;
+  return check_intra(c);
 }
 
 int mav1hip_sync(mav1hip_ctx *c) {
   if (!c) return MAV1HIP_EINVAL;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  return MAV1HIP_OK;
+  return check_intra(c);
 }
 
 int mav1hip_motion_search(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *ref,

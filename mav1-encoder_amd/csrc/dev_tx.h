@@ -534,12 +534,16 @@ DEVI void tx_forward(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp,
     tx_forward_m(c, resid, rs, tmp, coef, txSz, mats + (rk * 5 + lw - 2) * 4096, mats + (ck * 5 + lh - 2) * 4096);
 }
 
-/* Dead-zone quantiser (k_fwd.h:66-90) into the decoder layout (stride tw). */
-DEVI void tx_quantize(const TxCtx &c, const int32_t *coef_, int32_t *q_, int txSz, int dcQ, int acQ) {
+/* The quantiser's reciprocal of q: floor(2^44 / q) + 1. */
+DEVI unsigned long long tx_qinv(int q) { return (1ull << 44) / (unsigned)q + 1; }
+
+/* Dead-zone quantiser (k_fwd.h:66-90) into the decoder layout (stride tw);
+ * invDc / invAc: tx_qinv of dcQ / acQ. */
+DEVI void tx_quantize(const TxCtx &c, const int32_t *coef_, int32_t *q_, int txSz, int dcQ, int acQ,
+                      unsigned long long invDc, unsigned long long invAc) {
   const lds_i32 *coef = lds(coef_);
   lds_i32 *q = lds(q_);
   const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
-  const unsigned long long invDc = (1ull << 44) / (unsigned)dcQ + 1, invAc = (1ull << 44) / (unsigned)acQ + 1;
   for (int k = c.tid; k < th * tw; k += c.nt) {
     const bool dc = k == 0;
     const int qv = dc ? dcQ : acQ, rnd = dc ? 26 : 22;

@@ -37,13 +37,17 @@ hipError_t launch_edge_types(hipStream_t st, const Mav1FrameParams &fp, Mav1Bloc
 hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
                               const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold,
                               int32_t *coefs, const int32_t *mats, int *work, int workgroups);
-int recon_intra_steps(const Mav1FrameParams &fp);
-/* Anti-diagonals of one tile column: its width in superblocks + 2 (rows - 1). */
-int recon_intra_tile_steps(const Mav1FrameParams &fp, int tile);
-/* Anti-diagonal t of tile column `tile`. */
-hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, int t, int tile, Mav1BlockInfo *blocks,
-                              const Mav1SbInfo *sbs, const uint8_t *src, uint8_t *rec, const uint8_t *ref,
-                              const uint8_t *gold, int32_t *coefs, const int32_t *mats);
+/* Intra phase work space (ints) and whether the frame fits its scheduler. */
+size_t intra_work_ints(const Mav1FrameParams &fp);
+int intra_sched_fits(const Mav1FrameParams &fp);
+/* The intra phase: the scheduling pass, then `workgroups` persistent
+ * workgroups over the superblocks holding intra blocks; iwork[2] becomes
+ * non-zero if a dependency wait ran out. */
+hipError_t launch_intra_sched(hipStream_t st, const Mav1FrameParams &fp, const Mav1BlockInfo *blocks,
+                              const Mav1SbInfo *sbs, int *iwork);
+hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
+                              const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold,
+                              int32_t *coefs, const int32_t *mats, int *iwork, int workgroups);
 
 /* filters.hip */
 hipError_t launch_filter_info(hipStream_t st, const Mav1FrameParams &fp, const Mav1BlockInfo *blocks,

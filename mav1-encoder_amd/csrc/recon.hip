@@ -78,6 +78,9 @@ struct SiWave {
   long long aSse[33];
 };
 
+/* The scans of the nine transform sizes up to 16x16, back to back. */
+#define SCAN_SMALL 784
+
 struct RLds {
   union {
     SmallTx sm[RW];
@@ -98,6 +101,8 @@ struct RLds {
   uint8_t save[64 * 64];
   uint8_t tsrc[64 * 64], tpred[64 * 64]; /* the transform block's source and prediction */
   int32_t smats[TX_SMATS];
+  int16_t scans[3 * SCAN_SMALL]; /* scans of the transforms up to 16x16: default, mrow, mcol */
+  unsigned long long qinv[3][2]; /* per plane the quantiser reciprocals of dc_q / ac_q */
   int ival[4];
   Mav1BlockInfo blk;
   /* intra phase: the superblock's reconstruction with the row above it and
@@ -161,14 +166,30 @@ DEVI const int16_t *scan_of(int txSz, int kind) {
     default: return Default_Scan_32x32;
   }
 }
+/* Offset of a transform size's scan among the staged small scans, -1 for
+ * the 32- and 64-point sizes. */
+DEVI int small_scan_off(int txSz) {
+  switch (txSz) {
+    case TX_4X4: return 0;
+    case TX_8X8: return 16;
+    case TX_16X16: return 80;
+    case TX_4X8: return 336;
+    case TX_8X4: return 368;
+    case TX_8X16: return 400;
+    case TX_16X8: return 528;
+    case TX_4X16: return 656;
+    case TX_16X4: return 720;
+    default: return -1;
+  }
+}
 DEVI const int16_t *get_scan(int txSz, int t) {
   if (txSz == TX_16X64) return Default_Scan_16x32;
   if (txSz == TX_64X16) return Default_Scan_32x16;
   if (Tx_Size_Sqr_Up[txSz] == TX_64X64) return Default_Scan_32x32;
-  if (t == IDTX) return scan_of(txSz, 0);
-  if (t == V_DCT || t == V_ADST || t == V_FLIPADST) return scan_of(txSz, 1);
-  if (t == H_DCT || t == H_ADST || t == H_FLIPADST) return scan_of(txSz, 2);
-  return scan_of(txSz, 0);
+  const int kind = (t == V_DCT || t == V_ADST || t == V_FLIPADST) ? 1 : (t == H_DCT || t == H_ADST || t == H_FLIPADST) ? 2 : 0;
+  const int off = small_scan_off(txSz);
+  if (off >= 0) return gL.scans + kind * SCAN_SMALL + off;
+  return scan_of(txSz, kind);
 }
 DEVI int get_tx_set(int txSz, int isInter, int reduced) {
   const int sq = Tx_Size_Sqr[txSz], up = Tx_Size_Sqr_Up[txSz];
@@ -244,7 +265,7 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
   const int dcQ = fp.dc_q[plane], acQ = fp.ac_q[plane];
   tx_residual(c, spx, px, S, x, y, lw, lh, t, resid, rs);
   tx_forward(c, resid, rs, tmp, coef, txSz, t, mats, gL.smats);
-  tx_quantize(c, coef, q, txSz, dcQ, acQ);
+  tx_quantize(c, coef, q, txSz, dcQ, acQ, gL.qinv[plane][0], gL.qinv[plane][1]);
   const int16_t *scan = get_scan(txSz, t);
   const int eob = tx_eob(c, q, scan, seg_eob(txSz));
   if (direct) {
@@ -1092,7 +1113,8 @@ DEVI void si_step(const Mav1FrameParams &fp, int ls, int lk, int cs, int ck, int
   }
   __syncthreads();
   /* P2: the luma transform types, one per wave */
-  int bestType = DCT_DCT, bestEob = 0;
+  int bestType = DCT_DCT, bestEob = 0, bestWave = 0;
+  bool reuse = false; /* the chosen candidate's wave holds its inverse transform */
   const SiGeo gy = lk == 1 ? si_geo(L.slot[ls], 0) : SiGeo{};
   if (lk == 1) {
     const Mav1BlockInfo &b = L.slot[ls];
@@ -1104,6 +1126,7 @@ DEVI void si_step(const Mav1FrameParams &fp, int ls, int lk, int cs, int ck, int
         if (in_set(set, t, 0)) cands[nc++] = t;
     (void)b;
     const bool direct = nc == 1;
+    reuse = !direct && nc <= RW;
     const int tw = d_min(32, 1 << gy.lw), th = d_min(32, 1 << gy.lh);
     const TxCtx c = wave_ctx();
     long long bestCost = -1;
@@ -1129,6 +1152,7 @@ DEVI void si_step(const Mav1FrameParams &fp, int ls, int lk, int cs, int ck, int
       if (pick >= 0) {
         bestEob = L.candEob[pick];
         bestType = bestEob > 0 ? cands[pick] : DCT_DCT;
+        bestWave = pick - r0;
         const int32_t *from = L.u.sm[pick - r0].q;
         for (int k = tid; k < tw * th; k += RT) L.qbest[k] = from[k];
       }
@@ -1143,7 +1167,10 @@ DEVI void si_step(const Mav1FrameParams &fp, int ls, int lk, int cs, int ck, int
     const int tw = d_min(32, 1 << gy.lw), th = d_min(32, 1 << gy.lh);
     int32_t *co = coefs + b.coef_off[0];
     for (int k = c.tid; k < tw * th; k += 64) co[k] = L.qbest[k];
-    if (bestEob > 0) {
+    /* the chroma waves code into their own candidate buffers meanwhile */
+    if (bestEob > 0 && reuse && (ck != 1 || bestWave < SI_WU)) {
+      tx_add(c, gy.px, gy.S, gy.sx, gy.sy, gy.lw, gy.lh, bestType, L.u.sm[bestWave].resid, SRS);
+    } else if (bestEob > 0) {
       tx_dequant(c, L.qbest, W.dq, SQS, gy.txSz, fp.dc_q[0], fp.ac_q[0]);
       tx_inverse(c, W.dq, SQS, W.resid, SRS, gy.txSz, bestType);
       tx_add(c, gy.px, gy.S, gy.sx, gy.sy, gy.lw, gy.lh, bestType, W.resid, SRS);
@@ -1171,11 +1198,24 @@ DEVI void si_step(const Mav1FrameParams &fp, int ls, int lk, int cs, int ck, int
   __syncthreads();
 }
 
-DEVI void stage_smats(const int32_t *__restrict__ mats) {
+/* The workgroup's constant tables in LDS: the short forward matrices, the
+ * small transforms' scans and the quantiser reciprocals. */
+DEVI void stage_smats(const Mav1FrameParams &fp, const int32_t *__restrict__ mats) {
   for (int q = threadIdx.x; q < TX_SMATS; q += RT) {
     const int kind = q / 336, r = q - kind * 336;
     const int n = r < 16 ? 2 : r < 80 ? 3 : 4, idx = r - (n == 2 ? 0 : n == 3 ? 16 : 80);
     gL.smats[q] = mats[(kind * 5 + n - 2) * 4096 + idx];
+  }
+  const int sizes[9] = {TX_4X4, TX_8X8, TX_16X16, TX_4X8, TX_8X4, TX_8X16, TX_16X8, TX_4X16, TX_16X4};
+  for (int q = threadIdx.x; q < 3 * SCAN_SMALL; q += RT) {
+    const int kind = q / SCAN_SMALL, r = q - kind * SCAN_SMALL;
+    int k = 8;
+    while (small_scan_off(sizes[k]) > r) k--;
+    gL.scans[q] = scan_of(sizes[k], kind)[r - small_scan_off(sizes[k])];
+  }
+  if (threadIdx.x < 6) {
+    const int p = threadIdx.x >> 1;
+    gL.qinv[p][threadIdx.x & 1] = tx_qinv((threadIdx.x & 1) ? fp.ac_q[p] : fp.dc_q[p]);
   }
 }
 
@@ -1201,7 +1241,7 @@ __global__ __launch_bounds__(RT) void k_recon_inter(const Mav1FrameParams fp, Ma
                                                     int *work) {
   RLds &L = gL;
   const int tid = threadIdx.x;
-  stage_smats(mats);
+  stage_smats(fp, mats);
   const int count = work[0];
   for (;;) {
     if (tid == 0) L.ival[2] = atomicAdd(&work[1], 1);
@@ -1218,27 +1258,81 @@ __global__ __launch_bounds__(RT) void k_recon_inter(const Mav1FrameParams fp, Ma
   }
 }
 
-/* Intra phase: the superblocks of anti-diagonal t of one tile column, one
- * workgroup each; grid = superblock rows. */
-__global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
-                                                    const Mav1SbInfo *__restrict__ sbs, const uint8_t *__restrict__ src,
-                                                    uint8_t *rec, const uint8_t *__restrict__ ref,
-                                                    const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
-                                                    const int32_t *__restrict__ mats, int t, int tile) {
+/* ---- intra phase: one dataflow launch ----
+ * k_intra_sched lists the superblocks holding intra blocks in order of
+ * T = column + 2 row, then row. Every superblock whose reconstruction another
+ * one reads (left, above-left, above, above-right) has a smaller T, so the
+ * list is a topological order of the dependencies. k_recon_intra's
+ * workgroups take the list's superblocks one at a time (an atomic counter);
+ * before coding one they wait for its neighbours in the same tile column
+ * that hold intra blocks, and they publish each finished superblock with an
+ * agent-scope release. A workgroup only waits for superblocks taken earlier
+ * by running workgroups, so every wait ends; it is still bounded, and a
+ * wait that runs out sets iwork[IW_ERR] instead of hanging.
+ * iwork: [0] count, [1] next, [2] error, [4 ..] the list, then per
+ * superblock "holds intra blocks", then per superblock "done". */
+#define IW_COUNT 0
+#define IW_NEXT 1
+#define IW_ERR 2
+#define IW_HDR 4
+#define SCHED_T 512
+#define SCHED_KEYS 20480 /* (sb_cols + 2 sb_rows) sb_rows, checked on the host */
+
+__global__ __launch_bounds__(SCHED_T) void k_intra_sched(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
+                                                        const Mav1SbInfo *__restrict__ sbs, int *__restrict__ iwork) {
+  __shared__ uint16_t dense[SCHED_KEYS];
+  __shared__ int cnt[SCHED_T + 1];
+  const int n = fp.sb_cols * fp.sb_rows, tid = threadIdx.x;
+  const int K = (fp.sb_cols + 2 * fp.sb_rows) * fp.sb_rows;
+  int *list = iwork + IW_HDR, *has = list + n, *done = has + n;
+  for (int k = tid; k < K; k += SCHED_T) dense[k] = 0;
+  __syncthreads();
+  for (int sb = tid; sb < n; sb += SCHED_T) {
+    int h = 0;
+    for (int i = 0; i < sbs[sb].num_blocks; i++) h |= !blocks[sbs[sb].first_block + i].is_inter;
+    has[sb] = h;
+    done[sb] = 0;
+    const int r = sb / fp.sb_cols, c = sb - r * fp.sb_cols;
+    if (h) dense[(c + 2 * r) * fp.sb_rows + r] = (uint16_t)(sb + 1);
+  }
+  __syncthreads();
+  /* compaction in key order: per thread a contiguous range of keys */
+  const int C = (K + SCHED_T - 1) / SCHED_T, k0 = tid * C, k1 = d_min(K, k0 + C);
+  int m = 0;
+  for (int k = k0; k < k1; k++) m += dense[k] != 0;
+  cnt[tid + 1] = m;
+  __syncthreads();
+  if (tid == 0) {
+    cnt[0] = 0;
+    for (int i = 1; i <= SCHED_T; i++) cnt[i] += cnt[i - 1];
+    iwork[IW_COUNT] = cnt[SCHED_T];
+    iwork[IW_NEXT] = 0;
+    iwork[IW_ERR] = 0;
+  }
+  __syncthreads();
+  int o = cnt[tid];
+  for (int k = k0; k < k1; k++)
+    if (dense[k]) list[o++] = dense[k] - 1;
+}
+
+/* Waits (thread 0) until *flag is set; false when maxSpin polls ran out. */
+DEVI bool wait_flag(const int *flag, int maxSpin) {
+  for (int it = 0; it < maxSpin; it++) {
+    if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) return true;
+    __builtin_amdgcn_s_sleep(4);
+  }
+  return false;
+}
+
+/* Codes the intra blocks of superblock (r, c) (recon_sb, k_recon.h:396-440):
+ * BlockDecoded flags, LDS tiles, blocks in decode order, tiles back. */
+DEVI void intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo &S,
+                   const uint8_t *__restrict__ src, uint8_t *rec, const uint8_t *__restrict__ ref,
+                   const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs, const int32_t *__restrict__ mats,
+                   int r, int c) {
   RLds &L = gL;
   const int tid = threadIdx.x;
-  const int r = blockIdx.x;
-  const int c0 = tile * fp.tile_w_sb, cl = t - 2 * r;
-  if (cl < 0 || cl >= fp.tile_w_sb || c0 + cl >= fp.sb_cols || r >= fp.sb_rows) return;
-  const int sbIdx = r * fp.sb_cols + c0 + cl;
-  const Mav1SbInfo &S = sbs[sbIdx];
-  stage_smats(mats);
-  if (tid == 0) L.ival[1] = 0;
-  __syncthreads();
-  for (int i = tid; i < S.num_blocks; i += RT)
-    if (!blocks[S.first_block + i].is_inter) L.ival[1] = 1;
-  __syncthreads();
-  if (!L.ival[1]) return;
+  const int c0 = 0, cl = c;
   /* clear_block_decoded_flags (k_recon.h:402-414) */
   const int mr = r * 16, mc = (c0 + cl) * 16;
   const int tileEnd = d_min(fp.mi_cols, tile_start(fp, mc) + fp.tile_w_sb * 16);
@@ -1308,6 +1402,48 @@ __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Ma
   }
 }
 
+__global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
+                                                    const Mav1SbInfo *__restrict__ sbs, const uint8_t *__restrict__ src,
+                                                    uint8_t *rec, const uint8_t *__restrict__ ref,
+                                                    const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs,
+                                                    const int32_t *__restrict__ mats, int *__restrict__ iwork,
+                                                    int maxSpin) {
+  RLds &L = gL;
+  const int tid = threadIdx.x, n = fp.sb_cols * fp.sb_rows;
+  const int *list = iwork + IW_HDR, *has = list + n;
+  int *done = iwork + IW_HDR + 2 * n;
+  /* the first superblock before anything else: workgroups without one leave
+   * at once */
+  if (tid == 0) L.ival[2] = atomicAdd(&iwork[IW_NEXT], 1);
+  __syncthreads();
+  if (L.ival[2] >= iwork[IW_COUNT]) return;
+  stage_smats(fp, mats);
+  for (;;) {
+    const int idx = L.ival[2];
+    __syncthreads();
+    if (idx >= iwork[IW_COUNT]) break;
+    const int sb = list[idx], r = sb / fp.sb_cols, c = sb - r * fp.sb_cols;
+    if (tid == 0) {
+      /* the neighbours whose reconstruction this superblock reads */
+      const int ts = (c / fp.tile_w_sb) * fp.tile_w_sb, te = d_min(fp.sb_cols, ts + fp.tile_w_sb);
+      const int dep[4] = {c > ts ? sb - 1 : -1, r > 0 && c > ts ? sb - fp.sb_cols - 1 : -1, r > 0 ? sb - fp.sb_cols : -1,
+                          r > 0 && c + 1 < te ? sb - fp.sb_cols + 1 : -1};
+      for (int k = 0; k < 4; k++)
+        if (dep[k] >= 0 && has[dep[k]] && !wait_flag(&done[dep[k]], maxSpin)) atomicOr(&iwork[IW_ERR], 1);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    intra_sb(fp, blocks, sbs[sb], src, rec, ref, gold, coefs, mats, r, c);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) {
+      __hip_atomic_store(&done[sb], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      L.ival[2] = atomicAdd(&iwork[IW_NEXT], 1);
+    }
+    __syncthreads();
+  }
+}
+
 /* Edge filter types (src/encoder.c:319-346): smooth flags per mi, then per
  * block the above / left neighbours' flags into interp_filter bits 4 / 5. */
 __global__ __launch_bounds__(64) void k_edge_flags(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
@@ -1355,17 +1491,22 @@ hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
   return hipGetLastError();
 }
 
-int recon_intra_steps(const Mav1FrameParams &fp) { return fp.tile_w_sb + 2 * (fp.sb_rows - 1); }
 
-int recon_intra_tile_steps(const Mav1FrameParams &fp, int tile) {
-  const int rest = fp.sb_cols - tile * fp.tile_w_sb, w = rest < fp.tile_w_sb ? rest : fp.tile_w_sb;
-  return w + 2 * (fp.sb_rows - 1);
+size_t intra_work_ints(const Mav1FrameParams &fp) { return IW_HDR + 3 * (size_t)fp.sb_cols * fp.sb_rows; }
+int intra_sched_fits(const Mav1FrameParams &fp) {
+  return (fp.sb_cols + 2 * fp.sb_rows) * fp.sb_rows <= SCHED_KEYS && fp.sb_cols * fp.sb_rows < 65535;
 }
 
-hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, int t, int tile, Mav1BlockInfo *blocks,
-                              const Mav1SbInfo *sbs, const uint8_t *src, uint8_t *rec, const uint8_t *ref,
-                              const uint8_t *gold, int32_t *coefs, const int32_t *mats) {
-  hipLaunchKernelGGL(k_recon_intra, dim3(fp.sb_rows), dim3(RT), 0, st, fp, blocks, sbs, src, rec, ref, gold, coefs,
-                     mats, t, tile);
+hipError_t launch_intra_sched(hipStream_t st, const Mav1FrameParams &fp, const Mav1BlockInfo *blocks,
+                              const Mav1SbInfo *sbs, int *iwork) {
+  hipLaunchKernelGGL(k_intra_sched, dim3(1), dim3(SCHED_T), 0, st, fp, blocks, sbs, iwork);
+  return hipGetLastError();
+}
+
+hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
+                              const uint8_t *src, uint8_t *rec, const uint8_t *ref, const uint8_t *gold,
+                              int32_t *coefs, const int32_t *mats, int *iwork, int workgroups) {
+  hipLaunchKernelGGL(k_recon_intra, dim3(workgroups), dim3(RT), 0, st, fp, blocks, sbs, src, rec, ref, gold, coefs,
+                     mats, iwork, 1 << 20);
   return hipGetLastError();
 }
