@@ -668,6 +668,81 @@ __global__ __launch_bounds__(LR_T) void k_lr_path(const Mav1FrameParams fp, cons
  * units at a time; the sixteen self-guided costs of a unit are scored one
  * per lane. */
 #define LR_CB 32
+
+/* The unit record of a choice: kind (RESTORE_*), self-guided set. */
+DEVI Mav1LrUnit lr_unit_of(const LrUnitStat &st, int kind, int set, const int *w) {
+  Mav1LrUnit un = {};
+  un.type = kind;
+  if (kind == RESTORE_WIENER)
+    for (int k = 0; k < 6; k++) un.wiener[k / 3][k % 3] = w[k];
+  if (kind == RESTORE_SGRPROJ) {
+    un.set = set;
+    un.xqd[0] = st.xqd[set][0];
+    un.xqd[1] = st.xqd[set][1];
+  }
+  return un;
+}
+
+/* Speculative choices of staged units li .. nb-1 under the current
+ * references sRef / sRefX: sMis[u] = the unit's refinement assumed another
+ * Wiener reference, sDec[u] = kind | set << 4 (NONE, then WIENER, then the
+ * sets in order, first strict minimum; the lr_test pattern). Sixteen lanes
+ * per unit, one per self-guided set. */
+DEVI void lr_spec(const Mav1FrameParams &fp, const LrUnitStat *sSt, int li, int nb, int b0, int lrTest,
+                  const int *sRef, const int *sRefX, int *sDec, int *sMis, int tid) {
+  const int set = tid & 15;
+  const int has0 = SGR_PARAM(set, 0) != 0, has1 = SGR_PARAM(set, 2) != 0;
+  const int xlo0 = Sgrproj_Xqd_Min[0], xhi0 = Sgrproj_Xqd_Max[0] + 1;
+  const int xlo1 = Sgrproj_Xqd_Min[1], xhi1 = Sgrproj_Xqd_Max[1] + 1;
+  for (int q = tid; q < (nb - li) * 16; q += LR_T) {
+    const int u = li + (q >> 4);
+    const LrUnitStat &st = sSt[u];
+    int bits = 2 + SGRPROJ_PARAMS_BITS;
+    if (has0) bits += lr_subexp_bits(xlo0, xhi0, SGRPROJ_PRJ_SUBEXP_K, sRefX[0], st.xqd[set][0]);
+    if (has1) bits += lr_subexp_bits(xlo1, xhi1, SGRPROJ_PRJ_SUBEXP_K, sRefX[1], st.xqd[set][1]);
+    long long sc = lr_rd(fp, st.sseSgr[set], bits);
+    int si = set;
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) {
+      const long long oc = __shfl_xor(sc, m);
+      const int oi = __shfl_xor(si, m);
+      if (oc < sc || (oc == sc && oi < si)) {
+        sc = oc;
+        si = oi;
+      }
+    }
+    if (set == 0) {
+      int mis = 0;
+      for (int k = 0; k < 6; k++) mis |= st.refAssumed[k] != sRef[k];
+      long long best = lr_rd(fp, st.sseNone, 1);
+      int kind = RESTORE_NONE, bs = 0;
+      if (st.costStar < best) {
+        best = st.costStar;
+        kind = RESTORE_WIENER;
+      }
+      if (sc < best) {
+        kind = RESTORE_SGRPROJ;
+        bs = si;
+      }
+      if (lrTest) {
+        if (!((b0 + u) & 1)) {
+          kind = RESTORE_WIENER;
+        } else {
+          long long e = -1;
+          kind = RESTORE_SGRPROJ;
+          for (int k = 0; k < 16; k++)
+            if (e < 0 || st.sseSgr[k] < e) {
+              e = st.sseSgr[k];
+              bs = k;
+            }
+        }
+      }
+      sDec[u] = kind | bs << 4;
+      sMis[u] = mis;
+    }
+  }
+}
+
 __global__ __launch_bounds__(LR_T) void k_lr_chain(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
                                                    const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
                                                    const LrUnitStat *__restrict__ stats, Mav1LrUnit *__restrict__ units,
@@ -680,7 +755,7 @@ __global__ __launch_bounds__(LR_T) void k_lr_chain(const Mav1FrameParams fp, con
   __shared__ Mav1LrUnit sUn[LR_CB];
   __shared__ int sRef[6], sRefX[2], sW[6];
   __shared__ long long sCost, sSse;
-  __shared__ int sRerun, sStop;
+  __shared__ int sRerun, sStop, sDec[LR_CB], sMis[LR_CB], sNext, sReason;
   if (tid < 6) sRef[tid] = cs.ref[tid];
   if (tid < 2) sRefX[tid] = cs.refX[tid];
   if (tid == 0) sStop = -1;
@@ -697,6 +772,50 @@ __global__ __launch_bounds__(LR_T) void k_lr_chain(const Mav1FrameParams fp, con
     for (int q = tid; q < words; q += LR_T) ((int *)sSt)[q] = ((const int *)(stats + off + b0))[q];
     __syncthreads();
     for (int li = 0; li < nb; li++) {
+      /* the units from li on under the current references, accepted up to
+       * the first that changes a reference or whose refinement assumed
+       * another one */
+      lr_spec(fp, sSt, li, nb, b0, lrTest, sRef, sRefX, sDec, sMis, tid);
+      __syncthreads();
+      if (tid == 0) {
+        int k = li, reason = 0;
+        for (; k < nb; k++) {
+          if (sMis[k]) {
+            reason = 2;
+            break;
+          }
+          const LrUnitStat &st = sSt[k];
+          const int kind = sDec[k] & 15, set = sDec[k] >> 4;
+          int change = 0;
+          if (kind == RESTORE_WIENER)
+            for (int j = 0; j < 6; j++)
+              if (p == 0 || j % 3) {
+                change |= sRef[j] != st.wStar[j];
+                sRef[j] = st.wStar[j];
+              }
+          if (kind == RESTORE_SGRPROJ)
+            for (int j = 0; j < 2; j++) {
+              change |= sRefX[j] != st.xqd[set][j];
+              sRefX[j] = st.xqd[set][j];
+            }
+          sUn[k] = lr_unit_of(st, kind, set, st.wStar);
+          used |= kind != RESTORE_NONE;
+          if (change) {
+            k++;
+            reason = 1;
+            break;
+          }
+        }
+        sNext = k;
+        sReason = reason;
+      }
+      __syncthreads();
+      li = sNext;
+      if (sReason != 2) {
+        li--; /* the loop's increment resumes at sNext */
+        continue;
+      }
+      /* unit li: its refinement assumed another reference */
       const int l = b0 + li, u = off + l;
       const LrUnitStat &st = sSt[li];
       if (tid == 0) {
