@@ -46,7 +46,8 @@ struct LrLds {
       int16_t f0[LR_MAXPX], f1[LR_MAXPX]; /* self-guided filter outputs */
     } sg;
     struct {
-      double phi[3][LR_T + 128], tgt[LR_T + 128]; /* one block row of fit terms, in sample order */
+      double h[2][1024];           /* a block row's 7-tap filtered samples (two rows in flight) */
+      double ph[2][4][LR_T + 128]; /* a block row's fit terms, phi[0..2] then target, in sample order */
     } wf;
     struct {
       int16_t mid[2][LR_ROWS * 96]; /* Wiener horizontal pass: current taps, candidate taps */
@@ -453,44 +454,77 @@ DEVI int lr_solve(int n, double M[3][3], double v[3], double *x) {
 /* Separable least-squares Wiener fit (lr_search.c:132-180): four
  * alternating solves; the sums run over the unit's samples in the
  * reference's order, one lane per accumulator. */
+/* The 7-tap sums of block row br under filter f, each in the reference's
+ * order (t = 0 .. 6): vert, H(x, yy) = sum_t f[t] img(x + t - 3, yy) for the
+ * row's samples' rows -3 .. +3 (row stride uw); else V(xx, y) = sum_t f[t]
+ * img(xx, y + t - 3) for their columns -3 .. +3 (row stride uw + 6). */
+DEVI void lr_wf_filter(const LrGeo &g, int br, int vert, const double *f, double *h, int t0, int nt) {
+  const int y0 = g.y0 + br * g.bh, hh = d_min(g.bh, g.endY - y0 + 1), uw = g.x1 - g.x0, s = lr_seg_of(g, y0);
+  const int n = vert ? (hh + 6) * uw : hh * (uw + 6), rw = vert ? uw : uw + 6;
+  for (int k = t0; k < n; k += nt) {
+    const int r = k / rw, c = k - r * rw;
+    double sum = 0;
+    if (vert) {
+#pragma unroll
+      for (int t = 0; t < 7; t++) sum += f[t] * (double)lr_img(g, s, g.x0 + c + t - 3, y0 - 3 + r);
+    } else {
+#pragma unroll
+      for (int t = 0; t < 7; t++) sum += f[t] * (double)lr_img(g, s, g.x0 - 3 + c, y0 + r + t - 3);
+    }
+    h[k] = sum;
+  }
+}
+/* The fit terms of block row br's samples in the reference's sample order
+ * (lr_search.c:150-160): phi[a - first] = Z[a] + Z[6 - a] - 2 Z[3] and the
+ * target src - Z[3], Z from the row's 7-tap sums. */
+DEVI void lr_wf_phi(const LrGeo &g, int br, int vert, int first, const double *h, double (*ph)[LR_T + 128], int t0,
+                    int nt) {
+  const int y0 = g.y0 + br * g.bh, uw = g.x1 - g.x0, cnt = lr_row_count(g, br);
+  for (int q = t0; q < cnt; q += nt) {
+    int x, y;
+    lr_row_sample(g, br, q, x, y);
+    double Z[7];
+#pragma unroll
+    for (int a = 0; a < 7; a++)
+      Z[a] = vert ? h[(y - y0 + a) * uw + x - g.x0] : h[(y - y0) * (uw + 6) + x - g.x0 + a];
+    for (int a = first; a < 3; a++) ph[a - first][q] = Z[a] + Z[6 - a] - 2 * Z[3];
+    ph[3][q] = (double)gLr.srcu[(y - g.y0) * uw + x - g.x0] - Z[3];
+  }
+}
+
+/* Separable least-squares Wiener fit (lr_search.c:132-180): four
+ * alternating solves. Each solve's sums run over the unit's samples in the
+ * reference's order, one lane of the last wave per accumulator; the other
+ * waves prepare the fit terms two block rows ahead (filter a row, then its
+ * terms, then its sums: a three-step pipeline with one barrier a step). */
 DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t *src, int *out, int tid) {
+  (void)fp;
+  (void)src;
   const int first = g.p ? 1 : 0, nc = 3 - first, nacc = nc * nc + nc;
-  const int S = fp.stride[g.p], o = fp.offset[g.p];
+  const int wave = tid >> 6, lane = tid & 63, NCT = LR_T - 64;
   double fv[7] = {0, 0, 0, 1, 0, 0, 0}, fh[7] = {0, 0, 0, 1, 0, 0, 0};
   for (int it = 0; it < 4; it++) {
     const int vert = !(it & 1);
-    double acc = 0; /* lane tid < nacc: M[a][b] (a * nc + b), then V[a] */
-    for (int br = 0; br < g.nbr; br++) {
-      const int cnt = lr_row_count(g, br);
-      for (int c0 = 0; c0 < cnt; c0 += LR_T) {
-        const int q = c0 + tid;
-        if (q < cnt) {
-          int x, y;
-          lr_row_sample(g, br, q, x, y);
-          const int s = lr_seg_of(g, y);
-          double Z[7];
-          for (int a = 0; a < 7; a++) {
-            double sum = 0;
-            for (int t = 0; t < 7; t++)
-              sum += vert ? fh[t] * (double)lr_img(g, s, x + t - 3, y + a - 3) : fv[t] * (double)lr_img(g, s, x + a - 3, y + t - 3);
-            Z[a] = sum;
-          }
-          for (int a = first; a < 3; a++) gLr.u.wf.phi[a - first][q - c0] = Z[a] + Z[6 - a] - 2 * Z[3];
-          gLr.u.wf.tgt[q - c0] = (double)gLr.srcu[(y - g.y0) * (g.x1 - g.x0) + x - g.x0] - Z[3];
-        }
-        __syncthreads();
-        if (tid < nacc) {
-          const int n = d_min(LR_T, cnt - c0);
-          const int a = tid < nc * nc ? tid / nc : tid - nc * nc, b = tid < nc * nc ? tid - (tid / nc) * nc : -1;
-          typedef __attribute__((address_space(3))) const double lds_f64;
-          const lds_f64 *pa = (lds_f64 *)gLr.u.wf.phi[a], *pb = (lds_f64 *)(b >= 0 ? gLr.u.wf.phi[b] : gLr.u.wf.tgt);
+    double f[7];
+#pragma unroll
+    for (int t = 0; t < 7; t++) f[t] = vert ? fh[t] : fv[t];
+    double acc = 0; /* lane < nacc of the last wave: M[a][b] (a * nc + b), then V[a] */
+    for (int st = 0; st < g.nbr + 2; st++) {
+      if (wave < LR_T / 64 - 1) {
+        if (st < g.nbr) lr_wf_filter(g, st, vert, f, gLr.u.wf.h[st & 1], tid, NCT);
+        if (st >= 1 && st - 1 < g.nbr)
+          lr_wf_phi(g, st - 1, vert, first, gLr.u.wf.h[(st - 1) & 1], gLr.u.wf.ph[(st - 1) & 1], tid, NCT);
+      } else if (st >= 2 && lane < nacc) {
+        const int br = st - 2, n = lr_row_count(g, br);
+        const int a = lane < nc * nc ? lane / nc : lane - nc * nc, b = lane < nc * nc ? lane - (lane / nc) * nc : 3;
+        typedef __attribute__((address_space(3))) const double lds_f64;
+        const lds_f64 *pa = (lds_f64 *)gLr.u.wf.ph[br & 1][a], *pb = (lds_f64 *)gLr.u.wf.ph[br & 1][b];
 #pragma unroll 8
-          for (int k = 0; k < n; k++) acc += pa[k] * pb[k];
-        }
-        __syncthreads();
+        for (int k = 0; k < n; k++) acc += pa[k] * pb[k];
       }
+      __syncthreads();
     }
-    if (tid < nacc) gLr.acc[tid] = acc;
+    if (wave == LR_T / 64 - 1 && lane < nacc) gLr.acc[lane] = acc;
     __syncthreads();
     if (tid == 0) {
       double M[3][3], V[3], x[3] = {0, 0, 0};
