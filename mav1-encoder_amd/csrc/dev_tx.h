@@ -444,13 +444,40 @@ DEVI void itx_cols(int32_t *res_, int rs, int w, int tid, int nt) {
     }                                                                       \
   } while (0)
 
+/* The dispatch over kind and length of the transforms up to 16 points. */
+#define ITX_DISPATCH_SMALL(FN, kind, n, ...)                                \
+  do {                                                                      \
+    if (n == 2) {                                                           \
+      if (kind == 0) FN<0, 2>(__VA_ARGS__);                                 \
+      else if (kind == 1) FN<1, 2>(__VA_ARGS__);                            \
+      else FN<2, 2>(__VA_ARGS__);                                           \
+    } else if (n == 3) {                                                    \
+      if (kind == 0) FN<0, 3>(__VA_ARGS__);                                 \
+      else if (kind == 1) FN<1, 3>(__VA_ARGS__);                            \
+      else FN<2, 3>(__VA_ARGS__);                                           \
+    } else {                                                                \
+      if (kind == 0) FN<0, 4>(__VA_ARGS__);                                 \
+      else if (kind == 1) FN<1, 4>(__VA_ARGS__);                            \
+      else FN<2, 4>(__VA_ARGS__);                                           \
+    }                                                                       \
+  } while (0)
+
+/* SMALL: the caller's transforms are at most 16x16, so only those lengths
+ * are compiled in (smaller code where the small-block path runs). */
+template <bool SMALL = false>
 DEVI void tx_inverse(const TxCtx &c, const int32_t *dq, int qs, int32_t *res, int rs, int txSz, int txType) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
   const int rect = (lw - lh == 1) || (lh - lw == 1);
   const int rk = row_kind(txType), ck = col_kind(txType);
-  ITX_DISPATCH(itx_rows, rk, lw, dq, qs, res, rs, 1 << lh, rect, (int)Transform_Row_Shift[txSz], c.tid, c.nt);
-  tx_sync(c);
-  ITX_DISPATCH(itx_cols, ck, lh, res, rs, 1 << lw, c.tid, c.nt);
+  if constexpr (SMALL) {
+    ITX_DISPATCH_SMALL(itx_rows, rk, lw, dq, qs, res, rs, 1 << lh, rect, (int)Transform_Row_Shift[txSz], c.tid, c.nt);
+    tx_sync(c);
+    ITX_DISPATCH_SMALL(itx_cols, ck, lh, res, rs, 1 << lw, c.tid, c.nt);
+  } else {
+    ITX_DISPATCH(itx_rows, rk, lw, dq, qs, res, rs, 1 << lh, rect, (int)Transform_Row_Shift[txSz], c.tid, c.nt);
+    tx_sync(c);
+    ITX_DISPATCH(itx_cols, ck, lh, res, rs, 1 << lw, c.tid, c.nt);
+  }
   tx_sync(c);
 }
 
@@ -497,41 +524,48 @@ DEVI void fwd_cols(const TxCtx &c, const int32_t *tmp_, int32_t *coef_, int th, 
 #define TX_SMATS (3 * 336)
 DEVI int smat_off(int kind, int n) { return kind * 336 + (n == 2 ? 0 : n == 3 ? 16 : 80); }
 
-template <class MP>
+template <class MP, bool SMALL = false>
 DEVI void tx_forward_m(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int32_t *coef, int txSz, MP Mh,
                        MP Mv) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], h = 1 << lh;
   const int ltw = d_min(5, lw), th = d_min(32, h);
-  switch (lw) {
-    case 2: fwd_rows<4>(c, resid, rs, tmp, h, ltw, Mh); break;
-    case 3: fwd_rows<8>(c, resid, rs, tmp, h, ltw, Mh); break;
-    case 4: fwd_rows<16>(c, resid, rs, tmp, h, ltw, Mh); break;
-    case 5: fwd_rows<32>(c, resid, rs, tmp, h, ltw, Mh); break;
-    default: fwd_rows<64>(c, resid, rs, tmp, h, ltw, Mh); break;
+  if (lw == 2) fwd_rows<4>(c, resid, rs, tmp, h, ltw, Mh);
+  else if (lw == 3) fwd_rows<8>(c, resid, rs, tmp, h, ltw, Mh);
+  else if (SMALL || lw == 4) fwd_rows<16>(c, resid, rs, tmp, h, ltw, Mh);
+  else if constexpr (!SMALL) {
+    if (lw == 5) fwd_rows<32>(c, resid, rs, tmp, h, ltw, Mh);
+    else fwd_rows<64>(c, resid, rs, tmp, h, ltw, Mh);
   }
   tx_sync(c);
   const int sh = Transform_Row_Shift[txSz] + 4;
   const bool rect = (lw - lh == 1) || (lh - lw == 1);
-  switch (lh) {
-    case 2: fwd_cols<4>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
-    case 3: fwd_cols<8>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
-    case 4: fwd_cols<16>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
-    case 5: fwd_cols<32>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
-    default: fwd_cols<64>(c, tmp, coef, th, ltw, Mv, sh, rect); break;
+  if (lh == 2) fwd_cols<4>(c, tmp, coef, th, ltw, Mv, sh, rect);
+  else if (lh == 3) fwd_cols<8>(c, tmp, coef, th, ltw, Mv, sh, rect);
+  else if (SMALL || lh == 4) fwd_cols<16>(c, tmp, coef, th, ltw, Mv, sh, rect);
+  else if constexpr (!SMALL) {
+    if (lh == 5) fwd_cols<32>(c, tmp, coef, th, ltw, Mv, sh, rect);
+    else fwd_cols<64>(c, tmp, coef, th, ltw, Mv, sh, rect);
   }
   tx_sync(c);
 }
 
 /* smats (may be NULL): the packed short matrices in LDS, used when both
- * lengths are at most 16; otherwise the full matrices in global memory. */
+ * lengths are at most 16; otherwise the full matrices in global memory.
+ * SMALL: both lengths are at most 16 and smats is given. */
+template <bool SMALL = false>
 DEVI void tx_forward(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int32_t *coef, int txSz, int txType,
                      const int32_t *__restrict__ mats, const int32_t *smats) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
   const int rk = row_kind(txType), ck = col_kind(txType);
-  if (smats && lw <= 4 && lh <= 4)
-    tx_forward_m(c, resid, rs, tmp, coef, txSz, lds(smats) + smat_off(rk, lw), lds(smats) + smat_off(ck, lh));
-  else
-    tx_forward_m(c, resid, rs, tmp, coef, txSz, mats + (rk * 5 + lw - 2) * 4096, mats + (ck * 5 + lh - 2) * 4096);
+  if constexpr (SMALL) {
+    tx_forward_m<const lds_i32 *, true>(c, resid, rs, tmp, coef, txSz, lds(smats) + smat_off(rk, lw),
+                                        lds(smats) + smat_off(ck, lh));
+  } else {
+    if (smats && lw <= 4 && lh <= 4)
+      tx_forward_m(c, resid, rs, tmp, coef, txSz, lds(smats) + smat_off(rk, lw), lds(smats) + smat_off(ck, lh));
+    else
+      tx_forward_m(c, resid, rs, tmp, coef, txSz, mats + (rk * 5 + lw - 2) * 4096, mats + (ck * 5 + lh - 2) * 4096);
+  }
 }
 
 /* The quantiser's reciprocal of q: floor(2^44 / q) + 1. */

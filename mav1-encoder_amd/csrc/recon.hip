@@ -111,6 +111,7 @@ struct RLds {
   alignas(16) uint8_t tile[2][TILE_BYTES];
   SiWave si[3];             /* the small-block path: luma wave, U wave, V wave */
   Mav1BlockInfo slot[2];    /* its blocks in flight */
+  int mq[64][3], nmq;       /* inter blocks whose chroma flags follow the pending chroma: bsize, mi_row, mi_col */
 };
 
 /* The working set of the workgroup that reconstructs a block / superblock.
@@ -257,6 +258,7 @@ DEVI void tx_add(const TxCtx &c, uint8_t *px, int S, int x, int y, int lw, int l
 /* One candidate type of one transform block (k_recon.h:60-173 loop body):
  * levels into q, returns the eob; the RD cost 16 * SSE + lambda * bits / 8
  * in *costOut (0 when `direct`: a single type and no depth search). */
+template <bool SMALL>
 NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__restrict__ spx, const uint8_t *px, int S,
                   int x, int y, int txSz, int plane, int t, bool direct, int32_t *resid, int rs, int32_t *tmp,
                   int32_t *coef, int32_t *q, int32_t *dq, int qs, const int32_t *__restrict__ mats,
@@ -264,7 +266,7 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
   const int dcQ = fp.dc_q[plane], acQ = fp.ac_q[plane];
   tx_residual(c, spx, px, S, x, y, lw, lh, t, resid, rs);
-  tx_forward(c, resid, rs, tmp, coef, txSz, t, mats, gL.smats);
+  tx_forward<SMALL>(c, resid, rs, tmp, coef, txSz, t, mats, gL.smats);
   tx_quantize(c, coef, q, txSz, dcQ, acQ, gL.qinv[plane][0], gL.qinv[plane][1]);
   const int16_t *scan = get_scan(txSz, t);
   const int eob = tx_eob(c, q, scan, seg_eob(txSz));
@@ -276,7 +278,7 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
   const lds_i32 *rl = lds(resid);
   if (eob > 0) {
     tx_dequant(c, q, dq, qs, txSz, dcQ, acQ);
-    tx_inverse(c, dq, qs, resid, rs, txSz, t);
+    tx_inverse<SMALL>(c, dq, qs, resid, rs, txSz, t);
     const int fud = flip_ud(t), flr = flip_lr(t);
     for (int k = c.tid; k < w * h; k += c.nt) {
       const int i = k >> lw, j = k & (w - 1);
@@ -429,7 +431,7 @@ NOINL int code_tx_block(const Mav1FrameParams &fp, const uint8_t *__restrict__ s
     const int ci = r0 + (small ? wave : 0);
     if (ci < nc) {
       long long cost;
-      const int eob = tx_eval(c, fp, L.tsrc, L.tpred, w, 0, 0, txSz, plane, cands[ci], direct, resid, rs, tmp, coef,
+      const int eob = tx_eval<false>(c, fp, L.tsrc, L.tpred, w, 0, 0, txSz, plane, cands[ci], direct, resid, rs, tmp, coef,
                               qtry, dq, qs, mats, &cost);
       if (c.tid == 0) {
         L.candCost[ci] = cost;
@@ -947,11 +949,11 @@ DEVI bool si_eligible(const Mav1FrameParams &fp, const Mav1BlockInfo &b) {
 
 /* The BlockDecoded marks of one plane of block b, transform block by
  * transform block as code_plane sets them (one thread). */
-DEVI void si_mark(const Mav1FrameParams &fp, const Mav1BlockInfo &b, int plane) {
-  const int sub = plane > 0, bs = b.bsize;
+DEVI void si_mark_g(const Mav1FrameParams &fp, int bs, int miRow, int miCol, int plane) {
+  const int sub = plane > 0;
   const int txSz = fp.lossless ? TX_4X4 : plane ? uv_tx_size(bs) : Max_Tx_Size_Rect[bs];
   const int psz = SUBSAMPLED_SIZE(bs, sub, sub), n4w = Num_4x4_Blocks_Wide[psz], n4h = Num_4x4_Blocks_High[psz];
-  const int baseX = (b.mi_col >> sub) * 4, baseY = (b.mi_row >> sub) * 4;
+  const int baseX = (miCol >> sub) * 4, baseY = (miRow >> sub) * 4;
   const int stepX = Tx_Width[txSz] >> 2, stepY = Tx_Height[txSz] >> 2;
   for (int y = 0; y < n4h; y += stepY)
     for (int x = 0; x < n4w; x += stepX) {
@@ -961,6 +963,9 @@ DEVI void si_mark(const Mav1FrameParams &fp, const Mav1BlockInfo &b, int plane) 
       for (int i = 0; i < stepY; i++)
         for (int j = 0; j < stepX; j++) gL.bd[BDI(plane, (sbr >> sub) + i, (sbc >> sub) + j)] = 1;
     }
+}
+DEVI void si_mark(const Mav1FrameParams &fp, const Mav1BlockInfo &b, int plane) {
+  si_mark_g(fp, b.bsize, b.mi_row, b.mi_col, plane);
 }
 
 /* Geometry of plane `plane` of small block b. */
@@ -1074,7 +1079,7 @@ DEVI void si_code_chroma(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane,
   int t = Mode_To_Txfm[b.uv_mode];
   if (Tx_Size_Sqr_Up[g.txSz] > TX_32X32 || !in_set(set, t, 0)) t = DCT_DCT;
   long long cost;
-  const int eob = tx_eval(c, fp, g.spx, g.px, g.S, g.sx, g.sy, g.txSz, plane, t, true, T.resid, SRS, T.tmp, T.coef,
+  const int eob = tx_eval<true>(c, fp, g.spx, g.px, g.S, g.sx, g.sy, g.txSz, plane, t, true, T.resid, SRS, T.tmp, T.coef,
                           T.q, T.dq, SQS, mats, &cost);
   const int type = eob > 0 ? t : DCT_DCT;
   const int tw = d_min(32, 1 << g.lw), th = d_min(32, 1 << g.lh);
@@ -1082,7 +1087,7 @@ DEVI void si_code_chroma(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane,
   for (int k = c.tid; k < tw * th; k += 64) co[k] = T.q[k];
   if (eob > 0) {
     tx_dequant(c, T.q, T.dq, SQS, g.txSz, fp.dc_q[plane], fp.ac_q[plane]);
-    tx_inverse(c, T.dq, SQS, T.resid, SRS, g.txSz, type);
+    tx_inverse<true>(c, T.dq, SQS, T.resid, SRS, g.txSz, type);
     tx_add(c, g.px, g.S, g.sx, g.sy, g.lw, g.lh, type, T.resid, SRS);
   }
   if (c.tid == 0) {
@@ -1135,7 +1140,7 @@ DEVI void si_step(const Mav1FrameParams &fp, int ls, int lk, int cs, int ck, int
       if (ci < nc) {
         SmallTx &W = L.u.sm[wave];
         long long cost;
-        const int eob = tx_eval(c, fp, gy.spx, gy.px, gy.S, gy.sx, gy.sy, gy.txSz, 0, cands[ci], direct, W.resid, SRS,
+        const int eob = tx_eval<true>(c, fp, gy.spx, gy.px, gy.S, gy.sx, gy.sy, gy.txSz, 0, cands[ci], direct, W.resid, SRS,
                                 W.tmp, W.coef, W.q, W.dq, SQS, mats, &cost);
         if (c.tid == 0) {
           L.candCost[ci] = cost;
@@ -1172,7 +1177,7 @@ DEVI void si_step(const Mav1FrameParams &fp, int ls, int lk, int cs, int ck, int
       tx_add(c, gy.px, gy.S, gy.sx, gy.sy, gy.lw, gy.lh, bestType, L.u.sm[bestWave].resid, SRS);
     } else if (bestEob > 0) {
       tx_dequant(c, L.qbest, W.dq, SQS, gy.txSz, fp.dc_q[0], fp.ac_q[0]);
-      tx_inverse(c, W.dq, SQS, W.resid, SRS, gy.txSz, bestType);
+      tx_inverse<true>(c, W.dq, SQS, W.resid, SRS, gy.txSz, bestType);
       tx_add(c, gy.px, gy.S, gy.sx, gy.sy, gy.lw, gy.lh, bestType, W.resid, SRS);
     }
     if (c.tid == 0) {
@@ -1190,10 +1195,9 @@ DEVI void si_step(const Mav1FrameParams &fp, int ls, int lk, int cs, int ck, int
       if ((tid & 63) == 0) b.cfl_alpha_v = a;
     }
     si_code_chroma(fp, b, cplane, coefs, mats, L.u.sm[wave]);
-  }
-  if (ck == 2 && tid == 0) {
-    si_mark(fp, L.slot[cs], 1);
-    si_mark(fp, L.slot[cs], 2);
+    /* then the chroma flags of the inter blocks that follow it */
+    if ((tid & 63) == 0)
+      for (int k = 0; k < L.nmq; k++) si_mark_g(fp, L.mq[k][0], L.mq[k][1], L.mq[k][2], cplane);
   }
   __syncthreads();
 }
@@ -1363,24 +1367,45 @@ DEVI void intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks
    * pipeline (chroma one step behind luma), inter blocks' flags through
    * the same streams, any other intra block by the whole workgroup once the
    * pipeline has drained */
-  int pend = -1, pendKind = 0, pendIdx = 0;
+  /* inter blocks take no step: their luma flags are set at once, their
+   * chroma flags queued behind the pending chroma (or set at once when
+   * none is pending) */
+  int pend = -1, pendIdx = 0;
+  if (tid == 0) L.nmq = 0;
   for (int i = 0; i <= S.num_blocks; i++) {
     const int sl = pend == 0 ? 1 : 0;
     if (i < S.num_blocks && tid < 27)
       ((int32_t *)&L.slot[sl])[tid] = ((const int32_t *)(blocks + S.first_block + i))[tid];
     __syncthreads();
     const int kind = i == S.num_blocks ? 0 : L.slot[sl].is_inter ? 2 : si_eligible(fp, L.slot[sl]) ? 1 : 3;
-    if (kind == 1 || kind == 2) {
-      si_step(fp, sl, kind, pend < 0 ? 0 : pend, pend < 0 ? 0 : pendKind, coefs, mats);
-    } else if (pend >= 0) {
-      si_step(fp, 0, 0, pend, pendKind, coefs, mats);
+    if (kind == 2) {
+      if (tid == 0) {
+        const Mav1BlockInfo &b = L.slot[sl];
+        si_mark(fp, b, 0);
+        if (pend >= 0) {
+          L.mq[L.nmq][0] = b.bsize;
+          L.mq[L.nmq][1] = b.mi_row;
+          L.mq[L.nmq][2] = b.mi_col;
+          L.nmq++;
+        } else {
+          si_mark(fp, b, 1);
+          si_mark(fp, b, 2);
+        }
+      }
+      __syncthreads();
+      continue;
     }
-    if (pend >= 0 && pendKind == 1 && tid < 27)
+    if (kind == 1) {
+      si_step(fp, sl, 1, pend < 0 ? 0 : pend, pend < 0 ? 0 : 1, coefs, mats);
+    } else if (pend >= 0) {
+      si_step(fp, 0, 0, pend, 1, coefs, mats);
+    }
+    if (pend >= 0 && tid < 27)
       ((int32_t *)(blocks + S.first_block + pendIdx))[tid] = ((const int32_t *)&L.slot[pend])[tid];
+    if (tid == 0) L.nmq = 0;
     pend = -1;
-    if (kind == 1 || kind == 2) {
+    if (kind == 1) {
       pend = sl;
-      pendKind = kind;
       pendIdx = i;
     } else if (kind == 3) {
       if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)&L.slot[sl])[tid];
