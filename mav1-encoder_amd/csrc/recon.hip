@@ -20,6 +20,74 @@
  * transforms (32- and 64-point) use the whole workgroup per candidate. */
 #include <climits>
 
+#include "dev_common.h"
+
+/* The small AV1 tables that these kernels index with per-lane values live
+ * in LDS: in constant memory each such lookup is a vector load through the
+ * L2, and the stages chain several of them. Every workgroup copies them in
+ * first (stage_tabs); the names below then refer to the copies. */
+struct RTabs {
+  int16_t m2a[13], dr[90], iek[15], n4w[22], n4h[22], maxRect[22], maxDepth[22], split[19], subs[88];
+  int16_t txw[19], txh[19], txwl[19], txhl[19], sqr[19], sqrUp[19], rowShift[19], inIntra[48], inInter[64], m2t[14];
+  uint8_t sw[124];
+};
+__shared__ RTabs gT;
+template <class T>
+DEVI void tab_copy(int16_t *d, const T *s, int n, int tid, int nt) {
+  for (int k = tid; k < n; k += nt) d[k] = (int16_t)s[k];
+}
+DEVI void stage_tabs(int tid, int nt) {
+  tab_copy(gT.m2a, Mode_To_Angle, 13, tid, nt);
+  tab_copy(gT.dr, Dr_Intra_Derivative, 90, tid, nt);
+  tab_copy(gT.iek, Intra_Edge_Kernel, 15, tid, nt);
+  tab_copy(gT.n4w, Num_4x4_Blocks_Wide, 22, tid, nt);
+  tab_copy(gT.n4h, Num_4x4_Blocks_High, 22, tid, nt);
+  tab_copy(gT.maxRect, Max_Tx_Size_Rect, 22, tid, nt);
+  tab_copy(gT.maxDepth, Max_Tx_Depth, 22, tid, nt);
+  tab_copy(gT.split, Split_Tx_Size, 19, tid, nt);
+  tab_copy(gT.subs, Subsampled_Size, 88, tid, nt);
+  tab_copy(gT.txw, Tx_Width, 19, tid, nt);
+  tab_copy(gT.txh, Tx_Height, 19, tid, nt);
+  tab_copy(gT.txwl, Tx_Width_Log2, 19, tid, nt);
+  tab_copy(gT.txhl, Tx_Height_Log2, 19, tid, nt);
+  tab_copy(gT.sqr, Tx_Size_Sqr, 19, tid, nt);
+  tab_copy(gT.sqrUp, Tx_Size_Sqr_Up, 19, tid, nt);
+  tab_copy(gT.rowShift, Transform_Row_Shift, 19, tid, nt);
+  tab_copy(gT.inIntra, Tx_Type_In_Set_Intra, 48, tid, nt);
+  tab_copy(gT.inInter, Tx_Type_In_Set_Inter, 64, tid, nt);
+  tab_copy(gT.m2t, Mode_To_Txfm, 14, tid, nt);
+  const uint8_t *w[5] = {Sm_Weights_Tx_4x4, Sm_Weights_Tx_8x8, Sm_Weights_Tx_16x16, Sm_Weights_Tx_32x32,
+                         Sm_Weights_Tx_64x64};
+  for (int k = tid; k < 124; k += nt) {
+    const int lg = k < 4 ? 0 : k < 12 ? 1 : k < 28 ? 2 : k < 60 ? 3 : 4, base = (4 << lg) - 4;
+    gT.sw[k] = w[lg][k - base];
+  }
+}
+#define Mode_To_Angle gT.m2a
+#define Dr_Intra_Derivative gT.dr
+#define Intra_Edge_Kernel gT.iek
+#define Num_4x4_Blocks_Wide gT.n4w
+#define Num_4x4_Blocks_High gT.n4h
+#define Max_Tx_Size_Rect gT.maxRect
+#define Max_Tx_Depth gT.maxDepth
+#define Split_Tx_Size gT.split
+#define Subsampled_Size gT.subs
+#define Tx_Width gT.txw
+#define Tx_Height gT.txh
+#define Tx_Width_Log2 gT.txwl
+#define Tx_Height_Log2 gT.txhl
+#define Tx_Size_Sqr gT.sqr
+#define Tx_Size_Sqr_Up gT.sqrUp
+#define Transform_Row_Shift gT.rowShift
+#define Tx_Type_In_Set_Intra gT.inIntra
+#define Tx_Type_In_Set_Inter gT.inInter
+#define Mode_To_Txfm gT.m2t
+#define Sm_Weights_Tx_4x4 (gT.sw + 0)
+#define Sm_Weights_Tx_8x8 (gT.sw + 4)
+#define Sm_Weights_Tx_16x16 (gT.sw + 12)
+#define Sm_Weights_Tx_32x32 (gT.sw + 28)
+#define Sm_Weights_Tx_64x64 (gT.sw + 60)
+
 #include "dev_pred.h"
 #include "dev_tx.h"
 #include "internal.h"
@@ -1245,6 +1313,7 @@ __global__ __launch_bounds__(RT) void k_recon_inter(const Mav1FrameParams fp, Ma
                                                     int *work) {
   RLds &L = gL;
   const int tid = threadIdx.x;
+  stage_tabs(tid, RT);
   stage_smats(fp, mats);
   const int count = work[0];
   for (;;) {
@@ -1442,6 +1511,7 @@ __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Ma
   if (tid == 0) L.ival[2] = atomicAdd(&iwork[IW_NEXT], 1);
   __syncthreads();
   if (L.ival[2] >= iwork[IW_COUNT]) return;
+  stage_tabs(tid, RT);
   stage_smats(fp, mats);
   for (;;) {
     const int idx = L.ival[2];
@@ -1474,6 +1544,8 @@ __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Ma
 __global__ __launch_bounds__(64) void k_edge_flags(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
                                                    const Mav1SbInfo *__restrict__ sbs, uint8_t *ys, uint8_t *uvs) {
   const int sb = blockIdx.x, lane = threadIdx.x;
+  stage_tabs(lane, 64);
+  __syncthreads();
   if (lane >= sbs[sb].num_blocks) return;
   const Mav1BlockInfo b = blocks[sbs[sb].first_block + lane];
   const int sy = !b.is_inter && (b.y_mode == SMOOTH_PRED || b.y_mode == SMOOTH_V_PRED || b.y_mode == SMOOTH_H_PRED);
