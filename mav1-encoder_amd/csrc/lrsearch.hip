@@ -555,30 +555,36 @@ DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t
 }
 
 /* Per unit: SSE of NONE, the Wiener fit, and each self-guided set's
- * weights (sgr_fit, lr_search.c:183-231) and SSE. */
+ * weights (sgr_fit, lr_search.c:183-231) and SSE. Three workgroups per unit
+ * (blockIdx.y): NONE and the Wiener fit, then the sets 0-7 and 8-15. */
 __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
                                                  const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
                                                  LrUnitStat *__restrict__ stats) {
-  const int u = blockIdx.x, tid = threadIdx.x;
+  const int u = blockIdx.x, part = blockIdx.y, tid = threadIdx.x;
   LrGeo g;
   lr_geo(fp, u, g);
   lr_stage(fp, g, cdef, dbk, src, tid);
-  lr_xtab(tid);
   LrUnitStat &st = stats[u];
   const int S = fp.stride[g.p], o = fp.offset[g.p], uw = g.x1 - g.x0, uh = g.y1 - g.y0;
-  long long e = 0;
-  for (int q = tid; q < uw * uh; q += LR_T) {
-    const int a = o + (g.y0 + q / uw) * S + g.x0 + q % uw;
-    const int d = (int)cdef[a] - (int)gLr.srcu[q];
-    e += d * d;
+  if (part == 0) {
+    /* NONE and the Wiener fit */
+    long long e = 0;
+    for (int q = tid; q < uw * uh; q += LR_T) {
+      const int a = o + (g.y0 + q / uw) * S + g.x0 + q % uw;
+      const int d = (int)cdef[a] - (int)gLr.srcu[q];
+      e += d * d;
+    }
+    e = block_sum64(e, tid);
+    if (tid == 0) st.sseNone = e;
+    int w0[6];
+    lr_wiener_fit(fp, g, src, w0, tid);
+    if (tid < 6) st.w0[tid] = w0[tid];
+    return;
   }
-  e = block_sum64(e, tid);
-  if (tid == 0) st.sseNone = e;
-  int w0[6];
-  lr_wiener_fit(fp, g, src, w0, tid);
-  if (tid < 6) st.w0[tid] = w0[tid];
+  /* half of the self-guided sets */
+  lr_xtab(tid);
   const int nblk = g.nbr * g.nbc;
-  for (int set = 0; set < 16; set++) {
+  for (int set = (part - 1) * 8; set < part * 8; set++) {
     /* the radii are 2 (pass 0) and 1 (pass 1), or 0: pass off */
     const int r0 = SGR_PARAM(set, 0), r1 = SGR_PARAM(set, 2);
     int sc[2], one[2];
@@ -989,7 +995,7 @@ size_t lr_chain_bytes() { return 3 * sizeof(LrChain); }
 
 hipError_t launch_lr_fit(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
                          const uint8_t *dbk, void *stats, void *chain, int nunits) {
-  hipLaunchKernelGGL(k_lr_fit, dim3(nunits), dim3(LR_T), 0, st, fp, src, cdef, dbk, (LrUnitStat *)stats);
+  hipLaunchKernelGGL(k_lr_fit, dim3(nunits, 3), dim3(LR_T), 0, st, fp, src, cdef, dbk, (LrUnitStat *)stats);
   hipLaunchKernelGGL(k_lr_chain_init, dim3(1), dim3(64), 0, st, (LrChain *)chain);
   return hipGetLastError();
 }
