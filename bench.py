@@ -98,10 +98,11 @@ def algorithmic_bytes(kernel, fp, nref):
 BW = [4, 4, 8, 8, 8, 16, 16, 16, 32, 32, 32, 64, 64, 64, 128, 128, 4, 16, 8, 32, 16, 64]
 BH = [4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 64, 32, 64, 128, 64, 128, 16, 4, 32, 8, 64, 16]
 
-# HBM traffic per recon_intra launch over the ALTREF / leaf mix, from the
+# HBM traffic per recon_intra launch over the timed frames of the default run
+# (40 steps after 6 warm-up ones, ALTREF / leaf alternating), from the
 # FETCH_SIZE and WRITE_SIZE passes of profiles/recon_intra_1080p_turbo_pmc.md,
 # scaled by the frame copy's factors there (fetch x1.985, write x1.0).
-RECON_INTRA_TRAFFIC = 5988150
+RECON_INTRA_TRAFFIC = 6692434
 
 
 def intra_phase_bytes(fp, blocks, sbs):
@@ -182,21 +183,27 @@ def main():
             ctx.launch_restore()
         return mav1hip.BUF_LROUT if plan.restoration else mav1hip.BUF_REC
 
+    final_buf = [None]
+
     def step(kind, f, last, alt):
         ref, gold = slot_of[last], (slot_of[alt] if alt >= 0 else -1)
         ctx.bind_frames(f % uniq, ref, gold)
         ctx.set_params(params[kind])
-        final = run_stages(kind)
+        final = final_buf[0] = run_stages(kind)
         if kind == "arf":  # the final picture becomes a reference: the slot LAST does not hold
             target = uniq if ref != uniq else uniq + 1
             ctx.pool_store(target, final)
             slot_of[f] = target
 
-    # frame 0: the key frame (untimed), its reconstruction in the first slot
-    ctx.bind_frames(0, -1, -1)
-    ctx.set_params(params["key"])
-    ctx.pool_store(uniq, run_stages("key"))
-    slot_of[0] = uniq
+    def code_key():
+        """Frame 0: the key frame (untimed), its reconstruction in the first slot."""
+        slot_of.clear()
+        ctx.bind_frames(0, -1, -1)
+        ctx.set_params(params["key"])
+        ctx.pool_store(uniq, run_stages("key"))
+        slot_of[0] = uniq
+
+    code_key()
 
     for s in order[:args.warmup]:
         step(*s)
@@ -209,16 +216,24 @@ def main():
     ctx.sync()
     elapsed = time.perf_counter() - t0
     ctx.timing(False)
-    # the intra phase's algorithmic bytes: one more ALTREF + leaf pair,
-    # untimed, its block records read back after each frame
+    last_out = ctx.download(final_buf[0])
+    # the intra phase's algorithmic bytes of every timed frame: the whole
+    # sequence coded again, untimed, from the key frame on (the coding is
+    # deterministic), the block records read back after each timed frame
+    code_key()
     intra_bytes = []
-    for kind, f, last, alt in coding_order(args.warmup + args.steps + 2)[-2:]:
-        step(kind, f, last, alt)
+    for i, s in enumerate(order):
+        step(*s)
+        if i < args.warmup:
+            continue
         ctx.sync()
-        nsb = params[kind].sb_cols * params[kind].sb_rows
+        p = params[s[0]]
+        nsb = p.sb_cols * p.sb_rows
         blk = ctx.download(mav1hip.BUF_BLOCKS, np.int32, nsb * 64 * 27).reshape(-1, 27)
         sbi = ctx.download(mav1hip.BUF_SBS, np.int32, nsb * 89).reshape(-1, 89)
-        intra_bytes.append(intra_phase_bytes(params[kind], blk, sbi))
+        intra_bytes.append(intra_phase_bytes(p, blk, sbi))
+    if not np.array_equal(ctx.download(final_buf[0]), last_out):
+        raise RuntimeError("the second coding of the timed frames differs from the first")
     kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3", "analyze_level0",
                "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table", "edge_types", "recon_inter",
                "intra_sched", "recon_intra", "filter_info", "deblock", "cdef", "cdef_select", "lr_fit", "lr_path", "lr_chain", "lr_apply"]
@@ -234,7 +249,7 @@ def main():
     if dom in ("qpel_build", "dec_build"):
         per_launch = algorithmic_bytes(dom, params["arf"], 1)
     elif dom == "recon_intra":
-        per_launch = np.mean(intra_bytes)  # one launch per frame
+        per_launch = np.mean(intra_bytes)  # one launch per timed frame
     else:
         per_launch = np.mean([algorithmic_bytes(dom, params[k], 2 if k == "leaf" else 1)
                               for k, *_ in order[args.warmup:]])
