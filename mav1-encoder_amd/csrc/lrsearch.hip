@@ -29,6 +29,7 @@
 #define LR_MAXSEG 4
 #define LR_MAXPX (104 * 96)
 #define LR_T 256       /* threads per workgroup */
+#define LR_PH (4 * 95 + 8) /* a block row's samples (<= 4 rows x 95), + 8 read ahead by the ordered sums */
 
 struct LrGeo {
   int p, sub, x0, x1, y0, y1, endX, endY;
@@ -47,13 +48,13 @@ struct LrLds {
     } sg;
     struct {
       double h[2][1024];           /* a block row's 7-tap filtered samples (two rows in flight) */
-      double ph[2][4][LR_T + 128]; /* a block row's fit terms, phi[0..2] then target, in sample order */
+      double ph[2][4][LR_PH];      /* a block row's fit terms, phi[0..2] then target, in sample order */
     } wf;
     struct {
       int16_t mid[2][LR_ROWS * 96]; /* Wiener horizontal pass: current taps, candidate taps */
     } wp;
   } u;
-  long long red[LR_T / 64 * 6];
+  long long red[LR_T / 64 * 12];
   double acc[12];
   int itmp[16];
   int xtab[256]; /* a of the self-guided filter by z (7.17.3) */
@@ -102,27 +103,38 @@ DEVI void lr_geo(const Mav1FrameParams &fp, int u, LrGeo &g) {
   g.bh = 4 >> sub;
   g.nbr = (g.y1 - g.y0 + g.bh - 1) / g.bh;
   g.nbc = (g.x1 - g.x0 + g.bw - 1) / g.bw;
-  int k = ((g.y0 << sub) + 8) / 64, base = 0;
+  /* unrolled, every index a constant: the arrays stay in registers (the
+   * fields of segments past nseg are never read) */
+  int k = ((g.y0 << sub) + 8) / 64, base = 0, y = g.y0;
   g.nseg = 0;
-  for (int y = g.y0; y < g.y1 && g.nseg < LR_MAXSEG; k++) {
+#pragma unroll
+  for (int i = 0; i < LR_MAXSEG; i++, k++) {
     const int s0 = lr_stripe_row(k, sub), s1 = lr_stripe_row(k + 1, sub) - 1;
     const int a = d_max(y, s0), b = d_min(g.y1, s1 + 1);
-    g.segY0[g.nseg] = a;
-    g.segY1[g.nseg] = b;
-    g.segS0[g.nseg] = s0;
-    g.segS1[g.nseg] = s1;
-    g.segBase[g.nseg] = base;
-    base += (b - a + 6) * LR_PITCH;
-    g.nseg++;
-    y = b;
+    g.segY0[i] = a;
+    g.segY1[i] = b;
+    g.segS0[i] = s0;
+    g.segS1[i] = s1;
+    g.segBase[i] = base;
+    if (y < g.y1) {
+      base += (b - a + 6) * LR_PITCH;
+      g.nseg++;
+      y = b;
+    }
   }
 }
+
+/* v[s] by selects over values, so that a run-time segment index does not
+ * move the arrays to scratch memory (a select over the array's own
+ * elements would be folded back into an indexed load). */
+DEVI int lr_pick(int v0, int v1, int v2, int v3, int s) { return s == 0 ? v0 : s == 1 ? v1 : s == 2 ? v2 : v3; }
+#define lr_sel(v, s) lr_pick((v)[0], (v)[1], (v)[2], (v)[3], (s))
 
 DEVI int lr_seg_of(const LrGeo &g, int y) { return y < g.segY1[0] ? 0 : y < g.segY1[1] ? 1 : y < g.segY1[2] ? 2 : 3; }
 
 /* Sample (x, y) of segment s, -3 <= x - x0 < w + 3, -3 <= y - segY0 < h + 3. */
 DEVI int lr_img(const LrGeo &g, int s, int x, int y) {
-  return gLr.img[g.segBase[s] + (y - g.segY0[s] + 3) * LR_PITCH + (x - g.x0 + 3)];
+  return gLr.img[lr_sel(g.segBase, s) + (y - lr_sel(g.segY0, s) + 3) * LR_PITCH + (x - g.x0 + 3)];
 }
 
 /* get_source_sample (7.17.6) of every sample the unit's filters read, and
@@ -135,20 +147,21 @@ DEVI void lr_stage(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t *cde
     gLr.srcu[q] = src[o + (g.y0 + r) * S + g.x0 + q - r * uw];
   }
   for (int s = 0; s < g.nseg; s++) {
-    const int h = g.segY1[s] - g.segY0[s] + 6;
+    const int sy0 = lr_sel(g.segY0, s), ss0 = lr_sel(g.segS0, s), ss1 = lr_sel(g.segS1, s);
+    const int h = lr_sel(g.segY1, s) - sy0 + 6, sb = lr_sel(g.segBase, s);
     for (int q = tid; q < w * h; q += LR_T) {
       const int r = q / w, c = q - r * w;
       const int x = d_clip(0, g.endX, g.x0 - 3 + c);
-      int y = d_clip(0, g.endY, g.segY0[s] - 3 + r);
+      int y = d_clip(0, g.endY, sy0 - 3 + r);
       const uint8_t *pl = cdef;
-      if (y < g.segS0[s]) {
-        y = d_max(g.segS0[s] - 2, y);
+      if (y < ss0) {
+        y = d_max(ss0 - 2, y);
         pl = dbk;
-      } else if (y > g.segS1[s]) {
-        y = d_min(g.segS1[s] + 2, y);
+      } else if (y > ss1) {
+        y = d_min(ss1 + 2, y);
         pl = dbk;
       }
-      gLr.img[g.segBase[s] + r * LR_PITCH + c] = pl[o + y * S + x];
+      gLr.img[sb + r * LR_PITCH + c] = pl[o + y * S + x];
     }
   }
   __syncthreads();
@@ -236,8 +249,8 @@ DEVI void lr_xtab(int tid) {
 template <int R, int BS>
 DEVI void lr_ab(const LrGeo &g, int s, int bx, int by, int sc, int one, int A[6][6], int B[6][6]) {
   constexpr int P = BS + 2, N = P + 2 * R, D = 2 * R + 1, n = D * D;
-  const int xmax = g.x1 + 2, ymax = g.segY1[s] + 2;
-  const uint8_t *img = gLr.img + g.segBase[s] + (3 - g.segY0[s]) * LR_PITCH + (3 - g.x0); /* img[y * P + x] */
+  const int xmax = g.x1 + 2, ymax = lr_sel(g.segY1, s) + 2;
+  const uint8_t *img = gLr.img + lr_sel(g.segBase, s) + (3 - lr_sel(g.segY0, s)) * LR_PITCH + (3 - g.x0); /* img[y * P + x] */
   int xs[N], cs[N], cq[N];
 #pragma unroll
   for (int c = 0; c < N; c++) {
@@ -291,8 +304,8 @@ DEVI void lr_ab(const LrGeo &g, int s, int bx, int by, int sc, int one, int A[6]
  * pass 1: radius 1) into F (row stride 4). */
 template <int PASS, int BS>
 DEVI void lr_f(const LrGeo &g, int s, int bx, int by, const int A[6][6], const int B[6][6], int *F) {
-  const uint8_t *img = gLr.img + g.segBase[s] + (3 - g.segY0[s]) * LR_PITCH + (3 - g.x0);
-  const int xmax = g.x1 + 2, ymax = g.segY1[s] + 2;
+  const uint8_t *img = gLr.img + lr_sel(g.segBase, s) + (3 - lr_sel(g.segY0, s)) * LR_PITCH + (3 - g.x0);
+  const int xmax = g.x1 + 2, ymax = lr_sel(g.segY1, s) + 2;
 #pragma unroll
   for (int i = 0; i < BS; i++) {
     const int sh = SGRPROJ_SGR_BITS + ((PASS == 0 && (i & 1)) ? 4 : 5) - SGRPROJ_RST_BITS;
@@ -341,10 +354,10 @@ DEVI void lr_mid(const LrGeo &g, const int *hw, int m, int tid) {
   const int uw = g.x1 - g.x0;
   int rowBase = 0;
   for (int s = 0; s < g.nseg; s++) {
-    const int rows = g.segY1[s] - g.segY0[s] + 6;
+    const int rows = lr_sel(g.segY1, s) - lr_sel(g.segY0, s) + 6, sb = lr_sel(g.segBase, s);
     for (int q = tid; q < rows * uw; q += LR_T) {
       const int r = q / uw, c = q - r * uw;
-      const uint8_t *ip = gLr.img + g.segBase[s] + r * LR_PITCH + c;
+      const uint8_t *ip = gLr.img + sb + r * LR_PITCH + c;
       int acc = 0;
 #pragma unroll
       for (int t = 0; t < 7; t++) acc += hf[t] * ip[t];
@@ -368,14 +381,14 @@ DEVI long long lr_wiener_sse(const Mav1FrameParams &fp, const LrGeo &g, const in
   long long e = 0;
   int rowBase = 0;
   for (int s = 0; s < g.nseg; s++) {
-    const int h = g.segY1[s] - g.segY0[s];
+    const int sy0 = lr_sel(g.segY0, s), h = lr_sel(g.segY1, s) - sy0;
     for (int q = tid; q < h * uw; q += LR_T) {
       const int r = q / uw, c = q - r * uw;
       const int16_t *mp = gLr.u.wp.mid[m] + (rowBase + r) * 96 + c;
       int acc = 0;
 #pragma unroll
       for (int t = 0; t < 7; t++) acc += vf[t] * mp[t * 96];
-      const int d = d_clip1(d_round2(acc, 11)) - (int)gLr.srcu[(g.segY0[s] + r - g.y0) * uw + c];
+      const int d = d_clip1(d_round2(acc, 11)) - (int)gLr.srcu[(sy0 + r - g.y0) * uw + c];
       e += d * d;
     }
     rowBase += h + 6;
@@ -477,7 +490,7 @@ DEVI void lr_wf_filter(const LrGeo &g, int br, int vert, const double *f, double
 /* The fit terms of block row br's samples in the reference's sample order
  * (lr_search.c:150-160): phi[a - first] = Z[a] + Z[6 - a] - 2 Z[3] and the
  * target src - Z[3], Z from the row's 7-tap sums. */
-DEVI void lr_wf_phi(const LrGeo &g, int br, int vert, int first, const double *h, double (*ph)[LR_T + 128], int t0,
+DEVI void lr_wf_phi(const LrGeo &g, int br, int vert, int first, const double *h, double (*ph)[LR_PH], int t0,
                     int nt) {
   const int y0 = g.y0 + br * g.bh, uw = g.x1 - g.x0, cnt = lr_row_count(g, br);
   for (int q = t0; q < cnt; q += nt) {
@@ -492,11 +505,57 @@ DEVI void lr_wf_phi(const LrGeo &g, int br, int vert, int first, const double *h
   }
 }
 
+/* The first solve's sums (a vertical fit under the identity horizontal
+ * filter): every fit term is then an integer (|phi| <= 510, |target| <=
+ * 255), so every product and partial sum of the reference's FP64
+ * accumulation is an integer below 2^53 and exact, whatever the order.
+ * All lanes sum their samples in int32 (<= 39 samples x 510^2 each), the
+ * waves in int64; the NC * NC + NC sums land in gLr.acc as doubles. */
+template <int NC>
+DEVI void lr_wf_exact(const LrGeo &g, int tid) {
+  constexpr int FIRST = 3 - NC, NACC = NC * NC + NC;
+  const int uw = g.x1 - g.x0, n = uw * (g.y1 - g.y0);
+  int sum[NACC];
+#pragma unroll
+  for (int i = 0; i < NACC; i++) sum[i] = 0;
+  for (int q = tid; q < n; q += LR_T) {
+    const int r = q / uw, y = g.y0 + r, x = g.x0 + q - r * uw, s = lr_seg_of(g, y);
+    int Z[7];
+#pragma unroll
+    for (int a = 0; a < 7; a++) Z[a] = lr_img(g, s, x, y + a - 3);
+    int ph[NC + 1];
+#pragma unroll
+    for (int a = FIRST; a < 3; a++) ph[a - FIRST] = Z[a] + Z[6 - a] - 2 * Z[3];
+    ph[NC] = (int)gLr.srcu[q] - Z[3];
+#pragma unroll
+    for (int a = 0; a < NC; a++) {
+#pragma unroll
+      for (int b = 0; b < NC; b++) sum[a * NC + b] += ph[a] * ph[b];
+      sum[NC * NC + a] += ph[a] * ph[NC];
+    }
+  }
+  const int wave = tid >> 6;
+#pragma unroll
+  for (int i = 0; i < NACC; i++) {
+    const int v = wave_sum(sum[i]);
+    if ((tid & 63) == 0) gLr.red[wave * 12 + i] = v;
+  }
+  __syncthreads();
+  if (tid < NACC) {
+    long long t = 0;
+#pragma unroll
+    for (int w = 0; w < LR_T / 64; w++) t += gLr.red[w * 12 + tid];
+    gLr.acc[tid] = (double)t;
+  }
+}
+
 /* Separable least-squares Wiener fit (lr_search.c:132-180): four
- * alternating solves. Each solve's sums run over the unit's samples in the
- * reference's order, one lane of the last wave per accumulator; the other
- * waves prepare the fit terms two block rows ahead (filter a row, then its
- * terms, then its sums: a three-step pipeline with one barrier a step). */
+ * alternating solves. The first solve's sums are exact integers
+ * (lr_wf_exact). Each later solve's sums run over the unit's samples in
+ * the reference's order, one lane of the last wave per accumulator; the
+ * other waves prepare the fit terms two block rows ahead (filter a row,
+ * then its terms, then its sums: a three-step pipeline with one barrier a
+ * step). */
 DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t *src, int *out, int tid) {
   (void)fp;
   (void)src;
@@ -505,26 +564,55 @@ DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t
   double fv[7] = {0, 0, 0, 1, 0, 0, 0}, fh[7] = {0, 0, 0, 1, 0, 0, 0};
   for (int it = 0; it < 4; it++) {
     const int vert = !(it & 1);
-    double f[7];
+    if (it == 0) {
+      if (nc == 3) lr_wf_exact<3>(g, tid);
+      else lr_wf_exact<2>(g, tid);
+    } else {
+      double f[7];
 #pragma unroll
-    for (int t = 0; t < 7; t++) f[t] = vert ? fh[t] : fv[t];
-    double acc = 0; /* lane < nacc of the last wave: M[a][b] (a * nc + b), then V[a] */
-    for (int st = 0; st < g.nbr + 2; st++) {
-      if (wave < LR_T / 64 - 1) {
-        if (st < g.nbr) lr_wf_filter(g, st, vert, f, gLr.u.wf.h[st & 1], tid, NCT);
-        if (st >= 1 && st - 1 < g.nbr)
-          lr_wf_phi(g, st - 1, vert, first, gLr.u.wf.h[(st - 1) & 1], gLr.u.wf.ph[(st - 1) & 1], tid, NCT);
-      } else if (st >= 2 && lane < nacc) {
-        const int br = st - 2, n = lr_row_count(g, br);
-        const int a = lane < nc * nc ? lane / nc : lane - nc * nc, b = lane < nc * nc ? lane - (lane / nc) * nc : 3;
-        typedef __attribute__((address_space(3))) const double lds_f64;
-        const lds_f64 *pa = (lds_f64 *)gLr.u.wf.ph[br & 1][a], *pb = (lds_f64 *)gLr.u.wf.ph[br & 1][b];
-#pragma unroll 8
-        for (int k = 0; k < n; k++) acc += pa[k] * pb[k];
+      for (int t = 0; t < 7; t++) f[t] = vert ? fh[t] : fv[t];
+      double acc = 0; /* lane < nacc of the last wave: M[a][b] (a * nc + b), then V[a] */
+      for (int st = 0; st < g.nbr + 2; st++) {
+        if (wave < LR_T / 64 - 1) {
+          if (st < g.nbr) lr_wf_filter(g, st, vert, f, gLr.u.wf.h[st & 1], tid, NCT);
+          if (st >= 1 && st - 1 < g.nbr)
+            lr_wf_phi(g, st - 1, vert, first, gLr.u.wf.h[(st - 1) & 1], gLr.u.wf.ph[(st - 1) & 1], tid, NCT);
+        } else if (st >= 2 && lane < nacc) {
+          const int br = st - 2, n = lr_row_count(g, br), n8 = n & ~7;
+          const int a = lane < nc * nc ? lane / nc : lane - nc * nc, b = lane < nc * nc ? lane - (lane / nc) * nc : 3;
+          typedef __attribute__((address_space(3))) const double lds_f64;
+          const lds_f64 *pa = (lds_f64 *)gLr.u.wf.ph[br & 1][a], *pb = (lds_f64 *)gLr.u.wf.ph[br & 1][b];
+          /* in order, sample by sample; the next 8 terms load while these 8 add */
+          double xa[8], xb[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            xa[j] = pa[j];
+            xb[j] = pb[j];
+          }
+          for (int k = 0; k < n8; k += 8) {
+            double ya[8], yb[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+              ya[j] = pa[k + 8 + j];
+              yb[j] = pb[k + 8 + j];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) acc += xa[j] * xb[j];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+              /* opaque here, so the loads stay a batch ahead (the compiler
+               * would otherwise move them back next to their use) */
+              asm volatile("" : "+v"(ya[j]), "+v"(yb[j]));
+              xa[j] = ya[j];
+              xb[j] = yb[j];
+            }
+          }
+          for (int k = n8; k < n; k++) acc += pa[k] * pb[k];
+        }
+        __syncthreads();
       }
-      __syncthreads();
+      if (wave == LR_T / 64 - 1 && lane < nacc) gLr.acc[lane] = acc;
     }
-    if (wave == LR_T / 64 - 1 && lane < nacc) gLr.acc[lane] = acc;
     __syncthreads();
     if (tid == 0) {
       double M[3][3], V[3], x[3] = {0, 0, 0};
@@ -539,19 +627,24 @@ DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t
     }
     __syncthreads();
     if (!gLr.itmp[0]) break;
-    double *dst = vert ? fv : fh, sum = 0;
+    double sum = 0;
+#pragma unroll
     for (int a = 0; a < 3; a++) {
       const double v = gLr.acc[a];
-      dst[a] = dst[6 - a] = v;
+      if (vert) fv[a] = fv[6 - a] = v;
+      else fh[a] = fh[6 - a] = v;
       sum += v;
     }
-    dst[3] = 1 - 2 * sum;
+    if (vert) fv[3] = 1 - 2 * sum;
+    else fh[3] = 1 - 2 * sum;
     __syncthreads();
   }
+#pragma unroll
   for (int pass = 0; pass < 2; pass++)
+#pragma unroll
     for (int j = 0; j < 3; j++)
       out[pass * 3 + j] =
-          j < first ? 0 : d_clip(Wiener_Taps_Min[j], Wiener_Taps_Max[j], (int)lround((pass ? fh : fv)[j] * 128));
+          j < first ? 0 : d_clip(Wiener_Taps_Min[j], Wiener_Taps_Max[j], (int)lround((pass ? fh[j] : fv[j]) * 128));
 }
 
 /* Per unit: SSE of NONE, the Wiener fit, and each self-guided set's
