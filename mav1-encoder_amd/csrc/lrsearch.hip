@@ -578,36 +578,45 @@ DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t
           if (st >= 1 && st - 1 < g.nbr)
             lr_wf_phi(g, st - 1, vert, first, gLr.u.wf.h[(st - 1) & 1], gLr.u.wf.ph[(st - 1) & 1], tid, NCT);
         } else if (st >= 2 && lane < nacc) {
-          const int br = st - 2, n = lr_row_count(g, br), n8 = n & ~7;
+          const int br = st - 2, n = lr_row_count(g, br), n16 = n & ~15;
           const int a = lane < nc * nc ? lane / nc : lane - nc * nc, b = lane < nc * nc ? lane - (lane / nc) * nc : 3;
           typedef __attribute__((address_space(3))) const double lds_f64;
           const lds_f64 *pa = (lds_f64 *)gLr.u.wf.ph[br & 1][a], *pb = (lds_f64 *)gLr.u.wf.ph[br & 1][b];
-          /* in order, sample by sample; the next 8 terms load while these 8 add */
-          double xa[8], xb[8];
+          /* in order, sample by sample, in batches of 8 that alternate
+           * between two register sets: one batch adds while the next loads.
+           * The scheduling barriers keep each batch's loads ahead of the
+           * previous batch's adds, and the empty asm statements make each
+           * set opaque once its batch is needed (the compiler would
+           * otherwise move the loads back next to their use). */
+          double xa[8], xb[8], ya[8], yb[8];
 #pragma unroll
           for (int j = 0; j < 8; j++) {
             xa[j] = pa[j];
             xb[j] = pb[j];
           }
-          for (int k = 0; k < n8; k += 8) {
-            double ya[8], yb[8];
+          for (int k = 0; k < n16; k += 16) {
 #pragma unroll
             for (int j = 0; j < 8; j++) {
               ya[j] = pa[k + 8 + j];
               yb[j] = pb[k + 8 + j];
             }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int j = 0; j < 8; j++) acc += xa[j] * xb[j];
 #pragma unroll
+            for (int j = 0; j < 8; j++) asm volatile("" : "+v"(ya[j]), "+v"(yb[j]));
+#pragma unroll
             for (int j = 0; j < 8; j++) {
-              /* opaque here, so the loads stay a batch ahead (the compiler
-               * would otherwise move them back next to their use) */
-              asm volatile("" : "+v"(ya[j]), "+v"(yb[j]));
-              xa[j] = ya[j];
-              xb[j] = yb[j];
+              xa[j] = pa[k + 16 + j];
+              xb[j] = pb[k + 16 + j];
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < 8; j++) acc += ya[j] * yb[j];
+#pragma unroll
+            for (int j = 0; j < 8; j++) asm volatile("" : "+v"(xa[j]), "+v"(xb[j]));
           }
-          for (int k = n8; k < n; k++) acc += pa[k] * pb[k];
+          for (int k = n16; k < n; k++) acc += pa[k] * pb[k];
         }
         __syncthreads();
       }
