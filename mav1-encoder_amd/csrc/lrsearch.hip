@@ -686,6 +686,7 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
   /* half of the self-guided sets */
   lr_xtab(tid);
   const int nblk = g.nbr * g.nbc;
+  const int r_t = tid / uw, c_t = tid - r_t * uw, dr_t = LR_T / uw, dc_t = LR_T - dr_t * uw;
   for (int set = (part - 1) * 8; set < part * 8; set++) {
     /* the radii are 2 (pass 0) and 1 (pass 1), or 0: pass off */
     const int r0 = SGR_PARAM(set, 0), r1 = SGR_PARAM(set, 2);
@@ -722,12 +723,29 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
           gLr.u.sg.f1[pi] = (int16_t)(r1 ? F1[i * 4 + j] : 0);
         }
     }
-    h00 = block_sum64(h00, tid);
-    h01 = block_sum64(h01, tid);
-    h11 = block_sum64(h11, tid);
-    c0 = block_sum64(c0, tid);
-    c1 = block_sum64(c1, tid);
+    /* the five block sums in one exchange: each wave's sums into red, one
+     * barrier, lane 0 of the workgroup adds them (red is read again only
+     * after the barrier below) */
+    {
+      const long long part[5] = {h00, h01, h11, c0, c1};
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        const long long v = wave_sum64(part[i]);
+        if ((tid & 63) == 0) gLr.red[(tid >> 6) * 5 + i] = v;
+      }
+    }
+    __syncthreads();
     if (tid == 0) {
+      long long tot[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int w = 0; w < LR_T / 64; w++)
+#pragma unroll
+        for (int i = 0; i < 5; i++) tot[i] += gLr.red[w * 5 + i];
+      h00 = tot[0];
+      h01 = tot[1];
+      h11 = tot[2];
+      c0 = tot[3];
+      c1 = tot[4];
       /* the reference's double sums of integer terms are exact: the same
        * values as these integer sums */
       const double H00 = (double)h00, H01 = (double)h01, H11 = (double)h11, C0 = (double)c0, C1 = (double)c1;
@@ -753,16 +771,25 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
     }
     __syncthreads();
     const int xw0 = gLr.itmp[0], xw1 = gLr.itmp[1], xw2 = (1 << SGRPROJ_PRJ_BITS) - xw0 - xw1;
-    long long es = 0;
-    for (int q = tid; q < uw * uh; q += LR_T) {
-      const int y = g.y0 + q / uw, x = g.x0 + q % uw;
+    /* each lane's samples step by LR_T in row-major order (row r, column
+     * c, no division in the loop); at most 39 squared differences of
+     * 8-bit samples per lane, so the lane's sum fits an int */
+    int es = 0;
+    for (int q = tid, r = r_t, c = c_t; q < uw * uh; q += LR_T) {
+      const int y = g.y0 + r, x = g.x0 + c;
       const int uu = lr_img(g, lr_seg_of(g, y), x, y) << SGRPROJ_RST_BITS;
       const int v = xw1 * uu + (r0 ? xw0 * gLr.u.sg.f0[q] : xw0 * uu) + (r1 ? xw2 * gLr.u.sg.f1[q] : xw2 * uu);
       const int d = d_clip1(d_round2(v, SGRPROJ_RST_BITS + SGRPROJ_PRJ_BITS)) - (int)gLr.srcu[q];
       es += d * d;
+      c += dc_t;
+      r += dr_t;
+      if (c >= uw) {
+        c -= uw;
+        r++;
+      }
     }
-    es = block_sum64(es, tid);
-    if (tid == 0) st.sseSgr[set] = es;
+    const long long ess = block_sum64(es, tid);
+    if (tid == 0) st.sseSgr[set] = ess;
     __syncthreads();
   }
 }
