@@ -212,6 +212,36 @@ DEVI int subpel_cost_hp(const Mav1FrameParams &fp, const uint8_t *srcL, const ui
   return wave_sum(s) * st;
 }
 
+/* Sub-pel costs of up to 9 candidates at once from the quarter-pel planes:
+ * the lanes take (candidate, 4x4 unit) pairs, and each candidate's cost is
+ * the same sum as subpel_cost_hp's (every st-th unit, times st), added up
+ * in LDS. Integer sums, so the order does not matter. */
+template <int BW>
+DEVI void subpel_costs_hp(const Mav1FrameParams &fp, const uint8_t *srcL, const uint8_t *planes, int x, int y,
+                          const int *cr, const int *cc, int ncand, int st, int lane, int *cost) {
+  constexpr int NR = BW / 4, NU = NR * NR;
+  const int W = fp.frame_w, H = fp.frame_h, nue = NU / st;
+  if (lane < ncand) cost[lane] = 0;
+  __syncthreads();
+  for (int p = lane; p < ncand * nue; p += 64) {
+    const int c = p / nue, u = (p - c * nue) * st;
+    const int mvr = cr[c], mvc = cc[c];
+    const uint8_t *pl = planes + (size_t)(((mvr >> 1) & 3) * 4 + ((mvc >> 1) & 3)) * W * H;
+    const int ucol = u % NR, urow = u / NR;
+    const int iy = y + urow * 4 + (mvr >> 3), ix = x + ucol * 4 + (mvc >> 3);
+    int d[16];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const uint8_t *prow = pl + (size_t)d_clip(0, H - 1, iy + i) * W;
+      const uint8_t *srow = srcL + (urow * 4 + i) * BW + ucol * 4;
+#pragma unroll
+      for (int j = 0; j < 4; j++) d[4 * i + j] = (int)srow[j] - (int)prow[d_clip(0, W - 1, ix + j)];
+    }
+    atomicAdd(&cost[c], satd4x4_arr(d) * st);
+  }
+  __syncthreads();
+}
+
 /* Sub-pel cost at an even 1/8-pel vector without the quarter-pel planes
  * (k_me.h:247-258): the normative EIGHTTAP prediction of the block
  * (7.11.3, staged through LDS) and the SATD of all its 4x4 units. */
@@ -271,6 +301,7 @@ __global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const
   __shared__ uint8_t win[WW * WW];
   __shared__ uint8_t pwin[HP ? 1 : (BW + 7) * (BW + 8)];
   __shared__ int16_t pmid[HP ? 1 : (BW + 7) * BW];
+  __shared__ int scost[9];
   const int lane = threadIdx.x;
   const int r = blockIdx.x / nodesPerRef, node = blockIdx.x - r * nodesPerRef;
   const int sb = node >> (2 * LEVEL), i = node & (N * N - 1);
@@ -379,9 +410,47 @@ __global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const
     if constexpr (HP) return subpel_cost_hp<BW>(fp, srcL, pl, m.x, m.y, r_, c_, st, lane);
     else return subpel_cost_pred<BW>(fp, srcL, refY, m.x, m.y, r_, c_, pwin, pmid, lane);
   };
-  int bestS = subpel(mvr, mvc);
   const int minStep = (BW >= 32 && TB(fp, MAV1_T_ME_HALF01)) ? 4 : 2;
   const bool cross = TB(fp, MAV1_T_ME_CROSS);
+  if constexpr (HP) {
+    /* The ring's costs do not depend on each other: all of a ring at once
+     * (the centre with the first ring), then the serial walk over them in
+     * ring order with the strict <, which keeps its first minimum. */
+    int bestS = 0;
+    for (int step = 4; step >= minStep; step >>= 1) {
+      const int cr0 = mvr, cc0 = mvc, first = step == 4;
+      int vr[9], vc[9], n = 0;
+      if (first) {
+        vr[0] = cr0;
+        vc[0] = cc0;
+        n = 1;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        if (cross && !(j == 1 || j == 3 || j == 4 || j == 6)) continue;
+        vr[n] = cr0 + (j < 3 ? -1 : j < 5 ? 0 : 1) * step;
+        vc[n] = cc0 + ((j == 0 || j == 3 || j == 5) ? -1 : (j == 1 || j == 6) ? 0 : 1) * step;
+        n++;
+      }
+      subpel_costs_hp<BW>(fp, srcL, pl, m.x, m.y, vr, vc, n, st, lane, scost);
+      if (first) bestS = scost[0];
+      for (int k = first; k < n; k++) {
+        const int c = scost[k];
+        if (c < bestS) {
+          bestS = c;
+          mvr = vr[k];
+          mvc = vc[k];
+        }
+      }
+      __syncthreads();
+    }
+    if (lane == 0) {
+      out[0] = mvr;
+      out[1] = mvc;
+    }
+    return;
+  }
+  int bestS = subpel(mvr, mvc);
   for (int step = 4; step >= minStep; step >>= 1) {
     const int cr = mvr, cc = mvc;
     for (int j = 0; j < (cross ? 4 : 8); j++) {
