@@ -30,6 +30,13 @@ struct mav1hip_ctx {
   size_t frame_bytes;
   int device;
   hipStream_t stream;
+  /* Analysis runs on `side` when motion search of the same frame was just
+   * enqueued: analysis level k waits for me_done[k] only, so it overlaps
+   * motion-search level k + 1; `stream` waits for ana_done before it goes
+   * on. me_pending: me_done holds this frame's levels. */
+  hipStream_t side;
+  hipEvent_t me_done[4], ana_done;
+  int me_pending;
 
   char err[512];
   char devname[256];
@@ -124,6 +131,7 @@ static int timed(mav1hip_ctx *c, const char *name, F &&launch) {
 
 static int collect_timing(mav1hip_ctx *c) {
   if (c->pending.empty()) return MAV1HIP_OK;
+  HIPCHK(c, hipStreamSynchronize(c->side));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (auto &t : c->pending) {
     float ms = 0.f;
@@ -169,10 +177,23 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   }
   c->num_sbs = c->sb_cols * c->sb_rows;
   int rc = MAV1HIP_OK;
-  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipStreamCreate(&c->side) != hipSuccess) {
     if (err && errlen) snprintf(err, errlen, "cannot create a HIP stream");
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return nullptr;
+  }
+  for (int i = 0; i < 5; i++) {
+    hipEvent_t &e = i < 4 ? c->me_done[i] : c->ana_done;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      if (err && errlen) snprintf(err, errlen, "cannot create a HIP event");
+      for (int j = 0; j < i; j++) (void)hipEventDestroy(j < 4 ? c->me_done[j] : c->ana_done);
+      (void)hipStreamDestroy(c->side);
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      return nullptr;
+    }
   }
 
   hipDeviceProp_t prop;
@@ -280,6 +301,7 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
 
 void mav1hip_destroy(mav1hip_ctx *c) {
   if (!c) return;
+  if (c->side) (void)hipStreamSynchronize(c->side);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto &t : c->pending) {
     (void)hipEventDestroy(t.start);
@@ -316,6 +338,9 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipFree(c->d_lrtype);
   (void)hipFree(c->d_lrreruns);
   (void)hipFree(c->d_pool);
+  for (int i = 0; i < 4; i++) (void)hipEventDestroy(c->me_done[i]);
+  (void)hipEventDestroy(c->ana_done);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -397,6 +422,7 @@ int mav1hip_set_params(mav1hip_ctx *c, const Mav1FrameParams *fp) {
   if (rc) return rc;
   c->fp = *fp;
   c->fp_set = 1;
+  c->me_pending = 0;
   HIPCHK(c, hipMemcpyAsync(c->d_lrtype, fp->lr_type, sizeof(int) * 3, hipMemcpyHostToDevice, c->stream));
   return MAV1HIP_OK;
 }
@@ -425,12 +451,16 @@ int mav1hip_launch_motion_search(mav1hip_ctx *c) {
       if ((rc = timed(c, "dec_build", [&] { return launch_dec_build(c->stream, fp, c->b_src, srcDec); }))) return rc;
   }
   static const char *names[4] = {"me_level0", "me_level1", "me_level2", "me_level3"};
-  for (int level = 0; level < 4; level++)
+  c->me_pending = 0;
+  for (int level = 0; level < 4; level++) {
     if ((rc = timed(c, names[level], [&] {
            return launch_me_level(c->stream, level, fp, c->b_src, c->b_ref, c->b_gold, c->d_planes, c->entry_bytes,
                                   srcDec, c->d_mvfield, c->d_mvgold, nref);
          })))
       return rc;
+    HIPCHK(c, hipEventRecord(c->me_done[level], c->stream));
+  }
+  c->me_pending = 1;
   return MAV1HIP_OK;
 }
 
@@ -439,14 +469,30 @@ int mav1hip_launch_analyze(mav1hip_ctx *c) {
   const Mav1FrameParams &fp = c->fp;
   static const char *names[4] = {"analyze_level0", "analyze_level1", "analyze_level2", "analyze_level3"};
   int rc;
-  for (int level = 0; level < 4; level++)
-    if ((rc = timed(c, names[level], [&] {
-           return launch_analyze(c->stream, level, fp, c->b_src, c->b_ref, c->b_gold, c->d_mvfield, c->d_mvgold,
-                                 c->d_acosts, c->d_ainfos);
+  /* Level k reads the motion field of level k only (and the costs of the
+   * levels above it): after this frame's motion search it runs on the side
+   * stream behind me_done[k], beside motion-search level k + 1. Level 0's
+   * wait also orders it after everything enqueued on the main stream
+   * before this frame's motion search. */
+  const bool overlap = c->me_pending != 0;
+  c->me_pending = 0;
+  hipStream_t st = overlap ? c->side : c->stream;
+  for (int level = 0; level < 4; level++) {
+    if (overlap) HIPCHK(c, hipStreamWaitEvent(c->side, c->me_done[level], 0));
+    if ((rc = timed_on(c, st, names[level], [&] {
+           return launch_analyze(st, level, fp, c->b_src, c->b_ref, c->b_gold, c->d_mvfield, c->d_mvgold, c->d_acosts,
+                                 c->d_ainfos);
          })))
       return rc;
-  return timed(c, "analyze_table",
-               [&] { return launch_analyze_table(c->stream, fp, c->d_acosts, c->d_ainfos, c->d_blocks, c->d_sbs); });
+  }
+  if ((rc = timed_on(c, st, "analyze_table",
+                     [&] { return launch_analyze_table(st, fp, c->d_acosts, c->d_ainfos, c->d_blocks, c->d_sbs); })))
+    return rc;
+  if (overlap) {
+    HIPCHK(c, hipEventRecord(c->ana_done, c->side));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ana_done, 0));
+  }
+  return MAV1HIP_OK;
 }
 
 int mav1hip_analyze(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *ref,
@@ -752,6 +798,7 @@ int mav1hip_recon(mav1hip_ctx *c, const Mav1FrameParams *fp, Mav1BlockInfo *bloc
 
 int mav1hip_sync(mav1hip_ctx *c) {
   if (!c) return MAV1HIP_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->side));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return check_intra(c);
 }
