@@ -150,19 +150,27 @@ def main():
     uniq = min(nframes, 50)
     frames = [synth.pad_frame(geo, gen.frame(t)) for t in range(uniq)]
 
-    ctx = mav1hip.Context(cfg)
-    # pool: the source frames, then two reference slots (LAST / ALTREF reconstructions)
-    ctx.pool_alloc(uniq + 2)
-    for i, f in enumerate(frames):
-        ctx.pool_upload(i, f)
+    # Two frame engines, as the reference's driver runs a stream's mini-GOP
+    # (encoder.c:1150-1224: engine encoders with a backend instance each, on
+    # their own queues): engine A codes the key frame and the ALTREF chain,
+    # engine B the leaf frames. Each engine is a context with its own stream
+    # and buffers; an ALTREF picture passes from A's pool to B's with
+    # mav1hip_pool_import, ordered on both streams.
+    engA, engB = mav1hip.Context(cfg), mav1hip.Context(cfg)
+    # pool: the source frames, then two reference slots
+    for eng in (engA, engB):
+        eng.pool_alloc(uniq + 2)
+        for i, f in enumerate(frames):
+            eng.pool_upload(i, f)
     params, plans = {}, {}
     for kind in ("key", "arf", "leaf"):
         fr = mav1hip.turbo_frame(kind, 0 if kind == "key" else 1)
         fp, npre = mav1hip.setup_params(cfg, fr)
         params[kind], plans[kind] = fp, mav1hip.filter_plan(cfg, fr, fp, npre)
-    slot_of = {}
+    slot_of = {}    # frame -> engine A's slot holding its final picture
+    b_slot_of = {}  # frame -> engine B's slot holding its imported copy
 
-    def run_stages(kind):
+    def run_stages(ctx, kind):
         plan = plans[kind]
         if kind != "key":
             ctx.launch_motion_search()
@@ -183,40 +191,63 @@ def main():
             ctx.launch_restore()
         return mav1hip.BUF_LROUT if plan.restoration else mav1hip.BUF_REC
 
-    final_buf = [None]
+    last_run = [None, None]  # engine and final buffer of the last coded frame
 
     def step(kind, f, last, alt):
-        ref, gold = slot_of[last], (slot_of[alt] if alt >= 0 else -1)
-        ctx.bind_frames(f % uniq, ref, gold)
-        ctx.set_params(params[kind])
-        final = final_buf[0] = run_stages(kind)
-        if kind == "arf":  # the final picture becomes a reference: the slot LAST does not hold
-            target = uniq if ref != uniq else uniq + 1
-            ctx.pool_store(target, final)
+        if kind == "arf":  # engine A: LAST is the previous ALTREF (or the key frame)
+            ref = slot_of[last]
+            engA.bind_frames(f % uniq, ref, -1)
+            engA.set_params(params[kind])
+            final = run_stages(engA, kind)
+            target = uniq if ref != uniq else uniq + 1  # the slot LAST does not hold
+            engA.pool_store(target, final)
             slot_of[f] = target
+            last_run[:] = [engA, final]
+            return
+        # engine B: LAST and ALTREF come from engine A, each imported once
+        # into the slot the other one does not hold
+        engB.bind_frames(f % uniq, -1, -1)
+        for need, other in ((last, alt), (alt, last)):
+            if need not in b_slot_of:
+                keep = b_slot_of.get(other)
+                slot = uniq if keep != uniq else uniq + 1
+                for g in [g for g, sl in b_slot_of.items() if sl == slot]:
+                    del b_slot_of[g]
+                engB.pool_import(slot, engA, slot_of[need])
+                b_slot_of[need] = slot
+        engB.bind_frames(f % uniq, b_slot_of[last], b_slot_of[alt])
+        engB.set_params(params[kind])
+        last_run[:] = [engB, run_stages(engB, kind)]
 
     def code_key():
-        """Frame 0: the key frame (untimed), its reconstruction in the first slot."""
+        """Frame 0: the key frame (untimed) on engine A, its reconstruction in A's first slot."""
         slot_of.clear()
-        ctx.bind_frames(0, -1, -1)
-        ctx.set_params(params["key"])
-        ctx.pool_store(uniq, run_stages("key"))
+        b_slot_of.clear()
+        engA.bind_frames(0, -1, -1)
+        engA.set_params(params["key"])
+        engA.pool_store(uniq, run_stages(engA, "key"))
         slot_of[0] = uniq
+
+    def sync():
+        engA.sync()
+        engB.sync()
 
     code_key()
 
     for s in order[:args.warmup]:
         step(*s)
-    ctx.sync()
-    ctx.timing(True)
-    ctx.timing_reset()
+    sync()
+    for eng in (engA, engB):
+        eng.timing(True)
+        eng.timing_reset()
     t0 = time.perf_counter()
     for s in order[args.warmup:]:
         step(*s)
-    ctx.sync()
+    sync()
     elapsed = time.perf_counter() - t0
-    ctx.timing(False)
-    last_out = ctx.download(final_buf[0])
+    for eng in (engA, engB):
+        eng.timing(False)
+    last_out = last_run[0].download(last_run[1])
     # the intra phase's algorithmic bytes of every timed frame: the whole
     # sequence coded again, untimed, from the key frame on (the coding is
     # deterministic), the block records read back after each timed frame
@@ -226,18 +257,24 @@ def main():
         step(*s)
         if i < args.warmup:
             continue
-        ctx.sync()
+        eng = last_run[0]
+        eng.sync()
         p = params[s[0]]
         nsb = p.sb_cols * p.sb_rows
-        blk = ctx.download(mav1hip.BUF_BLOCKS, np.int32, nsb * 64 * 27).reshape(-1, 27)
-        sbi = ctx.download(mav1hip.BUF_SBS, np.int32, nsb * 89).reshape(-1, 89)
+        blk = eng.download(mav1hip.BUF_BLOCKS, np.int32, nsb * 64 * 27).reshape(-1, 27)
+        sbi = eng.download(mav1hip.BUF_SBS, np.int32, nsb * 89).reshape(-1, 89)
         intra_bytes.append(intra_phase_bytes(p, blk, sbi))
-    if not np.array_equal(ctx.download(final_buf[0]), last_out):
+    sync()
+    if not np.array_equal(last_run[0].download(last_run[1]), last_out):
         raise RuntimeError("the second coding of the timed frames differs from the first")
     kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3", "analyze_level0",
                "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table", "edge_types", "recon_inter",
                "intra_sched", "recon_intra", "filter_info", "deblock", "cdef", "cdef_select", "lr_fit", "lr_path", "lr_chain", "lr_apply"]
-    stats = {k: ctx.timing_get(k) for k in kernels}
+    stats = {}
+    for k in kernels:
+        na, ta = engA.timing_get(k)
+        nb, tb = engB.timing_get(k)
+        stats[k] = (na + nb, ta + tb)
     fps = args.steps / elapsed
 
     # dominant kernel: the largest total time in the timed region
@@ -279,7 +316,9 @@ def main():
             "frame_kinds": "alternating hidden ALTREF (1 ref) / leaf (2 refs); leaf frames are not filtered",
             "references": "the reconstructed, filtered pictures of the referenced frames (closed loop from a coded key frame)",
             "filters": "deblocking + CDEF + loop restoration on hidden ALTREF frames (leaf frames skip them)",
-            "device": ctx.device_name,
+            "engines": "2 (engine A: key frame + ALTREF chain; engine B: leaf frames), one context and stream each, "
+                       "as the reference's driver runs a stream's mini-GOP on engine encoders",
+            "device": engA.device_name,
         },
         "kernels_ms_per_launch": {k: round(v[1] / max(v[0], 1), 5) for k, v in stats.items()},
         "roofline": {
@@ -298,7 +337,8 @@ def main():
         },
         "cpu_baseline": cpu,
     }
-    ctx.close()
+    engA.close()
+    engB.close()
     print(json.dumps(line))
 
 

@@ -286,6 +286,15 @@ int mav1hip_bind_frames(mav1hip_ctx *c, int32_t src_idx, int32_t ref_idx, int32_
  * pool frame idx on the context's stream: a reconstructed picture becomes a
  * reference for the frames that follow. */
 int mav1hip_pool_store(mav1hip_ctx *c, int32_t idx, int32_t buf);
+/* Copies pool frame src_idx of context src into pool frame dst_idx of
+ * context dst (same device and frame geometry), ordered after all work
+ * enqueued on src so far and before the later work of both contexts.
+ * Stands in for an engine's slot copy (encoder.c:1088-1118, the reference
+ * frames copied into the engine's ref / gold buffers before its frame
+ * runs): frame engines are separate contexts, each with its own stream, and
+ * a reference picture passes from the engine that coded it to the engine
+ * that reads it. Calls for one pair of contexts must not overlap. */
+int mav1hip_pool_import(mav1hip_ctx *dst, int32_t dst_idx, mav1hip_ctx *src, int32_t src_idx);
 
 /* Per-kernel timing with HIP events recorded on the context's stream
  * around every launch while enabled. mav1hip_timing_get returns the number

@@ -37,6 +37,8 @@ struct mav1hip_ctx {
   hipStream_t side;
   hipEvent_t me_done[4], ana_done;
   int me_pending;
+  /* cross-context ordering of mav1hip_pool_import */
+  hipEvent_t xfer;
 
   char err[512];
   char devname[256];
@@ -184,11 +186,11 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
     delete c;
     return nullptr;
   }
-  for (int i = 0; i < 5; i++) {
-    hipEvent_t &e = i < 4 ? c->me_done[i] : c->ana_done;
+  for (int i = 0; i < 6; i++) {
+    hipEvent_t &e = i < 4 ? c->me_done[i] : i == 4 ? c->ana_done : c->xfer;
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
       if (err && errlen) snprintf(err, errlen, "cannot create a HIP event");
-      for (int j = 0; j < i; j++) (void)hipEventDestroy(j < 4 ? c->me_done[j] : c->ana_done);
+      for (int j = 0; j < i; j++) (void)hipEventDestroy(j < 4 ? c->me_done[j] : j == 4 ? c->ana_done : c->xfer);
       (void)hipStreamDestroy(c->side);
       (void)hipStreamDestroy(c->stream);
       delete c;
@@ -340,6 +342,7 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipFree(c->d_pool);
   for (int i = 0; i < 4; i++) (void)hipEventDestroy(c->me_done[i]);
   (void)hipEventDestroy(c->ana_done);
+  (void)hipEventDestroy(c->xfer);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -866,6 +869,27 @@ int mav1hip_pool_store(mav1hip_ctx *c, int32_t idx, int32_t buf) {
     return fail(c, MAV1HIP_EINVAL, "pool_store: frame %d is bound to a stage input", idx);
   HIPCHK(c, hipMemcpyAsync(dst, buf == MAV1HIP_BUF_REC ? c->d_rec : c->d_lrout, c->frame_bytes,
                            hipMemcpyDeviceToDevice, c->stream));
+  return MAV1HIP_OK;
+}
+
+int mav1hip_pool_import(mav1hip_ctx *dst, int32_t dst_idx, mav1hip_ctx *src, int32_t src_idx) {
+  if (!dst || !src || dst == src) return fail(dst, MAV1HIP_EINVAL, "pool_import: needs two contexts");
+  if (dst_idx < 0 || dst_idx >= dst->pool_n) return fail(dst, MAV1HIP_EINVAL, "pool_import: bad index %d", dst_idx);
+  if (src_idx < 0 || src_idx >= src->pool_n)
+    return fail(dst, MAV1HIP_EINVAL, "pool_import: bad source index %d", src_idx);
+  if (dst->device != src->device || dst->frame_bytes != src->frame_bytes)
+    return fail(dst, MAV1HIP_EINVAL, "pool_import: contexts differ in device or frame geometry");
+  uint8_t *to = dst->d_pool + (size_t)dst_idx * dst->frame_bytes;
+  if (to == dst->b_src || to == dst->b_ref || to == dst->b_gold)
+    return fail(dst, MAV1HIP_EINVAL, "pool_import: frame %d is bound to a stage input", dst_idx);
+  /* after everything enqueued on src so far (the picture is complete), and
+   * before src's later work (which may overwrite the source frame) */
+  HIPCHK(dst, hipEventRecord(src->xfer, src->stream));
+  HIPCHK(dst, hipStreamWaitEvent(dst->stream, src->xfer, 0));
+  HIPCHK(dst, hipMemcpyAsync(to, src->d_pool + (size_t)src_idx * src->frame_bytes, dst->frame_bytes,
+                             hipMemcpyDeviceToDevice, dst->stream));
+  HIPCHK(dst, hipEventRecord(dst->xfer, dst->stream));
+  HIPCHK(dst, hipStreamWaitEvent(src->stream, dst->xfer, 0));
   return MAV1HIP_OK;
 }
 

@@ -114,6 +114,7 @@ def _load():
         "mav1hip_pool_upload": (C.c_int, [P, i32, P]),
         "mav1hip_bind_frames": (C.c_int, [P, i32, i32, i32]),
         "mav1hip_pool_store": (C.c_int, [P, i32, i32]),
+        "mav1hip_pool_import": (C.c_int, [P, i32, P, i32]),
         "mav1hip_timing_enable": (C.c_int, [P, i32]),
         "mav1hip_timing_reset": (C.c_int, [P]),
         "mav1hip_timing_get": (C.c_int, [P, C.c_char_p, C.POINTER(i32), C.POINTER(C.c_double)]),
@@ -391,6 +392,11 @@ class Context:
 
     def pool_store(self, idx, buf):
         self._chk(lib().mav1hip_pool_store(self.h, idx, buf), "pool_store")
+
+    def pool_import(self, idx, src, src_idx):
+        """Copy pool frame src_idx of Context src into this context's pool
+        frame idx, ordered on both contexts' streams."""
+        self._chk(lib().mav1hip_pool_import(self.h, idx, src.h, src_idx), "pool_import")
 
     def timing(self, on=True):
         self._chk(lib().mav1hip_timing_enable(self.h, 1 if on else 0), "timing_enable")

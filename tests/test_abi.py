@@ -57,3 +57,9 @@ def test_create_rejects_bad_size():
         assert "size" in str(e) or "device" in str(e)
     else:
         raise AssertionError("context created for an 8x8 frame")
+
+
+def test_pool_import_rejects_missing_contexts():
+    lib = C.CDLL(mav1hip.LIB_PATH)
+    lib.mav1hip_pool_import.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
+    assert lib.mav1hip_pool_import(None, 0, None, 0) == -1  # MAV1HIP_EINVAL
