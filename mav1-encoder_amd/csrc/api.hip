@@ -39,6 +39,8 @@ struct mav1hip_ctx {
   int me_pending;
   /* cross-context ordering of mav1hip_pool_import */
   hipEvent_t xfer;
+  /* the intra phase's scheduling pass on `side` beside the inter phase */
+  hipEvent_t rec_go, sched_done;
 
   char err[512];
   char devname[256];
@@ -186,11 +188,12 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
     delete c;
     return nullptr;
   }
-  for (int i = 0; i < 6; i++) {
-    hipEvent_t &e = i < 4 ? c->me_done[i] : i == 4 ? c->ana_done : c->xfer;
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+  hipEvent_t *evs[8] = {&c->me_done[0], &c->me_done[1], &c->me_done[2], &c->me_done[3],
+                        &c->ana_done,   &c->xfer,       &c->rec_go,     &c->sched_done};
+  for (int i = 0; i < 8; i++) {
+    if (hipEventCreateWithFlags(evs[i], hipEventDisableTiming) != hipSuccess) {
       if (err && errlen) snprintf(err, errlen, "cannot create a HIP event");
-      for (int j = 0; j < i; j++) (void)hipEventDestroy(j < 4 ? c->me_done[j] : j == 4 ? c->ana_done : c->xfer);
+      for (int j = 0; j < i; j++) (void)hipEventDestroy(*evs[j]);
       (void)hipStreamDestroy(c->side);
       (void)hipStreamDestroy(c->stream);
       delete c;
@@ -343,6 +346,8 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   for (int i = 0; i < 4; i++) (void)hipEventDestroy(c->me_done[i]);
   (void)hipEventDestroy(c->ana_done);
   (void)hipEventDestroy(c->xfer);
+  (void)hipEventDestroy(c->rec_go);
+  (void)hipEventDestroy(c->sched_done);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -543,13 +548,22 @@ int mav1hip_launch_edge_types(mav1hip_ctx *c) {
 int mav1hip_launch_recon(mav1hip_ctx *c) {
   if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "recon: no frame parameters set");
   const Mav1FrameParams &fp = c->fp;
-  int rc = timed(c, "recon_inter", [&] {
+  /* The scheduling pass reads only is_inter of the block records, which the
+   * inter phase writes back unchanged: it runs on the side stream beside the
+   * inter phase, after everything enqueued so far (the previous frame's
+   * intra phase included, whose work space it resets). */
+  HIPCHK(c, hipEventRecord(c->rec_go, c->stream));
+  HIPCHK(c, hipStreamWaitEvent(c->side, c->rec_go, 0));
+  int rc = timed_on(c, c->side, "intra_sched",
+                    [&] { return launch_intra_sched(c->side, fp, c->d_blocks, c->d_sbs, c->d_iwork); });
+  if (rc) return rc;
+  HIPCHK(c, hipEventRecord(c->sched_done, c->side));
+  rc = timed(c, "recon_inter", [&] {
     return launch_recon_inter(c->stream, fp, c->d_blocks, c->d_sbs, c->b_src, c->d_rec, c->b_ref, c->b_gold,
                               c->d_coefs, c->d_mats, c->d_rwork, 2 * c->num_cu);
   });
   if (rc) return rc;
-  rc = timed(c, "intra_sched", [&] { return launch_intra_sched(c->stream, fp, c->d_blocks, c->d_sbs, c->d_iwork); });
-  if (rc) return rc;
+  HIPCHK(c, hipStreamWaitEvent(c->stream, c->sched_done, 0));
   return timed(c, "recon_intra", [&] {
     return launch_recon_intra(c->stream, fp, c->d_blocks, c->d_sbs, c->b_src, c->d_rec, c->b_ref, c->b_gold,
                               c->d_coefs, c->d_mats, c->d_iwork, c->num_cu);
