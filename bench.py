@@ -170,12 +170,14 @@ def main():
     slot_of = {}    # frame -> engine A's slot holding its final picture
     b_slot_of = {}  # frame -> engine B's slot holding its imported copy
 
-    def run_stages(ctx, kind):
-        plan = plans[kind]
+    def early_stages(ctx, kind):
         if kind != "key":
             ctx.launch_motion_search()
         ctx.launch_analyze()
         ctx.launch_edge_types()
+
+    def late_stages(ctx, kind):
+        plan = plans[kind]
         ctx.launch_recon()
         if plan.filter_info:
             ctx.launch_filter_info()
@@ -191,21 +193,45 @@ def main():
             ctx.launch_restore()
         return mav1hip.BUF_LROUT if plan.restoration else mav1hip.BUF_REC
 
-    last_run = [None, None]  # engine and final buffer of the last coded frame
+    def run_stages(ctx, kind):
+        early_stages(ctx, kind)
+        return late_stages(ctx, kind)
 
-    def step(kind, f, last, alt):
+    last_run = [None, None]  # engine and final buffer of the last coded frame
+    # A leaf frame waits here for the next ALTREF: it is enqueued after that
+    # ALTREF's motion search, analysis and edge types, so its imports (which
+    # order it after everything engine A has enqueued) hold it until then and
+    # it runs beside the ALTREF's reconstruction, whose intra phase leaves
+    # most CUs idle. Only the enqueue order changes, not the results.
+    pending = []
+
+    def step(kind, f, last, alt, overlap=True):
         if kind == "arf":  # engine A: LAST is the previous ALTREF (or the key frame)
             ref = slot_of[last]
             engA.bind_frames(f % uniq, ref, -1)
             engA.set_params(params[kind])
-            final = run_stages(engA, kind)
+            early_stages(engA, kind)
+            if pending:
+                run_leaf(*pending.pop())
+            final = late_stages(engA, kind)
             target = uniq if ref != uniq else uniq + 1  # the slot LAST does not hold
             engA.pool_store(target, final)
             slot_of[f] = target
             last_run[:] = [engA, final]
             return
+        if overlap:
+            pending.append((f, last, alt))
+        else:
+            run_leaf(f, last, alt)
+
+    def flush():
+        if pending:
+            run_leaf(*pending.pop())
+
+    def run_leaf(f, last, alt):
         # engine B: LAST and ALTREF come from engine A, each imported once
         # into the slot the other one does not hold
+        kind = "leaf"
         engB.bind_frames(f % uniq, -1, -1)
         for need, other in ((last, alt), (alt, last)):
             if need not in b_slot_of:
@@ -236,6 +262,7 @@ def main():
 
     for s in order[:args.warmup]:
         step(*s)
+    flush()
     sync()
     for eng in (engA, engB):
         eng.timing(True)
@@ -243,6 +270,7 @@ def main():
     t0 = time.perf_counter()
     for s in order[args.warmup:]:
         step(*s)
+    flush()
     sync()
     elapsed = time.perf_counter() - t0
     for eng in (engA, engB):
@@ -254,7 +282,7 @@ def main():
     code_key()
     intra_bytes = []
     for i, s in enumerate(order):
-        step(*s)
+        step(*s, overlap=False)
         if i < args.warmup:
             continue
         eng = last_run[0]
@@ -317,7 +345,8 @@ def main():
             "references": "the reconstructed, filtered pictures of the referenced frames (closed loop from a coded key frame)",
             "filters": "deblocking + CDEF + loop restoration on hidden ALTREF frames (leaf frames skip them)",
             "engines": "2 (engine A: key frame + ALTREF chain; engine B: leaf frames), one context and stream each, "
-                       "as the reference's driver runs a stream's mini-GOP on engine encoders",
+                       "as the reference's driver runs a stream's mini-GOP on engine encoders; a leaf is enqueued "
+                       "behind the next ALTREF's analysis and runs beside its reconstruction",
             "device": engA.device_name,
         },
         "kernels_ms_per_launch": {k: round(v[1] / max(v[0], 1), 5) for k, v in stats.items()},
