@@ -301,6 +301,9 @@ int mav1hip_pool_import(mav1hip_ctx *dst, int32_t dst_idx, mav1hip_ctx *src, int
  * of launches of the named kernel and their total duration in ms since the
  * last reset (synchronises the stream). */
 int mav1hip_timing_enable(mav1hip_ctx *c, int32_t on);
+/* Restricts the events to the launches of one kernel (NULL: every kernel);
+ * each timed launch adds two event records to its stream. */
+int mav1hip_timing_select(mav1hip_ctx *c, const char *kernel);
 int mav1hip_timing_reset(mav1hip_ctx *c);
 int mav1hip_timing_get(mav1hip_ctx *c, const char *kernel, int32_t *launches, double *total_ms);
 

@@ -77,6 +77,7 @@ struct mav1hip_ctx {
   int pool_n;
   /* timing */
   int timing;
+  std::string timing_only; /* empty: every kernel */
   std::vector<TimedLaunch> pending;
   std::vector<hipEvent_t> free_events;
   std::map<std::string, std::pair<int, double>> stats;
@@ -115,14 +116,15 @@ static hipEvent_t take_event(mav1hip_ctx *c) {
 template <class F>
 static int timed_on(mav1hip_ctx *c, hipStream_t st, const char *name, F &&launch) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (c->timing) {
+  const bool on = c->timing && (c->timing_only.empty() || c->timing_only == name);
+  if (on) {
     a = take_event(c);
     b = take_event(c);
     if (a && b) HIPCHK(c, hipEventRecord(a, st));
   }
   hipError_t e = launch();
   if (e != hipSuccess) return fail(c, MAV1HIP_EDEVICE, "launch of %s: %s", name, hipGetErrorString(e));
-  if (c->timing && a && b) {
+  if (on && a && b) {
     HIPCHK(c, hipEventRecord(b, st));
     c->pending.push_back({name, a, b});
   }
@@ -910,6 +912,12 @@ int mav1hip_pool_import(mav1hip_ctx *dst, int32_t dst_idx, mav1hip_ctx *src, int
 int mav1hip_timing_enable(mav1hip_ctx *c, int32_t on) {
   if (!c) return MAV1HIP_EINVAL;
   c->timing = on != 0;
+  return MAV1HIP_OK;
+}
+
+int mav1hip_timing_select(mav1hip_ctx *c, const char *kernel) {
+  if (!c) return MAV1HIP_EINVAL;
+  c->timing_only = kernel ? kernel : "";
   return MAV1HIP_OK;
 }
 

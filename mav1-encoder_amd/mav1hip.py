@@ -116,6 +116,7 @@ def _load():
         "mav1hip_pool_store": (C.c_int, [P, i32, i32]),
         "mav1hip_pool_import": (C.c_int, [P, i32, P, i32]),
         "mav1hip_timing_enable": (C.c_int, [P, i32]),
+        "mav1hip_timing_select": (C.c_int, [P, C.c_char_p]),
         "mav1hip_timing_reset": (C.c_int, [P]),
         "mav1hip_timing_get": (C.c_int, [P, C.c_char_p, C.POINTER(i32), C.POINTER(C.c_double)]),
         "mav1hip_filter_plan": (i32, [C.POINTER(Config), C.POINTER(Frame), C.POINTER(FrameParams), i32,
@@ -400,6 +401,10 @@ class Context:
 
     def timing(self, on=True):
         self._chk(lib().mav1hip_timing_enable(self.h, 1 if on else 0), "timing_enable")
+
+    def timing_select(self, kernel=None):
+        """Time only the launches of `kernel` (None: every kernel)."""
+        self._chk(lib().mav1hip_timing_select(self.h, kernel.encode() if kernel else None), "timing_select")
 
     def timing_reset(self):
         self._chk(lib().mav1hip_timing_reset(self.h), "timing_reset")
