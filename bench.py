@@ -285,9 +285,10 @@ def main():
     flush()
     sync()
     warm = kernel_stats()
-    dom = max(warm, key=lambda k: warm[k][1])
+    dom = max(warm, key=lambda k: warm[k][1]) if any(v[0] for v in warm.values()) else None
     # the timed region: events around the dominant kernel's launches only
-    # (each timed launch adds two event records to its stream)
+    # (each timed launch adds two event records to its stream); without a
+    # warm-up, around every launch, and the dominant kernel named after
     timing(True, dom)
     t0 = time.perf_counter()
     for s in order[args.warmup:]:
@@ -295,7 +296,10 @@ def main():
     flush()
     sync()
     elapsed = time.perf_counter() - t0
-    n_launch, tot_ms = kernel_stats()[dom]
+    timed = kernel_stats()
+    if dom is None:
+        dom = max(timed, key=lambda k: timed[k][1])
+    n_launch, tot_ms = timed[dom]
     timing(False)
     last_out = last_run[0].download(last_run[1])
     # the intra phase's algorithmic bytes of every timed frame: the whole
