@@ -69,6 +69,7 @@ struct mav1hip_ctx {
   int num_cu;
   size_t decgrid_bytes;
   int lr_units;
+  int lr_gen; /* generation of the last restoration fit launch */
   long entry_bytes;
   size_t planes_bytes;
   /* frames the stages read (own buffers or pool frames) */
@@ -260,6 +261,9 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
     }
     rc |= dmalloc(c, (void **)&c->d_lrunits, sizeof(Mav1LrUnit) * (size_t)c->lr_units);
     rc |= dmalloc(c, &c->d_lrstats, lr_stat_bytes() * (size_t)c->lr_units);
+    /* zeroed once: the fit's generation numbers start at 1 */
+    if (rc == MAV1HIP_OK && hipMemset(c->d_lrstats, 0, lr_stat_bytes() * (size_t)c->lr_units) != hipSuccess)
+      rc = fail(c, MAV1HIP_EDEVICE, "hipMemset of the restoration statistics failed");
     rc |= dmalloc(c, &c->d_lrchain, lr_chain_bytes());
     rc |= dmalloc(c, (void **)&c->d_lrtype, sizeof(int) * 3);
     rc |= dmalloc(c, (void **)&c->d_lrreruns, sizeof(int) * 3);
@@ -632,20 +636,23 @@ int mav1hip_launch_lr_select(mav1hip_ctx *c, int32_t lr_test) {
   if (!c || !c->fp_set) return fail(c, MAV1HIP_EINVAL, "lr select: no frame parameters set");
   int rc = check_lr_params(c, c->fp);
   if (rc) return rc;
+  c->lr_gen = c->lr_gen % 0x7ffffffe + 1;
   if ((rc = timed(c, "lr_fit", [&] {
-         return launch_lr_fit(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrchain, c->lr_units);
+         return launch_lr_fit(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrchain, c->lr_units,
+                              c->lr_gen);
        })))
     return rc;
-  /* refinement under the default reference, then up to LR_ROUNDS - 1
-   * chain launches that stop at a reference change, each followed by the
-   * refinement of the remaining units under the new reference; the last
-   * chain launch refines any further mismatching unit itself */
+  /* the fit launch refines under the default reference; then up to
+   * LR_ROUNDS - 1 chain launches that stop at a reference change, each
+   * followed by the refinement of the remaining units under the new
+   * reference; the last chain launch refines any further mismatching unit
+   * itself */
   const int LR_ROUNDS = 3;
   for (int r = 0; r < LR_ROUNDS; r++) {
-    if ((rc = timed(c, "lr_path", [&] {
-           return launch_lr_path(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrchain,
-                                 c->lr_units);
-         })))
+    if (r > 0 && (rc = timed(c, "lr_path", [&] {
+                    return launch_lr_path(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrchain,
+                                          c->lr_units);
+                  })))
       return rc;
     if ((rc = timed(c, "lr_chain", [&] {
            return launch_lr_chain(c->stream, c->fp, c->b_src, c->d_rec, c->d_dbk, c->d_lrstats, c->d_lrunits,

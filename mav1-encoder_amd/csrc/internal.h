@@ -66,9 +66,11 @@ hipError_t launch_decision_grid(hipStream_t st, const Mav1FrameParams &fp, const
 /* lrsearch.hip */
 size_t lr_stat_bytes();
 size_t lr_chain_bytes();
-/* The fit of every unit, and the chain state reset. */
+/* The fit of every unit with its refinement under the default reference,
+ * and the chain state reset. gen: a number that differs from the previous
+ * launch's on the same statistics buffer (the buffer starts zeroed). */
 hipError_t launch_lr_fit(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
-                         const uint8_t *dbk, void *stats, void *chain, int nunits);
+                         const uint8_t *dbk, void *stats, void *chain, int nunits, int gen);
 /* Refinement of the units from each plane's chain restart point on. */
 hipError_t launch_lr_path(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
                           const uint8_t *dbk, void *stats, const void *chain, int nunits);

@@ -71,6 +71,8 @@ struct LrUnitStat {
   int wStar[6];       /* refined taps */
   long long sseStar;  /* SSE of the refined taps */
   long long costStar; /* their cost under refAssumed */
+  int fitGen;         /* generation of the launch whose fit w0 holds */
+  int pad;
 };
 
 /* Per plane: where the unit chain stands between its launches. */
@@ -657,16 +659,54 @@ DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t
 }
 
 /* Per unit: SSE of NONE, the Wiener fit, and each self-guided set's
- * weights (sgr_fit, lr_search.c:183-231) and SSE. Three workgroups per unit
- * (blockIdx.y): NONE and the Wiener fit, then the sets 0-7 and 8-15. */
+ * weights (sgr_fit, lr_search.c:183-231) and SSE. Four workgroups per unit
+ * (blockIdx.y): NONE and the Wiener fit, the sets 0-7, the sets 8-15, and
+ * the greedy refinement of the fitted taps under the default reference
+ * (what k_lr_path computes before the first chain launch). The refinement
+ * waits for its unit's fit, published with generation `gen`; every fit
+ * workgroup is dispatched before any refinement workgroup (blockIdx.y
+ * order) and waits for nothing, so the wait ends. Should it run out after
+ * maxSpin polls, the refinement records an assumed reference no chain
+ * holds, and the chain refines the unit again: the result never depends on
+ * the wait. */
 __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
                                                  const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
-                                                 LrUnitStat *__restrict__ stats) {
+                                                 LrUnitStat *__restrict__ stats, int gen, int maxSpin) {
   const int u = blockIdx.x, part = blockIdx.y, tid = threadIdx.x;
   LrGeo g;
   lr_geo(fp, u, g);
   lr_stage(fp, g, cdef, dbk, src, tid);
   LrUnitStat &st = stats[u];
+  if (part == 3) {
+    if (tid == 0) {
+      int ok = 0;
+      for (int it = 0; it < maxSpin && !ok; it++) {
+        ok = __hip_atomic_load(&st.fitGen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen;
+        if (!ok) __builtin_amdgcn_s_sleep(4);
+      }
+      gLr.itmp[0] = ok;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int ok = gLr.itmp[0];
+    int w[6], ref[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      w[k] = st.w0[k];
+      ref[k] = Wiener_Taps_Mid[k % 3];
+    }
+    long long sse = 0, cost = 0;
+    if (ok) cost = lr_refine(fp, g, w, ref, src, &sse, tid);
+    if (tid == 0) {
+      for (int k = 0; k < 6; k++) {
+        st.wStar[k] = w[k];
+        st.refAssumed[k] = ok ? ref[k] : INT_MIN;
+      }
+      st.sseStar = sse;
+      st.costStar = cost;
+    }
+    return;
+  }
   const int S = fp.stride[g.p], o = fp.offset[g.p], uw = g.x1 - g.x0, uh = g.y1 - g.y0;
   if (part == 0) {
     /* NONE and the Wiener fit */
@@ -681,6 +721,10 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
     int w0[6];
     lr_wiener_fit(fp, g, src, w0, tid);
     if (tid < 6) st.w0[tid] = w0[tid];
+    /* publish the taps to this unit's refinement workgroup */
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&st.fitGen, gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     return;
   }
   /* half of the self-guided sets */
@@ -1123,8 +1167,9 @@ size_t lr_stat_bytes() { return sizeof(LrUnitStat); }
 size_t lr_chain_bytes() { return 3 * sizeof(LrChain); }
 
 hipError_t launch_lr_fit(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *cdef,
-                         const uint8_t *dbk, void *stats, void *chain, int nunits) {
-  hipLaunchKernelGGL(k_lr_fit, dim3(nunits, 3), dim3(LR_T), 0, st, fp, src, cdef, dbk, (LrUnitStat *)stats);
+                         const uint8_t *dbk, void *stats, void *chain, int nunits, int gen) {
+  hipLaunchKernelGGL(k_lr_fit, dim3(nunits, 4), dim3(LR_T), 0, st, fp, src, cdef, dbk, (LrUnitStat *)stats, gen,
+                     1 << 20);
   hipLaunchKernelGGL(k_lr_chain_init, dim3(1), dim3(64), 0, st, (LrChain *)chain);
   return hipGetLastError();
 }
