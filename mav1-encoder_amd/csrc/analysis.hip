@@ -1,15 +1,16 @@
 /* Open-loop mode and partition analysis on gfx950 (behaviour of
  * src/kernels/k_analysis.h; per-SB driver src/backend_cpu.c:58-74).
  *
- * Pass 1: one wave64 per table item (superblock, node, slot), one launch per
- * level (a node's depth pruning reads its ancestors' costs). The item's
+ * Pass 1: one workgroup per table item (superblock, node, slot), one launch
+ * per level (a node's depth pruning reads its ancestors' costs): four waves
+ * for the 64x64 items, a wave64 for the smaller ones. The item's
  * source block (luma + both chroma planes) is staged in LDS; inter
  * predictions run through an LDS reference window (8-tap horizontal pass
  * into int16 LDS rows, vertical pass into LDS bytes); intra predictions are
  * produced per sample inside the SATD loop from LDS edge arrays. 4x4 units
- * are spread over the lanes and every SATD is a wave reduction, so each
- * cost is a wave-uniform value and the mode decisions (first minimum, ties
- * as the serial code) are taken identically by all lanes.
+ * are spread over the threads and every SATD is a workgroup reduction, so
+ * each cost is a uniform value and the mode decisions (first minimum, ties
+ * as the serial code) are taken identically by all threads.
  *
  * Pass 2: one wave per superblock runs the bottom-up partition choice over
  * the 85 nodes (lanes over the nodes of a level) and writes the blocks in
@@ -116,12 +117,15 @@ struct AnaLds {
 };
 
 template <int LEVEL>
-__global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+__global__ __launch_bounds__(LEVEL == 0 ? 256 : 64) void k_analyze_level(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
                                                       const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold,
                                                       const int32_t *__restrict__ mvf, const int32_t *__restrict__ mvg,
                                                       long long *__restrict__ costs, Mav1BlockInfo *__restrict__ infos,
                                                       int nslots) {
   constexpr int BW = 64 >> LEVEL, N = 1 << LEVEL;
+  /* threads per item: four waves for the 64x64 level (510 items at 1080p),
+   * one wave below */
+  constexpr int NT = LEVEL == 0 ? 256 : 64;
   __shared__ AnaLds<BW> L;
   const int lane = threadIdx.x;
   const int perSb = N * N * nslots;
@@ -167,11 +171,11 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
   const int lw = Mi_Width_Log2[bs] + 2, lh = Mi_Height_Log2[bs] + 2, w = 1 << lw, h = 1 << lh;
   const int x = miCol * 4, y = miRow * 4, lam = fp.ac_q[0] / 16;
   const int cx = x >> 1, cy = y >> 1, clw = lw - 1, clh = lh - 1, cw = w >> 1, ch = h >> 1;
-  for (int q = lane; q < w * h; q += 64) {
+  for (int q = lane; q < w * h; q += NT) {
     int yy = q >> lw, xx = q & (w - 1);
     L.srcY[yy * BW + xx] = src[fp.offset[0] + (y + yy) * fp.stride[0] + x + xx];
   }
-  for (int q = lane; q < 2 * cw * ch; q += 64) {
+  for (int q = lane; q < 2 * cw * ch; q += NT) {
     int pl = q / (cw * ch), e = q - pl * cw * ch, yy = e >> clw, xx = e & (cw - 1);
     L.srcC[pl][yy * (BW / 2) + xx] = src[fp.offset[1 + pl] + (cy + yy) * fp.stride[1] + cx + xx];
   }
@@ -194,13 +198,13 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
       const int st = sub ? BW / 2 : BW;
       const uint8_t *sblk = sub ? L.srcC[pl - 1] : L.srcY;
       const int usub = TB(fp, MAV1_T_ANA_USUB) && bw >= 32;
-      wave_predict_inter(ref + fp.offset[pl], S, fp.frame_w, fp.frame_h, bx, by, bw, bh, lr, lc, sub, EIGHTTAP, L.win,
+      wave_predict_inter<NT>(ref + fp.offset[pl], S, fp.frame_w, fp.frame_h, bx, by, bw, bh, lr, lc, sub, EIGHTTAP, L.win,
                          L.mid, L.pa, st, nullptr, 11, lane);
       if (want > 1)
-        wave_predict_inter(gold + fp.offset[pl], S, fp.frame_w, fp.frame_h, bx, by, bw, bh, gr, gc, sub, EIGHTTAP,
+        wave_predict_inter<NT>(gold + fp.offset[pl], S, fp.frame_w, fp.frame_h, bx, by, bw, bh, gr, gc, sub, EIGHTTAP,
                            L.win, L.mid, L.pb, st, nullptr, 11, lane);
       int a0, a1, a2;
-      wave_satd3(sblk, L.pa, L.pb, st, bw, bh, usub, want, lane, &a0, &a1, &a2);
+      wave_satd3<NT>(sblk, L.pa, L.pb, st, bw, bh, usub, want, lane, &a0, &a1, &a2);
       s0 += a0;
       s1 += a1;
       s2 += a2;
@@ -239,15 +243,15 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
   int *AY = L.edge[0][0] + EDGE_NEG, *LY = L.edge[0][1] + EDGE_NEG;
   const int maxXY = fp.mi_cols * 4 - 1, maxYY = fp.mi_rows * 4 - 1;
   if (!skipIntra) {
-    wave_intra_edges(src + fp.offset[0], fp.stride[0], x, y, w, h, x > 0, y > 0, 0, 0, maxXY, maxYY, AY, LY, lane);
-    const int dcY = wave_intra_dc(AY, LY, w, h, lw, lh, x > 0, y > 0, lane);
+    wave_intra_edges<NT>(src + fp.offset[0], fp.stride[0], x, y, w, h, x > 0, y > 0, 0, 0, maxXY, maxYY, AY, LY, lane);
+    const int dcY = wave_intra_dc<NT>(AY, LY, w, h, lw, lh, x > 0, y > 0, lane);
     for (int m = DC_PRED; m <= PAETH_PRED; m++) {
       if (tIntra && m != DC_PRED && m != V_PRED && m != H_PRED && m != SMOOTH_PRED) continue;
       IntraMode im;
       intra_mode_setup(im, m, 0, lw, lh);
       im.dcv = dcY;
       im.corner = AY[-1];
-      long long cst = wave_intra_satd(im, AY, LY, L.srcY, BW, lane) + (long long)lam * mode_bits8(m) / 8;
+      long long cst = wave_intra_satd<NT>(im, AY, LY, L.srcY, BW, lane) + (long long)lam * mode_bits8(m) / 8;
       if (best < 0 || cst < best) {
         best = cst;
         bestMode = m;
@@ -262,7 +266,7 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
       intra_mode_setup(im, bestMode, d, lw, lh);
       im.corner = AY[-1];
       im.dcv = 0;
-      long long cst = wave_intra_satd(im, AY, LY, L.srcY, BW, lane) + (long long)lam * (mode_bits8(bestMode) + 16) / 8;
+      long long cst = wave_intra_satd<NT>(im, AY, LY, L.srcY, BW, lane) + (long long)lam * (mode_bits8(bestMode) + 16) / 8;
       if (cst < best) {
         best = cst;
         bestDelta = d;
@@ -279,9 +283,9 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
   if (!skipUv) {
     for (int pl = 0; pl < 2; pl++) {
       int *A = L.edge[1 + pl][0] + EDGE_NEG, *Lc = L.edge[1 + pl][1] + EDGE_NEG;
-      wave_intra_edges(src + fp.offset[1 + pl], fp.stride[1], cx, cy, cw, ch, cx > 0, cy > 0, 0, 0, maxXC, maxYC, A, Lc,
+      wave_intra_edges<NT>(src + fp.offset[1 + pl], fp.stride[1], cx, cy, cw, ch, cx > 0, cy > 0, 0, 0, maxXC, maxYC, A, Lc,
                        lane);
-      dcC[pl] = wave_intra_dc(A, Lc, cw, ch, clw, clh, cx > 0, cy > 0, lane);
+      dcC[pl] = wave_intra_dc<NT>(A, Lc, cw, ch, clw, clh, cx > 0, cy > 0, lane);
     }
     for (int m = DC_PRED; m <= PAETH_PRED; m++) {
       if (m >= V_PRED && m <= D67_PRED && m != V_PRED && m != H_PRED && m != bestMode) continue;
@@ -293,7 +297,7 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
         intra_mode_setup(im, m, d, clw, clh);
         im.dcv = dcC[pl];
         im.corner = L.edge[1 + pl][0][EDGE_NEG - 1];
-        cst += wave_intra_satd(im, L.edge[1 + pl][0] + EDGE_NEG, L.edge[1 + pl][1] + EDGE_NEG, L.srcC[pl], BW / 2, lane);
+        cst += wave_intra_satd<NT>(im, L.edge[1 + pl][0] + EDGE_NEG, L.edge[1 + pl][1] + EDGE_NEG, L.srcC[pl], BW / 2, lane);
       }
       if (bestUv < 0 || cst < bestUv) {
         bestUv = cst;
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
     long long cst = (long long)lam * 40 / 8;
     const int nC = cw * ch;
     int lsum = 0;
-    for (int q = lane; q < nC; q += 64) {
+    for (int q = lane; q < nC; q += NT) {
       int yy = q >> clw, xx = q & (cw - 1);
       int ly = d_min(yy << 1, h - 2), lx = d_min(xx << 1, w - 2);
       const uint8_t *l = L.srcY + ly * BW + lx;
@@ -316,7 +320,7 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
       L.dcp[q] = v;
       lsum += v;
     }
-    lsum = wave_sum(lsum);
+    lsum = group_sum<NT>(lsum);
     const int avg = d_round2(lsum, clw + clh);
     __syncthreads();
     for (int pl = 0; pl < 2; pl++) {
@@ -326,19 +330,19 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
       int bestA = 0;
       for (int a = -16; a <= 16; a++) {
         int e = 0;
-        for (int q = lane; q < nC; q += 64) {
+        for (int q = lane; q < nC; q += NT) {
           int yy = q >> clw, xx = q & (cw - 1);
           int dd = d_clip1(dcv + d_round2signed(a * (L.dcp[q] - avg), 6)) - sC[yy * (BW / 2) + xx];
           e += dd * dd;
         }
-        long long sse = wave_sum(e);
+        long long sse = group_sum<NT>(e);
         if (bestS < 0 || sse < bestS) {
           bestS = sse;
           bestA = a;
         }
       }
       int acc = 0;
-      for (int u = lane; u < nC / 16; u += 64) {
+      for (int u = lane; u < nC / 16; u += NT) {
         int nx = cw >> 2, uy = u / nx, ux = u - uy * nx, dd[16];
 #pragma unroll
         for (int ii = 0; ii < 4; ii++)
@@ -350,7 +354,7 @@ __global__ __launch_bounds__(64) void k_analyze_level(const Mav1FrameParams fp, 
           }
         acc += satd4x4_arr(dd);
       }
-      cst += wave_sum(acc);
+      cst += group_sum<NT>(acc);
     }
     if (cst < bestUv) {
       bestUv = cst;
@@ -531,7 +535,7 @@ hipError_t launch_analyze(hipStream_t st, int level, const Mav1FrameParams &fp, 
   const int nslots = analysis_level_slots(fp, level);
   const int items = fp.sb_cols * fp.sb_rows * (1 << (2 * level)) * nslots;
   switch (level) {
-    case 0: hipLaunchKernelGGL(k_analyze_level<0>, dim3(items), dim3(64), 0, st, fp, src, ref, gold, mvf, mvg, costs, infos, nslots); break;
+    case 0: hipLaunchKernelGGL(k_analyze_level<0>, dim3(items), dim3(256), 0, st, fp, src, ref, gold, mvf, mvg, costs, infos, nslots); break;
     case 1: hipLaunchKernelGGL(k_analyze_level<1>, dim3(items), dim3(64), 0, st, fp, src, ref, gold, mvf, mvg, costs, infos, nslots); break;
     case 2: hipLaunchKernelGGL(k_analyze_level<2>, dim3(items), dim3(64), 0, st, fp, src, ref, gold, mvf, mvg, costs, infos, nslots); break;
     default: hipLaunchKernelGGL(k_analyze_level<3>, dim3(items), dim3(64), 0, st, fp, src, ref, gold, mvf, mvg, costs, infos, nslots); break;

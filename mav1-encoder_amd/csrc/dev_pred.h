@@ -3,16 +3,38 @@
  * LDS-staged reference window, intra edge preparation (7.11.2 steps 1-3,
  * the edge filter and upsampling of 7.11.2.9-12) and the per-sample intra
  * predictor, and 4x4 Hadamard SATD over the units of a block.
- * All functions are called by every lane of a wave64 workgroup. */
+ * All functions are called by every thread of a workgroup of NT threads
+ * (a wave64 by default); `lane` is the thread's index in the workgroup. */
 #ifndef MAV1_DEV_PRED_H
 #define MAV1_DEV_PRED_H
 
 #include "dev_common.h"
 
+/* Sum over the NT threads of the workgroup; every thread gets it. Beyond
+ * one wave, the waves' sums meet in LDS between two barriers (the first
+ * keeps a previous sum's readers ahead of the writes). */
+template <int NT>
+DEVI int group_sum(int v) {
+  v = wave_sum(v);
+  if constexpr (NT == 64) {
+    return v;
+  } else {
+    __shared__ int part[NT / 64];
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int t = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) t += part[i];
+    return t;
+  }
+}
+
 /* Inter prediction of a w x h block of one plane into dst (row stride ds).
  * win / mid are LDS scratch of (h + 7) x (w + 8) bytes and (h + 7) x w
  * int16; round1 11 writes clipped bytes to dst, round1 7 (compound) writes
  * the unclipped int32 values to dst32 instead. */
+template <int NT = 64>
 DEVI void wave_predict_inter(const uint8_t *__restrict__ plane, int S, int refW, int refH, int x, int y, int w, int h,
                              int mvr, int mvc, int sub, int filter, uint8_t *win, int16_t *mid, uint8_t *dst, int ds,
                              int32_t *dst32, int round1, int lane) {
@@ -31,12 +53,12 @@ DEVI void wave_predict_inter(const uint8_t *__restrict__ plane, int S, int refW,
     kv[t] = tv[t];
   }
   const int WW = w + 8, nr = h + 7;
-  for (int k = lane; k < nr * (w + 7); k += 64) {
+  for (int k = lane; k < nr * (w + 7); k += NT) {
     int r = k / (w + 7), c = k - r * (w + 7);
     win[r * WW + c] = plane[d_clip(0, lastY, iy - 3 + r) * S + d_clip(0, lastX, ix - 3 + c)];
   }
   __syncthreads();
-  for (int k = lane; k < nr * w; k += 64) {
+  for (int k = lane; k < nr * w; k += NT) {
     int r = k / w, c = k - r * w;
     const uint8_t *q = win + r * WW + c;
     int acc = 0;
@@ -45,7 +67,7 @@ DEVI void wave_predict_inter(const uint8_t *__restrict__ plane, int S, int refW,
     mid[k] = (int16_t)d_round2(acc, 3);
   }
   __syncthreads();
-  for (int k = lane; k < h * w; k += 64) {
+  for (int k = lane; k < h * w; k += NT) {
     int r = k / w, c = k - r * w;
     int acc = 0;
 #pragma unroll
@@ -60,10 +82,11 @@ DEVI void wave_predict_inter(const uint8_t *__restrict__ plane, int S, int refW,
 
 /* SATD of the 4x4 units of s against p (both row stride st); `sub` keeps the
  * units with even (row + column) parity and doubles the sum. */
+template <int NT = 64>
 DEVI int wave_satd(const uint8_t *s, const uint8_t *p, int st, int w, int h, int sub, int lane) {
   const int nx = w >> 2, nu = nx * (h >> 2);
   int acc = 0;
-  for (int u = lane; u < nu; u += 64) {
+  for (int u = lane; u < nu; u += NT) {
     int uy = u / nx, ux = u - uy * nx;
     if (sub && ((ux + uy) & 1)) continue;
     const uint8_t *a = s + uy * 4 * st + ux * 4, *b = p + uy * 4 * st + ux * 4;
@@ -74,15 +97,16 @@ DEVI int wave_satd(const uint8_t *s, const uint8_t *p, int st, int w, int h, int
       for (int j = 0; j < 4; j++) d[4 * i + j] = (int)a[i * st + j] - (int)b[i * st + j];
     acc += satd4x4_arr(d);
   }
-  return wave_sum(acc) * (sub ? 2 : 1);
+  return group_sum<NT>(acc) * (sub ? 2 : 1);
 }
 
 /* SATD of s against pa, pb and their rounded average, unit by unit. */
+template <int NT = 64>
 DEVI void wave_satd3(const uint8_t *s, const uint8_t *pa, const uint8_t *pb, int st, int w, int h, int sub, int want,
                      int lane, int *r0, int *r1, int *r2) {
   const int nx = w >> 2, nu = nx * (h >> 2);
   int a0 = 0, a1 = 0, a2 = 0;
-  for (int u = lane; u < nu; u += 64) {
+  for (int u = lane; u < nu; u += NT) {
     int uy = u / nx, ux = u - uy * nx;
     if (sub && ((ux + uy) & 1)) continue;
     int o = uy * 4 * st + ux * 4;
@@ -101,9 +125,9 @@ DEVI void wave_satd3(const uint8_t *s, const uint8_t *pa, const uint8_t *pb, int
     if (want > 2) a2 += satd4x4_arr(d2);
   }
   int m = sub ? 2 : 1;
-  *r0 = wave_sum(a0) * m;
-  *r1 = want > 1 ? wave_sum(a1) * m : 0;
-  *r2 = want > 2 ? wave_sum(a2) * m : 0;
+  *r0 = group_sum<NT>(a0) * m;
+  *r1 = want > 1 ? group_sum<NT>(a1) * m : 0;
+  *r2 = want > 2 ? group_sum<NT>(a2) * m : 0;
 }
 
 /* ---- intra ---- */
@@ -111,10 +135,11 @@ DEVI void wave_satd3(const uint8_t *s, const uint8_t *pa, const uint8_t *pb, int
 
 /* Neighbouring samples (7.11.2 steps 1-3) of a w x h block at plane (x, y):
  * A[0 .. w+h-1] above, L[0 .. w+h-1] left, A[-1] = L[-1] = the corner. */
+template <int NT = 64>
 DEVI void wave_intra_edges(const uint8_t *px, int S, int x, int y, int w, int h, int hl, int ha, int har, int hbl,
                            int maxX, int maxY, int *A, int *L, int lane) {
   const int aLim = d_min(maxX, x + (har ? 2 * w : w) - 1), lLim = d_min(maxY, y + (hbl ? 2 * h : h) - 1);
-  for (int i = lane; i < w + h; i += 64) {
+  for (int i = lane; i < w + h; i += NT) {
     A[i] = ha ? px[(y - 1) * S + d_min(aLim, x + i)] : hl ? px[y * S + x - 1] : 127;
     L[i] = hl ? px[d_min(lLim, y + i) * S + x - 1] : ha ? px[(y - 1) * S + x] : 129;
   }
@@ -153,11 +178,12 @@ DEVI void intra_mode_setup(IntraMode &m, int mode, int delta, int lw, int lh) {
 }
 
 /* DC value (7.11.2.5 DC_PRED) from the edges; every lane gets it. */
+template <int NT = 64>
 DEVI int wave_intra_dc(const int *A, const int *L, int w, int h, int lw, int lh, int hl, int ha, int lane) {
   int s = 0;
-  for (int i = lane; i < w; i += 64) s += ha ? A[i] : 0;
-  for (int i = lane; i < h; i += 64) s += hl ? L[i] : 0;
-  s = wave_sum(s);
+  for (int i = lane; i < w; i += NT) s += ha ? A[i] : 0;
+  for (int i = lane; i < h; i += NT) s += hl ? L[i] : 0;
+  s = group_sum<NT>(s);
   if (ha && hl) return (s + ((w + h) >> 1)) / (w + h);
   if (hl) return d_clip1((s + (h >> 1)) >> lh);
   if (ha) return d_clip1((s + (w >> 1)) >> lw);
@@ -212,10 +238,11 @@ DEVI int intra_px(const IntraMode &m, const int *A, const int *L, int i, int j) 
 }
 
 /* SATD of an intra mode's prediction against the source block s (stride st). */
+template <int NT = 64>
 DEVI int wave_intra_satd(const IntraMode &m, const int *A, const int *L, const uint8_t *s, int st, int lane) {
   const int w = 1 << m.lw, h = 1 << m.lh, nx = w >> 2, nu = nx * (h >> 2);
   int acc = 0;
-  for (int u = lane; u < nu; u += 64) {
+  for (int u = lane; u < nu; u += NT) {
     int uy = u / nx, ux = u - uy * nx;
     int d[16];
 #pragma unroll
@@ -227,7 +254,7 @@ DEVI int wave_intra_satd(const IntraMode &m, const int *A, const int *L, const u
       }
     acc += satd4x4_arr(d);
   }
-  return wave_sum(acc);
+  return group_sum<NT>(acc);
 }
 
 #endif
