@@ -3,7 +3,7 @@
  *
  * Pass 1: one workgroup per table item (superblock, node, slot), one launch
  * per level (a node's depth pruning reads its ancestors' costs): four waves
- * for the 64x64 items, a wave64 for the smaller ones. The item's
+ * for the 64x64 items, two for the 32x32 ones, a wave64 for the smaller ones. The item's
  * source block (luma + both chroma planes) is staged in LDS; inter
  * predictions run through an LDS reference window (8-tap horizontal pass
  * into int16 LDS rows, vertical pass into LDS bytes); intra predictions are
@@ -117,15 +117,15 @@ struct AnaLds {
 };
 
 template <int LEVEL>
-__global__ __launch_bounds__(LEVEL == 0 ? 256 : 64) void k_analyze_level(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+__global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_analyze_level(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
                                                       const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold,
                                                       const int32_t *__restrict__ mvf, const int32_t *__restrict__ mvg,
                                                       long long *__restrict__ costs, Mav1BlockInfo *__restrict__ infos,
                                                       int nslots) {
   constexpr int BW = 64 >> LEVEL, N = 1 << LEVEL;
   /* threads per item: four waves for the 64x64 level (510 items at 1080p),
-   * one wave below */
-  constexpr int NT = LEVEL == 0 ? 256 : 64;
+   * two for the 32x32 level, one below */
+  constexpr int NT = LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64;
   __shared__ AnaLds<BW> L;
   const int lane = threadIdx.x;
   const int perSb = N * N * nslots;
@@ -536,7 +536,7 @@ hipError_t launch_analyze(hipStream_t st, int level, const Mav1FrameParams &fp, 
   const int items = fp.sb_cols * fp.sb_rows * (1 << (2 * level)) * nslots;
   switch (level) {
     case 0: hipLaunchKernelGGL(k_analyze_level<0>, dim3(items), dim3(256), 0, st, fp, src, ref, gold, mvf, mvg, costs, infos, nslots); break;
-    case 1: hipLaunchKernelGGL(k_analyze_level<1>, dim3(items), dim3(64), 0, st, fp, src, ref, gold, mvf, mvg, costs, infos, nslots); break;
+    case 1: hipLaunchKernelGGL(k_analyze_level<1>, dim3(items), dim3(128), 0, st, fp, src, ref, gold, mvf, mvg, costs, infos, nslots); break;
     case 2: hipLaunchKernelGGL(k_analyze_level<2>, dim3(items), dim3(64), 0, st, fp, src, ref, gold, mvf, mvg, costs, infos, nslots); break;
     default: hipLaunchKernelGGL(k_analyze_level<3>, dim3(items), dim3(64), 0, st, fp, src, ref, gold, mvf, mvg, costs, infos, nslots); break;
   }
