@@ -102,7 +102,7 @@ BH = [4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 64, 32, 64, 128, 64, 128, 16, 4, 32, 8,
 # (40 steps after 6 warm-up ones, ALTREF / leaf alternating), from the
 # FETCH_SIZE and WRITE_SIZE passes of profiles/recon_intra_1080p_turbo_pmc.md,
 # scaled by the frame copy's factors there (fetch x1.985, write x1.0).
-RECON_INTRA_TRAFFIC = 6697611
+RECON_INTRA_TRAFFIC = 6617313
 
 
 def intra_phase_bytes(fp, blocks, sbs):
