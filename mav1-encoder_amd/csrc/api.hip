@@ -30,10 +30,12 @@ struct mav1hip_ctx {
   size_t frame_bytes;
   int device;
   hipStream_t stream;
-  /* Analysis runs on `side` when motion search of the same frame was just
-   * enqueued: analysis level k waits for me_done[k] only, so it overlaps
-   * motion-search level k + 1; `stream` waits for ana_done before it goes
-   * on. me_pending: me_done holds this frame's levels. */
+  /* Analysis runs on `side` when motion search of the same frame was the
+   * last work enqueued: analysis level k waits for me_done[k] only, so it
+   * overlaps motion-search level k + 1; `stream` waits for ana_done before
+   * it goes on. me_pending: me_done holds this frame's levels and nothing
+   * was enqueued on `stream` since (every other launch, copy or parameter
+   * change clears it). */
   hipStream_t side;
   hipEvent_t me_done[4], ana_done;
   int me_pending;
@@ -117,6 +119,7 @@ static hipEvent_t take_event(mav1hip_ctx *c) {
 template <class F>
 static int timed_on(mav1hip_ctx *c, hipStream_t st, const char *name, F &&launch) {
   hipEvent_t a = nullptr, b = nullptr;
+  if (st == c->stream) c->me_pending = 0; /* analysis may overlap only what directly precedes it: motion search */
   const bool on = c->timing && (c->timing_only.empty() || c->timing_only == name);
   if (on) {
     a = take_event(c);
@@ -399,6 +402,7 @@ size_t mav1hip_planes_entry_bytes(const mav1hip_ctx *c) { return c ? (size_t)c->
 
 int mav1hip_upload(mav1hip_ctx *c, int32_t buf, const void *host, size_t bytes) {
   if (!c || !host) return MAV1HIP_EINVAL;
+  c->me_pending = 0;
   size_t cap;
   void *d = buf_ptr(c, buf, &cap);
   if (!d || bytes > cap) return fail(c, MAV1HIP_EINVAL, "upload: buffer %d holds %zu bytes, %zu given", buf, cap, bytes);
@@ -599,6 +603,7 @@ int mav1hip_launch_cdef(mav1hip_ctx *c, int32_t nsets) {
 
 int mav1hip_launch_save_deblocked(mav1hip_ctx *c) {
   if (!c) return MAV1HIP_EINVAL;
+  c->me_pending = 0;
   HIPCHK(c, hipMemcpyAsync(c->d_dbk, c->d_rec, c->frame_bytes, hipMemcpyDeviceToDevice, c->stream));
   return MAV1HIP_OK;
 }
@@ -869,6 +874,7 @@ int mav1hip_pool_alloc(mav1hip_ctx *c, int32_t n) {
 
 int mav1hip_pool_upload(mav1hip_ctx *c, int32_t idx, const uint8_t *frame) {
   if (!c || !frame || idx < 0 || idx >= c->pool_n) return fail(c, MAV1HIP_EINVAL, "pool_upload: bad index %d", idx);
+  c->me_pending = 0;
   HIPCHK(c, hipMemcpyAsync(c->d_pool + (size_t)idx * c->frame_bytes, frame, c->frame_bytes, hipMemcpyHostToDevice,
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -886,6 +892,7 @@ int mav1hip_bind_frames(mav1hip_ctx *c, int32_t s, int32_t r, int32_t g) {
 
 int mav1hip_pool_store(mav1hip_ctx *c, int32_t idx, int32_t buf) {
   if (!c || idx < 0 || idx >= c->pool_n) return fail(c, MAV1HIP_EINVAL, "pool_store: bad index %d", idx);
+  c->me_pending = 0;
   if (buf != MAV1HIP_BUF_REC && buf != MAV1HIP_BUF_LROUT) return fail(c, MAV1HIP_EINVAL, "pool_store: buffer %d", buf);
   uint8_t *dst = c->d_pool + (size_t)idx * c->frame_bytes;
   if (dst == c->b_src || dst == c->b_ref || dst == c->b_gold)
@@ -905,6 +912,7 @@ int mav1hip_pool_import(mav1hip_ctx *dst, int32_t dst_idx, mav1hip_ctx *src, int
   uint8_t *to = dst->d_pool + (size_t)dst_idx * dst->frame_bytes;
   if (to == dst->b_src || to == dst->b_ref || to == dst->b_gold)
     return fail(dst, MAV1HIP_EINVAL, "pool_import: frame %d is bound to a stage input", dst_idx);
+  dst->me_pending = src->me_pending = 0;
   /* after everything enqueued on src so far (the picture is complete), and
    * before src's later work (which may overwrite the source frame) */
   HIPCHK(dst, hipEventRecord(src->xfer, src->stream));
