@@ -83,3 +83,34 @@ def test_analysis_other_presets(gpu, variant):
     else:
         cfg, fp = _params(w, h, "arf", knob_turbo_mask=0x47CBF1 & ~0x400000 & ~0x1)
     run_case(cfg, fp, frames)
+
+
+def test_device_analysis_sees_upload_after_motion_search(gpu):
+    """On the device path analysis overlaps the frame's motion search (a
+    second stream) only when it directly follows it. A source upload enqueued
+    in between must be visible to the analysis: it equals the oracle's on
+    the new source with the motion field the GPU searched on the old one."""
+    w, h = 352, 288
+    cfg, fp = _params(w, h, "arf")
+    geo, frames = libs.make_sequence(w, h, 3, 17, n_objects=8, max_vel=10)
+    nsb = fp.sb_cols * fp.sb_rows
+    ctx = mav1hip.Context(cfg)
+    try:
+        ctx.set_params(fp)
+        ctx.upload(mav1hip.BUF_SRC, frames[1])
+        ctx.upload(mav1hip.BUF_REF, frames[0])
+        ctx.launch_motion_search()
+        ctx.upload(mav1hip.BUF_SRC, frames[2])
+        ctx.launch_analyze()
+        ctx.sync()
+        mvf = ctx.download(mav1hip.BUF_MVFIELD, np.int32, nsb * 170)
+        blocks = ctx.download(mav1hip.BUF_BLOCKS, np.int32, nsb * 64 * 27).reshape(-1, 27)
+        sbs = ctx.download(mav1hip.BUF_SBS, np.int32, nsb * 89).reshape(-1, 89)
+    finally:
+        ctx.close()
+    o = libs.Impl("oracle")
+    _, _, ob, os_ = o.analyze(fp, frames[2], frames[0], frames[2], mvf, np.zeros_like(mvf))
+    assert np.array_equal(sbs, os_), "superblock records differ"
+    for s in range(nsb):
+        n = os_[s, 1]
+        assert np.array_equal(blocks[s * 64:s * 64 + n], ob[s * 64:s * 64 + n]), f"blocks of SB {s} differ"
