@@ -443,6 +443,8 @@ void oracle_analyze(const Mav1FrameParams *fp, const uint8_t *src, const uint8_t
                     const int32_t *mvf, const int32_t *mvg, int64_t *costs, Mav1BlockInfo *infos,
                     Mav1BlockInfo *blocks, Mav1SbInfo *sbs) {
   int nsb = fp->sb_cols * fp->sb_rows;
+  /* a superblock's items and table read only its own entries */
+#pragma omp parallel for schedule(dynamic, 2)
   for (int sb = 0; sb < nsb; sb++) {
     for (int level = 0; level < 4; level++) {
       int ns = level_slots(fp, level);

@@ -110,6 +110,9 @@ static void lf_pass(const Mav1FrameParams *fp, uint8_t *rec, const uint8_t *info
   if (!lvl) return;
   const int sub = plane > 0, S = fp->stride[plane], step = 1 << sub;
   uint8_t *base = rec + fp->offset[plane];
+  /* edges of one pass touch disjoint samples (an edge filters at most half
+   * the transform size on either side), so mi rows are independent */
+#pragma omp parallel for schedule(static)
   for (int r0 = 0; r0 < fp->mi_rows; r0 += step)
     for (int c0 = 0; c0 < fp->mi_cols; c0 += step) {
       const int x = c0 * 4, y = r0 * 4;
@@ -222,6 +225,7 @@ static void cdef_plane(const Mav1FrameParams *fp, const uint8_t *cur, uint8_t *o
  * src/backend_cpu.c:171-176): skip blocks and all-zero sets are copies. */
 void oracle_cdef(const Mav1FrameParams *fp, const uint8_t *cur, uint8_t *out, int64_t frameBytes, const uint8_t *info,
                  int n) {
+#pragma omp parallel for schedule(static)
   for (int r = 0; r < fp->mi_rows; r += 2)
     for (int c = 0; c < fp->mi_cols; c += 2) {
       const int skip = (info[r * fp->mi_cols + c] >> 5) & 1;
@@ -252,6 +256,7 @@ void oracle_cdef(const Mav1FrameParams *fp, const uint8_t *cur, uint8_t *out, in
  * minimum), copied into rec over the mi-aligned area. */
 void oracle_cdef_select(const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdefbuf, int64_t frameBytes,
                         int n, uint8_t *rec, int8_t *choice) {
+#pragma omp parallel for schedule(static)
   for (int sr = 0; sr < fp->sb_rows; sr++)
     for (int sc = 0; sc < fp->sb_cols; sc++) {
       int64_t best = -1;

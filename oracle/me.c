@@ -17,6 +17,7 @@ void oracle_qpel_build(const Mav1FrameParams *fp, const uint8_t *ref, uint8_t *p
   const uint8_t *luma = ref + fp->offset[0];
   int W = fp->frame_w, H = fp->frame_h, S = fp->stride[0];
   size_t n = (size_t)W * H;
+#pragma omp parallel for schedule(static)
   for (int y = 0; y < H; y++)
     for (int x = 0; x < W; x++) {
       int hz[4][8];
@@ -278,6 +279,8 @@ void oracle_motion_search(const Mav1FrameParams *fp, const uint8_t *src, const u
   for (int level = 0; level < 4; level++)
     for (int r = 0; r < nref; r++) {
       Refs R = {fp, src, r ? gold : ref, buf + r * eb, srcDec, buf + r * eb + n16};
+      /* superblocks are independent within a level */
+#pragma omp parallel for schedule(dynamic, 4)
       for (int sb = 0; sb < nsb; sb++)
         for (int i = 0; i < (1 << (2 * level)); i++) search_node(&R, sb, level, i, r, r ? mvgold : mvfield);
     }

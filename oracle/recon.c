@@ -99,7 +99,7 @@ static int code_tx_block(const Mav1FrameParams *fp, const uint8_t *src, uint8_t 
                          int64_t *outCost) {
   int w = Tx_Width[txSz], h = Tx_Height[txSz], tw = o_min(32, w), th = o_min(32, h);
   int dcQ = fp->dc_q[plane], acQ = fp->ac_q[plane];
-  static int32_t resid[64 * 64], coef[64 * 64], dq[64 * 64], res[64 * 64], qtry[1024], qbest[1024];
+  static _Thread_local int32_t resid[64 * 64], coef[64 * 64], dq[64 * 64], res[64 * 64], qtry[1024], qbest[1024];
   if (fp->lossless) {
     int32_t q16[16];
     for (int i = 0; i < 4; i++)
@@ -311,7 +311,7 @@ static void recon_block(const Mav1FrameParams *fp, Mav1BlockInfo *b, const uint8
     int baseX = (b->mi_col >> sub) * 4, baseY = (b->mi_row >> sub) * 4;
     int w = BLOCK_WIDTH(bs) >> sub, h = BLOCK_HEIGHT(bs) >> sub;
     if (comp) {
-      static int32_t p0[64 * 64], p1[64 * 64];
+      static _Thread_local int32_t p0[64 * 64], p1[64 * 64];
       o_predict_inter_raw(r0 + fp->offset[plane], S, fp->frame_w, fp->frame_h, p0, w, baseX, baseY, w, h, b->mv_row,
                           b->mv_col, sub, sub, b->interp_filter);
       o_predict_inter_raw(gold + fp->offset[plane], S, fp->frame_w, fp->frame_h, p1, w, baseX, baseY, w, h, b->mv1_row,
@@ -338,7 +338,7 @@ static void recon_block(const Mav1FrameParams *fp, Mav1BlockInfo *b, const uint8
     }
     /* tx_depth RD: code luma at each depth from the same state, keep the best */
     uint8_t *px = rec + fp->offset[0] + baseY * S + baseX;
-    static uint8_t save[64 * 64];
+    static _Thread_local uint8_t save[64 * 64];
     uint8_t bdSave[BDD * BDD];
     for (int i = 0; i < h; i++) memcpy(save + i * w, px + i * S, (size_t)w);
     memcpy(bdSave, bd, sizeof bdSave);
@@ -383,8 +383,16 @@ static void recon_sb(const Mav1FrameParams *fp, const Mav1SbInfo *sb, Mav1BlockI
 
 void oracle_recon(const Mav1FrameParams *fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs, const uint8_t *src,
                   uint8_t *rec, const uint8_t *ref, const uint8_t *gold, int32_t *coefs, const int32_t *mats) {
-  int nsb = fp->sb_cols * fp->sb_rows;
-  for (int s = 0; s < nsb; s++) recon_sb(fp, &sbs[s], blocks, src, rec, ref, gold, coefs, mats);
+  /* Raster order, run as a wavefront: a superblock reads the reconstruction
+   * of its left, above-left, above and above-right neighbours only, and each
+   * of those has a smaller col + 2 * row, so the superblocks of one such
+   * diagonal are independent once the earlier diagonals are done. */
+  const int nd = fp->sb_cols + 2 * (fp->sb_rows - 1), nsb = fp->sb_cols * fp->sb_rows;
+  for (int d = 0; d < nd; d++) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int s = 0; s < nsb; s++)
+      if (sbs[s].sb_col + 2 * sbs[s].sb_row == d) recon_sb(fp, &sbs[s], blocks, src, rec, ref, gold, coefs, mats);
+  }
 }
 
 /* Intra edge filter types (get_filter_type, spec 7.11.2.8) of every block

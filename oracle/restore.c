@@ -349,6 +349,14 @@ static void sgr_fit(const Unit *U, int set, int xqd[2]) {
   xqd[1] = o_clip(Sgrproj_Xqd_Min[1], Sgrproj_Xqd_Max[1], (1 << SGRPROJ_PRJ_BITS) - xqd[0] - (r1 ? w2 : 0));
 }
 
+/* The reference-independent results of one unit: the SSE of NONE, the
+ * fitted Wiener filter and its SSE, each self-guided set's fitted weights
+ * and SSE. */
+typedef struct {
+  int64_t none_sse, w_sse, sg_sse[1 << SGRPROJ_PARAMS_BITS];
+  Mav1LrUnit w, sg[1 << SGRPROJ_PARAMS_BITS];
+} Fit;
+
 static int64_t rd_cost(const Mav1FrameParams *fp, int64_t sse, int bits) { return sse * 16 + (int64_t)fp->lambda * bits; }
 
 /* lr_select (lr_search.c:235-359): per plane, units in raster order with
@@ -380,6 +388,33 @@ void oracle_lr_select(Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cd
         rc[2 * k] = r;
         rc[2 * k + 1] = c;
       }
+    /* The fits and the SSEs of the fitted candidates do not depend on the
+     * running subexp references, so they are computed for every unit first
+     * (in parallel: units cover disjoint samples, also of tmp); the
+     * reference-dependent part (bit costs, the greedy tap refinement) then
+     * runs in raster order exactly as the serial search does. */
+    Fit *fit = malloc(sizeof(Fit) * (size_t)(nu > 0 ? nu : 1));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int u = 0; u < nu; u++) {
+      const Unit U = {fp, src, cdef, dbk, tmp, p, start[u + 1] - start[u], rc + 2 * start[u]};
+      Fit *F = &fit[u];
+      Mav1LrUnit t;
+      memset(&t, 0, sizeof(t));
+      t.type = RESTORE_NONE;
+      F->none_sse = unit_sse(&U, &t);
+      memset(&F->w, 0, sizeof(F->w));
+      F->w.type = RESTORE_WIENER;
+      wiener_fit(&U, F->w.wiener);
+      F->w_sse = unit_sse(&U, &F->w);
+      for (int set = 0; set < (1 << SGRPROJ_PARAMS_BITS); set++) {
+        Mav1LrUnit *sg = &F->sg[set];
+        memset(sg, 0, sizeof(*sg));
+        sg->type = RESTORE_SGRPROJ;
+        sg->set = set;
+        sgr_fit(&U, set, sg->xqd);
+        F->sg_sse[set] = unit_sse(&U, sg);
+      }
+    }
     int refW[2][3], refX[2], used = 0;
     for (int i = 0; i < 2; i++) {
       refX[i] = Sgrproj_Xqd_Mid[i];
@@ -387,17 +422,15 @@ void oracle_lr_select(Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cd
     }
     for (int u = 0; u < nu; u++) {
       const Unit U = {fp, src, cdef, dbk, tmp, p, start[u + 1] - start[u], rc + 2 * start[u]};
+      const Fit *F = &fit[u];
       int tw[2][3], tx[2];
-      Mav1LrUnit best, w;
+      Mav1LrUnit best, w = F->w;
       memset(&best, 0, sizeof(best));
       best.type = RESTORE_NONE;
-      int64_t bestCost = rd_cost(fp, unit_sse(&U, &best), 1);
-      memset(&w, 0, sizeof(w));
-      w.type = RESTORE_WIENER;
-      wiener_fit(&U, w.wiener);
+      int64_t bestCost = rd_cost(fp, F->none_sse, 1);
       memcpy(tw, refW, sizeof(tw));
       memcpy(tx, refX, sizeof(tx));
-      int64_t wc = rd_cost(fp, unit_sse(&U, &w), 2 + unit_bits(p, &w, tw, tx));
+      int64_t wc = rd_cost(fp, F->w_sse, 2 + unit_bits(p, &w, tw, tx));
       for (int pass = 0; pass < 2; pass++)
         for (int j = p ? 1 : 0; j < 3; j++)
           for (int d = -1; d <= 1; d += 2) {
@@ -417,14 +450,10 @@ void oracle_lr_select(Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cd
         best = w;
       }
       for (int set = 0; set < (1 << SGRPROJ_PARAMS_BITS); set++) {
-        Mav1LrUnit sg;
-        memset(&sg, 0, sizeof(sg));
-        sg.type = RESTORE_SGRPROJ;
-        sg.set = set;
-        sgr_fit(&U, set, sg.xqd);
+        const Mav1LrUnit sg = F->sg[set];
         memcpy(tw, refW, sizeof(tw));
         memcpy(tx, refX, sizeof(tx));
-        const int64_t sc = rd_cost(fp, unit_sse(&U, &sg), 2 + unit_bits(p, &sg, tw, tx));
+        const int64_t sc = rd_cost(fp, F->sg_sse[set], 2 + unit_bits(p, &sg, tw, tx));
         if (sc < bestCost) {
           bestCost = sc;
           best = sg;
@@ -435,15 +464,10 @@ void oracle_lr_select(Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cd
         else {
           int64_t bs = -1;
           for (int set = 0; set < (1 << SGRPROJ_PARAMS_BITS); set++) {
-            Mav1LrUnit sg;
-            memset(&sg, 0, sizeof(sg));
-            sg.type = RESTORE_SGRPROJ;
-            sg.set = set;
-            sgr_fit(&U, set, sg.xqd);
-            const int64_t v = unit_sse(&U, &sg);
+            const int64_t v = F->sg_sse[set];
             if (bs < 0 || v < bs) {
               bs = v;
-              best = sg;
+              best = F->sg[set];
             }
           }
         }
@@ -453,6 +477,7 @@ void oracle_lr_select(Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cd
       used |= best.type != RESTORE_NONE;
     }
     fp->lr_type[p] = used ? RESTORE_SWITCHABLE : RESTORE_NONE;
+    free(fit);
     free(start);
     free(fill);
   }
