@@ -3,14 +3,17 @@
  * the dependencies of src/backend_cpu.c:81-86) and the intra edge filter
  * types of the frame driver (src/encoder.c:319-346).
  *
- * Inter blocks never read reconstructed neighbours: the inter phase codes
- * every inter block of the frame at once, one 512-thread workgroup per
- * block. Intra blocks read the reconstruction of earlier blocks: the intra
- * phase runs one launch per anti-diagonal t = column-in-tile + 2 * row of
- * the superblock grid (left, above and above-right superblocks are on
- * earlier diagonals, tile columns are independent), one workgroup per
- * superblock walking its blocks in decode order with the BlockDecoded flags
- * in LDS.
+ * Inter blocks never read reconstructed neighbours: k_recon_list gathers
+ * every inter block of the frame into a work list, and the persistent
+ * 512-thread workgroups of k_recon_inter take blocks off it with an atomic
+ * counter. Intra blocks read the reconstruction of earlier blocks: the
+ * intra phase is one dataflow launch. k_intra_sched lists the superblocks
+ * holding intra blocks in an order where every neighbour a superblock reads
+ * (left, above-left, above, above-right within its tile column) comes
+ * first; the persistent workgroups of k_recon_intra take list entries and
+ * wait (bounded, with an error word) for the done flags of those
+ * neighbours, then walk the superblock's blocks in decode order with the
+ * BlockDecoded flags and the reconstruction tiles in LDS.
  *
  * Inside a workgroup the candidates of every search run side by side: the
  * transform types of a block of at most 16x16 one per wave (each wave codes
