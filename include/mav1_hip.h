@@ -204,6 +204,24 @@ int mav1hip_restore(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *cd
 int mav1hip_lr_select(mav1hip_ctx *c, const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdef,
                       const uint8_t *dbk, int32_t lr_test, Mav1LrUnit *units, int32_t lr_type[3], int32_t reruns[3]);
 
+/* No counterpart in the reference: mav1hip_lr_select for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). It needs no context and
+ * calls no allocator: src, cdef and dbk (padded frames), units (one record
+ * per unit, fp->lr_unit_off indexing) and lr_type (int32[3]) are DEVICE
+ * pointers the caller owns; the search's scratch is a static device array
+ * sized for MAV1HIP_STATIC_LR_UNITS restoration units (about 1.2 MB per
+ * device, held until the process ends), taken and cleared on the first call
+ * on each device. The kernels are enqueued on `stream` (a hipStream_t; NULL
+ * is the null stream) and the call returns without waiting. More units
+ * than the scratch holds return MAV1HIP_EINVAL. Every call on one device
+ * shares that device's scratch: calls that overlap on one device race.
+ * err (may be NULL) receives a message on failure. */
+#define MAV1HIP_STATIC_LR_UNITS 3072
+int mav1hip_lr_select_static(const Mav1FrameParams *fp, const uint8_t *src, const uint8_t *cdef, const uint8_t *dbk,
+                             int32_t lr_test, Mav1LrUnit *units, int32_t *lr_type, void *stream, char *err,
+                             size_t errlen);
+
 /* The per-frame decision record of the driver's dump_decisions
  * (src/encoder.c:659-704, MAV1_DUMP_DEC): 11 int32 (magic 'MDEC',
  * order_hint, last_hint, arf_hint, frame_type 0 key / 1 inter, frame_kind,

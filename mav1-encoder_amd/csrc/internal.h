@@ -64,6 +64,10 @@ hipError_t launch_decision_grid(hipStream_t st, const Mav1FrameParams &fp, const
                                 const Mav1SbInfo *sbs, uint8_t *grid);
 
 /* lrsearch.hip */
+/* Upper bounds of lr_stat_bytes() and lr_chain_bytes() (the static scratch
+ * of mav1hip_lr_select_static is laid out with them). */
+#define LR_STAT_MAX_BYTES 384
+#define LR_CHAIN_MAX_BYTES 192
 size_t lr_stat_bytes();
 size_t lr_chain_bytes();
 /* The fit of every unit with its refinement under the default reference,

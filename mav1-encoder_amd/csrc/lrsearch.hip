@@ -1163,6 +1163,8 @@ __global__ void k_lr_chain_init(LrChain *__restrict__ chain) {
   chain[p] = c;
 }
 
+static_assert(sizeof(LrUnitStat) <= LR_STAT_MAX_BYTES, "LR_STAT_MAX_BYTES too small");
+static_assert(3 * sizeof(LrChain) <= LR_CHAIN_MAX_BYTES, "LR_CHAIN_MAX_BYTES too small");
 size_t lr_stat_bytes() { return sizeof(LrUnitStat); }
 size_t lr_chain_bytes() { return 3 * sizeof(LrChain); }
 

@@ -19,6 +19,7 @@ TURBO_DEFAULT = 0x47CBF1
  BUF_SBS, BUF_REC, BUF_COEFS, BUF_LFMASK, BUF_CDEFBUF, BUF_DBK, BUF_LROUT, BUF_LRUNITS, BUF_CDEF_CHOICE,
  BUF_DECGRID) = range(19)
 MAX_CDEF_SETS = 8
+STATIC_LR_UNITS = 3072  # MAV1HIP_STATIC_LR_UNITS
 
 
 class FrameParams(C.Structure):
@@ -136,6 +137,8 @@ def _load():
         "mav1hip_decision_record": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, P]),
         "mav1hip_launch_decision_grid": (C.c_int, [P]),
         "mav1hip_lr_select": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, i32, P, P, P]),
+        "mav1hip_lr_select_static": (C.c_int, [C.POINTER(FrameParams), P, P, P, i32, P, P, P, C.c_char_p,
+                                               C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -156,6 +159,18 @@ def lib():
 
 def ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def lr_select_static(fp, d_src, d_cdef, d_dbk, lr_test, d_units, d_lr_type, stream=None):
+    """mav1hip_lr_select_static: the restoration search on device buffers
+    (integer device addresses) with the library's static scratch, enqueued
+    on `stream` (an integer hipStream_t, None: the null stream)."""
+    err = C.create_string_buffer(512)
+    rc = lib().mav1hip_lr_select_static(C.byref(fp), C.c_void_p(d_src), C.c_void_p(d_cdef), C.c_void_p(d_dbk),
+                                        lr_test, C.c_void_p(d_units), C.c_void_p(d_lr_type),
+                                        C.c_void_p(stream) if stream else None, err, len(err))
+    if rc != 0:
+        raise RuntimeError(f"lr_select_static failed ({rc}): {err.value.decode(errors='replace')}")
 
 
 def default_config(width, height, turbo=1, **kw):
