@@ -183,6 +183,18 @@ struct RLds {
   SiWave si[3];             /* the small-block path: luma wave, U wave, V wave */
   Mav1BlockInfo slot[2];    /* its blocks in flight */
   int mq[64][3], nmq;       /* inter blocks whose chroma flags follow the pending chroma: bsize, mi_row, mi_col */
+  /* the level schedule of a superblock whose intra blocks all take the
+   * small-block path: its block records, per block the dependency level
+   * (0: inter), the BlockDecoded bits its prediction reads (above-right,
+   * below-left per plane) and the schedule (intra blocks by level) */
+  Mav1BlockInfo sbb[64];
+  int plv[64], pav[64], pord[64], pn, pok;
+  SiWave psw[12];           /* luma predictions of a step (4), chroma of the step before (8) */
+  int32_t pq[4][256];       /* per luma block of a step: the chosen levels */
+  long long pcost[4];       /* running best cost, eob, type and candidate wave */
+  int peob[4], ptype[4], pwin[4];
+  long long rcost[RW];      /* this round's candidates: cost and eob per wave */
+  int reob[RW];
 };
 
 /* The working set of the workgroup that reconstructs a block / superblock.
@@ -1061,7 +1073,7 @@ DEVI SiGeo si_geo(const Mav1BlockInfo &b, int plane) {
 /* Prediction of one plane of small block b by one wave (code_plane's
  * intra_tx_predict call); for CfL also the alphas' SSEs (cfl_tx), and for U
  * the chosen alpha applied. */
-DEVI void si_predict(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane, SiWave &W) {
+DEVI void si_predict(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane, SiWave &W, int har = -1, int hbl = -1) {
   const TxCtx c = wave_ctx();
   const int lane = c.tid;
   const SiGeo g = si_geo(b, plane);
@@ -1071,9 +1083,9 @@ DEVI void si_predict(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane, SiW
   const int isCfl = plane > 0 && b.uv_mode == UV_CFL_PRED;
   intra_tx_predict(c, g.px, g.S, g.sx, g.sy, g.lw, g.lh, plane == 0 ? b.y_mode : (isCfl ? DC_PRED : b.uv_mode),
                    plane == 0 ? b.angle_y : b.angle_uv, availL, availU,
-                   gL.bd[BDI(plane, (sbr >> sub) - 1, (sbc >> sub) + stepX)],
-                   gL.bd[BDI(plane, (sbr >> sub) + stepY, (sbc >> sub) - 1)], ((fp.mi_cols * 4) >> sub) - 1,
-                   ((fp.mi_rows * 4) >> sub) - 1, fp.enable_intra_edge_filter,
+                   har >= 0 ? har : gL.bd[BDI(plane, (sbr >> sub) - 1, (sbc >> sub) + stepX)],
+                   hbl >= 0 ? hbl : gL.bd[BDI(plane, (sbr >> sub) + stepY, (sbc >> sub) - 1)],
+                   ((fp.mi_cols * 4) >> sub) - 1, ((fp.mi_rows * 4) >> sub) - 1, fp.enable_intra_edge_filter,
                    (b.interp_filter >> (plane == 0 ? 4 : 5)) & 1, W.eA, W.eL, W.ecp);
   if (!isCfl) return;
   /* CfL (7.11.5) over the DC prediction: the luma average of each sample,
@@ -1143,7 +1155,7 @@ DEVI int si_cfl_apply(const Mav1BlockInfo &b, int plane, SiWave &W, int excludeZ
 /* The chroma plane of small block b by one wave: its single transform type
  * (code_plane), levels, reconstruction, flags. */
 DEVI void si_code_chroma(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane, int32_t *__restrict__ coefs,
-                         const int32_t *__restrict__ mats, SmallTx &T) {
+                         const int32_t *__restrict__ mats, SmallTx &T, bool mark = true) {
   const TxCtx c = wave_ctx();
   const SiGeo g = si_geo(b, plane);
   const int set = get_tx_set(g.txSz, 0, fp.reduced_tx_set);
@@ -1163,7 +1175,7 @@ DEVI void si_code_chroma(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane,
   }
   if (c.tid == 0) {
     b.eob[plane] = eob > 0;
-    si_mark(fp, b, plane);
+    if (mark) si_mark(fp, b, plane);
   }
 }
 
@@ -1400,6 +1412,266 @@ DEVI bool wait_flag(const int *flag, int maxSpin) {
   return false;
 }
 
+/* ---- the level schedule of a superblock ----
+ * When every intra block of a superblock takes the small-block path (8x8 ..
+ * 16x16, one transform block per plane, no tx_depth search) and no block is
+ * smaller than 8x8, its intra blocks are coded by dependency level instead
+ * of strictly one after another. A block's level is one more than the
+ * highest level of the earlier (decode order) intra blocks of the
+ * superblock that its prediction may read: those covering the column left
+ * of it down to twice its height, the row above it right to twice its
+ * width, or the corner; blocks of one level are therefore independent. The
+ * BlockDecoded flags its prediction reads (above-right, below-left, per
+ * plane) are those of decode order: the superblock's initial flags, or set
+ * where an earlier block of the superblock covers the cell. Covering whole
+ * blocks equals marking their transform blocks inside the frame, and a cell
+ * outside the frame never changes a prediction (the edge fetch clamps to
+ * the frame). Each step predicts the luma of up to four blocks of one
+ * level, one wave each; then, behind a barrier, the chroma of the previous
+ * step's blocks (U and V a wave each; CfL reads the luma finished one step
+ * earlier); then the luma transforms and reconstructions beside the chroma
+ * coding. Results equal the decode-order pipeline's. */
+/* luma blocks coded side by side in one step */
+#define LVL_MAXW 4
+DEVI void lvl_schedule(const Mav1FrameParams &fp, int n) {
+  RLds &L = gL;
+  const int tid = threadIdx.x;
+  /* the BlockDecoded bits each intra block's prediction reads */
+  for (int q = tid; q < n * 6; q += RT) {
+    const int i = q / 6, p = (q >> 1) % 3, f = q & 1, sub = p > 0;
+    const Mav1BlockInfo &b = L.sbb[i];
+    if (b.is_inter) continue;
+    const int txSz = p ? uv_tx_size(b.bsize) : Max_Tx_Size_Rect[b.bsize];
+    const int stepX = Tx_Width[txSz] >> 2, stepY = Tx_Height[txSz] >> 2;
+    const int by = (b.mi_row & 15) >> sub, bx = (b.mi_col & 15) >> sub;
+    const int y = f ? by + stepY : by - 1, x = f ? bx - 1 : bx + stepX;
+    int v = L.bd[BDI(p, y, x)];
+    for (int j = 0; j < i && !v && y >= 0 && x >= 0; j++) {
+      const Mav1BlockInfo &o = L.sbb[j];
+      const int psz = SUBSAMPLED_SIZE(o.bsize, sub, sub);
+      const int r0 = (o.mi_row & 15) >> sub, c0 = (o.mi_col & 15) >> sub;
+      v = y >= r0 && y < r0 + Num_4x4_Blocks_High[psz] && x >= c0 && x < c0 + Num_4x4_Blocks_Wide[psz];
+    }
+    if (v) atomicOr(&L.pav[i], 1 << (2 * p + f));
+  }
+  /* levels, in decode order on wave 0 (lane j: block j as a dependency) */
+  if (tid < 64) {
+    const int j = tid;
+    int myLv = 0, rj = 0, cj = 0, hj = 0, wj = 0, intraJ = 0;
+    if (j < n) {
+      const Mav1BlockInfo &o = L.sbb[j];
+      rj = o.mi_row & 15;
+      cj = o.mi_col & 15;
+      hj = Num_4x4_Blocks_High[o.bsize];
+      wj = Num_4x4_Blocks_Wide[o.bsize];
+      intraJ = !o.is_inter;
+    }
+    for (int i = 0; i < n; i++) {
+      const Mav1BlockInfo &b = L.sbb[i];
+      if (b.is_inter) continue;
+      const int r = b.mi_row & 15, c = b.mi_col & 15, h = Num_4x4_Blocks_High[b.bsize], w = Num_4x4_Blocks_Wide[b.bsize];
+      int m = 0;
+      if (j < i && intraJ) {
+        const bool left = c - 1 >= cj && c - 1 < cj + wj && r - 1 < rj + hj && r + 2 * h > rj;
+        const bool above = r - 1 >= rj && r - 1 < rj + hj && c - 1 < cj + wj && c + 2 * w > cj;
+        if (left || above) m = myLv;
+      }
+#pragma unroll
+      for (int sh = 32; sh >= 1; sh >>= 1) m = d_max(m, __shfl_xor(m, sh));
+      if (j == i) myLv = m + 1;
+    }
+    if (j < n) L.plv[j] = myLv;
+  }
+  __syncthreads();
+  /* the schedule: intra blocks by level, then decode order */
+  if (tid < n && L.plv[tid] > 0) {
+    const int lv = L.plv[tid];
+    int pos = 0;
+    for (int k = 0; k < n; k++) {
+      const int l = L.plv[k];
+      pos += l > 0 && (l < lv || (l == lv && k < tid));
+    }
+    L.pord[pos] = tid;
+  }
+  if (tid == 64) {
+    int cnt = 0;
+    for (int k = 0; k < n; k++) cnt += L.plv[k] > 0;
+    L.pn = cnt;
+  }
+  __syncthreads();
+}
+
+/* Number of luma transform-type candidates of a small intra block, and the
+ * one at index ci (code_plane's candidate list). */
+DEVI int lvl_ncands(const Mav1FrameParams &fp, int txSz) {
+  const int set = get_tx_set(txSz, 0, fp.reduced_tx_set);
+  if (!fp.tx_search || set == TX_SET_DCTONLY) return 1;
+  int n = 0;
+  for (int t = 0; t < 16; t++) n += in_set(set, t, 0) ? 1 : 0;
+  return n;
+}
+DEVI int lvl_cand(const Mav1FrameParams &fp, int txSz, int ci) {
+  const int set = get_tx_set(txSz, 0, fp.reduced_tx_set);
+  if (!fp.tx_search || set == TX_SET_DCTONLY) return DCT_DCT;
+  for (int t = 0; t < 16; t++)
+    if (in_set(set, t, 0) && ci-- == 0) return t;
+  return DCT_DCT;
+}
+
+/* The chosen luma type of small block b on one wave: levels out,
+ * reconstruction into the tile, the block's transform fields. */
+DEVI void lvl_luma_out(const Mav1FrameParams &fp, Mav1BlockInfo &b, const int32_t *q, int type, int eob,
+                       int32_t *__restrict__ coefs, SmallTx &W) {
+  const TxCtx c = wave_ctx();
+  const SiGeo g = si_geo(b, 0);
+  const int tw = d_min(32, 1 << g.lw), th = d_min(32, 1 << g.lh);
+  int32_t *co = coefs + b.coef_off[0];
+  for (int k = c.tid; k < tw * th; k += 64) co[k] = q[k];
+  if (eob > 0) {
+    tx_dequant(c, q, W.dq, SQS, g.txSz, fp.dc_q[0], fp.ac_q[0]);
+    tx_inverse<true>(c, W.dq, SQS, W.resid, SRS, g.txSz, type);
+    tx_add(c, g.px, g.S, g.sx, g.sy, g.lw, g.lh, type, W.resid, SRS);
+  }
+  if (c.tid == 0) {
+    b.tx_types_lo = (int32_t)(((uint32_t)b.tx_types_lo & ~15u) | (uint32_t)type);
+    b.tx_type = type;
+    b.eob[0] = eob > 0;
+  }
+}
+
+/* A chroma plane of the previous step's block k (task index 2k + plane - 1)
+ * on one wave: V's CfL alpha (U's is known), then its coding. */
+DEVI void lvl_chroma(const Mav1FrameParams &fp, int prev, int k, int32_t *__restrict__ coefs,
+                     const int32_t *__restrict__ mats) {
+  RLds &L = gL;
+  const int plane = 1 + (k & 1), wave = threadIdx.x >> 6;
+  Mav1BlockInfo &b = L.sbb[L.pord[prev + (k >> 1)]];
+  if (plane == 2 && b.uv_mode == UV_CFL_PRED) {
+    const int a = si_cfl_apply(b, 2, L.psw[4 + k], b.cfl_alpha_u == 0);
+    if ((threadIdx.x & 63) == 0) b.cfl_alpha_v = a;
+  }
+  si_code_chroma(fp, b, plane, coefs, mats, L.u.sm[wave], false);
+}
+
+/* One step of the level schedule: luma of blocks pord[cur .. cur + ncur)
+ * (one level), chroma of blocks pord[prev .. prev + nprev). */
+DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const int32_t *__restrict__ mats, int cur,
+                   int ncur, int prev, int nprev) {
+  RLds &L = gL;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ntask = ncur + 2 * nprev;
+  /* P1: the luma predictions of this step's blocks, one wave each */
+  for (int t = wave; t < ncur; t += RW) {
+    const int i = L.pord[cur + t];
+    Mav1BlockInfo &b = L.sbb[i];
+    if (lane == 0) {
+      b.tx_depth = 0;
+      b.interp_filter &= 0x30;
+    }
+    const int av = L.pav[i];
+    si_predict(fp, b, 0, L.psw[t], av & 1, (av >> 1) & 1);
+  }
+  __syncthreads();
+  /* P2: the chroma predictions of the previous step's blocks (U's CfL alpha
+   * at once), on the waves after the luma ones */
+  for (int t = wave + ncur; t < ntask; t += RW) {
+    const int k = t - ncur, plane = 1 + (k & 1), i = L.pord[prev + (k >> 1)];
+    Mav1BlockInfo &b = L.sbb[i];
+    const int av = L.pav[i];
+    si_predict(fp, b, plane, L.psw[4 + k], (av >> (2 * plane)) & 1, (av >> (2 * plane + 1)) & 1);
+    if (plane == 1 && b.uv_mode == UV_CFL_PRED) {
+      const int a = si_cfl_apply(b, 1, L.psw[4 + k], 0);
+      if (lane == 0) b.cfl_alpha_u = a;
+    }
+  }
+  __syncthreads();
+  /* luma candidates: offsets per block */
+  int off[5], total = 0;
+  bool single = true;
+  for (int k = 0; k < 4; k++) {
+    off[k] = total;
+    if (k < ncur) {
+      const int nc = lvl_ncands(fp, Max_Tx_Size_Rect[L.sbb[L.pord[cur + k]].bsize]);
+      total += nc;
+      single = single && nc == 1;
+    }
+  }
+  off[4] = total;
+  if (single) {
+    /* P3: one type per block: evaluation and reconstruction on the block's
+     * wave, beside the chroma coding */
+    const TxCtx c = wave_ctx();
+    for (int t = wave; t < ntask; t += RW) {
+      if (t < ncur) {
+        Mav1BlockInfo &b = L.sbb[L.pord[cur + t]];
+        const SiGeo g = si_geo(b, 0);
+        SmallTx &W = L.u.sm[wave];
+        long long cost;
+        const int eob = tx_eval<true>(c, fp, g.spx, g.px, g.S, g.sx, g.sy, g.txSz, 0, DCT_DCT, true, W.resid, SRS,
+                                      W.tmp, W.coef, W.q, W.dq, SQS, mats, &cost);
+        lvl_luma_out(fp, b, W.q, DCT_DCT, eob, coefs, W);
+      } else {
+        lvl_chroma(fp, prev, t - ncur, coefs, mats);
+      }
+    }
+    __syncthreads();
+    return;
+  }
+  /* several types: rounds of one candidate per wave, the best kept per
+   * block (strictly smaller cost wins, candidates in list order) */
+  if (tid < ncur) L.pcost[tid] = -1;
+  for (int r0 = 0; r0 < total; r0 += RW) {
+    const int t = r0 + wave;
+    if (t < total) {
+      int k = 0;
+      while (t >= off[k + 1]) k++;
+      Mav1BlockInfo &b = L.sbb[L.pord[cur + k]];
+      const SiGeo g = si_geo(b, 0);
+      SmallTx &W = L.u.sm[wave];
+      long long cost;
+      const int eob = tx_eval<true>(wave_ctx(), fp, g.spx, g.px, g.S, g.sx, g.sy, g.txSz, 0,
+                                    lvl_cand(fp, g.txSz, t - off[k]), false, W.resid, SRS, W.tmp, W.coef, W.q, W.dq,
+                                    SQS, mats, &cost);
+      if (lane == 0) {
+        L.rcost[wave] = cost;
+        L.reob[wave] = eob;
+      }
+    }
+    __syncthreads();
+    if (tid < ncur) {
+      const int k = tid;
+      int win = -1;
+      for (int u = d_max(off[k], r0); u < d_min(off[k + 1], r0 + RW); u++)
+        if (L.pcost[k] < 0 || L.rcost[u - r0] < L.pcost[k]) {
+          L.pcost[k] = L.rcost[u - r0];
+          L.peob[k] = L.reob[u - r0];
+          const int ty = lvl_cand(fp, Max_Tx_Size_Rect[L.sbb[L.pord[cur + k]].bsize], u - off[k]);
+          L.ptype[k] = L.reob[u - r0] > 0 ? ty : DCT_DCT;
+          win = u - r0;
+        }
+      L.pwin[k] = win;
+    }
+    __syncthreads();
+    for (int k = 0; k < ncur; k++) {
+      const int win = L.pwin[k];
+      if (win < 0) continue;
+      const Mav1BlockInfo &b = L.sbb[L.pord[cur + k]];
+      const int txSz = Max_Tx_Size_Rect[b.bsize];
+      const int n = d_min(32, Tx_Width[txSz]) * d_min(32, Tx_Height[txSz]);
+      for (int e = tid; e < n; e += RT) L.pq[k][e] = L.u.sm[win].q[e];
+    }
+    __syncthreads();
+  }
+  /* P3: the chosen luma types' reconstruction beside the chroma coding */
+  for (int t = wave; t < ntask; t += RW) {
+    if (t < ncur)
+      lvl_luma_out(fp, L.sbb[L.pord[cur + t]], L.pq[t], L.ptype[t], L.peob[t], coefs, L.u.sm[wave]);
+    else
+      lvl_chroma(fp, prev, t - ncur, coefs, mats);
+  }
+  __syncthreads();
+}
+
 /* Codes the intra blocks of superblock (r, c) (recon_sb, k_recon.h:396-440):
  * BlockDecoded flags, LDS tiles, blocks in decode order, tiles back. */
 DEVI void intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo &S,
@@ -1434,7 +1706,39 @@ DEVI void intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks
       if (ty > 0 && tx >= 4) ts[q] = *(const uint32_t *)(src + o);
     }
   }
+  /* the block records; the level schedule applies when every intra block
+   * takes the small-block path and no block is smaller than 8x8 */
+  const int nb = S.num_blocks;
+  for (int q = tid; q < nb * 27; q += RT) ((int32_t *)L.sbb)[q] = ((const int32_t *)(blocks + S.first_block))[q];
+  for (int q = tid; q < 64; q += RT) L.pav[q] = 0;
+  if (tid == 0) L.pok = 1;
   __syncthreads();
+  if (tid < nb) {
+    const Mav1BlockInfo &b = L.sbb[tid];
+    const int bs = b.bsize;
+    if (bs < BLOCK_8X8 || bs == BLOCK_4X16 || bs == BLOCK_16X4 || (!b.is_inter && !si_eligible(fp, b))) L.pok = 0;
+  }
+  __syncthreads();
+  if (L.pok) {
+    lvl_schedule(fp, nb);
+    int prev = 0, nprev = 0, cur = 0;
+    const int pn = L.pn;
+    for (;;) {
+      int ncur = 0;
+      if (cur < pn) {
+        const int lv = L.plv[L.pord[cur]];
+        while (ncur < LVL_MAXW && cur + ncur < pn && L.plv[L.pord[cur + ncur]] == lv) ncur++;
+      }
+      if (!ncur && !nprev) break;
+      lvl_step(fp, coefs, mats, cur, ncur, prev, nprev);
+      prev = cur;
+      nprev = ncur;
+      cur += ncur;
+    }
+    for (int q = tid; q < nb * 27; q += RT)
+      if (!L.sbb[q / 27].is_inter) ((int32_t *)(blocks + S.first_block))[q] = ((const int32_t *)L.sbb)[q];
+    __syncthreads();
+  } else {
   /* blocks in decode order: small intra blocks through the two-stage
    * pipeline (chroma one step behind luma), inter blocks' flags through
    * the same streams, any other intra block by the whole workgroup once the
@@ -1486,6 +1790,7 @@ DEVI void intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks
       if (tid < 27) ((int32_t *)(blocks + S.first_block + i))[tid] = ((const int32_t *)&L.blk)[tid];
     }
     __syncthreads();
+  }
   }
   /* the superblock back into the frame */
   for (int p = 0; p < 3; p++) {
