@@ -1,21 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the HIP analysis backend on one MI355X.
 
-Workload: the 1080p configuration (1920x1080 8-bit 4:2:0, synthetic content
-with 12 objects moving up to +-48 px/frame, luma noise sigma 3), encoded with
---turbo -q 125: frame 0 is a key frame, then mini-GOPs of 2 in coding order
-(hidden ALTREF frame predicting from LAST, then a non-reference leaf frame
-predicting from LAST and ALTREF). A step is one frame of that sequence
-through the backend stages built on the GPU so far (listed in
-config["stages"]), with every source frame already resident in HBM (a device
-frame pool filled before the timed region). Reference pictures are the
-reconstructed, filtered pictures of the referenced frames, kept in two
-device slots (closed loop, starting from a coded key frame).
+Workload: the 1080p stream (1920x1080 8-bit 4:2:0, synthetic content with 12
+objects moving up to +-48 px/frame, luma noise sigma 3), encoded with
+--turbo -q 125 --keyint 50 and loop restoration on, through the sequence
+driver (mav1hip_seq_*): frame 0 is a key frame, then ALTREF mini-GOPs of 2
+(hidden ALTREF, then a non-reference leaf frame), and a regular frame closes
+a stream of even length. A step is one frame; the default 50 steps are the
+whole 50-frame stream, key frame included. Source frames are resident in
+HBM when the timed region starts; every frame's outputs for the CPU side
+(block and superblock records, levels, CDEF choices, restoration units, the
+final picture) are copied to pinned host memory inside the timed region.
+Reference pictures are the driver's own reconstructions (closed loop).
 
-Prints one JSON line (rank 0) with the throughput in frames/s, the roofline
-of the dominant kernel (per-launch durations from HIP events recorded on the
-context's stream during the timed region) and the reference CPU baseline
-(the reference's own kernels, single-threaded, on a bounded sample).
+Prints one JSON line with the throughput in frames/s, the same stream with
+the sources sent from host memory (PCIe included), the roofline of the
+dominant kernel (per-launch durations from HIP events recorded on the
+engines' streams during the timed region) and the CPU baseline (the
+oracle's serial driver and stages on the host cores, on a bounded sample).
 """
 import argparse
 import ctypes as C
@@ -98,11 +100,10 @@ def algorithmic_bytes(kernel, fp, nref):
 BW = [4, 4, 8, 8, 8, 16, 16, 16, 32, 32, 32, 64, 64, 64, 128, 128, 4, 16, 8, 32, 16, 64]
 BH = [4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 64, 32, 64, 128, 64, 128, 16, 4, 32, 8, 64, 16]
 
-# HBM traffic per recon_intra launch over the timed frames of the default run
-# (40 steps after 6 warm-up ones, ALTREF / leaf alternating), from the
-# FETCH_SIZE and WRITE_SIZE passes of profiles/recon_intra_1080p_turbo_pmc.md,
-# scaled by the frame copy's factors there (fetch x1.985, write x1.0).
-RECON_INTRA_TRAFFIC = 6617313
+# HBM traffic per launch of the dominant kernel over the timed frames of the
+# default run, from the FETCH_SIZE and WRITE_SIZE passes of the committed
+# rocprofv3 summary (profiles/), scaled by the frame copy's factors there.
+TRAFFIC = {}
 
 
 def intra_phase_bytes(fp, blocks, sbs):
@@ -131,218 +132,129 @@ def intra_phase_bytes(fp, blocks, sbs):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=6)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-frames", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pcie", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
+    import torch
 
     # deblocking + CDEF + loop restoration on (turbo turns restoration off unless asked for)
-    cfg = mav1hip.default_config(W, H, turbo=1, qindex=Q, restoration=1)
+    cfg = mav1hip.default_config(W, H, turbo=1, qindex=Q, restoration=1, keyint=50)
     geo = synth.Geometry(W, H, cfg.tile_cols_log2)
-    order = coding_order(args.warmup + args.steps)
-    nframes = max(max(f, l, a) for _, f, l, a in order) + 1
     gen = synth.config_synth(2, seed=2024)
-    uniq = min(nframes, 50)
-    frames = [synth.pad_frame(geo, gen.frame(t)) for t in range(uniq)]
-
-    # Two frame engines, as the reference's driver runs a stream's mini-GOP
-    # (encoder.c:1150-1224: engine encoders with a backend instance each, on
-    # their own queues): engine A codes the key frame and the ALTREF chain,
-    # engine B the leaf frames. Each engine is a context with its own stream
-    # and buffers; an ALTREF picture passes from A's pool to B's with
-    # mav1hip_pool_import, ordered on both streams.
-    engA, engB = mav1hip.Context(cfg), mav1hip.Context(cfg)
-    # pool: the source frames, then two reference slots
-    for eng in (engA, engB):
-        eng.pool_alloc(uniq + 2)
-        for i, f in enumerate(frames):
-            eng.pool_upload(i, f)
-    params, plans = {}, {}
-    for kind in ("key", "arf", "leaf"):
-        fr = mav1hip.turbo_frame(kind, 0 if kind == "key" else 1)
-        fp, npre = mav1hip.setup_params(cfg, fr)
-        params[kind], plans[kind] = fp, mav1hip.filter_plan(cfg, fr, fp, npre)
-    slot_of = {}    # frame -> engine A's slot holding its final picture
-    b_slot_of = {}  # frame -> engine B's slot holding its imported copy
-
-    def early_stages(ctx, kind):
-        if kind != "key":
-            ctx.launch_motion_search()
-        ctx.launch_analyze()
-        ctx.launch_edge_types()
-
-    def late_stages(ctx, kind):
-        plan = plans[kind]
-        ctx.launch_recon()
-        if plan.filter_info:
-            ctx.launch_filter_info()
-            if plan.deblock:
-                ctx.launch_deblock()
-            ctx.launch_cdef(plan.cdef_sets)
-            if plan.restoration:
-                ctx.launch_save_deblocked()
-            if plan.cdef_select:
-                ctx.launch_cdef_select(plan.cdef_sets)
-        if plan.restoration:
-            ctx.launch_lr_select(0)
-            ctx.launch_restore()
-        return mav1hip.BUF_LROUT if plan.restoration else mav1hip.BUF_REC
-
-    def run_stages(ctx, kind):
-        early_stages(ctx, kind)
-        return late_stages(ctx, kind)
-
-    last_run = [None, None]  # engine and final buffer of the last coded frame
-    # A leaf frame waits here for the next ALTREF: it is enqueued after that
-    # ALTREF's motion search, analysis and edge types, so its imports (which
-    # order it after everything engine A has enqueued) hold it until then and
-    # it runs beside the ALTREF's reconstruction, whose intra phase leaves
-    # most CUs idle. Only the enqueue order changes, not the results.
-    pending = []
-
-    def step(kind, f, last, alt, overlap=True):
-        if kind == "arf":  # engine A: LAST is the previous ALTREF (or the key frame)
-            ref = slot_of[last]
-            engA.bind_frames(f % uniq, ref, -1)
-            engA.set_params(params[kind])
-            early_stages(engA, kind)
-            if pending:
-                run_leaf(*pending.pop())
-            final = late_stages(engA, kind)
-            target = uniq if ref != uniq else uniq + 1  # the slot LAST does not hold
-            engA.pool_store(target, final)
-            slot_of[f] = target
-            last_run[:] = [engA, final]
-            return
-        if overlap:
-            pending.append((f, last, alt))
-        else:
-            run_leaf(f, last, alt)
-
-    def flush():
-        if pending:
-            run_leaf(*pending.pop())
-
-    def run_leaf(f, last, alt):
-        # engine B: LAST and ALTREF come from engine A, each imported once
-        # into the slot the other one does not hold
-        kind = "leaf"
-        engB.bind_frames(f % uniq, -1, -1)
-        for need, other in ((last, alt), (alt, last)):
-            if need not in b_slot_of:
-                keep = b_slot_of.get(other)
-                slot = uniq if keep != uniq else uniq + 1
-                for g in [g for g, sl in b_slot_of.items() if sl == slot]:
-                    del b_slot_of[g]
-                engB.pool_import(slot, engA, slot_of[need])
-                b_slot_of[need] = slot
-        engB.bind_frames(f % uniq, b_slot_of[last], b_slot_of[alt])
-        engB.set_params(params[kind])
-        last_run[:] = [engB, run_stages(engB, kind)]
-
-    def code_key():
-        """Frame 0: the key frame (untimed) on engine A, its reconstruction in A's first slot."""
-        slot_of.clear()
-        b_slot_of.clear()
-        engA.bind_frames(0, -1, -1)
-        engA.set_params(params["key"])
-        engA.pool_store(uniq, run_stages(engA, "key"))
-        slot_of[0] = uniq
-
-    def sync():
-        engA.sync()
-        engB.sync()
+    n = max(args.steps, args.warmup, args.cpu_frames)
+    planes = [gen.frame(t) for t in range(n)]
+    padded = [synth.pad_frame(geo, p) for p in planes]
+    dev = [torch.from_numpy(f).to("cuda:0") for f in padded]
+    torch.cuda.synchronize()
 
     kernels = ["qpel_build", "dec_build", "me_level0", "me_level1", "me_level2", "me_level3", "analyze_level0",
                "analyze_level1", "analyze_level2", "analyze_level3", "analyze_table", "edge_types", "recon_inter",
-               "intra_sched", "recon_intra", "filter_info", "deblock", "cdef", "cdef_select", "lr_fit", "lr_path", "lr_chain", "lr_apply"]
+               "intra_sched", "recon_intra", "filter_info", "deblock", "cdef", "cdef_select", "lr_fit", "lr_path",
+               "lr_chain", "lr_apply"]
 
-    def kernel_stats():
-        out = {}
+    devname = [""]
+
+    def run(nframes, device=True, timed=None, one_at_a_time=False, on_frame=None):
+        """The stream's first nframes frames through a new driver; returns
+        (elapsed s, receive times, per-kernel (launches, ms) over both
+        engines, the last frame's final picture)."""
+        seq = mav1hip.Sequence(cfg)
+        engines = [seq.engine(0), seq.engine(1)]
+        devname[0] = engines[0].device_name
+        for e in engines:
+            e.timing_select(timed if timed != "all" else None)
+            e.timing(timed is not None)
+            e.timing_reset()
+        stamps, last = [], [None]
+
+        def take(r):
+            stamps.append(time.perf_counter())
+            last[0] = r
+            if on_frame is not None:
+                on_frame(r)
+
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(nframes):
+            while not (seq.send_device(dev[i].data_ptr()) if device else seq.send(planes[i])):
+                take(seq.receive_raw(True))
+            while True:
+                r = seq.receive_raw(one_at_a_time)
+                if r is None:
+                    break
+                take(r)
+        while not seq.send(None):
+            take(seq.receive_raw(True))
+        while True:
+            r = seq.receive_raw(True)
+            if r is None:
+                break
+            take(r)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        fb = last[0].fp.offset[2] + last[0].fp.stride[2] * last[0].fp.plane_h[2]
+        final = np.ctypeslib.as_array(C.cast(last[0].recon, C.POINTER(C.c_uint8)), (fb,)).copy()
+        stats = {}
         for k in kernels:
-            na, ta = engA.timing_get(k)
-            nb, tb = engB.timing_get(k)
-            out[k] = (na + nb, ta + tb)
-        return out
+            tot = [e.timing_get(k) for e in engines]
+            stats[k] = (sum(t[0] for t in tot), sum(t[1] for t in tot))
+        seq.close()
+        return elapsed, [s - t0 for s in stamps], stats, final
 
-    def timing(on, only=None):
-        for eng in (engA, engB):
-            eng.timing_select(only)
-            eng.timing(on)
-            eng.timing_reset()
-
-    code_key()
-
-    # warm-up, every kernel timed: it names the dominant kernel (largest total)
-    timing(True)
-    for s in order[:args.warmup]:
-        step(*s)
-    flush()
-    sync()
-    warm = kernel_stats()
-    dom = max(warm, key=lambda k: warm[k][1]) if any(v[0] for v in warm.values()) else None
-    # the timed region: events around the dominant kernel's launches only
-    # (each timed launch adds two event records to its stream); without a
-    # warm-up, around every launch, and the dominant kernel named after
-    timing(True, dom)
-    t0 = time.perf_counter()
-    for s in order[args.warmup:]:
-        step(*s)
-    flush()
-    sync()
-    elapsed = time.perf_counter() - t0
-    timed = kernel_stats()
-    if dom is None:
-        dom = max(timed, key=lambda k: timed[k][1])
-    n_launch, tot_ms = timed[dom]
-    timing(False)
-    last_out = last_run[0].download(last_run[1])
-    # the intra phase's algorithmic bytes of every timed frame: the whole
-    # sequence coded again, untimed, from the key frame on (the coding is
-    # deterministic), the block records read back after each timed frame;
-    # every kernel timed, one frame at a time, for the per-kernel table
-    code_key()
-    timing(True)
-    intra_bytes = []
-    for i, s in enumerate(order):
-        step(*s, overlap=False)
-        if i < args.warmup:
-            continue
-        eng = last_run[0]
-        eng.sync()
-        p = params[s[0]]
-        nsb = p.sb_cols * p.sb_rows
-        blk = eng.download(mav1hip.BUF_BLOCKS, np.int32, nsb * 64 * 27).reshape(-1, 27)
-        sbi = eng.download(mav1hip.BUF_SBS, np.int32, nsb * 89).reshape(-1, 89)
-        intra_bytes.append(intra_phase_bytes(p, blk, sbi))
-    sync()
-    if not np.array_equal(last_run[0].download(last_run[1]), last_out):
-        raise RuntimeError("the second coding of the timed frames differs from the first")
-    stats = kernel_stats()  # the second coding, frame by frame
-    timing(False)
+    # warm-up stream, every kernel timed: names the dominant kernel (largest total)
+    _, _, warm, _ = run(args.warmup, timed="all")
+    dom = max(warm, key=lambda k: warm[k][1])
+    # the timed stream: events around the dominant kernel's launches only
+    elapsed, stamps, timed_stats, final = run(args.steps, timed=dom)
+    n_launch, tot_ms = timed_stats[dom]
     fps = args.steps / elapsed
+    steady = (args.steps - 1) / (stamps[-1] - stamps[0]) if len(stamps) > 1 else None
+    # the same stream again, one frame at a time with every kernel timed:
+    # the per-kernel table, the block records of every frame (the intra
+    # phase's algorithmic bytes) and a check that the coding is repeatable
+    intra_bytes = []
+
+    def on_frame(r):
+        fp = r.fp
+        nsb = fp.sb_cols * fp.sb_rows
+        blk = np.ctypeslib.as_array(C.cast(r.blocks, C.POINTER(C.c_int32)), (nsb * 64 * 27,)).reshape(-1, 27)
+        sbi = np.ctypeslib.as_array(C.cast(r.sbs, C.POINTER(C.c_int32)), (nsb * 89,)).reshape(-1, 89)
+        intra_bytes.append(intra_phase_bytes(fp, blk, sbi))
+
+    _, _, stats, final2 = run(args.steps, timed="all", one_at_a_time=True, on_frame=on_frame)
+    if not np.array_equal(final, final2):
+        raise RuntimeError("the second coding of the stream differs from the first")
+    pcie = None
+    if not args.no_pcie:
+        e2, _, _, final3 = run(args.steps, device=False)
+        if not np.array_equal(final, final3):
+            raise RuntimeError("the stream sent from host memory codes differently")
+        pcie = round(args.steps / e2, 3)
 
     # the dominant kernel's mean launch duration over the timed region
     avg_s = tot_ms / 1e3 / max(n_launch, 1)
-    # average algorithmic bytes per launch over the frames of the timed region
-    nref_avg = np.mean([2 if k == "leaf" else 1 for k, *_ in order[args.warmup:]])
-    if dom in ("qpel_build", "dec_build"):
-        per_launch = algorithmic_bytes(dom, params["arf"], 1)
-    elif dom == "recon_intra":
-        per_launch = np.mean(intra_bytes)  # one launch per timed frame
+    params = {}
+    for kind in ("key", "arf", "leaf"):
+        fr = mav1hip.turbo_frame(kind, 0 if kind == "key" else 1)
+        params[kind], _ = mav1hip.setup_params(cfg, fr)
+    if dom == "recon_intra":
+        per_launch = float(np.mean(intra_bytes))  # one launch per frame
     else:
-        per_launch = np.mean([algorithmic_bytes(dom, params[k], 2 if k == "leaf" else 1)
-                              for k, *_ in order[args.warmup:]])
+        kinds = ["key"] + [k for k, *_ in coding_order(args.steps - 1)]
+        per_launch = float(np.mean([algorithmic_bytes(dom, params[k], 2 if k == "leaf" else 1) for k in kinds]))
     achieved = per_launch / avg_s / 1e9
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.cpu_seconds, frames, params, uniq)
+        cpu = cpu_baseline(cfg, padded, args.cpu_frames, args.cpu_runs)
 
     line = {
         "metric": "1080p encode fps, 1 MI355X: plain HIP vs batched+pruned at ≤1% BD-rate Δ",
@@ -358,20 +270,25 @@ def main():
         "dtype": "u8",
         "data": "synthetic (seeded generator: 6 sinusoids + 12 moving rectangles + noise)",
         "config": {
-            "workload": "1920x1080 8-bit 4:2:0, --turbo -q 125, turbo GOP (ARF + leaf), plain HIP path",
+            "workload": f"1920x1080 8-bit 4:2:0, --turbo -q 125 --keyint 50, loop restoration on: a {args.steps}-frame "
+                        "stream through the sequence driver, key frame included (key frame, ALTREF / leaf mini-GOPs, "
+                        "a regular frame closing an even-length stream), plain HIP path",
             "stages": ["motion_search", "analyze", "edge_types", "recon", "filter_info", "deblock", "cdef",
                        "cdef_select", "lr_select", "lr_apply"],
-            "frame_kinds": "alternating hidden ALTREF (1 ref) / leaf (2 refs); leaf frames are not filtered",
-            "references": "the reconstructed, filtered pictures of the referenced frames (closed loop from a coded key frame)",
-            "filters": "deblocking + CDEF + loop restoration on hidden ALTREF frames (leaf frames skip them)",
-            "engines": "2 (engine A: key frame + ALTREF chain; engine B: leaf frames), one context and stream each, "
-                       "as the reference's driver runs a stream's mini-GOP on engine encoders; a leaf is enqueued "
-                       "behind the next ALTREF's analysis and runs beside its reconstruction",
-            "device": engA.device_name,
+            "io": "sources resident in HBM; per frame the blocks, superblock records, levels, CDEF choices, "
+                  "restoration units and final picture copied to pinned host memory inside the timed region",
+            "references": "the driver's own reconstructed, filtered pictures (closed loop from the key frame)",
+            "engines": "2 (key, regular and ALTREF frames on engine 0, leaf frames on engine 1, each leaf beside the "
+                       "next ALTREF's reconstruction), as the reference's driver runs a stream's mini-GOP on engine "
+                       "encoders",
+            "device": devname[0],
         },
+        "steady_fps": None if steady is None else round(steady, 3),
+        "pcie_inclusive_fps": pcie,
         "kernels_ms_per_launch": {k: round(v[1] / max(v[0], 1), 5) for k, v in stats.items()},
-        "kernels_ms_note": "per-kernel means from the untimed second coding, one frame at a time (no engine "
-                           "overlap); the timed region records events around the dominant kernel only",
+        "kernels_launches": {k: v[0] for k, v in stats.items()},
+        "kernels_ms_note": "per-kernel means from a second coding of the stream, one frame at a time with every "
+                           "launch timed; the timed region records events around the dominant kernel only",
         "roofline": {
             "kernel": dom,
             "bound": "hbm",
@@ -379,89 +296,45 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
-            "traffic": RECON_INTRA_TRAFFIC if dom == "recon_intra" else None,
+            "traffic": TRAFFIC.get(dom),
             "algorithmic_bytes_per_launch": int(per_launch),
             "avg_launch_ms": round(avg_s * 1e3, 5),
+            "launches": n_launch,
             "note": ("latency-bound dependency chain (superblocks in dataflow order, blocks in decode order), "
                      "integer VALU, no MFMA: the HBM fraction is small by construction" if dom == "recon_intra" else
                      "integer-VALU bound, no MFMA: the HBM fraction is small by construction"),
         },
         "cpu_baseline": cpu,
     }
-    engA.close()
-    engB.close()
     print(json.dumps(line))
 
 
-def cpu_baseline(seconds, frames, params, uniq):
-    """The reference's own kernels (oracle/_ref: motion search, analysis,
-    reconstruction, deblocking, CDEF, restoration search and apply; the edge
-    filter types, filter info and CDEF choice from the oracle, as the
-    reference's are in its frame driver), serial on one host core, over the
-    same frame sequence until `seconds` have passed (whole mini-GOPs)."""
-    path = os.path.join(ROOT, "oracle", "_ref", "libmav1ref.so")
-    kind, pre = "reference", "ref_"
-    if not os.path.exists(path):
-        path, kind, pre = os.path.join(ROOT, "oracle", "liboracle.so"), "port", "oracle_"
-    lib = C.CDLL(path)
-    me, ana, rec_fn = getattr(lib, pre + "motion_search"), getattr(lib, pre + "analyze"), getattr(lib, pre + "recon")
-    lf, cdef = getattr(lib, pre + "loop_filter"), getattr(lib, pre + "cdef")
-    lrsel, lrapp = getattr(lib, pre + "lr_select"), getattr(lib, pre + "lr_apply")
-    nunits = sum(params["arf"].lr_unit_rows[p] * params["arf"].lr_unit_cols[p] for p in range(3))
-    units = np.zeros(nunits * 12, np.int32)
-    dbk, tmp, lrout = np.zeros_like(frames[0]), np.zeros_like(frames[0]), np.zeros_like(frames[0])
-    # the driver's edge filter types, filter info and CDEF choice (the reference's are in its frame driver)
-    orc = C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
-    fb = frames[0].size
-    info = np.zeros(params["arf"].mi_rows * params["arf"].mi_cols, np.uint8)
-    cbuf = np.zeros(2 * fb, np.uint8)
-    choice = np.zeros(params["arf"].sb_cols * params["arf"].sb_rows, np.int8)
-    mats = mav1hip.forward_matrices()
-    rec = np.zeros_like(frames[0])
-    coefs = np.zeros(params["arf"].sb_cols * params["arf"].sb_rows * 6144, np.int32)
-    nsb = params["arf"].sb_cols * params["arf"].sb_rows
-    mv = np.zeros(nsb * 170, np.int32)
-    mvg = np.zeros(nsb * 170, np.int32)
-    costs = np.zeros(nsb * 1105, np.int64)
-    infos = np.zeros(nsb * 1105 * 27, np.int32)
-    blocks = np.zeros(nsb * 64 * 27, np.int32)
-    sbs = np.zeros(nsb * 89, np.int32)
-    done = 0
-    t0 = time.perf_counter()
-    for kind_f, f, last, alt in coding_order(64):
-        fp = params[kind_f]
-        s, r, g = (mav1hip.ptr(frames[f % uniq]), mav1hip.ptr(frames[last % uniq]),
-                   mav1hip.ptr(frames[alt % uniq]) if alt >= 0 else None)
-        me(C.byref(fp), s, r, g, mav1hip.ptr(mv), mav1hip.ptr(mvg))
-        ana(C.byref(fp), s, r, g, mav1hip.ptr(mv), mav1hip.ptr(mvg), mav1hip.ptr(costs), mav1hip.ptr(infos),
-            mav1hip.ptr(blocks), mav1hip.ptr(sbs))
-        orc.oracle_edge_types(C.byref(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs))
-        rec_fn(C.byref(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs), s, mav1hip.ptr(rec), r, g, mav1hip.ptr(coefs),
-               mav1hip.ptr(mats))
-        if kind_f == "arf":
-            orc.oracle_filter_info(C.byref(fp), mav1hip.ptr(blocks), mav1hip.ptr(sbs), mav1hip.ptr(info))
-            lf(C.byref(fp), mav1hip.ptr(rec), mav1hip.ptr(info))
-            cdef(C.byref(fp), mav1hip.ptr(rec), mav1hip.ptr(cbuf), C.c_long(fb), mav1hip.ptr(info), 2)
-            dbk[:] = rec
-            orc.oracle_cdef_select(C.byref(fp), s, mav1hip.ptr(cbuf), C.c_int64(fb), 2, mav1hip.ptr(rec),
-                                   mav1hip.ptr(choice))
-            f2 = mav1hip.FrameParams.from_buffer_copy(fp)
-            lrsel(C.byref(f2), s, mav1hip.ptr(rec), mav1hip.ptr(dbk), mav1hip.ptr(tmp), mav1hip.ptr(units), 0)
-            lrapp(C.byref(f2), mav1hip.ptr(rec), mav1hip.ptr(dbk), mav1hip.ptr(lrout), mav1hip.ptr(units))
-        done += 1
-        if time.perf_counter() - t0 >= seconds and done % 2 == 0:
-            break
-    dt = time.perf_counter() - t0
+def cpu_baseline(cfg, padded, nframes, runs):
+    """The oracle's serial driver and stages (oracle/seq.py over
+    oracle/liboracle.so, whose independent superblocks, rows and units run
+    on OpenMP threads) on the host cores: the stream's first nframes frames
+    in closed loop, the median of `runs` runs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import seq as oracle_seq
+    times = []
+    for _ in range(runs):
+        o = oracle_seq.OracleSequence(mav1hip, cfg)
+        t0 = time.perf_counter()
+        for f in padded[:nframes]:
+            o.send(f)
+        o.end()
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     return {
-        "value": round(done / dt, 4),
+        "value": round(nframes / dt, 4),
         "unit": "frames/s",
-        "cores": 1,
-        "kind": kind,
-        "sample": f"{done} frames of the same 1080p turbo GOP (ARF/leaf alternating), motion search + analysis + edge types + "
-                  f"reconstruction + (ARF) filter info, deblocking, CDEF, CDEF choice, restoration search and apply "
-                  f"(the search alone takes ~45 s per 1080p frame, so the sample is whole mini-GOPs past the time "
-                  f"bound); references are source pictures, "
-                  f"{dt:.1f} s on one host core ({os.cpu_count()} visible)",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"the stream's first {nframes} frames (key frame, ALTREF, leaf) through the oracle's driver in "
+                  f"closed loop, every stage of the GPU path (the restoration search dominates: ~4 s per 1080p "
+                  f"frame on 8 cores), median of {runs} runs ({', '.join(f'{t:.1f}' for t in times)} s) on "
+                  f"{cores} OpenMP threads ({os.cpu_count()} CPUs visible)",
     }
 
 

@@ -314,6 +314,102 @@ int mav1hip_pool_store(mav1hip_ctx *c, int32_t idx, int32_t buf);
  * that reads it. Calls for one pair of contexts must not overlap. */
 int mav1hip_pool_import(mav1hip_ctx *dst, int32_t dst_idx, mav1hip_ctx *src, int32_t src_idx);
 
+/* ------------------------------------------------------------------------
+ * Sequence driver: the encoder's frame driver on HIP streams and events
+ * ------------------------------------------------------------------------
+ * The queue interface of the encoder (mav1_send_frame /
+ * mav1_receive_packet, src/mav1.h:55-64) for the backend path: frames go in
+ * in display order, the driver forms the ALTREF mini-GOPs and the coding
+ * order of src/encoder.c:1252-1293 (alt_drain, pyr_code: key frames every
+ * keyint, hidden ALTREF frames, pyramid middle frames, leaf frames, a
+ * regular frame when a mini-GOP cannot be completed), picks each frame's
+ * parameters (alt_code, :1004-1030; setup_params) and runs every stage of
+ * run_frame_core (:598-647) on the device. Reference pictures stay in HBM:
+ * each of the 8 reference slots maps to a picture on the engine that coded
+ * it, and an engine that reads a slot another engine refreshed copies it
+ * once (the engine slot copy of :1088-1118, ordered by events). Two frame
+ * engines (contexts with their own streams) code a stream as the
+ * reference's engines do (:1150-1224): the key frames, regular frames and
+ * the ALTREF chain on the first, the other frames on the second, each
+ * running beside the next ALTREF's reconstruction. The outputs the CPU side
+ * reads (the entropy coder's inputs and the final picture) are staged on
+ * the device and copied to pinned host memory on a copy stream.
+ *
+ * Only ALTREF configurations (altref >= 2, which --turbo implies) are
+ * driven; altref 0 returns NULL from mav1hip_seq_create. */
+typedef struct mav1hip_seq mav1hip_seq;
+
+/* One coded frame, in coding order (the order its OBU is written). The
+ * pointers are pinned host buffers owned by the driver, valid until the next
+ * mav1hip_seq_receive or mav1hip_seq_destroy. */
+typedef struct {
+  int32_t coded_index;   /* 0, 1, 2, ... in coding order */
+  int32_t disp;          /* display index of the frame */
+  int32_t is_key, frame_kind, nonref, hidden;
+  int32_t order_hint;
+  int32_t refresh_frame_flags;
+  int32_t last_slot, arf_slot;  /* reference slots read as LAST / ALTREF (-1: none) */
+  int32_t last_hint, arf_hint;  /* their order hints (-1: none) */
+  int32_t engine;               /* the frame engine that coded it */
+  Mav1FrameParams fp;           /* parameters, lr_type as the restoration search chose */
+  Mav1HipFilterPlan plan;       /* the in-loop filter stages that ran */
+  const Mav1BlockInfo *blocks;  /* num_sbs * 64 records (sbs[s] lists superblock s's) */
+  const Mav1SbInfo *sbs;        /* num_sbs */
+  const int32_t *coefs;         /* num_sbs * 6144 quantised levels */
+  const int8_t *cdef_choice;    /* num_sbs (0 when the frame skips CDEF) */
+  const Mav1LrUnit *lr_units;   /* one record per unit when restoration ran */
+  const uint8_t *recon;         /* final picture (padded layout, frame_bytes) */
+  const uint8_t *decision;      /* decision record (mav1hip_decision_record layout, tile_bytes 0) or NULL */
+  size_t decision_bytes;
+} Mav1HipCodedFrame;
+
+/* Creates the driver with its two engines for frames of cfg's size.
+ * lr_test: Mav1Config.lr_test. dump_decisions: produce the per-frame
+ * decision record (MAV1_DUMP_DEC). NULL and a message in err on failure. */
+mav1hip_seq *mav1hip_seq_create(const Mav1HipConfig *cfg, int32_t lr_test, int32_t dump_decisions, char *err,
+                                size_t errlen);
+void mav1hip_seq_destroy(mav1hip_seq *s);
+const char *mav1hip_seq_error(const mav1hip_seq *s);
+/* Sends one I420 frame (planes / strides as mav1_send_frame; the frame is
+ * copied at once), or ends the stream with planes == NULL. Frames whose
+ * mini-GOP is complete are enqueued on the device at once. Returns
+ * MAV1HIP_EBUSY when every output slot holds a frame not yet received: the
+ * frame was not taken; receive and send it again. */
+#define MAV1HIP_EBUSY (-5)
+int mav1hip_seq_send_frame(mav1hip_seq *s, const uint8_t *const planes[3], const int32_t strides[3]);
+/* Sends a frame already resident in device memory in the padded layout
+ * (frame_bytes, edges replicated as load_source does, src/encoder.c:297-310).
+ * It must stay unchanged until the frame is received. */
+int mav1hip_seq_send_device_frame(mav1hip_seq *s, const uint8_t *frame);
+/* The next coded frame in coding order: 0 and *out filled, 1 when none is
+ * ready (wait == 0) or none is left (the stream ended and every frame was
+ * received), a negative code on failure. wait != 0 blocks until the next
+ * frame's outputs are in host memory. */
+int mav1hip_seq_receive(mav1hip_seq *s, Mav1HipCodedFrame *out, int32_t wait);
+/* The context of engine i (0: key frames, regular frames and the ALTREF
+ * chain; 1: the other frames), for the timing calls below; owned by the
+ * driver. NULL for another index. */
+mav1hip_ctx *mav1hip_seq_engine(mav1hip_seq *s, int32_t i);
+
+/* ------------------------------------------------------------------------
+ * Input side and quality measure (host only)
+ * ------------------------------------------------------------------------ */
+/* The y4m reader of the command-line front end (tools/mav1enc.c:67-77):
+ * opens a YUV4MPEG2 file, reads W, H and F (frame rate num:den, default
+ * 30:1) from its header and refuses colour spaces other than 4:2:0. NULL
+ * and a message in err on failure. */
+typedef struct mav1hip_y4m mav1hip_y4m;
+mav1hip_y4m *mav1hip_y4m_open(const char *path, int32_t *width, int32_t *height, int32_t *fps_num, int32_t *fps_den,
+                              char *err, size_t errlen);
+/* Reads the next frame into I420 planes (rows of the visible width at the
+ * given strides): 0 on success, 1 at the end of the file, MAV1HIP_EINVAL
+ * on a truncated or malformed frame. */
+int mav1hip_y4m_read(mav1hip_y4m *y, uint8_t *const planes[3], const int32_t strides[3]);
+void mav1hip_y4m_close(mav1hip_y4m *y);
+/* PSNR of n samples (tools/mav1enc.c:22-34): 10 log10(255^2 n / SSE) with
+ * the SSE summed exactly in integers; 99 when the samples are equal. */
+double mav1hip_psnr(const uint8_t *a, const uint8_t *b, size_t n);
+
 /* Per-kernel timing with HIP events recorded on the context's stream
  * around every launch while enabled. mav1hip_timing_get returns the number
  * of launches of the named kernel and their total duration in ms since the

@@ -12,10 +12,10 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmav1hip.so")
 
-HIP_SOURCES = ["api.hip", "me.hip", "analysis.hip", "recon.hip", "filters.hip", "lrsearch.hip", "noalloc.hip"]
+HIP_SOURCES = ["api.hip", "me.hip", "analysis.hip", "recon.hip", "filters.hip", "lrsearch.hip", "noalloc.hip", "seq.hip"]
 # sources whose floating-point results must round exactly as written (no FMA contraction)
 STRICT_FP = {"lrsearch.hip"}
-C_SOURCES = ["params.c", "fwdmats.c"]
+C_SOURCES = ["params.c", "fwdmats.c", "y4m.c"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -954,3 +954,46 @@ int mav1hip_timing_get(mav1hip_ctx *c, const char *kernel, int32_t *launches, do
 }
 
 }  // extern "C"
+
+/* Library-internal access for the sequence driver (seq.hip). */
+hipStream_t ctx_stream(mav1hip_ctx *c) { return c->stream; }
+int ctx_device(const mav1hip_ctx *c) { return c->device; }
+size_t ctx_frame_bytes(const mav1hip_ctx *c) { return c->frame_bytes; }
+void *ctx_buffer(mav1hip_ctx *c, int buf, size_t *bytes) { return buf_ptr(c, buf, bytes); }
+const int *ctx_intra_error(const mav1hip_ctx *c) { return c->d_iwork + 2; }
+const int *ctx_lrtype(const mav1hip_ctx *c) { return c->d_lrtype; }
+int ctx_bind_ptrs(mav1hip_ctx *c, uint8_t *src, uint8_t *ref, uint8_t *gold) {
+  if (!c || !src) return MAV1HIP_EINVAL;
+  c->b_src = src;
+  c->b_ref = ref ? ref : c->d_ref;
+  c->b_gold = gold ? gold : c->d_gold;
+  return MAV1HIP_OK;
+}
+/* Copies a device frame buffer into dst on the context's stream. */
+int ctx_store(mav1hip_ctx *c, uint8_t *dst, int buf) {
+  if (!c || !dst || (buf != MAV1HIP_BUF_REC && buf != MAV1HIP_BUF_LROUT))
+    return fail(c, MAV1HIP_EINVAL, "store: buffer %d", buf);
+  if (dst == c->b_src || dst == c->b_ref || dst == c->b_gold)
+    return fail(c, MAV1HIP_EINVAL, "store: the frame is bound to a stage input");
+  c->me_pending = 0;
+  HIPCHK(c, hipMemcpyAsync(dst, buf == MAV1HIP_BUF_REC ? c->d_rec : c->d_lrout, c->frame_bytes,
+                           hipMemcpyDeviceToDevice, c->stream));
+  return MAV1HIP_OK;
+}
+/* REC copied into LROUT on the context's stream (the driver's copy before
+ * the restoration stage, src/encoder.c:636: LROUT outside the restored
+ * area keeps the reconstruction). */
+int ctx_rec_to_lrout(mav1hip_ctx *c) {
+  if (!c) return MAV1HIP_EINVAL;
+  c->me_pending = 0;
+  HIPCHK(c, hipMemcpyAsync(c->d_lrout, c->d_rec, c->frame_bytes, hipMemcpyDeviceToDevice, c->stream));
+  return MAV1HIP_OK;
+}
+/* CDEF choices set to 0 (the driver's memset when the frame skips CDEF,
+ * src/encoder.c:624-627). */
+int ctx_clear_choice(mav1hip_ctx *c) {
+  if (!c) return MAV1HIP_EINVAL;
+  c->me_pending = 0;
+  HIPCHK(c, hipMemsetAsync(c->d_choice, 0, (size_t)c->num_sbs, c->stream));
+  return MAV1HIP_OK;
+}

@@ -84,4 +84,20 @@ hipError_t launch_lr_chain(hipStream_t st, const Mav1FrameParams &fp, const uint
                            const uint8_t *dbk, const void *stats, Mav1LrUnit *units, int *lrType, int lrTest,
                            int *reruns, void *chain, int stop);
 
+/* api.hip: library-internal access to a context for the sequence driver */
+struct mav1hip_ctx;
+hipStream_t ctx_stream(mav1hip_ctx *c);
+int ctx_device(const mav1hip_ctx *c);
+size_t ctx_frame_bytes(const mav1hip_ctx *c);
+void *ctx_buffer(mav1hip_ctx *c, int buf, size_t *bytes);
+const int *ctx_intra_error(const mav1hip_ctx *c);
+/* restoration types of the last restoration search (int[3], device) */
+const int *ctx_lrtype(const mav1hip_ctx *c);
+/* Points the stage inputs at device frames (ref / gold NULL: the context's
+ * own buffers). */
+int ctx_bind_ptrs(mav1hip_ctx *c, uint8_t *src, uint8_t *ref, uint8_t *gold);
+int ctx_store(mav1hip_ctx *c, uint8_t *dst, int buf);
+int ctx_rec_to_lrout(mav1hip_ctx *c);
+int ctx_clear_choice(mav1hip_ctx *c);
+
 #endif

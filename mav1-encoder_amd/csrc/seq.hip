@@ -1,0 +1,778 @@
+/* Sequence driver (include/mav1_hip.h, mav1hip_seq_*): the encoder's frame
+ * driver for the backend path, on HIP streams and events.
+ *
+ * What it restates from the reference:
+ *  - the display-order input queue and the coding order of ALTREF
+ *    mini-GOPs: alt_drain / pyr_code (src/encoder.c:1229-1293);
+ *  - each frame's state (frame kind, order hint, reference slots,
+ *    refresh flags, compound): alt_code (:1004-1030), then frame_qindex and
+ *    setup_params;
+ *  - the stages of run_frame_core (:598-647) and their filter decisions;
+ *  - engines: the ALTREF chain on one engine and the other frames on a
+ *    second (:1046-1052), slot copies into the engine that reads a slot
+ *    (:1088-1118), slot writes after the core (:1128-1140).
+ *
+ * How it runs on the GPU:
+ *  - an engine is a context (its own streams and buffers) plus kPics
+ *    device pictures; each of the 8 reference slots maps, per engine, to
+ *    one of its pictures and the generation of the slot it holds. A frame
+ *    that refreshes a slot stores its final picture into a picture no slot
+ *    of its engine maps to; an engine that reads a slot coded elsewhere
+ *    copies it once on the transfer stream, ordered by events after the
+ *    picture's writer and before the reader and any later writer;
+ *  - a frame of the second engine is enqueued after the next ALTREF's
+ *    motion search, analysis and edge types, so it runs beside that
+ *    ALTREF's reconstruction (whose intra phase leaves most CUs idle);
+ *  - every frame's outputs are copied on its engine's stream into a device
+ *    staging slot, then to pinned host memory on the download stream, so
+ *    the engines never wait for PCIe; host-sent frames go up on the upload
+ *    stream into a per-engine ring of source frames.
+ * All streams have the default priority. */
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <deque>
+#include <vector>
+
+#include "../../include/mav1_hip.h"
+#include "internal.h"
+
+namespace {
+
+constexpr int kEngines = 2;
+constexpr int kPics = 10; /* pictures per engine: 8 slots, the frame's own store, one spare */
+constexpr int kSrcRing = 4;
+
+struct Input {
+  int disp;
+  int stage;          /* host staging slot, -1 for a device frame */
+  const uint8_t *dev; /* the caller's device frame */
+};
+
+/* A frame in coding order with everything alt_code decides. */
+struct Frame {
+  Input in;
+  int is_key, kind, last_slot, arf_slot, refresh, hidden, nonref;
+  int order_hint, last_hint, arf_hint;
+  int engine;
+  int out;        /* output slot */
+  long long coded; /* coding index */
+  Mav1HipFrame fr;
+  Mav1FrameParams fp;
+  Mav1HipFilterPlan plan;
+  int npresets;
+  uint8_t *src;   /* device source frame the stages read */
+  int src_ring;   /* ring entry of the engine (-1: the caller's device frame) */
+};
+
+struct OutSlot {
+  uint8_t *d = nullptr; /* device staging */
+  uint8_t *h = nullptr; /* pinned host mirror */
+  hipEvent_t staged = nullptr, landed = nullptr;
+  int busy = 0;
+  Mav1HipCodedFrame info;
+  int32_t hdr7[7];
+};
+
+struct Engine {
+  mav1hip_ctx *ctx = nullptr;
+  hipStream_t st = nullptr;
+  uint8_t *pics = nullptr;
+  int map[8];
+  unsigned have[8];
+  uint8_t *ring = nullptr; /* kSrcRing source frames */
+  hipEvent_t ring_free[kSrcRing] = {};
+  int ring_used[kSrcRing] = {};
+  int ring_next = 0;
+  hipEvent_t ev = nullptr; /* scratch */
+};
+
+}  // namespace
+
+struct mav1hip_seq {
+  Mav1HipConfig cfg;
+  int lr_test = 0, dump = 0;
+  char err[512] = {0};
+  size_t frame_bytes = 0;
+  int num_sbs = 0, lr_units = 0;
+  size_t grid_bytes = 0;
+  int32_t stride[3], offset[3], plane_h[3];
+  Engine eng[kEngines];
+  hipStream_t up = nullptr, down = nullptr, xfer = nullptr;
+  hipEvent_t xev = nullptr;
+  /* host staging of sent frames */
+  std::vector<uint8_t *> stage;
+  std::vector<hipEvent_t> stage_done;
+  std::vector<int> stage_busy;
+  std::deque<Input> inq;
+  int in_disp = 0, ended = 0;
+  int key_disp = 0, anchor = 0;
+  unsigned gen = 0;
+  unsigned slot_gen[8] = {};
+  int slot_owner[8];
+  int slot_hint[8];
+  /* outputs */
+  std::vector<OutSlot> outs;
+  std::deque<int> order; /* output slots in coding order, not yet received */
+  int held = -1;         /* slot handed to the caller by the last receive */
+  long long coded = 0;
+  std::vector<Frame> pending; /* second-engine frames waiting for the next ALTREF's early stages */
+  size_t o_blocks, o_sbs, o_coefs, o_choice, o_units, o_lrtype, o_err, o_pic, o_grid, o_end;
+};
+
+namespace {
+
+int seq_fail(mav1hip_seq *s, int code, const char *fmt, ...) {
+  if (s) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(s->err, sizeof s->err, fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+#define SCHK(s, call)                                                                                       \
+  do {                                                                                                      \
+    hipError_t e_ = (call);                                                                                 \
+    if (e_ != hipSuccess)                                                                                   \
+      return seq_fail(s, MAV1HIP_EDEVICE, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+/* A context call's status, with the context's message. */
+int cchk(mav1hip_seq *s, int eng, int rc) {
+  if (rc == MAV1HIP_OK) return rc;
+  return seq_fail(s, rc, "engine %d: %s", eng, mav1hip_error(s->eng[eng].ctx));
+}
+#define CCHK(s, eng, call)              \
+  do {                                  \
+    int rc_ = cchk(s, eng, (call));     \
+    if (rc_) return rc_;                \
+  } while (0)
+
+size_t al(size_t n) { return (n + 255) & ~(size_t)255; }
+
+uint8_t *pic(mav1hip_seq *s, int e, int idx) { return s->eng[e].pics + (size_t)idx * s->frame_bytes; }
+
+/* A picture of engine e that no slot maps to and that is none of `avoid`. */
+int free_pic(mav1hip_seq *s, int e, const int *avoid, int navoid) {
+  for (int i = 0; i < kPics; i++) {
+    bool used = false;
+    for (int k = 0; k < 8 && !used; k++) used = s->eng[e].map[k] == i;
+    for (int j = 0; j < navoid && !used; j++) used = avoid[j] == i;
+    if (!used) return i;
+  }
+  return -1;
+}
+
+/* The picture of engine e holding slot k's current version, copied there
+ * from the engine that wrote it when e holds an older one. */
+int slot_pic(mav1hip_seq *s, int e, int k, const int *avoid, int navoid, int *idx) {
+  Engine &E = s->eng[e];
+  const unsigned g = s->slot_gen[k];
+  if (E.map[k] >= 0 && E.have[k] == g) {
+    *idx = E.map[k];
+    return MAV1HIP_OK;
+  }
+  /* another slot of this engine may hold the same picture (a key frame
+   * refreshes every slot) */
+  for (int j = 0; j < 8; j++)
+    if (E.map[j] >= 0 && E.have[j] == g && s->slot_gen[j] == g) {
+      E.map[k] = E.map[j];
+      E.have[k] = g;
+      *idx = E.map[k];
+      return MAV1HIP_OK;
+    }
+  const int o = s->slot_owner[k];
+  if (o < 0 || o == e || s->eng[o].map[k] < 0 || s->eng[o].have[k] != g)
+    return seq_fail(s, MAV1HIP_EINVAL, "reference slot %d has no picture", k);
+  E.map[k] = -1;
+  const int b = free_pic(s, e, avoid, navoid);
+  if (b < 0) return seq_fail(s, MAV1HIP_EINVAL, "engine %d: no free picture", e);
+  /* the copy after everything the owner enqueued so far (the picture is
+   * complete), before this engine's next work and before the owner's later
+   * work (which may overwrite the picture) */
+  Engine &O = s->eng[o];
+  SCHK(s, hipEventRecord(O.ev, O.st));
+  SCHK(s, hipStreamWaitEvent(s->xfer, O.ev, 0));
+  SCHK(s, hipEventRecord(E.ev, E.st));
+  SCHK(s, hipStreamWaitEvent(s->xfer, E.ev, 0));
+  SCHK(s, hipMemcpyAsync(pic(s, e, b), pic(s, o, O.map[k]), s->frame_bytes, hipMemcpyDeviceToDevice, s->xfer));
+  SCHK(s, hipEventRecord(s->xev, s->xfer));
+  SCHK(s, hipStreamWaitEvent(E.st, s->xev, 0));
+  SCHK(s, hipStreamWaitEvent(O.st, s->xev, 0));
+  E.map[k] = b;
+  E.have[k] = g;
+  *idx = b;
+  return MAV1HIP_OK;
+}
+
+int free_outs(const mav1hip_seq *s) {
+  int n = 0;
+  for (auto &o : s->outs) n += !o.busy;
+  return n;
+}
+
+/* alt_code's frame state (src/encoder.c:1004-1030) and its parameters. */
+int make_frame(mav1hip_seq *s, const Input &in, int isKey, int kind, int lastSlot, int arfSlot, int refresh,
+               int hidden, Frame *f) {
+  memset(f, 0, sizeof *f);
+  f->in = in;
+  if (isKey) s->key_disp = in.disp;
+  f->is_key = isKey;
+  f->kind = isKey ? 0 : kind;
+  f->refresh = isKey ? 0xFF : refresh;
+  f->hidden = hidden;
+  f->nonref = s->cfg.leaf && !isKey && refresh == 0;
+  f->order_hint = in.disp - s->key_disp;
+  f->last_slot = isKey ? -1 : lastSlot;
+  f->arf_slot = isKey ? -1 : arfSlot;
+  f->last_hint = isKey ? -1 : s->slot_hint[lastSlot];
+  f->arf_hint = !isKey && arfSlot >= 0 ? s->slot_hint[arfSlot] : -1;
+  f->fr.is_key = isKey;
+  f->fr.frame_kind = f->kind;
+  f->fr.nonref = f->nonref;
+  f->fr.order_hint = f->order_hint;
+  f->fr.reference_select = !isKey && kind >= 2 && arfSlot >= 0;
+  f->fr.gold_differs = 0;
+  /* the reference chain on engine 0; pyramid middle frames and leaf frames
+   * on engine 1 (src/encoder.c:1046-1052) */
+  const int chain = f->refresh && !(f->kind == 3);
+  f->engine = chain ? 0 : 1;
+  const int q = mav1hip_frame_qindex(&s->cfg, &f->fr);
+  f->npresets = mav1hip_setup_params(&s->cfg, &f->fr, q, &f->fp);
+  if (f->npresets < 0) return seq_fail(s, MAV1HIP_EINVAL, "setup_params failed");
+  mav1hip_filter_plan(&s->cfg, &f->fr, &f->fp, f->npresets, &f->plan);
+  /* the slots' order hints change when the frame is handed over, as alt_code
+   * updates them at once */
+  for (int k = 0; k < 8; k++)
+    if ((f->refresh >> k) & 1) s->slot_hint[k] = f->order_hint;
+  /* output slot */
+  f->out = -1;
+  for (int i = 0; i < (int)s->outs.size(); i++)
+    if (!s->outs[i].busy) {
+      f->out = i;
+      break;
+    }
+  if (f->out < 0) return seq_fail(s, MAV1HIP_EBUSY, "no free output slot");
+  s->outs[f->out].busy = 1;
+  s->order.push_back(f->out);
+  f->coded = s->coded++;
+  return MAV1HIP_OK;
+}
+
+/* The source frame on the engine: the caller's device frame, or the staged
+ * host frame uploaded into the engine's ring on the upload stream. */
+int bind_source(mav1hip_seq *s, Frame &f) {
+  Engine &E = s->eng[f.engine];
+  if (f.in.stage < 0) {
+    f.src = const_cast<uint8_t *>(f.in.dev);
+    f.src_ring = -1;
+    return MAV1HIP_OK;
+  }
+  const int r = E.ring_next;
+  E.ring_next = (E.ring_next + 1) % kSrcRing;
+  if (E.ring_used[r]) SCHK(s, hipStreamWaitEvent(s->up, E.ring_free[r], 0));
+  uint8_t *dst = E.ring + (size_t)r * s->frame_bytes;
+  SCHK(s, hipMemcpyAsync(dst, s->stage[f.in.stage], s->frame_bytes, hipMemcpyHostToDevice, s->up));
+  SCHK(s, hipEventRecord(s->stage_done[f.in.stage], s->up));
+  s->stage_busy[f.in.stage] = 2; /* reusable once stage_done has completed */
+  SCHK(s, hipStreamWaitEvent(E.st, s->stage_done[f.in.stage], 0));
+  E.ring_used[r] = 1;
+  f.src = dst;
+  f.src_ring = r;
+  return MAV1HIP_OK;
+}
+
+/* Source, references and parameters, then motion search, analysis and edge
+ * types. */
+int early_stages(mav1hip_seq *s, Frame &f) {
+  const int e = f.engine;
+  Engine &E = s->eng[e];
+  int rc;
+  if ((rc = bind_source(s, f))) return rc;
+  uint8_t *ref = nullptr, *gold = nullptr;
+  int avoid[2] = {-1, -1};
+  if (!f.is_key) {
+    int li, ai = -1;
+    if ((rc = slot_pic(s, e, f.last_slot, avoid, 0, &li))) return rc;
+    avoid[0] = li;
+    ref = pic(s, e, li);
+    if (f.arf_slot >= 0 && f.fp.golden) {
+      if ((rc = slot_pic(s, e, f.arf_slot, avoid, 1, &ai))) return rc;
+      gold = pic(s, e, ai);
+    }
+  }
+  CCHK(s, e, ctx_bind_ptrs(E.ctx, f.src, ref, gold));
+  CCHK(s, e, mav1hip_set_params(E.ctx, &f.fp));
+  if (!f.is_key) CCHK(s, e, mav1hip_launch_motion_search(E.ctx));
+  CCHK(s, e, mav1hip_launch_analyze(E.ctx));
+  if (s->cfg.intra_edge_filter) CCHK(s, e, mav1hip_launch_edge_types(E.ctx));
+  return MAV1HIP_OK;
+}
+
+int stage_copy(mav1hip_seq *s, Engine &E, uint8_t *dst, int buf, size_t bytes) {
+  size_t cap = 0;
+  void *src = ctx_buffer(E.ctx, buf, &cap);
+  if (!src || bytes > cap) return seq_fail(s, MAV1HIP_EINVAL, "staging buffer %d", buf);
+  SCHK(s, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, E.st));
+  return MAV1HIP_OK;
+}
+
+/* Reconstruction and the in-loop filters (run_frame_core,
+ * src/encoder.c:611-641), the outputs staged and sent down, the refreshed
+ * slots stored. */
+int late_stages(mav1hip_seq *s, Frame &f) {
+  const int e = f.engine;
+  Engine &E = s->eng[e];
+  mav1hip_ctx *c = E.ctx;
+  const Mav1HipFilterPlan &pl = f.plan;
+  CCHK(s, e, mav1hip_launch_recon(c));
+  if (pl.filter_info) {
+    CCHK(s, e, mav1hip_launch_filter_info(c));
+    if (pl.deblock) CCHK(s, e, mav1hip_launch_deblock(c));
+    if (pl.cdef_sets) CCHK(s, e, mav1hip_launch_cdef(c, pl.cdef_sets));
+  }
+  /* leafopt bit 4: CDEF skipped, every superblock codes set 0 */
+  const int leafopt = (s->cfg.altref && f.nonref) ? (s->cfg.knob_leafopt >= 0 ? s->cfg.knob_leafopt : 0x3F) : 0;
+  const int cdefOff = (leafopt & 4) != 0;
+  if (pl.restoration) CCHK(s, e, mav1hip_launch_save_deblocked(c));
+  if (pl.cdef_select)
+    CCHK(s, e, mav1hip_launch_cdef_select(c, pl.cdef_sets));
+  else if (cdefOff)
+    CCHK(s, e, ctx_clear_choice(c));
+  int fin = MAV1HIP_BUF_REC;
+  if (pl.restoration) {
+    CCHK(s, e, mav1hip_launch_lr_select(c, s->lr_test));
+    CCHK(s, e, ctx_rec_to_lrout(c));
+    CCHK(s, e, mav1hip_launch_restore(c));
+    fin = MAV1HIP_BUF_LROUT;
+  }
+  if (s->dump) CCHK(s, e, mav1hip_launch_decision_grid(c));
+  /* outputs into the staging slot, then down on the download stream */
+  OutSlot &o = s->outs[f.out];
+  int rc;
+  const size_t nb = sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)s->num_sbs;
+  if ((rc = stage_copy(s, E, o.d + s->o_blocks, MAV1HIP_BUF_BLOCKS, nb))) return rc;
+  if ((rc = stage_copy(s, E, o.d + s->o_sbs, MAV1HIP_BUF_SBS, sizeof(Mav1SbInfo) * (size_t)s->num_sbs))) return rc;
+  if ((rc = stage_copy(s, E, o.d + s->o_coefs, MAV1HIP_BUF_COEFS,
+                       sizeof(int32_t) * MAV1_COEFS_PER_SB * (size_t)s->num_sbs)))
+    return rc;
+  if ((rc = stage_copy(s, E, o.d + s->o_choice, MAV1HIP_BUF_CDEF_CHOICE, (size_t)s->num_sbs))) return rc;
+  if (pl.restoration &&
+      (rc = stage_copy(s, E, o.d + s->o_units, MAV1HIP_BUF_LRUNITS, sizeof(Mav1LrUnit) * (size_t)s->lr_units)))
+    return rc;
+  if ((rc = stage_copy(s, E, o.d + s->o_pic, fin, s->frame_bytes))) return rc;
+  if (s->dump && (rc = stage_copy(s, E, o.d + s->o_grid, MAV1HIP_BUF_DECGRID, s->grid_bytes))) return rc;
+  /* the restoration types chosen on the device, and the intra phase's error word */
+  SCHK(s, hipMemcpyAsync(o.d + s->o_lrtype, ctx_lrtype(c), sizeof(int) * 3, hipMemcpyDeviceToDevice, E.st));
+  SCHK(s, hipMemcpyAsync(o.d + s->o_err, ctx_intra_error(c), sizeof(int), hipMemcpyDeviceToDevice, E.st));
+  SCHK(s, hipEventRecord(o.staged, E.st));
+  SCHK(s, hipStreamWaitEvent(s->down, o.staged, 0));
+  SCHK(s, hipMemcpyAsync(o.h, o.d, s->o_end, hipMemcpyDeviceToHost, s->down));
+  SCHK(s, hipEventRecord(o.landed, s->down));
+  /* the source ring entry is free once this frame's stages have read it */
+  if (f.src_ring >= 0) SCHK(s, hipEventRecord(E.ring_free[f.src_ring], E.st));
+  /* refreshed slots: the final picture into a picture no slot maps to */
+  if (f.refresh) {
+    const int b = free_pic(s, e, nullptr, 0);
+    if (b < 0) return seq_fail(s, MAV1HIP_EINVAL, "engine %d: no free picture", e);
+    CCHK(s, e, ctx_store(c, pic(s, e, b), fin));
+    s->gen++;
+    for (int k = 0; k < 8; k++)
+      if ((f.refresh >> k) & 1) {
+        s->slot_gen[k] = s->gen;
+        s->slot_owner[k] = e;
+        E.map[k] = b;
+        E.have[k] = s->gen;
+      }
+  }
+  /* what receive reports */
+  Mav1HipCodedFrame &info = o.info;
+  memset(&info, 0, sizeof info);
+  info.coded_index = (int32_t)f.coded;
+  info.disp = f.in.disp;
+  info.is_key = f.is_key;
+  info.frame_kind = f.kind;
+  info.nonref = f.nonref;
+  info.hidden = f.hidden;
+  info.order_hint = f.order_hint;
+  info.refresh_frame_flags = f.refresh;
+  info.last_slot = f.last_slot;
+  info.arf_slot = f.arf_slot;
+  info.last_hint = f.last_hint;
+  info.arf_hint = f.arf_hint;
+  info.engine = e;
+  info.fp = f.fp;
+  info.plan = f.plan;
+  o.hdr7[0] = f.order_hint;
+  o.hdr7[1] = f.last_hint;
+  o.hdr7[2] = f.arf_hint;
+  o.hdr7[3] = f.is_key ? 0 : 1;
+  o.hdr7[4] = f.kind;
+  o.hdr7[5] = f.nonref;
+  o.hdr7[6] = 0;
+  return MAV1HIP_OK;
+}
+
+int run_full(mav1hip_seq *s, Frame &f) {
+  int rc = early_stages(s, f);
+  return rc ? rc : late_stages(s, f);
+}
+
+int flush_pending(mav1hip_seq *s) {
+  for (auto &f : s->pending) {
+    int rc = run_full(s, f);
+    if (rc) return rc;
+  }
+  s->pending.clear();
+  return MAV1HIP_OK;
+}
+
+/* A frame handed over in coding order: second-engine frames wait for the
+ * next ALTREF's early stages; anything else runs now. */
+int code_frame(mav1hip_seq *s, const Input &in, int isKey, int kind, int lastSlot, int arfSlot, int refresh,
+               int hidden) {
+  Frame f;
+  int rc = make_frame(s, in, isKey, kind, lastSlot, arfSlot, refresh, hidden, &f);
+  if (rc) return rc;
+  if (f.engine != 0) {
+    s->pending.push_back(f);
+    return MAV1HIP_OK;
+  }
+  const bool arf = !f.is_key && f.kind == 1;
+  if (!arf && (rc = flush_pending(s))) return rc;
+  if ((rc = early_stages(s, f))) return rc;
+  if (arf && (rc = flush_pending(s))) return rc;
+  return late_stages(s, f);
+}
+
+/* pyr_code (src/encoder.c:1229-1250): the frames strictly between a (slot
+ * as) and b (slot bs). */
+int pyr_code(mav1hip_seq *s, int a, int as, int b, int bs, int depth) {
+  const int d0 = s->in_disp - (int)s->inq.size();
+  int rc;
+  if (b - a - 1 >= 3 && depth < 4) {
+    const int m = (a + b) / 2, ms = 4 + depth;
+    if ((rc = code_frame(s, s->inq[m - d0], 0, 3, as, bs, 1 << ms, 1))) return rc;
+    if ((rc = pyr_code(s, a, as, m, ms, depth + 1))) return rc;
+    return pyr_code(s, m, ms, b, bs, depth + 1);
+  }
+  for (int d = a + 1; d < b; d++)
+    if ((rc = code_frame(s, s->inq[d - d0], 0, 2, as, bs, 0, 0))) return rc;
+  return MAV1HIP_OK;
+}
+
+/* Frames the next mini-GOP of alt_drain would code (0: it cannot be formed
+ * yet). */
+int next_gop_len(const mav1hip_seq *s, int flush) {
+  if (s->inq.empty()) return 0;
+  const int N = s->cfg.altref;
+  const int d0 = s->in_disp - (int)s->inq.size();
+  const int isKey = d0 == 0 || (s->cfg.keyint > 0 && d0 % s->cfg.keyint == 0);
+  int L = isKey ? 1 : N;
+  if (!isKey && s->cfg.keyint > 0) L = std::min(L, (d0 / s->cfg.keyint + 1) * s->cfg.keyint - d0);
+  if ((int)s->inq.size() < L) {
+    if (!flush) return 0;
+    L = (int)s->inq.size();
+  }
+  return L;
+}
+
+/* alt_drain (src/encoder.c:1252-1293). */
+int drain(mav1hip_seq *s, int flush) {
+  int L;
+  while ((L = next_gop_len(s, flush)) > 0) {
+    if (free_outs(s) < L) return MAV1HIP_EBUSY;
+    const int d0 = s->in_disp - (int)s->inq.size();
+    const int isKey = d0 == 0 || (s->cfg.keyint > 0 && d0 % s->cfg.keyint == 0);
+    int rc;
+    if (isKey || L == 1) {
+      if ((rc = code_frame(s, s->inq[0], isKey, 0, s->anchor, -1, 0x01, 0))) return rc;
+      s->anchor = 0;
+    } else if (s->cfg.leaf) {
+      const int arfSlot = s->anchor == 2 ? 3 : 2;
+      if ((rc = code_frame(s, s->inq[L - 1], 0, 1, s->anchor, -1, 1 << arfSlot, 1))) return rc;
+      if ((rc = pyr_code(s, d0 - 1, s->anchor, d0 + L - 1, arfSlot, 0))) return rc;
+      s->anchor = arfSlot;
+    } else {
+      const int arfSlot = s->anchor == 2 ? 3 : 2;
+      if ((rc = code_frame(s, s->inq[L - 1], 0, 1, s->anchor, -1, 1 << arfSlot, 1))) return rc;
+      for (int i = 0; i < L - 1; i++)
+        if ((rc = code_frame(s, s->inq[i], 0, 2, i == 0 ? s->anchor : 0, arfSlot, 0x01, 0))) return rc;
+      s->anchor = arfSlot;
+    }
+    for (int i = 0; i < L; i++) s->inq.pop_front();
+  }
+  if (flush && s->inq.empty()) return flush_pending(s);
+  return MAV1HIP_OK;
+}
+
+/* A host staging slot for a sent frame (waits for its previous upload). */
+int take_stage(mav1hip_seq *s, int *idx) {
+  for (int i = 0; i < (int)s->stage.size(); i++) {
+    if (s->stage_busy[i] == 1) continue;
+    if (s->stage_busy[i] == 2) {
+      SCHK(s, hipEventSynchronize(s->stage_done[i]));
+      s->stage_busy[i] = 0;
+    }
+    s->stage_busy[i] = 1;
+    *idx = i;
+    return MAV1HIP_OK;
+  }
+  return seq_fail(s, MAV1HIP_EBUSY, "no free input staging slot");
+}
+
+void destroy(mav1hip_seq *s) {
+  if (!s) return;
+  for (auto &E : s->eng)
+    if (E.st) (void)hipStreamSynchronize(E.st);
+  if (s->up) (void)hipStreamSynchronize(s->up);
+  if (s->down) (void)hipStreamSynchronize(s->down);
+  if (s->xfer) (void)hipStreamSynchronize(s->xfer);
+  for (auto &E : s->eng) {
+    if (E.ctx) mav1hip_destroy(E.ctx);
+    (void)hipFree(E.pics);
+    (void)hipFree(E.ring);
+    for (auto &ev : E.ring_free)
+      if (ev) (void)hipEventDestroy(ev);
+    if (E.ev) (void)hipEventDestroy(E.ev);
+  }
+  for (auto &o : s->outs) {
+    (void)hipFree(o.d);
+    if (o.h) (void)hipHostFree(o.h);
+    if (o.staged) (void)hipEventDestroy(o.staged);
+    if (o.landed) (void)hipEventDestroy(o.landed);
+  }
+  for (auto *p : s->stage)
+    if (p) (void)hipHostFree(p);
+  for (auto ev : s->stage_done)
+    if (ev) (void)hipEventDestroy(ev);
+  if (s->xev) (void)hipEventDestroy(s->xev);
+  if (s->up) (void)hipStreamDestroy(s->up);
+  if (s->down) (void)hipStreamDestroy(s->down);
+  if (s->xfer) (void)hipStreamDestroy(s->xfer);
+  delete s;
+}
+
+int setup(mav1hip_seq *s) {
+  SCHK(s, hipStreamCreate(&s->up));
+  SCHK(s, hipStreamCreate(&s->down));
+  SCHK(s, hipStreamCreate(&s->xfer));
+  SCHK(s, hipEventCreateWithFlags(&s->xev, hipEventDisableTiming));
+  for (int e = 0; e < kEngines; e++) {
+    Engine &E = s->eng[e];
+    char err[512] = {0};
+    E.ctx = mav1hip_create(&s->cfg, err, sizeof err);
+    if (!E.ctx) return seq_fail(s, MAV1HIP_EDEVICE, "engine %d: %s", e, err);
+    E.st = ctx_stream(E.ctx);
+    SCHK(s, hipMalloc((void **)&E.pics, s->frame_bytes * kPics));
+    SCHK(s, hipMemset(E.pics, 0, s->frame_bytes * kPics));
+    SCHK(s, hipMalloc((void **)&E.ring, s->frame_bytes * kSrcRing));
+    for (auto &ev : E.ring_free) SCHK(s, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    SCHK(s, hipEventCreateWithFlags(&E.ev, hipEventDisableTiming));
+    for (int k = 0; k < 8; k++) {
+      E.map[k] = -1;
+      E.have[k] = 0;
+    }
+  }
+  for (int k = 0; k < 8; k++) {
+    s->slot_owner[k] = -1;
+    s->slot_hint[k] = 0;
+  }
+  /* output staging: blocks | sbs | coefs | choice | units | lrtype | err | picture | grid */
+  size_t o = 0;
+  s->o_blocks = o;
+  o += al(sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)s->num_sbs);
+  s->o_sbs = o;
+  o += al(sizeof(Mav1SbInfo) * (size_t)s->num_sbs);
+  s->o_coefs = o;
+  o += al(sizeof(int32_t) * MAV1_COEFS_PER_SB * (size_t)s->num_sbs);
+  s->o_choice = o;
+  o += al((size_t)s->num_sbs);
+  s->o_units = o;
+  o += al(sizeof(Mav1LrUnit) * (size_t)s->lr_units);
+  s->o_lrtype = o;
+  o += al(sizeof(int) * 3);
+  s->o_err = o;
+  o += al(sizeof(int));
+  s->o_pic = o;
+  o += al(s->frame_bytes);
+  /* the decision record's 44-byte header goes in front of the cells */
+  s->o_grid = o + 64;
+  o += al(64 + s->grid_bytes);
+  s->o_end = o;
+  const int nout = s->cfg.altref + 4;
+  s->outs.resize(nout);
+  for (auto &os : s->outs) {
+    SCHK(s, hipMalloc((void **)&os.d, s->o_end));
+    SCHK(s, hipMemset(os.d, 0, s->o_end));
+    SCHK(s, hipHostMalloc((void **)&os.h, s->o_end, hipHostMallocDefault));
+    SCHK(s, hipEventCreateWithFlags(&os.staged, hipEventDisableTiming));
+    SCHK(s, hipEventCreateWithFlags(&os.landed, hipEventDisableTiming));
+  }
+  /* input staging: the display-order queue plus the second engine's pending frames */
+  const int nstage = 2 * s->cfg.altref + 4;
+  s->stage.assign(nstage, nullptr);
+  s->stage_done.assign(nstage, nullptr);
+  s->stage_busy.assign(nstage, 0);
+  for (int i = 0; i < nstage; i++) {
+    SCHK(s, hipHostMalloc((void **)&s->stage[i], s->frame_bytes, hipHostMallocDefault));
+    SCHK(s, hipEventCreateWithFlags(&s->stage_done[i], hipEventDisableTiming));
+  }
+  return MAV1HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mav1hip_seq *mav1hip_seq_create(const Mav1HipConfig *cfg, int32_t lr_test, int32_t dump_decisions, char *err,
+                                size_t errlen) {
+  if (!cfg) {
+    if (err && errlen) snprintf(err, errlen, "no configuration");
+    return nullptr;
+  }
+  mav1hip_seq *s = new mav1hip_seq();
+  s->cfg = *cfg;
+  mav1hip_config_normalize(&s->cfg);
+  if (s->cfg.altref < 2) {
+    if (err && errlen) snprintf(err, errlen, "the sequence driver runs ALTREF configurations (altref >= 2)");
+    delete s;
+    return nullptr;
+  }
+  s->lr_test = lr_test;
+  s->dump = dump_decisions != 0;
+  int32_t mc, mr, sc, sr, tw;
+  s->frame_bytes = mav1hip_geometry(&s->cfg, &mc, &mr, &sc, &sr, &tw, s->stride, s->offset, s->plane_h);
+  if (!s->frame_bytes) {
+    if (err && errlen) snprintf(err, errlen, "unsupported frame size %dx%d", cfg->width, cfg->height);
+    delete s;
+    return nullptr;
+  }
+  s->num_sbs = sc * sr;
+  s->grid_bytes = (size_t)((mr + 1) >> 1) * ((mc + 1) >> 1) * 8;
+  {
+    Mav1HipFrame fr = {};
+    fr.is_key = 1;
+    Mav1FrameParams fp;
+    mav1hip_setup_params(&s->cfg, &fr, s->cfg.qindex, &fp);
+    s->lr_units = fp.lr_unit_off[2] + fp.lr_unit_rows[2] * fp.lr_unit_cols[2];
+  }
+  if (setup(s) != MAV1HIP_OK) {
+    if (err && errlen) snprintf(err, errlen, "%s", s->err);
+    destroy(s);
+    return nullptr;
+  }
+  return s;
+}
+
+void mav1hip_seq_destroy(mav1hip_seq *s) { destroy(s); }
+
+const char *mav1hip_seq_error(const mav1hip_seq *s) { return s ? s->err : "no sequence"; }
+
+mav1hip_ctx *mav1hip_seq_engine(mav1hip_seq *s, int32_t i) {
+  return s && i >= 0 && i < kEngines ? s->eng[i].ctx : nullptr;
+}
+
+static int send_input(mav1hip_seq *s, const Input &in) {
+  s->inq.push_back(in);
+  s->in_disp++;
+  return drain(s, 0);
+}
+
+int mav1hip_seq_send_frame(mav1hip_seq *s, const uint8_t *const planes[3], const int32_t strides[3]) {
+  if (!s) return MAV1HIP_EINVAL;
+  if (s->ended) return seq_fail(s, MAV1HIP_EINVAL, "send_frame: the stream has ended");
+  if (!planes) {
+    if (free_outs(s) < (int)s->inq.size()) return seq_fail(s, MAV1HIP_EBUSY, "end of stream: receive first");
+    s->ended = 1;
+    return drain(s, 1);
+  }
+  if (!strides || !planes[0] || !planes[1] || !planes[2]) return seq_fail(s, MAV1HIP_EINVAL, "send_frame: missing plane");
+  /* the frames a drain could code after this one need their output slots */
+  if (free_outs(s) < (int)s->inq.size() + 1) return seq_fail(s, MAV1HIP_EBUSY, "send_frame: receive first");
+  int st;
+  int rc = take_stage(s, &st);
+  if (rc) return rc;
+  /* load_source (src/encoder.c:297-310): edges replicated to the padded size */
+  uint8_t *dst0 = s->stage[st];
+  for (int p = 0; p < 3; p++) {
+    const int pw = p ? (s->cfg.width + 1) >> 1 : s->cfg.width;
+    const int ph = p ? (s->cfg.height + 1) >> 1 : s->cfg.height;
+    const int aw = s->stride[p], ah = s->plane_h[p];
+    uint8_t *dst = dst0 + s->offset[p];
+    for (int y = 0; y < ah; y++) {
+      const uint8_t *src = planes[p] + (size_t)(y < ph ? y : ph - 1) * strides[p];
+      uint8_t *d = dst + (size_t)y * aw;
+      memcpy(d, src, pw);
+      memset(d + pw, src[pw - 1], aw - pw);
+    }
+  }
+  return send_input(s, Input{s->in_disp, st, nullptr});
+}
+
+int mav1hip_seq_send_device_frame(mav1hip_seq *s, const uint8_t *frame) {
+  if (!s || !frame) return MAV1HIP_EINVAL;
+  if (s->ended) return seq_fail(s, MAV1HIP_EINVAL, "send_device_frame: the stream has ended");
+  if (free_outs(s) < (int)s->inq.size() + 1) return seq_fail(s, MAV1HIP_EBUSY, "send_device_frame: receive first");
+  return send_input(s, Input{s->in_disp, -1, frame});
+}
+
+int mav1hip_seq_receive(mav1hip_seq *s, Mav1HipCodedFrame *out, int32_t wait) {
+  if (!s || !out) return MAV1HIP_EINVAL;
+  if (s->held >= 0) {
+    s->outs[s->held].busy = 0;
+    s->held = -1;
+  }
+  /* frames still waiting for the next ALTREF run when the caller waits for them */
+  if (s->order.empty()) return 1;
+  const int i = s->order.front();
+  OutSlot &o = s->outs[i];
+  if (wait && !s->pending.empty()) {
+    bool queued = false;
+    for (auto &f : s->pending) queued |= f.out == i;
+    if (queued) {
+      int rc = flush_pending(s);
+      if (rc) return rc;
+    }
+  }
+  for (auto &f : s->pending)
+    if (f.out == i) return 1; /* not enqueued yet */
+  if (wait) {
+    SCHK(s, hipEventSynchronize(o.landed));
+  } else {
+    hipError_t q = hipEventQuery(o.landed);
+    if (q == hipErrorNotReady) return 1;
+    if (q != hipSuccess) return seq_fail(s, MAV1HIP_EDEVICE, "receive: %s", hipGetErrorString(q));
+  }
+  s->order.pop_front();
+  s->held = i;
+  int ierr = 0;
+  memcpy(&ierr, o.h + s->o_err, sizeof ierr);
+  if (ierr) return seq_fail(s, MAV1HIP_EDEVICE, "frame %d: an intra-phase dependency wait ran out", o.info.disp);
+  *out = o.info;
+  if (out->plan.restoration) memcpy(out->fp.lr_type, o.h + s->o_lrtype, sizeof(int) * 3);
+  out->blocks = (const Mav1BlockInfo *)(o.h + s->o_blocks);
+  out->sbs = (const Mav1SbInfo *)(o.h + s->o_sbs);
+  out->coefs = (const int32_t *)(o.h + s->o_coefs);
+  out->cdef_choice = (const int8_t *)(o.h + s->o_choice);
+  out->lr_units = out->plan.restoration ? (const Mav1LrUnit *)(o.h + s->o_units) : nullptr;
+  out->recon = o.h + s->o_pic;
+  if (s->dump) {
+    /* the record's header (src/encoder.c:659-704) in front of the cells */
+    uint8_t *g = o.h + s->o_grid - 44;
+    const int32_t hdr[11] = {0x4345444D, o.hdr7[0], o.hdr7[1], o.hdr7[2], o.hdr7[3], o.hdr7[4], o.hdr7[5],
+                             out->fp.base_q_idx, (out->fp.mi_rows + 1) >> 1, (out->fp.mi_cols + 1) >> 1, o.hdr7[6]};
+    memcpy(g, hdr, sizeof hdr);
+    out->decision = g;
+    out->decision_bytes = 44 + s->grid_bytes;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -137,6 +137,18 @@ def _load():
         "mav1hip_decision_record": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, P]),
         "mav1hip_launch_decision_grid": (C.c_int, [P]),
         "mav1hip_lr_select": (C.c_int, [P, C.POINTER(FrameParams), P, P, P, i32, P, P, P]),
+        "mav1hip_seq_create": (P, [C.POINTER(Config), i32, i32, C.c_char_p, C.c_size_t]),
+        "mav1hip_seq_destroy": (None, [P]),
+        "mav1hip_seq_error": (C.c_char_p, [P]),
+        "mav1hip_seq_send_frame": (C.c_int, [P, P, P]),
+        "mav1hip_seq_send_device_frame": (C.c_int, [P, P]),
+        "mav1hip_seq_receive": (C.c_int, [P, P, i32]),
+        "mav1hip_seq_engine": (P, [P, i32]),
+        "mav1hip_y4m_open": (P, [C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                 C.c_char_p, C.c_size_t]),
+        "mav1hip_y4m_read": (C.c_int, [P, P, P]),
+        "mav1hip_y4m_close": (None, [P]),
+        "mav1hip_psnr": (C.c_double, [P, P, C.c_size_t]),
         "mav1hip_lr_select_static": (C.c_int, [C.POINTER(FrameParams), P, P, P, i32, P, P, P, C.c_char_p,
                                                C.c_size_t]),
     }
@@ -223,6 +235,158 @@ class HipError(RuntimeError):
     pass
 
 
+def psnr(a, b):
+    """mav1hip_psnr over two equally sized uint8 arrays."""
+    a = np.ascontiguousarray(a, np.uint8)
+    b = np.ascontiguousarray(b, np.uint8)
+    if a.size != b.size:
+        raise ValueError("psnr: sizes differ")
+    return lib().mav1hip_psnr(ptr(a), ptr(b), a.size)
+
+
+class Y4mReader:
+    """mav1hip_y4m_*: I420 frames of a y4m file as (Y, U, V) arrays."""
+
+    def __init__(self, path):
+        w, h, fn, fd = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        err = C.create_string_buffer(256)
+        self.h_ = lib().mav1hip_y4m_open(path.encode(), C.byref(w), C.byref(h), C.byref(fn), C.byref(fd), err, 256)
+        if not self.h_:
+            raise HipError(err.value.decode())
+        self.width, self.height, self.fps = w.value, h.value, (fn.value, fd.value)
+
+    def read(self):
+        cw, ch = (self.width + 1) >> 1, (self.height + 1) >> 1
+        pl = [np.empty((self.height, self.width), np.uint8), np.empty((ch, cw), np.uint8), np.empty((ch, cw), np.uint8)]
+        arr = (C.c_void_p * 3)(*[p.ctypes.data for p in pl])
+        st = (C.c_int32 * 3)(self.width, cw, cw)
+        rc = lib().mav1hip_y4m_read(self.h_, arr, st)
+        if rc == 1:
+            return None
+        if rc:
+            raise HipError("y4m: truncated or malformed frame")
+        return tuple(pl)
+
+    def close(self):
+        if self.h_:
+            lib().mav1hip_y4m_close(self.h_)
+            self.h_ = None
+
+
+EBUSY = -5
+
+
+class CodedFrame(C.Structure):
+    """Mav1HipCodedFrame: one coded frame of the sequence driver."""
+    _fields_ = [(n, C.c_int32) for n in (
+        "coded_index", "disp", "is_key", "frame_kind", "nonref", "hidden", "order_hint", "refresh_frame_flags",
+        "last_slot", "arf_slot", "last_hint", "arf_hint", "engine")] + [
+        ("fp", FrameParams), ("plan", FilterPlan), ("blocks", C.c_void_p), ("sbs", C.c_void_p),
+        ("coefs", C.c_void_p), ("cdef_choice", C.c_void_p), ("lr_units", C.c_void_p), ("recon", C.c_void_p),
+        ("decision", C.c_void_p), ("decision_bytes", C.c_size_t)]
+
+
+def _host_array(p, dtype, count):
+    if not p:
+        return None
+    return np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dtype))), (count,)).copy()
+
+
+class Sequence:
+    """The sequence driver (mav1hip_seq_*): frames in, in display order;
+    coded frames out, in coding order, with the entropy coder's inputs."""
+
+    def __init__(self, cfg, lr_test=0, dump_decisions=False):
+        err = C.create_string_buffer(512)
+        h = lib().mav1hip_seq_create(C.byref(cfg), lr_test, int(bool(dump_decisions)), err, 512)
+        if not h:
+            raise HipError(err.value.decode())
+        self.h = h
+        self.cfg = cfg
+
+    def close(self):
+        if self.h:
+            lib().mav1hip_seq_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, what, rc):
+        return HipError(f"{what}: {lib().mav1hip_seq_error(self.h).decode()} (rc {rc})")
+
+    def engine(self, i):
+        """Engine i's context (owned by the driver), for the timing calls."""
+        h = lib().mav1hip_seq_engine(self.h, i)
+        if not h:
+            raise HipError(f"no engine {i}")
+        ctx = Context.__new__(Context)
+        ctx.h, ctx.cfg, ctx._borrowed = h, self.cfg, True
+        ctx.frame_bytes = lib().mav1hip_buffer_bytes(h, BUF_SRC)
+        ctx.mv_ints = lib().mav1hip_buffer_bytes(h, BUF_MVFIELD) // 4
+        return ctx
+
+    def send(self, planes):
+        """I420 planes (Y, U, V 2-D uint8 arrays) or None to end the stream.
+        Returns False when the driver needs a receive first (EBUSY)."""
+        if planes is None:
+            rc = lib().mav1hip_seq_send_frame(self.h, None, None)
+        else:
+            pl = [np.ascontiguousarray(p, np.uint8) for p in planes]
+            arr = (C.c_void_p * 3)(*[p.ctypes.data for p in pl])
+            st = (C.c_int32 * 3)(*[p.shape[1] for p in pl])
+            rc = lib().mav1hip_seq_send_frame(self.h, arr, st)
+        if rc == EBUSY:
+            return False
+        if rc:
+            raise self._err("send_frame", rc)
+        return True
+
+    def send_device(self, dev_ptr):
+        """A padded frame resident in device memory (integer address)."""
+        rc = lib().mav1hip_seq_send_device_frame(self.h, C.c_void_p(dev_ptr))
+        if rc == EBUSY:
+            return False
+        if rc:
+            raise self._err("send_device_frame", rc)
+        return True
+
+    def receive_raw(self, wait=True):
+        """The next CodedFrame (its pointers valid until the next call) or
+        None."""
+        out = CodedFrame()
+        rc = lib().mav1hip_seq_receive(self.h, C.byref(out), int(bool(wait)))
+        if rc == 1:
+            return None
+        if rc:
+            raise self._err("receive", rc)
+        return out
+
+    def receive(self, wait=True):
+        """The next coded frame as a dict of numpy copies, or None."""
+        out = self.receive_raw(wait)
+        if out is None:
+            return None
+        fp = out.fp
+        nsb = fp.sb_cols * fp.sb_rows
+        nunits = fp.lr_unit_off[2] + fp.lr_unit_rows[2] * fp.lr_unit_cols[2]
+        fb = fp.offset[2] + fp.stride[2] * fp.plane_h[2]
+        d = {n: getattr(out, n) for n, _ in CodedFrame._fields_[:13]}
+        d["fp"] = FrameParams.from_buffer_copy(fp)
+        d["plan"] = FilterPlan.from_buffer_copy(out.plan)
+        d["blocks"] = _host_array(out.blocks, np.int32, nsb * 64 * 27).reshape(-1, 27)
+        d["sbs"] = _host_array(out.sbs, np.int32, nsb * 89).reshape(-1, 89)
+        d["coefs"] = _host_array(out.coefs, np.int32, nsb * 6144)
+        d["cdef_choice"] = _host_array(out.cdef_choice, np.int8, nsb)
+        d["lr_units"] = None if not out.lr_units else _host_array(out.lr_units, np.int32, nunits * 12).reshape(-1, 12)
+        d["recon"] = _host_array(out.recon, np.uint8, fb)
+        d["decision"] = None if not out.decision else _host_array(out.decision, np.uint8, out.decision_bytes)
+        return d
+
+
 class Context:
     """A device context for frames of one size (mav1hip_create)."""
 
@@ -237,9 +401,9 @@ class Context:
         self.mv_ints = lib().mav1hip_buffer_bytes(h, BUF_MVFIELD) // 4
 
     def close(self):
-        if self.h:
+        if self.h and not getattr(self, "_borrowed", False):
             lib().mav1hip_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

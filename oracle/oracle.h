@@ -1,11 +1,15 @@
 /* ORACLE — test infrastructure only.
  *
- * A plain-C, single-threaded restatement of the mav1 analysis path
+ * A plain-C restatement of the mav1 analysis path
  * (the reference's src/kernels/k_*.h, backend_cpu.c and the host searches
  * of encoder.c / lr_search.c), written for this project to check the HIP
  * backend against. Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it, and only as the checker: the product path
  * (libmav1hip.so) never links or calls it.
+ *
+ * Independent superblocks, rows and units run on OpenMP threads; the
+ * results do not depend on the thread count (every parallel loop writes
+ * disjoint outputs, and the serial dependencies stay serial).
  *
  * Every function cites the reference file:line whose behaviour it follows.
  * The restatement is pinned against the reference itself: oracle/Makefile

@@ -174,6 +174,8 @@ LR_CASES = {
     "full_rd_136x104_test": ((136, 104, "inter", 7, {"turbo": 0, "multi_ref": 1}), 1),
     "key_204x116": ((204, 116, "key", 5, {"turbo": 0, "q": 60}), 0),
     "q80_arf_640x360": ((640, 360, "arf", 13, {"q": 80}), 0),
+    "q125_arf_1920x1080": ((1920, 1080, "arf", 21, {}), 0),
+    "q170_arf_1920x1080_test": ((1920, 1080, "arf", 22, {"q": 170}), 1),
 }
 
 
