@@ -29,6 +29,7 @@ struct mav1hip_ctx {
   int stride[3], offset[3], plane_h[3];
   size_t frame_bytes;
   int device;
+  int own_streams; /* the streams were created for the context (else the caller keeps them) */
   hipStream_t stream;
   /* Analysis runs on `side` when motion search of the same frame was the
    * last work enqueued: analysis level k waits for me_done[k] only, so it
@@ -164,7 +165,28 @@ static int dmalloc(mav1hip_ctx *c, void **p, size_t bytes) {
 
 extern "C" {
 
+static mav1hip_ctx *create_ctx(const Mav1HipConfig *cfg, hipStream_t extMain, hipStream_t extSide, char *err,
+                               size_t errlen);
+
 mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) {
+  return create_ctx(cfg, nullptr, nullptr, err, errlen);
+}
+
+}  // extern "C"
+
+/* A context on two streams the caller keeps (the sequence driver's
+ * process-wide streams): they are neither created nor destroyed here. */
+mav1hip_ctx *ctx_create_on_streams(const Mav1HipConfig *cfg, hipStream_t main, hipStream_t side, char *err,
+                                   size_t errlen) {
+  if (!main || !side) {
+    if (err && errlen) snprintf(err, errlen, "no streams");
+    return nullptr;
+  }
+  return create_ctx(cfg, main, side, err, errlen);
+}
+
+static mav1hip_ctx *create_ctx(const Mav1HipConfig *cfg, hipStream_t extMain, hipStream_t extSide, char *err,
+                               size_t errlen) {
   if (!cfg) {
     if (err && errlen) snprintf(err, errlen, "no configuration");
     return nullptr;
@@ -187,10 +209,15 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   }
   c->num_sbs = c->sb_cols * c->sb_rows;
   int rc = MAV1HIP_OK;
-  if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-      hipStreamCreate(&c->side) != hipSuccess) {
+  c->own_streams = extMain == nullptr;
+  if (!c->own_streams) {
+    c->stream = extMain;
+    c->side = extSide;
+  }
+  if (hipGetDevice(&c->device) != hipSuccess ||
+      (c->own_streams && (hipStreamCreate(&c->stream) != hipSuccess || hipStreamCreate(&c->side) != hipSuccess))) {
     if (err && errlen) snprintf(err, errlen, "cannot create a HIP stream");
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->own_streams && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return nullptr;
   }
@@ -200,8 +227,10 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
     if (hipEventCreateWithFlags(evs[i], hipEventDisableTiming) != hipSuccess) {
       if (err && errlen) snprintf(err, errlen, "cannot create a HIP event");
       for (int j = 0; j < i; j++) (void)hipEventDestroy(*evs[j]);
-      (void)hipStreamDestroy(c->side);
-      (void)hipStreamDestroy(c->stream);
+      if (c->own_streams) {
+        (void)hipStreamDestroy(c->side);
+        (void)hipStreamDestroy(c->stream);
+      }
       delete c;
       return nullptr;
     }
@@ -313,6 +342,8 @@ mav1hip_ctx *mav1hip_create(const Mav1HipConfig *cfg, char *err, size_t errlen) 
   return c;
 }
 
+extern "C" {
+
 void mav1hip_destroy(mav1hip_ctx *c) {
   if (!c) return;
   if (c->side) (void)hipStreamSynchronize(c->side);
@@ -357,8 +388,8 @@ void mav1hip_destroy(mav1hip_ctx *c) {
   (void)hipEventDestroy(c->xfer);
   (void)hipEventDestroy(c->rec_go);
   (void)hipEventDestroy(c->sched_done);
-  if (c->side) (void)hipStreamDestroy(c->side);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->own_streams && c->side) (void)hipStreamDestroy(c->side);
+  if (c->own_streams && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -957,6 +988,7 @@ int mav1hip_timing_get(mav1hip_ctx *c, const char *kernel, int32_t *launches, do
 
 /* Library-internal access for the sequence driver (seq.hip). */
 hipStream_t ctx_stream(mav1hip_ctx *c) { return c->stream; }
+hipStream_t ctx_side_stream(mav1hip_ctx *c) { return c->side; }
 int ctx_device(const mav1hip_ctx *c) { return c->device; }
 size_t ctx_frame_bytes(const mav1hip_ctx *c) { return c->frame_bytes; }
 void *ctx_buffer(mav1hip_ctx *c, int buf, size_t *bytes) { return buf_ptr(c, buf, bytes); }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/mav1_hip.h"
 #include "../../include/mav1_types.h"
 
 /* Motion-search planes of one reference: 16 quarter-pel planes
@@ -85,8 +86,12 @@ hipError_t launch_lr_chain(hipStream_t st, const Mav1FrameParams &fp, const uint
                            int *reruns, void *chain, int stop);
 
 /* api.hip: library-internal access to a context for the sequence driver */
-struct mav1hip_ctx;
+/* A context on two streams the caller keeps (neither created nor destroyed
+ * by the context). */
+mav1hip_ctx *ctx_create_on_streams(const Mav1HipConfig *cfg, hipStream_t main, hipStream_t side, char *err,
+                                   size_t errlen);
 hipStream_t ctx_stream(mav1hip_ctx *c);
+hipStream_t ctx_side_stream(mav1hip_ctx *c);
 int ctx_device(const mav1hip_ctx *c);
 size_t ctx_frame_bytes(const mav1hip_ctx *c);
 void *ctx_buffer(mav1hip_ctx *c, int buf, size_t *bytes);

@@ -18,15 +18,22 @@
  *    one of its pictures and the generation of the slot it holds. A frame
  *    that refreshes a slot stores its final picture into a picture no slot
  *    of its engine maps to; an engine that reads a slot coded elsewhere
- *    copies it once on the transfer stream, ordered by events after the
- *    picture's writer and before the reader and any later writer;
+ *    copies it once on its own stream, after an event that completes the
+ *    picture on the engine that wrote it. That engine waits for the copy
+ *    only when it next writes the picture (a per-picture read event), so it
+ *    never waits for the reading engine's earlier frames;
+ *  - the driver uses four streams per device, created once per process
+ *    (the GPU runs four hardware queues per process, and a stream gets its
+ *    queue when it is created): engine 0's main and side streams, engine
+ *    1's; uploads and downloads run on engine 1's side stream, which the
+ *    critical ALTREF chain never waits on;
  *  - a frame of the second engine is enqueued after the next ALTREF's
  *    motion search, analysis and edge types, so it runs beside that
  *    ALTREF's reconstruction (whose intra phase leaves most CUs idle);
  *  - every frame's outputs are copied on its engine's stream into a device
- *    staging slot, then to pinned host memory on the download stream, so
- *    the engines never wait for PCIe; host-sent frames go up on the upload
- *    stream into a per-engine ring of source frames.
+ *    staging slot (one launch), then to pinned host memory on the transfer
+ *    stream, so the engines never wait for PCIe; host-sent frames go up on
+ *    the transfer stream into a per-engine ring of source frames.
  * All streams have the default priority. */
 #include <hip/hip_runtime.h>
 
@@ -36,6 +43,7 @@
 
 #include <algorithm>
 #include <deque>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mav1_hip.h"
@@ -44,6 +52,7 @@
 namespace {
 
 constexpr int kEngines = 2;
+constexpr int kMaxDevices = 64;
 constexpr int kPics = 10; /* pictures per engine: 8 slots, the frame's own store, one spare */
 constexpr int kSrcRing = 4;
 
@@ -89,6 +98,8 @@ struct Engine {
   int ring_used[kSrcRing] = {};
   int ring_next = 0;
   hipEvent_t ev = nullptr; /* scratch */
+  hipEvent_t pic_read[kPics] = {}; /* the last copy of the picture to the other engine */
+  int pic_pend[kPics] = {};
 };
 
 }  // namespace
@@ -102,8 +113,7 @@ struct mav1hip_seq {
   size_t grid_bytes = 0;
   int32_t stride[3], offset[3], plane_h[3];
   Engine eng[kEngines];
-  hipStream_t up = nullptr, down = nullptr, xfer = nullptr;
-  hipEvent_t xev = nullptr;
+  hipStream_t tx = nullptr; /* transfers: the second engine's side stream */
   /* host staging of sent frames */
   std::vector<uint8_t *> stage;
   std::vector<hipEvent_t> stage_done;
@@ -193,18 +203,20 @@ int slot_pic(mav1hip_seq *s, int e, int k, const int *avoid, int navoid, int *id
   E.map[k] = -1;
   const int b = free_pic(s, e, avoid, navoid);
   if (b < 0) return seq_fail(s, MAV1HIP_EINVAL, "engine %d: no free picture", e);
-  /* the copy after everything the owner enqueued so far (the picture is
-   * complete), before this engine's next work and before the owner's later
-   * work (which may overwrite the picture) */
+  /* the copy on this engine's stream, after everything the owner enqueued
+   * so far (the picture is complete); the owner waits for it only before it
+   * next writes the picture */
   Engine &O = s->eng[o];
+  if (E.pic_pend[b]) {
+    SCHK(s, hipStreamWaitEvent(E.st, E.pic_read[b], 0));
+    E.pic_pend[b] = 0;
+  }
+  const int from = O.map[k];
   SCHK(s, hipEventRecord(O.ev, O.st));
-  SCHK(s, hipStreamWaitEvent(s->xfer, O.ev, 0));
-  SCHK(s, hipEventRecord(E.ev, E.st));
-  SCHK(s, hipStreamWaitEvent(s->xfer, E.ev, 0));
-  SCHK(s, hipMemcpyAsync(pic(s, e, b), pic(s, o, O.map[k]), s->frame_bytes, hipMemcpyDeviceToDevice, s->xfer));
-  SCHK(s, hipEventRecord(s->xev, s->xfer));
-  SCHK(s, hipStreamWaitEvent(E.st, s->xev, 0));
-  SCHK(s, hipStreamWaitEvent(O.st, s->xev, 0));
+  SCHK(s, hipStreamWaitEvent(E.st, O.ev, 0));
+  SCHK(s, hipMemcpyAsync(pic(s, e, b), pic(s, o, from), s->frame_bytes, hipMemcpyDeviceToDevice, E.st));
+  SCHK(s, hipEventRecord(O.pic_read[from], E.st));
+  O.pic_pend[from] = 1;
   E.map[k] = b;
   E.have[k] = g;
   *idx = b;
@@ -276,10 +288,10 @@ int bind_source(mav1hip_seq *s, Frame &f) {
   }
   const int r = E.ring_next;
   E.ring_next = (E.ring_next + 1) % kSrcRing;
-  if (E.ring_used[r]) SCHK(s, hipStreamWaitEvent(s->up, E.ring_free[r], 0));
+  if (E.ring_used[r]) SCHK(s, hipStreamWaitEvent(s->tx, E.ring_free[r], 0));
   uint8_t *dst = E.ring + (size_t)r * s->frame_bytes;
-  SCHK(s, hipMemcpyAsync(dst, s->stage[f.in.stage], s->frame_bytes, hipMemcpyHostToDevice, s->up));
-  SCHK(s, hipEventRecord(s->stage_done[f.in.stage], s->up));
+  SCHK(s, hipMemcpyAsync(dst, s->stage[f.in.stage], s->frame_bytes, hipMemcpyHostToDevice, s->tx));
+  SCHK(s, hipEventRecord(s->stage_done[f.in.stage], s->tx));
   s->stage_busy[f.in.stage] = 2; /* reusable once stage_done has completed */
   SCHK(s, hipStreamWaitEvent(E.st, s->stage_done[f.in.stage], 0));
   E.ring_used[r] = 1;
@@ -315,12 +327,58 @@ int early_stages(mav1hip_seq *s, Frame &f) {
   return MAV1HIP_OK;
 }
 
-int stage_copy(mav1hip_seq *s, Engine &E, uint8_t *dst, int buf, size_t bytes) {
+/* The frame's outputs into the staging slot in one launch: each segment a
+ * device buffer of the engine copied into the slot (and the final picture
+ * also into the picture that a refreshed slot will map to), 16 bytes per
+ * thread where source and destinations are 16-byte aligned. */
+constexpr int kMaxSegs = 10;
+struct StageSeg {
+  const uint8_t *src;
+  uint8_t *dst, *dst2;
+  unsigned long long bytes;
+};
+struct StageList {
+  StageSeg seg[kMaxSegs];
+  unsigned long long start[kMaxSegs + 1]; /* prefix sums of the sizes rounded up to 16 */
+  int n;
+};
+
+__device__ __forceinline__ bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+__global__ __launch_bounds__(256) void k_stage(const StageList L) {
+  const unsigned long long total = L.start[L.n], step = (unsigned long long)gridDim.x * 256 * 16;
+  for (unsigned long long o = ((unsigned long long)blockIdx.x * 256 + threadIdx.x) * 16; o < total; o += step) {
+    int s = 0;
+    while (o >= L.start[s + 1]) s++;
+    const StageSeg &g = L.seg[s];
+    const unsigned long long r = o - L.start[s];
+    if (r >= g.bytes) continue;
+    if (r + 16 <= g.bytes && al16(g.src + r) && al16(g.dst + r) && (!g.dst2 || al16(g.dst2 + r))) {
+      const uint4 v = *(const uint4 *)(g.src + r);
+      *(uint4 *)(g.dst + r) = v;
+      if (g.dst2) *(uint4 *)(g.dst2 + r) = v;
+    } else {
+      const unsigned long long e = r + 16 < g.bytes ? r + 16 : g.bytes;
+      for (unsigned long long q = r; q < e; q++) {
+        g.dst[q] = g.src[q];
+        if (g.dst2) g.dst2[q] = g.src[q];
+      }
+    }
+  }
+}
+
+int stage_add(mav1hip_seq *s, StageList &L, const void *src, uint8_t *dst, uint8_t *dst2, size_t bytes) {
+  if (!src || L.n >= kMaxSegs) return seq_fail(s, MAV1HIP_EINVAL, "staging: bad segment");
+  L.seg[L.n] = StageSeg{(const uint8_t *)src, dst, dst2, bytes};
+  L.start[L.n + 1] = L.start[L.n] + ((bytes + 15) & ~(size_t)15);
+  L.n++;
+  return MAV1HIP_OK;
+}
+int stage_buf(mav1hip_seq *s, StageList &L, Engine &E, int buf, uint8_t *dst, uint8_t *dst2, size_t bytes) {
   size_t cap = 0;
   void *src = ctx_buffer(E.ctx, buf, &cap);
   if (!src || bytes > cap) return seq_fail(s, MAV1HIP_EINVAL, "staging buffer %d", buf);
-  SCHK(s, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, E.st));
-  return MAV1HIP_OK;
+  return stage_add(s, L, src, dst, dst2, bytes);
 }
 
 /* Reconstruction and the in-loop filters (run_frame_core,
@@ -353,35 +411,52 @@ int late_stages(mav1hip_seq *s, Frame &f) {
     fin = MAV1HIP_BUF_LROUT;
   }
   if (s->dump) CCHK(s, e, mav1hip_launch_decision_grid(c));
-  /* outputs into the staging slot, then down on the download stream */
+  /* outputs into the staging slot (the final picture also into the picture
+   * its refreshed slots will map to), then down on the download stream */
   OutSlot &o = s->outs[f.out];
   int rc;
+  int b = -1;
+  if (f.refresh) {
+    b = free_pic(s, e, nullptr, 0);
+    if (b < 0) return seq_fail(s, MAV1HIP_EINVAL, "engine %d: no free picture", e);
+    if (E.pic_pend[b]) { /* a copy to the other engine still reads it */
+      SCHK(s, hipStreamWaitEvent(E.st, E.pic_read[b], 0));
+      E.pic_pend[b] = 0;
+    }
+  }
+  StageList SL;
+  SL.n = 0;
+  SL.start[0] = 0;
   const size_t nb = sizeof(Mav1BlockInfo) * MAV1_MAX_BLOCKS_PER_SB * (size_t)s->num_sbs;
-  if ((rc = stage_copy(s, E, o.d + s->o_blocks, MAV1HIP_BUF_BLOCKS, nb))) return rc;
-  if ((rc = stage_copy(s, E, o.d + s->o_sbs, MAV1HIP_BUF_SBS, sizeof(Mav1SbInfo) * (size_t)s->num_sbs))) return rc;
-  if ((rc = stage_copy(s, E, o.d + s->o_coefs, MAV1HIP_BUF_COEFS,
-                       sizeof(int32_t) * MAV1_COEFS_PER_SB * (size_t)s->num_sbs)))
+  if ((rc = stage_buf(s, SL, E, MAV1HIP_BUF_BLOCKS, o.d + s->o_blocks, nullptr, nb))) return rc;
+  if ((rc = stage_buf(s, SL, E, MAV1HIP_BUF_SBS, o.d + s->o_sbs, nullptr, sizeof(Mav1SbInfo) * (size_t)s->num_sbs)))
     return rc;
-  if ((rc = stage_copy(s, E, o.d + s->o_choice, MAV1HIP_BUF_CDEF_CHOICE, (size_t)s->num_sbs))) return rc;
+  if ((rc = stage_buf(s, SL, E, MAV1HIP_BUF_COEFS, o.d + s->o_coefs, nullptr,
+                      sizeof(int32_t) * MAV1_COEFS_PER_SB * (size_t)s->num_sbs)))
+    return rc;
+  if ((rc = stage_buf(s, SL, E, MAV1HIP_BUF_CDEF_CHOICE, o.d + s->o_choice, nullptr, (size_t)s->num_sbs))) return rc;
   if (pl.restoration &&
-      (rc = stage_copy(s, E, o.d + s->o_units, MAV1HIP_BUF_LRUNITS, sizeof(Mav1LrUnit) * (size_t)s->lr_units)))
+      (rc = stage_buf(s, SL, E, MAV1HIP_BUF_LRUNITS, o.d + s->o_units, nullptr, sizeof(Mav1LrUnit) * (size_t)s->lr_units)))
     return rc;
-  if ((rc = stage_copy(s, E, o.d + s->o_pic, fin, s->frame_bytes))) return rc;
-  if (s->dump && (rc = stage_copy(s, E, o.d + s->o_grid, MAV1HIP_BUF_DECGRID, s->grid_bytes))) return rc;
+  if ((rc = stage_buf(s, SL, E, fin, o.d + s->o_pic, b >= 0 ? pic(s, e, b) : nullptr, s->frame_bytes))) return rc;
+  if (s->dump && (rc = stage_buf(s, SL, E, MAV1HIP_BUF_DECGRID, o.d + s->o_grid, nullptr, s->grid_bytes))) return rc;
   /* the restoration types chosen on the device, and the intra phase's error word */
-  SCHK(s, hipMemcpyAsync(o.d + s->o_lrtype, ctx_lrtype(c), sizeof(int) * 3, hipMemcpyDeviceToDevice, E.st));
-  SCHK(s, hipMemcpyAsync(o.d + s->o_err, ctx_intra_error(c), sizeof(int), hipMemcpyDeviceToDevice, E.st));
+  if ((rc = stage_add(s, SL, ctx_lrtype(c), o.d + s->o_lrtype, nullptr, sizeof(int) * 3))) return rc;
+  if ((rc = stage_add(s, SL, ctx_intra_error(c), o.d + s->o_err, nullptr, sizeof(int)))) return rc;
+  {
+    const unsigned long long blocks16 = (SL.start[SL.n] / 16 + 255) / 256;
+    const int grid = (int)(blocks16 < 1024 ? blocks16 : 1024);
+    hipLaunchKernelGGL(k_stage, dim3(grid), dim3(256), 0, E.st, SL);
+    SCHK(s, hipGetLastError());
+  }
   SCHK(s, hipEventRecord(o.staged, E.st));
-  SCHK(s, hipStreamWaitEvent(s->down, o.staged, 0));
-  SCHK(s, hipMemcpyAsync(o.h, o.d, s->o_end, hipMemcpyDeviceToHost, s->down));
-  SCHK(s, hipEventRecord(o.landed, s->down));
+  SCHK(s, hipStreamWaitEvent(s->tx, o.staged, 0));
+  SCHK(s, hipMemcpyAsync(o.h, o.d, s->o_end, hipMemcpyDeviceToHost, s->tx));
+  SCHK(s, hipEventRecord(o.landed, s->tx));
   /* the source ring entry is free once this frame's stages have read it */
   if (f.src_ring >= 0) SCHK(s, hipEventRecord(E.ring_free[f.src_ring], E.st));
-  /* refreshed slots: the final picture into a picture no slot maps to */
+  /* refreshed slots now map to that picture */
   if (f.refresh) {
-    const int b = free_pic(s, e, nullptr, 0);
-    if (b < 0) return seq_fail(s, MAV1HIP_EINVAL, "engine %d: no free picture", e);
-    CCHK(s, e, ctx_store(c, pic(s, e, b), fin));
     s->gen++;
     for (int k = 0; k < 8; k++)
       if ((f.refresh >> k) & 1) {
@@ -531,9 +606,7 @@ void destroy(mav1hip_seq *s) {
   if (!s) return;
   for (auto &E : s->eng)
     if (E.st) (void)hipStreamSynchronize(E.st);
-  if (s->up) (void)hipStreamSynchronize(s->up);
-  if (s->down) (void)hipStreamSynchronize(s->down);
-  if (s->xfer) (void)hipStreamSynchronize(s->xfer);
+  if (s->tx) (void)hipStreamSynchronize(s->tx);
   for (auto &E : s->eng) {
     if (E.ctx) mav1hip_destroy(E.ctx);
     (void)hipFree(E.pics);
@@ -541,6 +614,8 @@ void destroy(mav1hip_seq *s) {
     for (auto &ev : E.ring_free)
       if (ev) (void)hipEventDestroy(ev);
     if (E.ev) (void)hipEventDestroy(E.ev);
+    for (auto &ev : E.pic_read)
+      if (ev) (void)hipEventDestroy(ev);
   }
   for (auto &o : s->outs) {
     (void)hipFree(o.d);
@@ -552,22 +627,37 @@ void destroy(mav1hip_seq *s) {
     if (p) (void)hipHostFree(p);
   for (auto ev : s->stage_done)
     if (ev) (void)hipEventDestroy(ev);
-  if (s->xev) (void)hipEventDestroy(s->xev);
-  if (s->up) (void)hipStreamDestroy(s->up);
-  if (s->down) (void)hipStreamDestroy(s->down);
-  if (s->xfer) (void)hipStreamDestroy(s->xfer);
   delete s;
 }
 
+/* The driver's four streams per device (engine 0 main and side, engine 1
+ * main and side), created once per process and shared by every sequence on
+ * the device: the GPU runs four hardware queues per process and assigns a
+ * stream its queue when the stream is created, so streams created later
+ * could share a queue with the ALTREF engine's. Sequences on one device
+ * therefore run one after another on these streams. */
+std::mutex g_streams_mu;
+hipStream_t g_streams[kMaxDevices][2 * kEngines];
+
+int device_streams(mav1hip_seq *s, hipStream_t **out) {
+  int dev = 0;
+  SCHK(s, hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return seq_fail(s, MAV1HIP_EDEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_streams_mu);
+  for (int i = 0; i < 2 * kEngines; i++)
+    if (!g_streams[dev][i]) SCHK(s, hipStreamCreate(&g_streams[dev][i]));
+  *out = g_streams[dev];
+  return MAV1HIP_OK;
+}
+
 int setup(mav1hip_seq *s) {
-  SCHK(s, hipStreamCreate(&s->up));
-  SCHK(s, hipStreamCreate(&s->down));
-  SCHK(s, hipStreamCreate(&s->xfer));
-  SCHK(s, hipEventCreateWithFlags(&s->xev, hipEventDisableTiming));
+  hipStream_t *st = nullptr;
+  int rc = device_streams(s, &st);
+  if (rc) return rc;
   for (int e = 0; e < kEngines; e++) {
     Engine &E = s->eng[e];
     char err[512] = {0};
-    E.ctx = mav1hip_create(&s->cfg, err, sizeof err);
+    E.ctx = ctx_create_on_streams(&s->cfg, st[2 * e], st[2 * e + 1], err, sizeof err);
     if (!E.ctx) return seq_fail(s, MAV1HIP_EDEVICE, "engine %d: %s", e, err);
     E.st = ctx_stream(E.ctx);
     SCHK(s, hipMalloc((void **)&E.pics, s->frame_bytes * kPics));
@@ -575,11 +665,13 @@ int setup(mav1hip_seq *s) {
     SCHK(s, hipMalloc((void **)&E.ring, s->frame_bytes * kSrcRing));
     for (auto &ev : E.ring_free) SCHK(s, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     SCHK(s, hipEventCreateWithFlags(&E.ev, hipEventDisableTiming));
+    for (auto &ev : E.pic_read) SCHK(s, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     for (int k = 0; k < 8; k++) {
       E.map[k] = -1;
       E.have[k] = 0;
     }
   }
+  s->tx = ctx_side_stream(s->eng[1].ctx);
   for (int k = 0; k < 8; k++) {
     s->slot_owner[k] = -1;
     s->slot_hint[k] = 0;
