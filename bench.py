@@ -100,10 +100,15 @@ def algorithmic_bytes(kernel, fp, nref):
 BW = [4, 4, 8, 8, 8, 16, 16, 16, 32, 32, 32, 64, 64, 64, 128, 128, 4, 16, 8, 32, 16, 64]
 BH = [4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 64, 32, 64, 128, 64, 128, 16, 4, 32, 8, 64, 16]
 
-# HBM traffic per launch of the dominant kernel over the timed frames of the
-# default run, from the FETCH_SIZE and WRITE_SIZE passes of the committed
-# rocprofv3 summary (profiles/), scaled by the frame copy's factors there.
-TRAFFIC = {}
+# HBM traffic per launch of the dominant kernel over the timed stream of the
+# default run, from the FETCH_SIZE and WRITE_SIZE passes of
+# profiles/recon_intra_1080p_turbo_stream_pmc.md, scaled by the known-size
+# copy's factors there (fetch x1.9992, write x1.0).
+TRAFFIC = {"recon_intra": 10252850}
+# VALUBusy (%) per launch over the timed stream, from the counter pass of the
+# same summary (100 * SQ_ACTIVE_INST_VALU / CU_NUM / GRBM_GUI_ACTIVE).
+VALU_BUSY = {"recon_intra": 0.05, "lr_fit": 6.11, "recon_inter": 2.15, "me_level0": 2.40, "me_level2": 5.05,
+             "me_level3": 9.39, "analyze_level0": 3.55, "qpel_build": 6.44, "cdef": 7.44}
 
 
 def intra_phase_bytes(fp, blocks, sbs):
@@ -138,7 +143,12 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 64 MiB device-to-device copy 5 times (the HBM counters' scale)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
@@ -148,10 +158,10 @@ def main():
     # deblocking + CDEF + loop restoration on (turbo turns restoration off unless asked for)
     cfg = mav1hip.default_config(W, H, turbo=1, qindex=Q, restoration=1, keyint=50)
     geo = synth.Geometry(W, H, cfg.tile_cols_log2)
-    gen = synth.config_synth(2, seed=2024)
     n = max(args.steps, args.warmup, args.cpu_frames)
-    planes = [gen.frame(t) for t in range(n)]
+    planes = generate_frames(n)
     padded = [synth.pad_frame(geo, p) for p in planes]
+    sha_ok = check_frame_hashes(planes)
     dev = [torch.from_numpy(f).to("cuda:0") for f in padded]
     torch.cuda.synchronize()
 
@@ -283,6 +293,7 @@ def main():
                        "encoders",
             "device": devname[0],
         },
+        "data_sha256_ok": sha_ok,
         "steady_fps": None if steady is None else round(steady, 3),
         "pcie_inclusive_fps": pcie,
         "kernels_ms_per_launch": {k: round(v[1] / max(v[0], 1), 5) for k, v in stats.items()},
@@ -300,6 +311,9 @@ def main():
             "algorithmic_bytes_per_launch": int(per_launch),
             "avg_launch_ms": round(avg_s * 1e3, 5),
             "launches": n_launch,
+            "valu_busy_pct": VALU_BUSY.get(dom),
+            "valu_busy_pct_other_kernels": {k: v for k, v in VALU_BUSY.items() if k != dom},
+            "counters": "profiles/recon_intra_1080p_turbo_stream_pmc.md",
             "note": ("latency-bound dependency chain (superblocks in dataflow order, blocks in decode order), "
                      "integer VALU, no MFMA: the HBM fraction is small by construction" if dom == "recon_intra" else
                      "integer-VALU bound, no MFMA: the HBM fraction is small by construction"),
@@ -307,6 +321,48 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
+
+
+def _gen_frame(t):
+    return synth.config_synth(2, seed=2024).frame(t)
+
+
+def generate_frames(n):
+    """The stream's first n frames (config-2 generator, seed 2024), made in
+    worker processes (no GPU is touched before they finish)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    workers = max(1, min(8, os.cpu_count() or 1, n))
+    with cf.ProcessPoolExecutor(workers, mp_context=mp.get_context("fork")) as ex:
+        return list(ex.map(_gen_frame, range(n)))
+
+
+def check_frame_hashes(planes):
+    """Whether the generated frames are the ones whose SHA-256 is committed
+    (tests/golden/synth_sha256.json): the same input on every host."""
+    import hashlib
+    path = os.path.join(ROOT, "tests", "golden", "synth_sha256.json")
+    want = json.load(open(path))["config2_seed2024"]["frames"]
+    for t, p in enumerate(planes[:len(want)]):
+        h = hashlib.sha256()
+        for a in p:
+            h.update(a.tobytes())
+        if h.hexdigest() != want[t]:
+            return False
+    return True
+
+
+def copy_calibration():
+    """A device-to-device copy of 64 MiB, five times: FETCH_SIZE and
+    WRITE_SIZE of this kernel over its known 64 MiB read and written give
+    the counters' scale."""
+    import torch
+    a = torch.ones(64 << 20, dtype=torch.uint8, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": 64 << 20, "launches": 5}))
 
 
 def cpu_baseline(cfg, padded, nframes, runs):
