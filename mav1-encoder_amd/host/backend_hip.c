@@ -96,6 +96,9 @@ static void hip_frame_core(Backend *b, Mav1Encoder *e) {
   if (!rc) rc = mav1hip_download(c, MAV1HIP_BUF_SBS, e->sbs.base, mav1hip_buffer_bytes(c, MAV1HIP_BUF_SBS));
   if (!rc) rc = mav1hip_download(c, MAV1HIP_BUF_COEFS, e->coefs.base, mav1hip_buffer_bytes(c, MAV1HIP_BUF_COEFS));
   if (!rc) rc = mav1hip_download(c, MAV1HIP_BUF_REC, e->rec.base, fb);
+  /* both streams drained, then the intra phase's error word: a dependency
+   * wait that ran out leaves the reconstruction wrong */
+  if (!rc) rc = mav1hip_sync(c);
   check(b, rc, "frame_core");
 }
 
