@@ -392,6 +392,36 @@ int mav1hip_seq_receive(mav1hip_seq *s, Mav1HipCodedFrame *out, int32_t wait);
 mav1hip_ctx *mav1hip_seq_engine(mav1hip_seq *s, int32_t i);
 
 /* ------------------------------------------------------------------------
+ * Batched streams: several sequences coded together on one GPU
+ * ------------------------------------------------------------------------
+ * No counterpart in the reference, which batches only within a stream (its
+ * engines, src/encoder.c:1150-1224): n independent streams, each with its
+ * own configuration (sizes may differ) and its own driver state, engines,
+ * reference pictures and outputs, as n mav1hip_seq would have. Their frames
+ * at the same position of their coding order are coded as one round: every
+ * stage is one launch over all of the round's frames (a table of per-frame
+ * parameters and buffers in device memory; the persistent inter and intra
+ * phases take work from every frame of the round). Each stream's coded
+ * frames equal what its own mav1hip_seq codes from the same frames.
+ * Rounds form when every stream that can still code has a frame ready:
+ * feed the streams in step (frame t to every stream, then frame t + 1).
+ * A receive that waits runs the frames that are ready without waiting for
+ * the other streams. At most 32 streams. */
+typedef struct mav1hip_batch mav1hip_batch;
+mav1hip_batch *mav1hip_batch_create(const Mav1HipConfig *cfgs, int32_t n, int32_t lr_test, int32_t dump_decisions,
+                                    char *err, size_t errlen);
+void mav1hip_batch_destroy(mav1hip_batch *b);
+const char *mav1hip_batch_error(const mav1hip_batch *b);
+/* As mav1hip_seq_send_frame / _send_device_frame / _receive for stream
+ * `stream` (0 .. n - 1). */
+int mav1hip_batch_send_frame(mav1hip_batch *b, int32_t stream, const uint8_t *const planes[3],
+                             const int32_t strides[3]);
+int mav1hip_batch_send_device_frame(mav1hip_batch *b, int32_t stream, const uint8_t *frame);
+int mav1hip_batch_receive(mav1hip_batch *b, int32_t stream, Mav1HipCodedFrame *out, int32_t wait);
+/* The sequence of stream i (owned by the batch; for mav1hip_seq_engine). */
+mav1hip_seq *mav1hip_batch_stream(mav1hip_batch *b, int32_t stream);
+
+/* ------------------------------------------------------------------------
  * Input side and quality measure (host only)
  * ------------------------------------------------------------------------ */
 /* The y4m reader of the command-line front end (tools/mav1enc.c:67-77):

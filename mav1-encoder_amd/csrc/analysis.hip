@@ -117,11 +117,7 @@ struct AnaLds {
 };
 
 template <int LEVEL>
-__global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_analyze_level(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
-                                                      const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold,
-                                                      const int32_t *__restrict__ mvf, const int32_t *__restrict__ mvg,
-                                                      long long *__restrict__ costs, Mav1BlockInfo *__restrict__ infos,
-                                                      int nslots) {
+DEVI void analyze_level_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold, const int32_t *__restrict__ mvf, const int32_t *__restrict__ mvg, long long *__restrict__ costs, Mav1BlockInfo *__restrict__ infos, int nslots, const int blk_x, const int blk_y, const int blk_z) {
   constexpr int BW = 64 >> LEVEL, N = 1 << LEVEL;
   /* threads per item: four waves for the 64x64 level (510 items at 1080p),
    * two for the 32x32 level, one below */
@@ -129,7 +125,7 @@ __global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_an
   __shared__ AnaLds<BW> L;
   const int lane = threadIdx.x;
   const int perSb = N * N * nslots;
-  const int sb = blockIdx.x / perSb, rem = blockIdx.x - sb * perSb;
+  const int sb = blk_x / perSb, rem = blk_x - sb * perSb;
   const int i = rem / nslots, slot = rem - i * nslots;
   const int sbRow = sb / fp.sb_cols, sbCol = sb - sbRow * fp.sb_cols;
   const int size4 = 16 >> LEVEL, half = size4 >> 1, node = d_node_base(LEVEL) + i;
@@ -391,6 +387,14 @@ __global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_an
     infos[idx] = b;
   }
 }
+template <int LEVEL>
+__global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_analyze_level(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+                                                      const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold,
+                                                      const int32_t *__restrict__ mvf, const int32_t *__restrict__ mvg,
+                                                      long long *__restrict__ costs, Mav1BlockInfo *__restrict__ infos,
+                                                      int nslots) {
+  analyze_level_body<LEVEL>(fp, src, ref, gold, mvf, mvg, costs, infos, nslots, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 /* Pass 2 (analyze_sb_table): one wave per superblock. The bottom-up DP runs
  * level by level with the lanes over the level's nodes (a node only reads
@@ -408,12 +412,10 @@ DEVI int wave_excl_scan(int v, int lane) {
   return x - v;
 }
 
-__global__ __launch_bounds__(64) void k_analyze_table(const Mav1FrameParams fp, const long long *__restrict__ costs,
-                                                      const Mav1BlockInfo *__restrict__ infos,
-                                                      Mav1BlockInfo *__restrict__ blocks, Mav1SbInfo *__restrict__ sbs) {
+DEVI void analyze_table_body(const Mav1FrameParams &fp, const long long *__restrict__ costs, const Mav1BlockInfo *__restrict__ infos, Mav1BlockInfo *__restrict__ blocks, Mav1SbInfo *__restrict__ sbs, const int blk_x, const int blk_y, const int blk_z) {
   __shared__ long long cost[MAV1_NUM_NODES], noneCost[MAV1_NUM_NODES];
   __shared__ int part[MAV1_NUM_NODES];
-  const int sb = blockIdx.x, lane = threadIdx.x;
+  const int sb = blk_x, lane = threadIdx.x;
   const int sbRow = sb / fp.sb_cols, sbCol = sb - sbRow * fp.sb_cols, lam = fp.ac_q[0] / 16;
   const long long *tc = costs + (long long)sb * MAV1_A_ITEMS;
   const Mav1BlockInfo *ti = infos + (long long)sb * MAV1_A_ITEMS;
@@ -528,6 +530,11 @@ __global__ __launch_bounds__(64) void k_analyze_table(const Mav1FrameParams fp, 
     S->sb_col = sbCol;
   }
 }
+__global__ __launch_bounds__(64) void k_analyze_table(const Mav1FrameParams fp, const long long *__restrict__ costs,
+                                                      const Mav1BlockInfo *__restrict__ infos,
+                                                      Mav1BlockInfo *__restrict__ blocks, Mav1SbInfo *__restrict__ sbs) {
+  analyze_table_body(fp, costs, infos, blocks, sbs, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 hipError_t launch_analyze(hipStream_t st, int level, const Mav1FrameParams &fp, const uint8_t *src, const uint8_t *ref,
                           const uint8_t *gold, const int32_t *mvf, const int32_t *mvg, long long *costs,
@@ -547,5 +554,44 @@ hipError_t launch_analyze_table(hipStream_t st, const Mav1FrameParams &fp, const
                                 const Mav1BlockInfo *infos, Mav1BlockInfo *blocks, Mav1SbInfo *sbs) {
   const int nsb = fp.sb_cols * fp.sb_rows;
   hipLaunchKernelGGL(k_analyze_table, dim3(nsb), dim3(64), 0, st, fp, costs, infos, blocks, sbs);
+  return hipGetLastError();
+}
+
+/* ---- batched launches (batch.h): the same kernels over several frames ---- */
+
+template <int LEVEL>
+__global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_analyze_level_b(
+    const BatchFrame *__restrict__ T, const BatchGrid g) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  analyze_level_body<LEVEL>(F.fp, F.src, F.ref, F.gold, F.mvf, F.mvg, F.acosts, F.ainfos, g.aux[b.f], b.x, b.y, b.z);
+}
+
+hipError_t launch_analyze_b(hipStream_t st, int level, const BatchFrame *T, const BatchFrame *H, int n) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) {
+    const int nslots = analysis_level_slots(H[f].fp, level);
+    batch_grid_add(g, H[f].fp.sb_cols * H[f].fp.sb_rows * (1 << (2 * level)) * nslots, 1, 1, nslots);
+  }
+  const dim3 grid(g.start[n]);
+  switch (level) {
+    case 0: hipLaunchKernelGGL(k_analyze_level_b<0>, grid, dim3(256), 0, st, T, g); break;
+    case 1: hipLaunchKernelGGL(k_analyze_level_b<1>, grid, dim3(128), 0, st, T, g); break;
+    case 2: hipLaunchKernelGGL(k_analyze_level_b<2>, grid, dim3(64), 0, st, T, g); break;
+    default: hipLaunchKernelGGL(k_analyze_level_b<3>, grid, dim3(64), 0, st, T, g); break;
+  }
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(64) void k_analyze_table_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  analyze_table_body(F.fp, F.acosts, F.ainfos, F.blocks, F.sbs, b.x, b.y, b.z);
+}
+
+hipError_t launch_analyze_table_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) batch_grid_add(g, H[f].fp.sb_cols * H[f].fp.sb_rows, 1, 1);
+  hipLaunchKernelGGL(k_analyze_table_b, dim3(g.start[n]), dim3(64), 0, st, T, g);
   return hipGetLastError();
 }

@@ -1029,3 +1029,45 @@ int ctx_clear_choice(mav1hip_ctx *c) {
   HIPCHK(c, hipMemsetAsync(c->d_choice, 0, (size_t)c->num_sbs, c->stream));
   return MAV1HIP_OK;
 }
+int ctx_batch_frame(mav1hip_ctx *c, BatchFrame *bf, int new_lr_gen) {
+  if (!c || !bf || !c->fp_set) return MAV1HIP_EINVAL;
+  memset(bf, 0, sizeof *bf);
+  bf->fp = c->fp;
+  bf->src = c->b_src;
+  bf->ref = c->b_ref;
+  bf->gold = c->b_gold;
+  bf->planes = c->d_planes;
+  bf->entry_bytes = c->entry_bytes;
+  bf->mvf = c->d_mvfield;
+  bf->mvg = c->d_mvgold;
+  bf->acosts = c->d_acosts;
+  bf->ainfos = c->d_ainfos;
+  bf->blocks = c->d_blocks;
+  bf->sbs = c->d_sbs;
+  bf->rec = c->d_rec;
+  bf->coefs = c->d_coefs;
+  bf->mats = c->d_mats;
+  bf->ys = c->d_ys;
+  bf->uvs = c->d_uvs;
+  bf->rwork = c->d_rwork;
+  bf->iwork = c->d_iwork;
+  bf->lfmask = c->d_lfmask;
+  bf->cdefbuf = c->d_cdefbuf;
+  bf->dbk = c->d_dbk;
+  bf->lrout = c->d_lrout;
+  bf->decgrid = c->d_decgrid;
+  bf->dirvar = c->d_dirvar;
+  bf->choice = c->d_choice;
+  bf->lrunits = c->d_lrunits;
+  bf->lrstats = c->d_lrstats;
+  bf->lrchain = c->d_lrchain;
+  bf->lrtype = c->d_lrtype;
+  bf->lrreruns = c->d_lrreruns;
+  bf->frame_bytes = (long long)c->frame_bytes;
+  bf->lr_units = c->lr_units;
+  if (new_lr_gen) c->lr_gen = c->lr_gen % 0x7ffffffe + 1;
+  bf->lr_gen = c->lr_gen;
+  c->me_pending = 0;
+  return MAV1HIP_OK;
+}
+int ctx_num_cu(const mav1hip_ctx *c) { return c ? c->num_cu : 0; }

@@ -16,9 +16,8 @@
 
 /* ---- filter info ---- */
 
-__global__ __launch_bounds__(64) void k_filter_info(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
-                                                    const Mav1SbInfo *__restrict__ sbs, uint8_t *__restrict__ info) {
-  const int sb = blockIdx.x, lane = threadIdx.x;
+DEVI void filter_info_body(const Mav1FrameParams &fp, const Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo *__restrict__ sbs, uint8_t *__restrict__ info, const int blk_x, const int blk_y, const int blk_z) {
+  const int sb = blk_x, lane = threadIdx.x;
   if (lane >= sbs[sb].num_blocks) return;
   const Mav1BlockInfo &b = blocks[sbs[sb].first_block + lane];
   const int skip = !b.eob[0] && !b.eob[1] && !b.eob[2];
@@ -27,6 +26,10 @@ __global__ __launch_bounds__(64) void k_filter_info(const Mav1FrameParams fp, co
   const int c1 = d_min(fp.mi_cols, b.mi_col + Num_4x4_Blocks_Wide[b.bsize]);
   for (int r = b.mi_row; r < r1; r++)
     for (int c = b.mi_col; c < c1; c++) info[r * fp.mi_cols + c] = v;
+}
+__global__ __launch_bounds__(64) void k_filter_info(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
+                                                    const Mav1SbInfo *__restrict__ sbs, uint8_t *__restrict__ info) {
+  filter_info_body(fp, blocks, sbs, info, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 /* ---- deblocking ---- */
@@ -108,12 +111,11 @@ DEVI void lf_sample(uint8_t *s, int step, int plane, int lvl, int sharp, int siz
 
 /* pass 0: x = vertical edge unit (4 samples), y = sample row;
  * pass 1: x = sample column, y = horizontal edge unit; z = plane. */
-__global__ __launch_bounds__(64) void k_deblock(const Mav1FrameParams fp, uint8_t *rec, const uint8_t *__restrict__ info,
-                                                int pass) {
-  const int plane = blockIdx.z, sub = plane > 0;
+DEVI void deblock_body(const Mav1FrameParams &fp, uint8_t *rec, const uint8_t *__restrict__ info, int pass, const int blk_x, const int blk_y, const int blk_z) {
+  const int plane = blk_z, sub = plane > 0;
   const int lvl = d_clip(0, MAX_LOOP_FILTER, fp.lf_level[plane == 0 ? pass : plane + 1]);
   if (!lvl) return;
-  const int gx = blockIdx.x * 64 + threadIdx.x, gy = blockIdx.y;
+  const int gx = blk_x * 64 + threadIdx.x, gy = blk_y;
   int c0, r0, sx, sy;
   if (pass == 0) {
     c0 = gx << sub;
@@ -140,15 +142,18 @@ __global__ __launch_bounds__(64) void k_deblock(const Mav1FrameParams fp, uint8_
   lf_sample(rec + fp.offset[plane] + sy * S + sx, pass == 0 ? 1 : S, plane, lvl, fp.lf_sharpness,
             d_min(plane ? 8 : 16, sz));
 }
+__global__ __launch_bounds__(64) void k_deblock(const Mav1FrameParams fp, uint8_t *rec, const uint8_t *__restrict__ info,
+                                                int pass) {
+  deblock_body(fp, rec, info, pass, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 /* ---- CDEF ---- */
 
 /* Direction and variance of every 8x8 block (7.15.2), one thread each;
  * dirvar = -1 for skip blocks, else dir | var << 3. */
-__global__ __launch_bounds__(64) void k_cdef_dir(const Mav1FrameParams fp, const uint8_t *__restrict__ cur,
-                                                 const uint8_t *__restrict__ info, int *__restrict__ dirvar) {
+DEVI void cdef_dir_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ cur, const uint8_t *__restrict__ info, int *__restrict__ dirvar, const int blk_x, const int blk_y, const int blk_z) {
   const int bc = fp.mi_cols >> 1, br = fp.mi_rows >> 1;
-  const int bi = blockIdx.x * 64 + threadIdx.x;
+  const int bi = blk_x * 64 + threadIdx.x;
   if (bi >= bc * br) return;
   const int r = (bi / bc) * 2, c = (bi % bc) * 2;
   if ((info[r * fp.mi_cols + c] >> 5) & 1) {
@@ -216,6 +221,10 @@ __global__ __launch_bounds__(64) void k_cdef_dir(const Mav1FrameParams fp, const
     if (d == ((best + 4) & 7)) opp = cost[d];
   dirvar[bi] = best | ((bestCost - opp) >> 10) << 3;
 }
+__global__ __launch_bounds__(64) void k_cdef_dir(const Mav1FrameParams fp, const uint8_t *__restrict__ cur,
+                                                 const uint8_t *__restrict__ info, int *__restrict__ dirvar) {
+  cdef_dir_body(fp, cur, info, dirvar, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 DEVI int cdef_constrain(int diff, int thr, int damping) {
   if (!thr) return 0;
@@ -227,11 +236,9 @@ DEVI int cdef_constrain(int diff, int thr, int damping) {
 /* One output sample per thread for every strength set (7.15.3 with the
  * preset handling of src/kernels/k_cdef.h:124-150): x = sample column,
  * y = sample row, z = plane, over the mi-covered area. */
-__global__ __launch_bounds__(64) void k_cdef(const Mav1FrameParams fp, const uint8_t *__restrict__ cur,
-                                             uint8_t *__restrict__ out, long long frameBytes,
-                                             const int *__restrict__ dirvar, int n) {
-  const int plane = blockIdx.z, sub = plane > 0;
-  const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y;
+DEVI void cdef_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ cur, uint8_t *__restrict__ out, long long frameBytes, const int *__restrict__ dirvar, int n, const int blk_x, const int blk_y, const int blk_z) {
+  const int plane = blk_z, sub = plane > 0;
+  const int x = blk_x * 64 + threadIdx.x, y = blk_y;
   const int W = (fp.mi_cols * 4) >> sub, H = (fp.mi_rows * 4) >> sub;
   if (x >= W || y >= H) return;
   const int S = fp.stride[plane], o = fp.offset[plane] + y * S + x;
@@ -280,16 +287,19 @@ __global__ __launch_bounds__(64) void k_cdef(const Mav1FrameParams fp, const uin
     dst[o] = (uint8_t)d_clip(lo, hi, v + ((8 + sum - (sum < 0)) >> 4));
   }
 }
+__global__ __launch_bounds__(64) void k_cdef(const Mav1FrameParams fp, const uint8_t *__restrict__ cur,
+                                             uint8_t *__restrict__ out, long long frameBytes,
+                                             const int *__restrict__ dirvar, int n) {
+  cdef_body(fp, cur, out, frameBytes, dirvar, n, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 /* Per superblock (256 threads): visible-area SSE of each set against the
  * source, first minimum, then the chosen set over the mi-aligned area
  * into rec. */
-__global__ __launch_bounds__(256) void k_cdef_select(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
-                                                     const uint8_t *__restrict__ cdefbuf, long long frameBytes, int n,
-                                                     uint8_t *__restrict__ rec, int8_t *__restrict__ choice) {
+DEVI void cdef_select_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, const uint8_t *__restrict__ cdefbuf, long long frameBytes, int n, uint8_t *__restrict__ rec, int8_t *__restrict__ choice, const int blk_x, const int blk_y, const int blk_z) {
   __shared__ long long part[4];
   __shared__ int bestK;
-  const int sb = blockIdx.x, sr = sb / fp.sb_cols, sc = sb % fp.sb_cols, tid = threadIdx.x;
+  const int sb = blk_x, sr = sb / fp.sb_cols, sc = sb % fp.sb_cols, tid = threadIdx.x;
   long long best = -1;
   int bk = 0;
   for (int k = 0; k < n; k++) {
@@ -330,6 +340,11 @@ __global__ __launch_bounds__(256) void k_cdef_select(const Mav1FrameParams fp, c
       rec[a] = o[a];
     }
   }
+}
+__global__ __launch_bounds__(256) void k_cdef_select(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+                                                     const uint8_t *__restrict__ cdefbuf, long long frameBytes, int n,
+                                                     uint8_t *__restrict__ rec, int8_t *__restrict__ choice) {
+  cdef_select_body(fp, src, cdefbuf, frameBytes, n, rec, choice, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 /* ---- loop restoration ---- */
@@ -452,10 +467,8 @@ DEVI void lr_sgr(const Mav1FrameParams &fp, int p, const LrBlk &b, const uint8_t
 
 /* One 4x4 luma block (2x2 chroma) per thread: x = block column, y = block
  * row, z = plane; lrType: the frame's restoration type per plane. */
-__global__ __launch_bounds__(64) void k_lr_apply(const Mav1FrameParams fp, const uint8_t *__restrict__ cdef,
-                                                 const uint8_t *__restrict__ dbk, uint8_t *__restrict__ out,
-                                                 const Mav1LrUnit *__restrict__ units, const int *__restrict__ lrType) {
-  const int p = blockIdx.z, col = blockIdx.x * 64 + threadIdx.x, row = blockIdx.y;
+DEVI void lr_apply_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk, uint8_t *__restrict__ out, const Mav1LrUnit *__restrict__ units, const int *__restrict__ lrType, const int blk_x, const int blk_y, const int blk_z) {
+  const int p = blk_z, col = blk_x * 64 + threadIdx.x, row = blk_y;
   if (col >= ((fp.frame_w + 3) >> 2)) return;
   LrBlk b;
   if (!lr_geom(fp, p, row, col, b)) return;
@@ -470,6 +483,11 @@ __global__ __launch_bounds__(64) void k_lr_apply(const Mav1FrameParams fp, const
         const int o = fp.offset[p] + (b.y + i) * fp.stride[p] + b.x + j;
         out[o] = cdef[o];
       }
+}
+__global__ __launch_bounds__(64) void k_lr_apply(const Mav1FrameParams fp, const uint8_t *__restrict__ cdef,
+                                                 const uint8_t *__restrict__ dbk, uint8_t *__restrict__ out,
+                                                 const Mav1LrUnit *__restrict__ units, const int *__restrict__ lrType) {
+  lr_apply_body(fp, cdef, dbk, out, units, lrType, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 /* ---- launchers ---- */
@@ -515,9 +533,8 @@ hipError_t launch_lr_apply(hipStream_t st, const Mav1FrameParams &fp, const uint
  * block covers, where no later block of the superblock (in decode order)
  * covers them too, so each cell holds the last block as the serial dump
  * leaves it. Cells outside every block are cleared first. */
-__global__ __launch_bounds__(64) void k_decision_grid(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
-                                                      const Mav1SbInfo *__restrict__ sbs, uint8_t *__restrict__ grid) {
-  const int sb = blockIdx.x, lane = threadIdx.x;
+DEVI void decision_grid_body(const Mav1FrameParams &fp, const Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo *__restrict__ sbs, uint8_t *__restrict__ grid, const int blk_x, const int blk_y, const int blk_z) {
+  const int sb = blk_x, lane = threadIdx.x;
   const int rows8 = (fp.mi_rows + 1) >> 1, cols8 = (fp.mi_cols + 1) >> 1;
   const int sr = sb / fp.sb_cols, sc = sb % fp.sb_cols;
   /* the superblock's 8x8 cells */
@@ -547,9 +564,139 @@ __global__ __launch_bounds__(64) void k_decision_grid(const Mav1FrameParams fp, 
       if (!later) *(uint2 *)(grid + ((size_t)r * cols8 + c) * 8) = make_uint2(lo, hi);
     }
 }
+__global__ __launch_bounds__(64) void k_decision_grid(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
+                                                      const Mav1SbInfo *__restrict__ sbs, uint8_t *__restrict__ grid) {
+  decision_grid_body(fp, blocks, sbs, grid, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 hipError_t launch_decision_grid(hipStream_t st, const Mav1FrameParams &fp, const Mav1BlockInfo *blocks,
                                 const Mav1SbInfo *sbs, uint8_t *grid) {
   hipLaunchKernelGGL(k_decision_grid, dim3(fp.sb_cols * fp.sb_rows), dim3(64), 0, st, fp, blocks, sbs, grid);
+  return hipGetLastError();
+}
+
+/* ---- batched launches (batch.h): the same kernels over several frames ---- */
+
+#define BATCH_FRAME()              \
+  const BatchIdx b = batch_idx(g); \
+  const BatchFrame &F = T[b.f]
+
+__global__ __launch_bounds__(64) void k_filter_info_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
+  BATCH_FRAME();
+  filter_info_body(F.fp, F.blocks, F.sbs, F.lfmask, b.x, b.y, b.z);
+}
+__global__ __launch_bounds__(64) void k_deblock_b(const BatchFrame *__restrict__ T, const BatchGrid g, int pass) {
+  BATCH_FRAME();
+  deblock_body(F.fp, F.rec, F.lfmask, pass, b.x, b.y, b.z);
+}
+__global__ __launch_bounds__(64) void k_cdef_dir_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
+  BATCH_FRAME();
+  cdef_dir_body(F.fp, F.rec, F.lfmask, F.dirvar, b.x, b.y, b.z);
+}
+__global__ __launch_bounds__(64) void k_cdef_b(const BatchFrame *__restrict__ T, const BatchGrid g, int n) {
+  BATCH_FRAME();
+  cdef_body(F.fp, F.rec, F.cdefbuf, F.frame_bytes, F.dirvar, n, b.x, b.y, b.z);
+}
+__global__ __launch_bounds__(256) void k_cdef_select_b(const BatchFrame *__restrict__ T, const BatchGrid g, int n) {
+  BATCH_FRAME();
+  cdef_select_body(F.fp, F.src, F.cdefbuf, F.frame_bytes, n, F.rec, F.choice, b.x, b.y, b.z);
+}
+__global__ __launch_bounds__(64) void k_lr_apply_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
+  BATCH_FRAME();
+  lr_apply_body(F.fp, F.rec, F.dbk, F.lrout, F.lrunits, F.lrtype, b.x, b.y, b.z);
+}
+__global__ __launch_bounds__(64) void k_decision_grid_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
+  BATCH_FRAME();
+  decision_grid_body(F.fp, F.blocks, F.sbs, F.decgrid, b.x, b.y, b.z);
+}
+
+/* The driver's frame copies and clears, 16 bytes per thread (the buffers
+ * are 256-byte aligned allocations; a tail shorter than 16 bytes is copied
+ * bytewise). */
+__global__ __launch_bounds__(256) void k_frame_op_b(const BatchFrame *__restrict__ T, const BatchGrid g, int which) {
+  BATCH_FRAME();
+  const long long o = ((long long)b.x * 256 + threadIdx.x) * 16;
+  if (which == 2) {
+    const long long nsb = (long long)F.fp.sb_cols * F.fp.sb_rows;
+    for (long long q = o; q < o + 16 && q < nsb; q++) F.choice[q] = 0;
+    return;
+  }
+  const uint8_t *src = F.rec;
+  uint8_t *dst = which == 0 ? F.dbk : F.lrout;
+  const long long n = F.frame_bytes;
+  if (o + 16 <= n) {
+    *(uint4 *)(dst + o) = *(const uint4 *)(src + o);
+  } else {
+    for (long long q = o; q < n; q++) dst[q] = src[q];
+  }
+}
+
+static BatchGrid sb_grid(const BatchFrame *H, int n) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) batch_grid_add(g, H[f].fp.sb_cols * H[f].fp.sb_rows, 1, 1);
+  return g;
+}
+
+hipError_t launch_filter_info_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {
+  const BatchGrid g = sb_grid(H, n);
+  hipLaunchKernelGGL(k_filter_info_b, dim3(g.start[n]), dim3(64), 0, st, T, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_deblock_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {
+  BatchGrid g0 = batch_grid(), g1 = batch_grid();
+  for (int f = 0; f < n; f++) {
+    const Mav1FrameParams &fp = H[f].fp;
+    batch_grid_add(g0, (fp.mi_cols + 63) / 64, fp.mi_rows * 4, 3);
+    batch_grid_add(g1, (fp.mi_cols * 4 + 63) / 64, fp.mi_rows, 3);
+  }
+  hipLaunchKernelGGL(k_deblock_b, dim3(g0.start[n]), dim3(64), 0, st, T, g0, 0);
+  hipLaunchKernelGGL(k_deblock_b, dim3(g1.start[n]), dim3(64), 0, st, T, g1, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_cdef_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int nsets) {
+  BatchGrid g0 = batch_grid(), g1 = batch_grid();
+  for (int f = 0; f < n; f++) {
+    const Mav1FrameParams &fp = H[f].fp;
+    const int nb = (fp.mi_cols >> 1) * (fp.mi_rows >> 1);
+    batch_grid_add(g0, (nb + 63) / 64, 1, 1);
+    batch_grid_add(g1, (fp.mi_cols * 4 + 63) / 64, fp.mi_rows * 4, 3);
+  }
+  hipLaunchKernelGGL(k_cdef_dir_b, dim3(g0.start[n]), dim3(64), 0, st, T, g0);
+  hipLaunchKernelGGL(k_cdef_b, dim3(g1.start[n]), dim3(64), 0, st, T, g1, nsets);
+  return hipGetLastError();
+}
+
+hipError_t launch_cdef_select_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int nsets) {
+  const BatchGrid g = sb_grid(H, n);
+  hipLaunchKernelGGL(k_cdef_select_b, dim3(g.start[n]), dim3(256), 0, st, T, g, nsets);
+  return hipGetLastError();
+}
+
+hipError_t launch_lr_apply_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) {
+    const Mav1FrameParams &fp = H[f].fp;
+    batch_grid_add(g, (((fp.frame_w + 3) >> 2) + 63) / 64, (fp.frame_h + 3) >> 2, 3);
+  }
+  hipLaunchKernelGGL(k_lr_apply_b, dim3(g.start[n]), dim3(64), 0, st, T, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_decision_grid_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {
+  const BatchGrid g = sb_grid(H, n);
+  hipLaunchKernelGGL(k_decision_grid_b, dim3(g.start[n]), dim3(64), 0, st, T, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_frame_op_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int which) {
+  if (which < 0 || which > 2) return hipErrorInvalidValue;
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) {
+    const long long bytes = which == 2 ? (long long)H[f].fp.sb_cols * H[f].fp.sb_rows : H[f].frame_bytes;
+    batch_grid_add(g, (int)((bytes + 16 * 256 - 1) / (16 * 256)), 1, 1);
+  }
+  hipLaunchKernelGGL(k_frame_op_b, dim3(g.start[n]), dim3(256), 0, st, T, g, which);
   return hipGetLastError();
 }

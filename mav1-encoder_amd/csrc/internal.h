@@ -8,6 +8,7 @@
 
 #include "../../include/mav1_hip.h"
 #include "../../include/mav1_types.h"
+#include "batch.h"
 
 /* Motion-search planes of one reference: 16 quarter-pel planes
  * (frame_w x frame_h each) followed by the 4x4-decimated plane. */
@@ -104,5 +105,11 @@ int ctx_bind_ptrs(mav1hip_ctx *c, uint8_t *src, uint8_t *ref, uint8_t *gold);
 int ctx_store(mav1hip_ctx *c, uint8_t *dst, int buf);
 int ctx_rec_to_lrout(mav1hip_ctx *c);
 int ctx_clear_choice(mav1hip_ctx *c);
+/* The context's frame as a batch table entry: its parameters (set by
+ * mav1hip_set_params), bound inputs and buffers. new_lr_gen: take the next
+ * restoration-fit generation, as a restoration search of this context does. */
+int ctx_batch_frame(mav1hip_ctx *c, BatchFrame *bf, int new_lr_gen);
+/* workgroups of the persistent inter and intra phases */
+int ctx_num_cu(const mav1hip_ctx *c);
 
 #endif

@@ -669,10 +669,8 @@ DEVI void lr_wiener_fit(const Mav1FrameParams &fp, const LrGeo &g, const uint8_t
  * maxSpin polls, the refinement records an assumed reference no chain
  * holds, and the chain refines the unit again: the result never depends on
  * the wait. */
-__global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
-                                                 const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
-                                                 LrUnitStat *__restrict__ stats, int gen, int maxSpin) {
-  const int u = blockIdx.x, part = blockIdx.y, tid = threadIdx.x;
+DEVI void lr_fit_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk, LrUnitStat *__restrict__ stats, int gen, int maxSpin, const int blk_x, const int blk_y, const int blk_z) {
+  const int u = blk_x, part = blk_y, tid = threadIdx.x;
   LrGeo g;
   lr_geo(fp, u, g);
   lr_stage(fp, g, cdef, dbk, src, tid);
@@ -837,14 +835,17 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const
     __syncthreads();
   }
 }
+__global__ __launch_bounds__(LR_T) void k_lr_fit(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+                                                 const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
+                                                 LrUnitStat *__restrict__ stats, int gen, int maxSpin) {
+  lr_fit_body(fp, src, cdef, dbk, stats, gen, maxSpin, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 /* Per unit: the greedy refinement of the fitted taps under the plane's
  * running reference, for the units from the chain's restart point on (all
  * of them in the first pass, where the reference is the default). */
-__global__ __launch_bounds__(LR_T) void k_lr_path(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
-                                                  const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
-                                                  LrUnitStat *__restrict__ stats, const LrChain *__restrict__ chain) {
-  const int u = blockIdx.x, tid = threadIdx.x;
+DEVI void lr_path_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk, LrUnitStat *__restrict__ stats, const LrChain *__restrict__ chain, const int blk_x, const int blk_y, const int blk_z) {
+  const int u = blk_x, tid = threadIdx.x;
   const int p = u < fp.lr_unit_off[1] ? 0 : u < fp.lr_unit_off[2] ? 1 : 2;
   const LrChain &cs = chain[p];
   if (cs.done || u - fp.lr_unit_off[p] < cs.start) return;
@@ -868,6 +869,11 @@ __global__ __launch_bounds__(LR_T) void k_lr_path(const Mav1FrameParams fp, cons
     st.sseStar = sse;
     st.costStar = cost;
   }
+}
+__global__ __launch_bounds__(LR_T) void k_lr_path(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+                                                  const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
+                                                  LrUnitStat *__restrict__ stats, const LrChain *__restrict__ chain) {
+  lr_path_body(fp, src, cdef, dbk, stats, chain, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 /* Per plane, units in raster order with the true running references: the
@@ -956,12 +962,8 @@ DEVI void lr_spec(const Mav1FrameParams &fp, const LrUnitStat *sSt, int li, int 
   }
 }
 
-__global__ __launch_bounds__(LR_T) void k_lr_chain(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
-                                                   const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
-                                                   const LrUnitStat *__restrict__ stats, Mav1LrUnit *__restrict__ units,
-                                                   int *__restrict__ lrType, int lrTest, int *__restrict__ reruns,
-                                                   LrChain *__restrict__ chain, int stop) {
-  const int p = blockIdx.x, tid = threadIdx.x;
+DEVI void lr_chain_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk, const LrUnitStat *__restrict__ stats, Mav1LrUnit *__restrict__ units, int *__restrict__ lrType, int lrTest, int *__restrict__ reruns, LrChain *__restrict__ chain, int stop, const int blk_x, const int blk_y, const int blk_z) {
+  const int p = blk_x, tid = threadIdx.x;
   LrChain &cs = chain[p];
   if (cs.done) return;
   __shared__ LrUnitStat sSt[LR_CB];
@@ -1153,6 +1155,13 @@ __global__ __launch_bounds__(LR_T) void k_lr_chain(const Mav1FrameParams fp, con
     reruns[p] = nrerun;
   }
 }
+__global__ __launch_bounds__(LR_T) void k_lr_chain(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+                                                   const uint8_t *__restrict__ cdef, const uint8_t *__restrict__ dbk,
+                                                   const LrUnitStat *__restrict__ stats, Mav1LrUnit *__restrict__ units,
+                                                   int *__restrict__ lrType, int lrTest, int *__restrict__ reruns,
+                                                   LrChain *__restrict__ chain, int stop) {
+  lr_chain_body(fp, src, cdef, dbk, stats, units, lrType, lrTest, reruns, chain, stop, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 __global__ void k_lr_chain_init(LrChain *__restrict__ chain) {
   const int p = threadIdx.x;
@@ -1188,5 +1197,64 @@ hipError_t launch_lr_chain(hipStream_t st, const Mav1FrameParams &fp, const uint
                            int *reruns, void *chain, int stop) {
   hipLaunchKernelGGL(k_lr_chain, dim3(3), dim3(LR_T), 0, st, fp, src, cdef, dbk, (const LrUnitStat *)stats, units,
                      lrType, lrTest, reruns, (LrChain *)chain, stop);
+  return hipGetLastError();
+}
+
+/* ---- batched launches (batch.h): the same kernels over several frames ---- */
+
+/* The fit over every frame's units: the workgroups in part-major order
+ * (g.start over units; part = block / units), so every fit workgroup of
+ * every frame is dispatched before any refinement workgroup, as in the
+ * single-frame launch. */
+__global__ __launch_bounds__(LR_T) void k_lr_fit_b(const BatchFrame *__restrict__ T, const BatchGrid g, int maxSpin) {
+  const int units = g.start[g.n], part = blockIdx.x / units;
+  const BatchIdx b = batch_idx(g, blockIdx.x - part * units);
+  const BatchFrame &F = T[b.f];
+  lr_fit_body(F.fp, F.src, F.rec, F.dbk, (LrUnitStat *)F.lrstats, F.lr_gen, maxSpin, b.x, part, 0);
+}
+__global__ void k_lr_chain_init_b(const BatchFrame *__restrict__ T) {
+  const int p = threadIdx.x;
+  if (p >= 3) return;
+  LrChain c = {};
+  for (int k = 0; k < 6; k++) c.ref[k] = Wiener_Taps_Mid[k % 3];
+  for (int k = 0; k < 2; k++) c.refX[k] = Sgrproj_Xqd_Mid[k];
+  ((LrChain *)T[blockIdx.x].lrchain)[p] = c;
+}
+__global__ __launch_bounds__(LR_T) void k_lr_path_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  lr_path_body(F.fp, F.src, F.rec, F.dbk, (LrUnitStat *)F.lrstats, (const LrChain *)F.lrchain, b.x, b.y, b.z);
+}
+__global__ __launch_bounds__(LR_T) void k_lr_chain_b(const BatchFrame *__restrict__ T, const BatchGrid g, int lrTest,
+                                                     int stop) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  lr_chain_body(F.fp, F.src, F.rec, F.dbk, (const LrUnitStat *)F.lrstats, F.lrunits, F.lrtype, lrTest, F.lrreruns,
+                (LrChain *)F.lrchain, stop, b.x, b.y, b.z);
+}
+
+static BatchGrid unit_grid(const BatchFrame *H, int n) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) batch_grid_add(g, H[f].lr_units, 1, 1);
+  return g;
+}
+
+hipError_t launch_lr_fit_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {
+  const BatchGrid g = unit_grid(H, n);
+  hipLaunchKernelGGL(k_lr_fit_b, dim3(g.start[n] * 4), dim3(LR_T), 0, st, T, g, 1 << 20);
+  hipLaunchKernelGGL(k_lr_chain_init_b, dim3(n), dim3(64), 0, st, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_lr_path_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {
+  const BatchGrid g = unit_grid(H, n);
+  hipLaunchKernelGGL(k_lr_path_b, dim3(g.start[n]), dim3(LR_T), 0, st, T, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_lr_chain_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int lrTest, int stop) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) batch_grid_add(g, 3, 1, 1);
+  hipLaunchKernelGGL(k_lr_chain_b, dim3(g.start[n]), dim3(LR_T), 0, st, T, g, lrTest, stop);
   return hipGetLastError();
 }

@@ -37,13 +37,12 @@ size_t me_dec_bytes(const Mav1FrameParams &fp) {
 #define QT_W 64
 #define QT_H 16
 
-__global__ __launch_bounds__(256) void k_qpel_build(const Mav1FrameParams fp, const uint8_t *__restrict__ ref,
-                                                    uint8_t *__restrict__ out) {
+DEVI void qpel_build_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ ref, uint8_t *__restrict__ out, const int blk_x, const int blk_y, const int blk_z) {
   __shared__ uint8_t tile[QT_H + 7][QT_W + 8];
   __shared__ int16_t hz[3][QT_H + 7][QT_W];
   const int W = fp.frame_w, H = fp.frame_h, S = fp.stride[0];
   const uint8_t *luma = ref + fp.offset[0];
-  const int x0 = blockIdx.x * QT_W, y0 = blockIdx.y * QT_H, tid = threadIdx.x;
+  const int x0 = blk_x * QT_W, y0 = blk_y * QT_H, tid = threadIdx.x;
   for (int i = tid; i < (QT_H + 7) * (QT_W + 7); i += 256) {
     int r = i / (QT_W + 7), c = i - r * (QT_W + 7);
     tile[r][c] = luma[d_clip(0, H - 1, y0 + r - 3) * S + d_clip(0, W - 1, x0 + c - 3)];
@@ -98,6 +97,10 @@ __global__ __launch_bounds__(256) void k_qpel_build(const Mav1FrameParams fp, co
     }
   }
 }
+__global__ __launch_bounds__(256) void k_qpel_build(const Mav1FrameParams fp, const uint8_t *__restrict__ ref,
+                                                    uint8_t *__restrict__ out) {
+  qpel_build_body(fp, ref, out, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 hipError_t launch_qpel_build(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *ref, uint8_t *planes16) {
   dim3 grid((fp.frame_w + QT_W - 1) / QT_W, (fp.frame_h + QT_H - 1) / QT_H);
@@ -106,10 +109,9 @@ hipError_t launch_qpel_build(hipStream_t st, const Mav1FrameParams &fp, const ui
 }
 
 /* 4x4-mean decimated plane (k_me.h:112-118 per sample). */
-__global__ __launch_bounds__(256) void k_dec_build(const Mav1FrameParams fp, const uint8_t *__restrict__ frame,
-                                                   uint8_t *__restrict__ dec) {
+DEVI void dec_build_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ frame, uint8_t *__restrict__ dec, const int blk_x, const int blk_y, const int blk_z) {
   const int W = fp.frame_w, H = fp.frame_h, dw = (W + 3) >> 2, dh = (H + 3) >> 2;
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int x = blk_x * 64 + (threadIdx.x & 63), y = blk_y * 4 + (threadIdx.x >> 6);
   if (x >= dw || y >= dh) return;
   const uint8_t *luma = frame + fp.offset[0];
   int s = 0;
@@ -120,6 +122,10 @@ __global__ __launch_bounds__(256) void k_dec_build(const Mav1FrameParams fp, con
     for (int j = 0; j < 4; j++) s += row[d_clip(0, W - 1, 4 * x + j)];
   }
   dec[(size_t)y * dw + x] = (uint8_t)((s + 8) >> 4);
+}
+__global__ __launch_bounds__(256) void k_dec_build(const Mav1FrameParams fp, const uint8_t *__restrict__ frame,
+                                                   uint8_t *__restrict__ dec) {
+  dec_build_body(fp, frame, dec, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 hipError_t launch_dec_build(hipStream_t st, const Mav1FrameParams &fp, const uint8_t *frame, uint8_t *dec) {
@@ -289,11 +295,7 @@ DEVI int subpel_cost_pred(const Mav1FrameParams &fp, const uint8_t *srcL, const 
 
 /* HP: sub-pel costs from the quarter-pel planes, else from predictions. */
 template <int LEVEL, bool HP>
-__global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
-                                                 const uint8_t *__restrict__ ref0, const uint8_t *__restrict__ ref1,
-                                                 const uint8_t *__restrict__ planes, long entryBytes,
-                                                 const uint8_t *__restrict__ srcDec, int32_t *__restrict__ field0,
-                                                 int32_t *__restrict__ field1, int nodesPerRef) {
+DEVI void me_level_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, const uint8_t *__restrict__ ref0, const uint8_t *__restrict__ ref1, const uint8_t *__restrict__ planes, long entryBytes, const uint8_t *__restrict__ srcDec, int32_t *__restrict__ field0, int32_t *__restrict__ field1, int nodesPerRef, const int blk_x, const int blk_y, const int blk_z) {
   constexpr int BW = 64 >> LEVEL;
   constexpr int N = 1 << LEVEL;
   constexpr int WW = BW + 2 * WIN_R;
@@ -303,7 +305,7 @@ __global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const
   __shared__ int16_t pmid[HP ? 1 : (BW + 7) * BW];
   __shared__ int scost[9];
   const int lane = threadIdx.x;
-  const int r = blockIdx.x / nodesPerRef, node = blockIdx.x - r * nodesPerRef;
+  const int r = blk_x / nodesPerRef, node = blk_x - r * nodesPerRef;
   const int sb = node >> (2 * LEVEL), i = node & (N * N - 1);
   const uint8_t *ref = r ? ref1 : ref0;
   int32_t *field = r ? field1 : field0;
@@ -470,6 +472,14 @@ __global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const
     out[1] = mvc;
   }
 }
+template <int LEVEL, bool HP>
+__global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const uint8_t *__restrict__ src,
+                                                 const uint8_t *__restrict__ ref0, const uint8_t *__restrict__ ref1,
+                                                 const uint8_t *__restrict__ planes, long entryBytes,
+                                                 const uint8_t *__restrict__ srcDec, int32_t *__restrict__ field0,
+                                                 int32_t *__restrict__ field1, int nodesPerRef) {
+  me_level_body<LEVEL, HP>(fp, src, ref0, ref1, planes, entryBytes, srcDec, field0, field1, nodesPerRef, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 template <int LEVEL>
 static void launch_level(hipStream_t st, bool hp, dim3 grid, const Mav1FrameParams &fp, const uint8_t *src,
@@ -494,6 +504,76 @@ hipError_t launch_me_level(hipStream_t st, int level, const Mav1FrameParams &fp,
     case 1: launch_level<1>(st, hp, grid, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0, field1, nodesPerRef); break;
     case 2: launch_level<2>(st, hp, grid, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0, field1, nodesPerRef); break;
     default: launch_level<3>(st, hp, grid, fp, src, ref0, ref1, planes, entryBytes, srcDec, field0, field1, nodesPerRef); break;
+  }
+  return hipGetLastError();
+}
+
+/* ---- batched launches (batch.h): the same kernels over several frames ---- */
+
+/* which 0: LAST's planes, 1: the second reference's. */
+__global__ __launch_bounds__(256) void k_qpel_build_b(const BatchFrame *__restrict__ T, const BatchGrid g, int which) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  qpel_build_body(F.fp, which ? F.gold : F.ref, F.planes + which * F.entry_bytes, b.x, b.y, b.z);
+}
+
+hipError_t launch_qpel_build_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int which) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++)
+    batch_grid_add(g, (H[f].fp.frame_w + QT_W - 1) / QT_W, (H[f].fp.frame_h + QT_H - 1) / QT_H, 1);
+  hipLaunchKernelGGL(k_qpel_build_b, dim3(g.start[n]), dim3(256), 0, st, T, g, which);
+  return hipGetLastError();
+}
+
+/* which 0 / 1: the decimated plane of LAST / the second reference behind
+ * its 16 quarter-pel planes; 2: the source's, after both entries. */
+__global__ __launch_bounds__(256) void k_dec_build_b(const BatchFrame *__restrict__ T, const BatchGrid g, int which) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  const size_t n16 = (size_t)16 * F.fp.frame_w * F.fp.frame_h;
+  const uint8_t *frame = which == 2 ? F.src : which ? F.gold : F.ref;
+  uint8_t *dec = which == 2 ? F.planes + 2 * F.entry_bytes : F.planes + which * F.entry_bytes + n16;
+  dec_build_body(F.fp, frame, dec, b.x, b.y, b.z);
+}
+
+hipError_t launch_dec_build_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int which) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) {
+    const int dw = (H[f].fp.frame_w + 3) >> 2, dh = (H[f].fp.frame_h + 3) >> 2;
+    batch_grid_add(g, (dw + 63) / 64, (dh + 3) / 4, 1);
+  }
+  hipLaunchKernelGGL(k_dec_build_b, dim3(g.start[n]), dim3(256), 0, st, T, g, which);
+  return hipGetLastError();
+}
+
+template <int LEVEL, bool HP>
+__global__ __launch_bounds__(64) void k_me_level_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  me_level_body<LEVEL, HP>(F.fp, F.src, F.ref, F.gold, F.planes, (long)F.entry_bytes, F.planes + 2 * F.entry_bytes,
+                           F.mvf, F.mvg, g.aux[b.f], b.x, b.y, b.z);
+}
+
+template <int LEVEL>
+static void launch_level_b(hipStream_t st, bool hp, const BatchFrame *T, const BatchGrid &g) {
+  if (hp)
+    hipLaunchKernelGGL((k_me_level_b<LEVEL, true>), dim3(g.start[g.n]), dim3(64), 0, st, T, g);
+  else
+    hipLaunchKernelGGL((k_me_level_b<LEVEL, false>), dim3(g.start[g.n]), dim3(64), 0, st, T, g);
+}
+
+hipError_t launch_me_level_b(hipStream_t st, int level, const BatchFrame *T, const BatchFrame *H, int n, int nref) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) {
+    const int nodesPerRef = H[f].fp.sb_cols * H[f].fp.sb_rows << (2 * level);
+    batch_grid_add(g, nodesPerRef * nref, 1, 1, nodesPerRef);
+  }
+  const bool hp = (H[0].fp.turbo & MAV1_T_ME_HPEL) != 0;
+  switch (level) {
+    case 0: launch_level_b<0>(st, hp, T, g); break;
+    case 1: launch_level_b<1>(st, hp, T, g); break;
+    case 2: launch_level_b<2>(st, hp, T, g); break;
+    default: launch_level_b<3>(st, hp, T, g); break;
   }
   return hipGetLastError();
 }

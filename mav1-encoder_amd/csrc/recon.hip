@@ -1311,12 +1311,15 @@ DEVI void stage_smats(const Mav1FrameParams &fp, const int32_t *__restrict__ mat
 __global__ void k_recon_list_reset(int *work) {
   if (threadIdx.x < 2) work[threadIdx.x] = 0;
 }
-__global__ __launch_bounds__(64) void k_recon_list(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
-                                                   const Mav1SbInfo *__restrict__ sbs, int *work) {
-  const int sb = blockIdx.x, k = threadIdx.x;
+DEVI void recon_list_body(const Mav1FrameParams &fp, const Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo *__restrict__ sbs, int *work, const int blk_x, const int blk_y, const int blk_z) {
+  const int sb = blk_x, k = threadIdx.x;
   if (k >= sbs[sb].num_blocks) return;
   const int bi = sbs[sb].first_block + k;
   if (blocks[bi].is_inter) work[2 + atomicAdd(&work[0], 1)] = bi;
+}
+__global__ __launch_bounds__(64) void k_recon_list(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
+                                                   const Mav1SbInfo *__restrict__ sbs, int *work) {
+  recon_list_body(fp, blocks, sbs, work, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 /* Inter phase: persistent workgroups take the inter blocks off the list
@@ -1366,8 +1369,7 @@ __global__ __launch_bounds__(RT) void k_recon_inter(const Mav1FrameParams fp, Ma
 #define SCHED_T 512
 #define SCHED_KEYS 20480 /* (sb_cols + 2 sb_rows) sb_rows, checked on the host */
 
-__global__ __launch_bounds__(SCHED_T) void k_intra_sched(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
-                                                        const Mav1SbInfo *__restrict__ sbs, int *__restrict__ iwork) {
+DEVI void intra_sched_body(const Mav1FrameParams &fp, const Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo *__restrict__ sbs, int *__restrict__ iwork, const int blk_x, const int blk_y, const int blk_z) {
   __shared__ uint16_t dense[SCHED_KEYS];
   __shared__ int cnt[SCHED_T + 1];
   const int n = fp.sb_cols * fp.sb_rows, tid = threadIdx.x;
@@ -1401,6 +1403,10 @@ __global__ __launch_bounds__(SCHED_T) void k_intra_sched(const Mav1FrameParams f
   int o = cnt[tid];
   for (int k = k0; k < k1; k++)
     if (dense[k]) list[o++] = dense[k] - 1;
+}
+__global__ __launch_bounds__(SCHED_T) void k_intra_sched(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
+                                                        const Mav1SbInfo *__restrict__ sbs, int *__restrict__ iwork) {
+  intra_sched_body(fp, blocks, sbs, iwork, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 /* Waits (thread 0) until *flag is set; false when maxSpin polls ran out. */
@@ -1849,9 +1855,8 @@ __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Ma
 
 /* Edge filter types (src/encoder.c:319-346): smooth flags per mi, then per
  * block the above / left neighbours' flags into interp_filter bits 4 / 5. */
-__global__ __launch_bounds__(64) void k_edge_flags(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
-                                                   const Mav1SbInfo *__restrict__ sbs, uint8_t *ys, uint8_t *uvs) {
-  const int sb = blockIdx.x, lane = threadIdx.x;
+DEVI void edge_flags_body(const Mav1FrameParams &fp, const Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo *__restrict__ sbs, uint8_t *ys, uint8_t *uvs, const int blk_x, const int blk_y, const int blk_z) {
+  const int sb = blk_x, lane = threadIdx.x;
   stage_tabs(lane, 64);
   __syncthreads();
   if (lane >= sbs[sb].num_blocks) return;
@@ -1864,10 +1869,12 @@ __global__ __launch_bounds__(64) void k_edge_flags(const Mav1FrameParams fp, con
       uvs[r * fp.mi_cols + c] = (uint8_t)su;
     }
 }
-__global__ __launch_bounds__(64) void k_edge_types(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
-                                                   const Mav1SbInfo *__restrict__ sbs, const uint8_t *ys,
-                                                   const uint8_t *uvs) {
-  const int sb = blockIdx.x, lane = threadIdx.x;
+__global__ __launch_bounds__(64) void k_edge_flags(const Mav1FrameParams fp, const Mav1BlockInfo *__restrict__ blocks,
+                                                   const Mav1SbInfo *__restrict__ sbs, uint8_t *ys, uint8_t *uvs) {
+  edge_flags_body(fp, blocks, sbs, ys, uvs, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+DEVI void edge_types_body(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo *__restrict__ sbs, const uint8_t *ys, const uint8_t *uvs, const int blk_x, const int blk_y, const int blk_z) {
+  const int sb = blk_x, lane = threadIdx.x;
   if (lane >= sbs[sb].num_blocks) return;
   Mav1BlockInfo *b = &blocks[sbs[sb].first_block + lane];
   const int cols = fp.mi_cols, rows = fp.mi_rows, r = b->mi_row, c = b->mi_col;
@@ -1875,6 +1882,11 @@ __global__ __launch_bounds__(64) void k_edge_types(const Mav1FrameParams fp, Mav
   const int fy = (au && ys[(r - 1) * cols + c]) || (al && ys[r * cols + c - 1]);
   const int fuv = (au && uvs[(r - 1) * cols + d_min(c + 1, cols - 1)]) || (al && uvs[d_min(r + 1, rows - 1) * cols + c - 1]);
   b->interp_filter = (b->interp_filter & 15) | fy << 4 | fuv << 5;
+}
+__global__ __launch_bounds__(64) void k_edge_types(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
+                                                   const Mav1SbInfo *__restrict__ sbs, const uint8_t *ys,
+                                                   const uint8_t *uvs) {
+  edge_types_body(fp, blocks, sbs, ys, uvs, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 hipError_t launch_edge_types(hipStream_t st, const Mav1FrameParams &fp, Mav1BlockInfo *blocks, const Mav1SbInfo *sbs,
@@ -1913,5 +1925,147 @@ hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
                               int32_t *coefs, const int32_t *mats, int *iwork, int workgroups) {
   hipLaunchKernelGGL(k_recon_intra, dim3(workgroups), dim3(RT), 0, st, fp, blocks, sbs, src, rec, ref, gold, coefs,
                      mats, iwork, 1 << 20);
+  return hipGetLastError();
+}
+
+/* ---- batched launches (batch.h): the same kernels over several frames ---- */
+
+__global__ __launch_bounds__(64) void k_edge_flags_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  edge_flags_body(F.fp, F.blocks, F.sbs, F.ys, F.uvs, b.x, b.y, b.z);
+}
+__global__ __launch_bounds__(64) void k_edge_types_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  edge_types_body(F.fp, F.blocks, F.sbs, F.ys, F.uvs, b.x, b.y, b.z);
+}
+__global__ void k_recon_list_reset_b(const BatchFrame *__restrict__ T) {
+  if (threadIdx.x < 2) T[blockIdx.x].rwork[threadIdx.x] = 0;
+}
+__global__ __launch_bounds__(64) void k_recon_list_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  recon_list_body(F.fp, F.blocks, F.sbs, F.rwork, b.x, b.y, b.z);
+}
+
+/* Inter phase over several frames: every workgroup takes the inter blocks
+ * of frame 0 off its list until the list is empty, then frame 1's, and so
+ * on; the frame-dependent LDS tables are staged when a workgroup takes its
+ * first block of a frame. */
+__global__ __launch_bounds__(RT) void k_recon_inter_b(const BatchFrame *__restrict__ T, int nfr) {
+  RLds &L = gL;
+  const int tid = threadIdx.x;
+  stage_tabs(tid, RT);
+  for (int f = 0; f < nfr; f++) {
+    const BatchFrame &F = T[f];
+    int *work = F.rwork;
+    const int count = work[0];
+    bool staged = false;
+    for (;;) {
+      if (tid == 0) L.ival[2] = atomicAdd(&work[1], 1);
+      __syncthreads();
+      const int idx = L.ival[2];
+      __syncthreads();
+      if (idx >= count) break;
+      if (!staged) {
+        stage_smats(F.fp, F.mats);
+        __syncthreads();
+        staged = true;
+      }
+      Mav1BlockInfo *gb = F.blocks + work[2 + idx];
+      if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)gb)[tid];
+      __syncthreads();
+      recon_block(F.fp, F.src, F.rec, false, F.ref, F.gold, F.coefs, F.mats, false, false);
+      if (tid < 27) ((int32_t *)gb)[tid] = ((const int32_t *)&L.blk)[tid];
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(SCHED_T) void k_intra_sched_b(const BatchFrame *__restrict__ T) {
+  const BatchFrame &F = T[blockIdx.x];
+  intra_sched_body(F.fp, F.blocks, F.sbs, F.iwork, 0, 0, 0);
+}
+
+/* Intra phase over several frames: the workgroups take frame 0's listed
+ * superblocks, then frame 1's, and so on (each frame's own counter, in its
+ * list's order). A workgroup waits only for superblocks of its frame that
+ * running workgroups took before, so every wait ends, as in the
+ * single-frame launch; a wait that runs out sets that frame's error word. */
+__global__ __launch_bounds__(RT) void k_recon_intra_b(const BatchFrame *__restrict__ T, int nfr, int maxSpin) {
+  RLds &L = gL;
+  const int tid = threadIdx.x;
+  int f = 0, staged = -1;
+  for (;;) {
+    if (tid == 0) {
+      int idx = 0;
+      while (f < nfr) {
+        idx = atomicAdd(&T[f].iwork[IW_NEXT], 1);
+        if (idx < T[f].iwork[IW_COUNT]) break;
+        f++;
+      }
+      L.ival[2] = idx;
+      L.ival[3] = f;
+    }
+    __syncthreads();
+    const int idx = L.ival[2];
+    f = L.ival[3];
+    __syncthreads();
+    if (f >= nfr) break;
+    const BatchFrame &F = T[f];
+    const Mav1FrameParams &fp = F.fp;
+    if (staged < 0) stage_tabs(tid, RT);
+    if (staged != f) {
+      stage_smats(fp, F.mats);
+      staged = f;
+    }
+    const int n = fp.sb_cols * fp.sb_rows;
+    const int *list = F.iwork + IW_HDR, *has = list + n;
+    int *done = F.iwork + IW_HDR + 2 * n;
+    const int sb = list[idx], r = sb / fp.sb_cols, c = sb - r * fp.sb_cols;
+    if (tid == 0) {
+      const int ts = (c / fp.tile_w_sb) * fp.tile_w_sb, te = d_min(fp.sb_cols, ts + fp.tile_w_sb);
+      const int dep[4] = {c > ts ? sb - 1 : -1, r > 0 && c > ts ? sb - fp.sb_cols - 1 : -1, r > 0 ? sb - fp.sb_cols : -1,
+                          r > 0 && c + 1 < te ? sb - fp.sb_cols + 1 : -1};
+      for (int k = 0; k < 4; k++)
+        if (dep[k] >= 0 && has[dep[k]] && !wait_flag(&done[dep[k]], maxSpin)) atomicOr(&F.iwork[IW_ERR], 1);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    intra_sb(fp, F.blocks, F.sbs[sb], F.src, F.rec, F.ref, F.gold, F.coefs, F.mats, r, c);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&done[sb], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+  }
+}
+
+hipError_t launch_edge_types_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) batch_grid_add(g, H[f].fp.sb_cols * H[f].fp.sb_rows, 1, 1);
+  hipLaunchKernelGGL(k_edge_flags_b, dim3(g.start[n]), dim3(64), 0, st, T, g);
+  hipLaunchKernelGGL(k_edge_types_b, dim3(g.start[n]), dim3(64), 0, st, T, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_recon_inter_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int workgroups) {
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) batch_grid_add(g, H[f].fp.sb_cols * H[f].fp.sb_rows, 1, 1);
+  hipLaunchKernelGGL(k_recon_list_reset_b, dim3(n), dim3(64), 0, st, T);
+  hipLaunchKernelGGL(k_recon_list_b, dim3(g.start[n]), dim3(64), 0, st, T, g);
+  hipLaunchKernelGGL(k_recon_inter_b, dim3(workgroups), dim3(RT), 0, st, T, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_intra_sched_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {
+  (void)H;
+  hipLaunchKernelGGL(k_intra_sched_b, dim3(n), dim3(SCHED_T), 0, st, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_recon_intra_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int workgroups) {
+  (void)H;
+  hipLaunchKernelGGL(k_recon_intra_b, dim3(workgroups), dim3(RT), 0, st, T, n, 1 << 20);
   return hipGetLastError();
 }

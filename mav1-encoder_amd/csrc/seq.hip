@@ -131,7 +131,40 @@ struct mav1hip_seq {
   int held = -1;         /* slot handed to the caller by the last receive */
   long long coded = 0;
   std::vector<Frame> pending; /* second-engine frames waiting for the next ALTREF's early stages */
+  mav1hip_batch *batch = nullptr; /* coded in batches with other sequences (frames queue in `jobs`) */
+  std::deque<Frame> jobs;
   size_t o_blocks, o_sbs, o_coefs, o_choice, o_units, o_lrtype, o_err, o_pic, o_grid, o_end;
+};
+
+namespace {
+constexpr int kTabs = 8; /* batch tables in flight */
+
+/* Frames of several sequences coded together: the same position of their
+ * coding order, the same stages. */
+struct Round {
+  int engine = 0;
+  int tab = -1;
+  std::vector<mav1hip_seq *> seqs;
+  std::vector<Frame> frames;
+};
+}  // namespace
+
+struct mav1hip_batch {
+  std::vector<mav1hip_seq *> seqs;
+  int lr_test = 0, dump = 0;
+  char err[512] = {0};
+  int num_cu = 0;
+  /* the frame tables: kTabs x kMaxBatch entries in device memory and their
+   * pinned host mirror; a table is reused once the round that used it has
+   * finished on the GPU */
+  BatchFrame *d_tab = nullptr, *h_tab = nullptr;
+  hipEvent_t tab_free[kTabs] = {};
+  int tab_used[kTabs] = {};
+  int tab_next = 0;
+  /* per engine: analysis beside motion search, the intra scheduling pass
+   * beside the inter phase */
+  hipEvent_t me_done[2][4] = {}, ana_done[2] = {}, rec_go[2] = {}, sched_done[2] = {};
+  std::vector<Round> pending; /* second-engine rounds waiting for the next ALTREF round's early stages */
 };
 
 namespace {
@@ -300,9 +333,8 @@ int bind_source(mav1hip_seq *s, Frame &f) {
   return MAV1HIP_OK;
 }
 
-/* Source, references and parameters, then motion search, analysis and edge
- * types. */
-int early_stages(mav1hip_seq *s, Frame &f) {
+/* The frame's source, references and parameters bound to its engine. */
+int bind_frame(mav1hip_seq *s, Frame &f) {
   const int e = f.engine;
   Engine &E = s->eng[e];
   int rc;
@@ -321,6 +353,16 @@ int early_stages(mav1hip_seq *s, Frame &f) {
   }
   CCHK(s, e, ctx_bind_ptrs(E.ctx, f.src, ref, gold));
   CCHK(s, e, mav1hip_set_params(E.ctx, &f.fp));
+  return MAV1HIP_OK;
+}
+
+/* Source, references and parameters, then motion search, analysis and edge
+ * types. */
+int early_stages(mav1hip_seq *s, Frame &f) {
+  const int e = f.engine;
+  Engine &E = s->eng[e];
+  int rc;
+  if ((rc = bind_frame(s, f))) return rc;
   if (!f.is_key) CCHK(s, e, mav1hip_launch_motion_search(E.ctx));
   CCHK(s, e, mav1hip_launch_analyze(E.ctx));
   if (s->cfg.intra_edge_filter) CCHK(s, e, mav1hip_launch_edge_types(E.ctx));
@@ -381,6 +423,8 @@ int stage_buf(mav1hip_seq *s, StageList &L, Engine &E, int buf, uint8_t *dst, ui
   return stage_add(s, L, src, dst, dst2, bytes);
 }
 
+int finish_frame(mav1hip_seq *s, Frame &f);
+
 /* Reconstruction and the in-loop filters (run_frame_core,
  * src/encoder.c:611-641), the outputs staged and sent down, the refreshed
  * slots stored. */
@@ -403,16 +447,24 @@ int late_stages(mav1hip_seq *s, Frame &f) {
     CCHK(s, e, mav1hip_launch_cdef_select(c, pl.cdef_sets));
   else if (cdefOff)
     CCHK(s, e, ctx_clear_choice(c));
-  int fin = MAV1HIP_BUF_REC;
   if (pl.restoration) {
     CCHK(s, e, mav1hip_launch_lr_select(c, s->lr_test));
     CCHK(s, e, ctx_rec_to_lrout(c));
     CCHK(s, e, mav1hip_launch_restore(c));
-    fin = MAV1HIP_BUF_LROUT;
   }
   if (s->dump) CCHK(s, e, mav1hip_launch_decision_grid(c));
-  /* outputs into the staging slot (the final picture also into the picture
-   * its refreshed slots will map to), then down on the download stream */
+  return finish_frame(s, f);
+}
+
+/* The frame's outputs into its staging slot (the final picture also into
+ * the picture its refreshed slots will map to), then down on the transfer
+ * stream; the refreshed slots then map to that picture. */
+int finish_frame(mav1hip_seq *s, Frame &f) {
+  const int e = f.engine;
+  Engine &E = s->eng[e];
+  mav1hip_ctx *c = E.ctx;
+  const Mav1HipFilterPlan &pl = f.plan;
+  const int fin = pl.restoration ? MAV1HIP_BUF_LROUT : MAV1HIP_BUF_REC;
   OutSlot &o = s->outs[f.out];
   int rc;
   int b = -1;
@@ -515,6 +567,10 @@ int code_frame(mav1hip_seq *s, const Input &in, int isKey, int kind, int lastSlo
   Frame f;
   int rc = make_frame(s, in, isKey, kind, lastSlot, arfSlot, refresh, hidden, &f);
   if (rc) return rc;
+  if (s->batch) { /* the batch codes it together with the other sequences' frames */
+    s->jobs.push_back(f);
+    return MAV1HIP_OK;
+  }
   if (f.engine != 0) {
     s->pending.push_back(f);
     return MAV1HIP_OK;
@@ -719,6 +775,257 @@ int setup(mav1hip_seq *s) {
   return MAV1HIP_OK;
 }
 
+/* ---- batches of sequences ----
+ * Each sequence keeps its own driver state, engines, slot pictures and
+ * outputs; its frames queue as jobs. A round takes the next job of every
+ * sequence (the same coding position when the streams are fed in step) and
+ * runs each stage as one launch over all of them (batch.h), on the streams
+ * of the jobs' engine, which every sequence on the device shares. Rounds of
+ * the second engine wait for the next ALTREF round's early stages, as single
+ * frames do. */
+
+int bfail(mav1hip_batch *b, int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(b->err, sizeof b->err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define BCHK(b, call)                                                                                       \
+  do {                                                                                                      \
+    hipError_t e_ = (call);                                                                                 \
+    if (e_ != hipSuccess)                                                                                   \
+      return bfail(b, MAV1HIP_EDEVICE, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+/* A sequence call's status, with the sequence's message. */
+int from_seq(mav1hip_batch *b, int i, int rc) {
+  if (rc != MAV1HIP_OK && rc != 1) snprintf(b->err, sizeof b->err, "stream %d: %s", i, b->seqs[i]->err);
+  return rc;
+}
+
+/* What decides a frame's stages: frames batched together agree on it. */
+struct Sig {
+  int v[16];
+};
+Sig sig_of(const mav1hip_seq *s, const Frame &f) {
+  Sig g;
+  memset(&g, 0, sizeof g);
+  const int leafopt = (s->cfg.altref && f.nonref) ? (s->cfg.knob_leafopt >= 0 ? s->cfg.knob_leafopt : 0x3F) : 0;
+  const int v[] = {f.engine,           f.is_key,          f.kind,
+                   f.fp.golden,        f.fp.turbo,        f.plan.filter_info,
+                   f.plan.deblock,     f.plan.cdef_sets,  f.plan.cdef_select,
+                   f.plan.restoration, (leafopt & 4) != 0, s->cfg.intra_edge_filter,
+                   f.fp.frame_is_intra};
+  memcpy(g.v, v, sizeof v);
+  return g;
+}
+bool same_sig(const Sig &a, const Sig &b) { return memcmp(&a, &b, sizeof a) == 0; }
+
+bool seq_finished(const mav1hip_seq *s) { return s->ended && s->inq.empty() && s->jobs.empty(); }
+
+int take_table(mav1hip_batch *b, int *t) {
+  const int i = b->tab_next;
+  b->tab_next = (i + 1) % kTabs;
+  if (b->tab_used[i]) {
+    BCHK(b, hipEventSynchronize(b->tab_free[i]));
+    b->tab_used[i] = 0;
+  }
+  *t = i;
+  return MAV1HIP_OK;
+}
+
+#define LCHK(b, what, call)                                                                         \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
+    if (e_ != hipSuccess) return bfail(b, MAV1HIP_EDEVICE, "batched %s: %s", what, hipGetErrorString(e_)); \
+  } while (0)
+
+/* Sources, references and parameters of every frame, the table, then motion
+ * search, analysis and edge types as batched launches. */
+int round_early(mav1hip_batch *b, Round &R) {
+  const int n = (int)R.frames.size(), e = R.engine;
+  for (int k = 0; k < n; k++) {
+    int rc = bind_frame(R.seqs[k], R.frames[k]);
+    if (rc) return bfail(b, rc, "%s", R.seqs[k]->err);
+  }
+  int rc = take_table(b, &R.tab);
+  if (rc) return rc;
+  BatchFrame *T = b->d_tab + (size_t)R.tab * kMaxBatch, *H = b->h_tab + (size_t)R.tab * kMaxBatch;
+  for (int k = 0; k < n; k++)
+    if (ctx_batch_frame(R.seqs[k]->eng[e].ctx, &H[k], R.frames[k].plan.restoration))
+      return bfail(b, MAV1HIP_EINVAL, "batch: frame %d has no parameters", k);
+  mav1hip_seq *s0 = R.seqs[0];
+  hipStream_t st = s0->eng[e].st, side = ctx_side_stream(s0->eng[e].ctx);
+  BCHK(b, hipMemcpyAsync(T, H, sizeof(BatchFrame) * n, hipMemcpyHostToDevice, st));
+  const Frame &f0 = R.frames[0];
+  const bool me = !f0.is_key;
+  if (me) {
+    const int turbo = f0.fp.turbo, nref = f0.fp.golden ? 2 : 1;
+    const bool hp = (turbo & MAV1_T_ME_HPEL) != 0, dec = (turbo & MAV1_T_ME_DEC) != 0;
+    if (!hp && dec) return bfail(b, MAV1HIP_EUNSUP, "motion search: turbo bit 32 without bit 16");
+    if (hp) {
+      for (int r = 0; r < nref; r++) {
+        LCHK(b, "qpel_build", launch_qpel_build_b(st, T, H, n, r));
+        if (dec) LCHK(b, "dec_build", launch_dec_build_b(st, T, H, n, r));
+      }
+      if (dec) LCHK(b, "dec_build", launch_dec_build_b(st, T, H, n, 2));
+    }
+    for (int level = 0; level < 4; level++) {
+      LCHK(b, "me_level", launch_me_level_b(st, level, T, H, n, nref));
+      BCHK(b, hipEventRecord(b->me_done[e][level], st));
+    }
+  }
+  /* analysis level k after motion-search level k, beside level k + 1 */
+  hipStream_t ast = me ? side : st;
+  for (int level = 0; level < 4; level++) {
+    if (me) BCHK(b, hipStreamWaitEvent(side, b->me_done[e][level], 0));
+    LCHK(b, "analyze", launch_analyze_b(ast, level, T, H, n));
+  }
+  LCHK(b, "analyze_table", launch_analyze_table_b(ast, T, H, n));
+  if (me) {
+    BCHK(b, hipEventRecord(b->ana_done[e], side));
+    BCHK(b, hipStreamWaitEvent(st, b->ana_done[e], 0));
+  }
+  if (s0->cfg.intra_edge_filter) LCHK(b, "edge_types", launch_edge_types_b(st, T, H, n));
+  return MAV1HIP_OK;
+}
+
+/* Reconstruction and the in-loop filters as batched launches, then every
+ * frame's outputs staged and sent down. */
+int round_late(mav1hip_batch *b, Round &R) {
+  const int n = (int)R.frames.size(), e = R.engine;
+  mav1hip_seq *s0 = R.seqs[0];
+  hipStream_t st = s0->eng[e].st, side = ctx_side_stream(s0->eng[e].ctx);
+  const BatchFrame *T = b->d_tab + (size_t)R.tab * kMaxBatch, *H = b->h_tab + (size_t)R.tab * kMaxBatch;
+  /* the scheduling pass beside the inter phase */
+  BCHK(b, hipEventRecord(b->rec_go[e], st));
+  BCHK(b, hipStreamWaitEvent(side, b->rec_go[e], 0));
+  LCHK(b, "intra_sched", launch_intra_sched_b(side, T, H, n));
+  BCHK(b, hipEventRecord(b->sched_done[e], side));
+  LCHK(b, "recon_inter", launch_recon_inter_b(st, T, H, n, 2 * b->num_cu));
+  BCHK(b, hipStreamWaitEvent(st, b->sched_done[e], 0));
+  LCHK(b, "recon_intra", launch_recon_intra_b(st, T, H, n, b->num_cu));
+  const Frame &f0 = R.frames[0];
+  const Mav1HipFilterPlan &pl = f0.plan;
+  if (pl.filter_info) {
+    LCHK(b, "filter_info", launch_filter_info_b(st, T, H, n));
+    if (pl.deblock) LCHK(b, "deblock", launch_deblock_b(st, T, H, n));
+    if (pl.cdef_sets) LCHK(b, "cdef", launch_cdef_b(st, T, H, n, pl.cdef_sets));
+  }
+  const int leafopt = (s0->cfg.altref && f0.nonref) ? (s0->cfg.knob_leafopt >= 0 ? s0->cfg.knob_leafopt : 0x3F) : 0;
+  const int cdefOff = (leafopt & 4) != 0;
+  if (pl.restoration) LCHK(b, "deblocked copy", launch_frame_op_b(st, T, H, n, 0));
+  if (pl.cdef_select)
+    LCHK(b, "cdef_select", launch_cdef_select_b(st, T, H, n, pl.cdef_sets));
+  else if (cdefOff)
+    LCHK(b, "cdef choice clear", launch_frame_op_b(st, T, H, n, 2));
+  if (pl.restoration) {
+    /* the fit, then chain launches with the refinements between them, as
+     * the single-frame restoration search */
+    const int LR_ROUNDS = 3;
+    LCHK(b, "lr_fit", launch_lr_fit_b(st, T, H, n));
+    for (int r = 0; r < LR_ROUNDS; r++) {
+      if (r > 0) LCHK(b, "lr_path", launch_lr_path_b(st, T, H, n));
+      LCHK(b, "lr_chain", launch_lr_chain_b(st, T, H, n, b->lr_test != 0, r < LR_ROUNDS - 1));
+    }
+    LCHK(b, "restoration copy", launch_frame_op_b(st, T, H, n, 1));
+    LCHK(b, "lr_apply", launch_lr_apply_b(st, T, H, n));
+  }
+  if (b->dump) LCHK(b, "decision_grid", launch_decision_grid_b(st, T, H, n));
+  BCHK(b, hipEventRecord(b->tab_free[R.tab], st));
+  b->tab_used[R.tab] = 1;
+  for (int k = 0; k < n; k++) {
+    int rc = finish_frame(R.seqs[k], R.frames[k]);
+    if (rc) return bfail(b, rc, "%s", R.seqs[k]->err);
+  }
+  return MAV1HIP_OK;
+}
+
+int flush_rounds(mav1hip_batch *b) {
+  for (auto &R : b->pending) {
+    int rc = round_early(b, R);
+    if (!rc) rc = round_late(b, R);
+    if (rc) return rc;
+  }
+  b->pending.clear();
+  return MAV1HIP_OK;
+}
+
+/* code_frame for a round */
+int exec_round(mav1hip_batch *b, Round &R) {
+  if (R.engine != 0) {
+    b->pending.push_back(R);
+    return MAV1HIP_OK;
+  }
+  const Frame &f0 = R.frames[0];
+  const bool arf = !f0.is_key && f0.kind == 1;
+  int rc;
+  if (!arf && (rc = flush_rounds(b))) return rc;
+  if ((rc = round_early(b, R))) return rc;
+  if (arf && (rc = flush_rounds(b))) return rc;
+  return round_late(b, R);
+}
+
+/* Runs the rounds that can be formed: every sequence that can still code
+ * a frame has a job (all: also rounds of the sequences that have one). */
+int pump(mav1hip_batch *b, int all) {
+  for (;;) {
+    int first = -1;
+    for (int i = 0; i < (int)b->seqs.size() && first < 0; i++)
+      if (!b->seqs[i]->jobs.empty()) first = i;
+    if (first < 0) break;
+    if (!all)
+      for (auto *s : b->seqs)
+        if (s->jobs.empty() && !seq_finished(s)) return MAV1HIP_OK;
+    Round R;
+    R.engine = b->seqs[first]->jobs.front().engine;
+    const Sig g0 = sig_of(b->seqs[first], b->seqs[first]->jobs.front());
+    for (auto *s : b->seqs)
+      if (!s->jobs.empty() && same_sig(sig_of(s, s->jobs.front()), g0) && (int)R.frames.size() < kMaxBatch) {
+        R.seqs.push_back(s);
+        R.frames.push_back(s->jobs.front());
+        s->jobs.pop_front();
+      }
+    int rc = exec_round(b, R);
+    if (rc) return rc;
+  }
+  bool done = true;
+  for (auto *s : b->seqs) done = done && seq_finished(s);
+  return done ? flush_rounds(b) : MAV1HIP_OK;
+}
+
+void batch_destroy(mav1hip_batch *b) {
+  if (!b) return;
+  for (auto *s : b->seqs) destroy(s);
+  for (auto &ev : b->tab_free)
+    if (ev) (void)hipEventDestroy(ev);
+  for (int e = 0; e < 2; e++) {
+    for (auto &ev : b->me_done[e])
+      if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : {b->ana_done[e], b->rec_go[e], b->sched_done[e]})
+      if (ev) (void)hipEventDestroy(ev);
+  }
+  (void)hipFree(b->d_tab);
+  if (b->h_tab) (void)hipHostFree(b->h_tab);
+  delete b;
+}
+
+int batch_setup(mav1hip_batch *b) {
+  BCHK(b, hipMalloc((void **)&b->d_tab, sizeof(BatchFrame) * kTabs * kMaxBatch));
+  BCHK(b, hipHostMalloc((void **)&b->h_tab, sizeof(BatchFrame) * kTabs * kMaxBatch, hipHostMallocDefault));
+  for (auto &ev : b->tab_free) BCHK(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  for (int e = 0; e < 2; e++) {
+    for (auto &ev : b->me_done[e]) BCHK(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    BCHK(b, hipEventCreateWithFlags(&b->ana_done[e], hipEventDisableTiming));
+    BCHK(b, hipEventCreateWithFlags(&b->rec_go[e], hipEventDisableTiming));
+    BCHK(b, hipEventCreateWithFlags(&b->sched_done[e], hipEventDisableTiming));
+  }
+  b->num_cu = ctx_num_cu(b->seqs[0]->eng[0].ctx);
+  return MAV1HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -865,6 +1172,78 @@ int mav1hip_seq_receive(mav1hip_seq *s, Mav1HipCodedFrame *out, int32_t wait) {
     out->decision_bytes = 44 + s->grid_bytes;
   }
   return 0;
+}
+
+mav1hip_batch *mav1hip_batch_create(const Mav1HipConfig *cfgs, int32_t n, int32_t lr_test, int32_t dump_decisions,
+                                    char *err, size_t errlen) {
+  if (!cfgs || n < 1 || n > kMaxBatch) {
+    if (err && errlen) snprintf(err, errlen, "a batch holds 1 to %d streams", kMaxBatch);
+    return nullptr;
+  }
+  mav1hip_batch *b = new mav1hip_batch();
+  b->lr_test = lr_test;
+  b->dump = dump_decisions != 0;
+  for (int i = 0; i < n; i++) {
+    char e2[512] = {0};
+    mav1hip_seq *s = mav1hip_seq_create(&cfgs[i], lr_test, dump_decisions, e2, sizeof e2);
+    if (!s) {
+      if (err && errlen) snprintf(err, errlen, "stream %d: %s", i, e2);
+      batch_destroy(b);
+      return nullptr;
+    }
+    s->batch = b;
+    b->seqs.push_back(s);
+  }
+  if (batch_setup(b) != MAV1HIP_OK) {
+    if (err && errlen) snprintf(err, errlen, "%s", b->err);
+    batch_destroy(b);
+    return nullptr;
+  }
+  return b;
+}
+
+void mav1hip_batch_destroy(mav1hip_batch *b) { batch_destroy(b); }
+
+const char *mav1hip_batch_error(const mav1hip_batch *b) { return b ? b->err : "no batch"; }
+
+int mav1hip_batch_send_frame(mav1hip_batch *b, int32_t stream, const uint8_t *const planes[3],
+                             const int32_t strides[3]) {
+  if (!b || stream < 0 || stream >= (int)b->seqs.size()) return MAV1HIP_EINVAL;
+  int rc = from_seq(b, stream, mav1hip_seq_send_frame(b->seqs[stream], planes, strides));
+  if (rc) return rc;
+  return pump(b, 0);
+}
+
+int mav1hip_batch_send_device_frame(mav1hip_batch *b, int32_t stream, const uint8_t *frame) {
+  if (!b || stream < 0 || stream >= (int)b->seqs.size()) return MAV1HIP_EINVAL;
+  int rc = from_seq(b, stream, mav1hip_seq_send_device_frame(b->seqs[stream], frame));
+  if (rc) return rc;
+  return pump(b, 0);
+}
+
+int mav1hip_batch_receive(mav1hip_batch *b, int32_t stream, Mav1HipCodedFrame *out, int32_t wait) {
+  if (!b || !out || stream < 0 || stream >= (int)b->seqs.size()) return MAV1HIP_EINVAL;
+  mav1hip_seq *s = b->seqs[stream];
+  if (!s->order.empty()) {
+    /* the next frame may still be a job or in a waiting round: with wait,
+     * run what can run now, in rounds of the sequences that have a job */
+    const int i0 = s->order.front();
+    bool queued = false;
+    for (auto &f : s->jobs) queued |= f.out == i0;
+    for (auto &R : b->pending)
+      for (size_t k = 0; k < R.seqs.size(); k++) queued |= R.seqs[k] == s && R.frames[k].out == i0;
+    if (queued) {
+      if (!wait) return 1;
+      int rc = pump(b, 1);
+      if (!rc) rc = flush_rounds(b);
+      if (rc) return rc;
+    }
+  }
+  return from_seq(b, stream, mav1hip_seq_receive(s, out, wait));
+}
+
+mav1hip_seq *mav1hip_batch_stream(mav1hip_batch *b, int32_t stream) {
+  return b && stream >= 0 && stream < (int)b->seqs.size() ? b->seqs[stream] : nullptr;
 }
 
 }  // extern "C"

@@ -144,6 +144,13 @@ def _load():
         "mav1hip_seq_send_device_frame": (C.c_int, [P, P]),
         "mav1hip_seq_receive": (C.c_int, [P, P, i32]),
         "mav1hip_seq_engine": (P, [P, i32]),
+        "mav1hip_batch_create": (P, [P, i32, i32, i32, C.c_char_p, C.c_size_t]),
+        "mav1hip_batch_destroy": (None, [P]),
+        "mav1hip_batch_error": (C.c_char_p, [P]),
+        "mav1hip_batch_send_frame": (C.c_int, [P, i32, P, P]),
+        "mav1hip_batch_send_device_frame": (C.c_int, [P, i32, P]),
+        "mav1hip_batch_receive": (C.c_int, [P, i32, P, i32]),
+        "mav1hip_batch_stream": (P, [P, i32]),
         "mav1hip_y4m_open": (P, [C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                  C.c_char_p, C.c_size_t]),
         "mav1hip_y4m_read": (C.c_int, [P, P, P]),
@@ -367,24 +374,94 @@ class Sequence:
 
     def receive(self, wait=True):
         """The next coded frame as a dict of numpy copies, or None."""
-        out = self.receive_raw(wait)
-        if out is None:
+        return coded_dict(self.receive_raw(wait))
+
+
+def coded_dict(out):
+    """A CodedFrame's fields and buffers as a dict of numpy copies (None for
+    None)."""
+    if out is None:
+        return None
+    fp = out.fp
+    nsb = fp.sb_cols * fp.sb_rows
+    nunits = fp.lr_unit_off[2] + fp.lr_unit_rows[2] * fp.lr_unit_cols[2]
+    fb = fp.offset[2] + fp.stride[2] * fp.plane_h[2]
+    d = {n: getattr(out, n) for n, _ in CodedFrame._fields_[:13]}
+    d["fp"] = FrameParams.from_buffer_copy(fp)
+    d["plan"] = FilterPlan.from_buffer_copy(out.plan)
+    d["blocks"] = _host_array(out.blocks, np.int32, nsb * 64 * 27).reshape(-1, 27)
+    d["sbs"] = _host_array(out.sbs, np.int32, nsb * 89).reshape(-1, 89)
+    d["coefs"] = _host_array(out.coefs, np.int32, nsb * 6144)
+    d["cdef_choice"] = _host_array(out.cdef_choice, np.int8, nsb)
+    d["lr_units"] = None if not out.lr_units else _host_array(out.lr_units, np.int32, nunits * 12).reshape(-1, 12)
+    d["recon"] = _host_array(out.recon, np.uint8, fb)
+    d["decision"] = None if not out.decision else _host_array(out.decision, np.uint8, out.decision_bytes)
+    return d
+
+
+class Batch:
+    """Several streams coded together (mav1hip_batch_*): every stage one
+    launch over the frames of all streams at the same coding position. Feed
+    the streams in step; each stream's coded frames equal its own
+    Sequence's."""
+
+    def __init__(self, cfgs, lr_test=0, dump_decisions=False):
+        arr = (Config * len(cfgs))(*cfgs)
+        err = C.create_string_buffer(512)
+        h = lib().mav1hip_batch_create(arr, len(cfgs), lr_test, int(bool(dump_decisions)), err, 512)
+        if not h:
+            raise HipError(err.value.decode())
+        self.h = h
+        self.cfgs = list(cfgs)
+
+    def close(self):
+        if self.h:
+            lib().mav1hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, what, rc):
+        return HipError(f"{what}: {lib().mav1hip_batch_error(self.h).decode()} (rc {rc})")
+
+    def send(self, i, planes):
+        """I420 planes for stream i, or None to end it. False on EBUSY."""
+        if planes is None:
+            rc = lib().mav1hip_batch_send_frame(self.h, i, None, None)
+        else:
+            pl = [np.ascontiguousarray(p, np.uint8) for p in planes]
+            arr = (C.c_void_p * 3)(*[p.ctypes.data for p in pl])
+            st = (C.c_int32 * 3)(*[p.shape[1] for p in pl])
+            rc = lib().mav1hip_batch_send_frame(self.h, i, arr, st)
+        if rc == EBUSY:
+            return False
+        if rc:
+            raise self._err("send_frame", rc)
+        return True
+
+    def send_device(self, i, dev_ptr):
+        rc = lib().mav1hip_batch_send_device_frame(self.h, i, C.c_void_p(dev_ptr))
+        if rc == EBUSY:
+            return False
+        if rc:
+            raise self._err("send_device_frame", rc)
+        return True
+
+    def receive_raw(self, i, wait=True):
+        out = CodedFrame()
+        rc = lib().mav1hip_batch_receive(self.h, i, C.byref(out), int(bool(wait)))
+        if rc == 1:
             return None
-        fp = out.fp
-        nsb = fp.sb_cols * fp.sb_rows
-        nunits = fp.lr_unit_off[2] + fp.lr_unit_rows[2] * fp.lr_unit_cols[2]
-        fb = fp.offset[2] + fp.stride[2] * fp.plane_h[2]
-        d = {n: getattr(out, n) for n, _ in CodedFrame._fields_[:13]}
-        d["fp"] = FrameParams.from_buffer_copy(fp)
-        d["plan"] = FilterPlan.from_buffer_copy(out.plan)
-        d["blocks"] = _host_array(out.blocks, np.int32, nsb * 64 * 27).reshape(-1, 27)
-        d["sbs"] = _host_array(out.sbs, np.int32, nsb * 89).reshape(-1, 89)
-        d["coefs"] = _host_array(out.coefs, np.int32, nsb * 6144)
-        d["cdef_choice"] = _host_array(out.cdef_choice, np.int8, nsb)
-        d["lr_units"] = None if not out.lr_units else _host_array(out.lr_units, np.int32, nunits * 12).reshape(-1, 12)
-        d["recon"] = _host_array(out.recon, np.uint8, fb)
-        d["decision"] = None if not out.decision else _host_array(out.decision, np.uint8, out.decision_bytes)
-        return d
+        if rc:
+            raise self._err("receive", rc)
+        return out
+
+    def receive(self, i, wait=True):
+        return coded_dict(self.receive_raw(i, wait))
 
 
 class Context:
