@@ -102,8 +102,12 @@ hipError_t launch_analyze_b(hipStream_t st, int level, const BatchFrame *T, cons
 hipError_t launch_analyze_table_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n);
 hipError_t launch_edge_types_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n);
 hipError_t launch_recon_inter_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int workgroups);
-hipError_t launch_intra_sched_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n);
-hipError_t launch_recon_intra_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int workgroups);
+/* The scheduling pass of every frame, then the batch's ready queue (q: 4 +
+ * qcap ints, qcap >= the round's superblocks) reset and filled with the
+ * superblocks that depend on none. */
+hipError_t launch_intra_sched_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int *q, int qcap);
+hipError_t launch_recon_intra_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int *q,
+                                int workgroups);
 hipError_t launch_filter_info_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n);
 hipError_t launch_deblock_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n);
 hipError_t launch_cdef_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int nsets);

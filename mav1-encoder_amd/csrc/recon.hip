@@ -21,6 +21,7 @@
  * rate, inverse transform and distortion), the nine dual-filter pairs over
  * three shared horizontal passes, the 33 CfL alphas over the waves. Larger
  * transforms (32- and 64-point) use the whole workgroup per candidate. */
+#include <algorithm>
 #include <climits>
 
 #include "dev_common.h"
@@ -1909,7 +1910,8 @@ hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
 }
 
 
-size_t intra_work_ints(const Mav1FrameParams &fp) { return IW_HDR + 3 * (size_t)fp.sb_cols * fp.sb_rows; }
+/* list, has, done and (batched form) the pending-dependency counts */
+size_t intra_work_ints(const Mav1FrameParams &fp) { return IW_HDR + 4 * (size_t)fp.sb_cols * fp.sb_rows; }
 int intra_sched_fits(const Mav1FrameParams &fp) {
   return (fp.sb_cols + 2 * fp.sb_rows) * fp.sb_rows <= SCHED_KEYS && fp.sb_cols * fp.sb_rows < 65535;
 }
@@ -1988,31 +1990,92 @@ __global__ __launch_bounds__(SCHED_T) void k_intra_sched_b(const BatchFrame *__r
   intra_sched_body(F.fp, F.blocks, F.sbs, F.iwork, 0, 0, 0);
 }
 
-/* Intra phase over several frames: the workgroups take frame 0's listed
- * superblocks, then frame 1's, and so on (each frame's own counter, in its
- * list's order). A workgroup waits only for superblocks of its frame that
- * running workgroups took before, so every wait ends, as in the
- * single-frame launch; a wait that runs out sets that frame's error word. */
-__global__ __launch_bounds__(RT) void k_recon_intra_b(const BatchFrame *__restrict__ T, int nfr, int maxSpin) {
+/* Intra phase over several frames: a dataflow queue. A superblock holding
+ * intra blocks enters the batch's ready queue when the last of its
+ * dependencies (the neighbours in its tile column whose reconstruction it
+ * reads and that hold intra blocks) is done, so a workgroup only ever takes
+ * a superblock it can code at once, of whichever frame; superblocks without
+ * such dependencies are queued first. q: [QH_HEAD] next entry to take,
+ * [QH_TAIL] next entry to fill, [QH_TOTAL] superblocks of the round, then
+ * the entries ((frame << 16) | superblock, -1 until filled). Every entry is
+ * filled once all superblocks before it in dependency order are done, so a
+ * workgroup's wait for its entry ends; the wait is still bounded, and one
+ * that runs out sets every frame's error word. */
+#define QH_HEAD 0
+#define QH_TAIL 1
+#define QH_TOTAL 2
+#define QH_ITEMS 4
+
+/* The superblocks whose reconstruction superblock sb reads (-1: none), as
+ * the single-frame intra phase waits for them. */
+DEVI void intra_deps(const Mav1FrameParams &fp, int sb, int dep[4]) {
+  const int r = sb / fp.sb_cols, c = sb - r * fp.sb_cols;
+  const int ts = (c / fp.tile_w_sb) * fp.tile_w_sb, te = d_min(fp.sb_cols, ts + fp.tile_w_sb);
+  dep[0] = c > ts ? sb - 1 : -1;
+  dep[1] = r > 0 && c > ts ? sb - fp.sb_cols - 1 : -1;
+  dep[2] = r > 0 ? sb - fp.sb_cols : -1;
+  dep[3] = r > 0 && c + 1 < te ? sb - fp.sb_cols + 1 : -1;
+}
+
+DEVI void queue_push(int *q, int v) {
+  const int slot = atomicAdd(&q[QH_TAIL], 1);
+  __hip_atomic_store(&q[QH_ITEMS + slot], v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_intra_queue_reset_b(int *__restrict__ q, int cap) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x) q[QH_ITEMS + i] = -1;
+  if (blockIdx.x == 0 && threadIdx.x < QH_ITEMS) q[threadIdx.x] = 0;
+}
+
+/* Per frame (one workgroup each, after the scheduling pass): the pending
+ * dependency count of every superblock holding intra blocks; those with none
+ * are queued. */
+__global__ __launch_bounds__(256) void k_intra_ready_b(const BatchFrame *__restrict__ T, int *__restrict__ q) {
+  const int f = blockIdx.x;
+  const BatchFrame &F = T[f];
+  const Mav1FrameParams &fp = F.fp;
+  const int n = fp.sb_cols * fp.sb_rows;
+  const int *has = F.iwork + IW_HDR + n;
+  int *cnt = F.iwork + IW_HDR + 3 * n;
+  int mine = 0;
+  for (int sb = threadIdx.x; sb < n; sb += blockDim.x) {
+    if (!has[sb]) continue;
+    int dep[4], k = 0;
+    intra_deps(fp, sb, dep);
+    for (int j = 0; j < 4; j++) k += dep[j] >= 0 && has[dep[j]];
+    cnt[sb] = k;
+    mine++;
+  }
+  __syncthreads(); /* every count stored before any superblock is queued */
+  for (int sb = threadIdx.x; sb < n; sb += blockDim.x)
+    if (has[sb] && cnt[sb] == 0) queue_push(q, f << 16 | sb);
+  if (mine) atomicAdd(&q[QH_TOTAL], mine);
+}
+
+__global__ __launch_bounds__(RT) void k_recon_intra_b(const BatchFrame *__restrict__ T, int nfr, int *__restrict__ q,
+                                                      int maxSpin) {
   RLds &L = gL;
   const int tid = threadIdx.x;
-  int f = 0, staged = -1;
+  int staged = -1;
   for (;;) {
     if (tid == 0) {
-      int idx = 0;
-      while (f < nfr) {
-        idx = atomicAdd(&T[f].iwork[IW_NEXT], 1);
-        if (idx < T[f].iwork[IW_COUNT]) break;
-        f++;
+      int v = -1;
+      const int idx = atomicAdd(&q[QH_HEAD], 1);
+      if (idx < q[QH_TOTAL]) {
+        for (int it = 0; it < maxSpin && v < 0; it++) {
+          v = __hip_atomic_load(&q[QH_ITEMS + idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          if (v < 0) __builtin_amdgcn_s_sleep(4);
+        }
+        if (v < 0)
+          for (int f = 0; f < nfr; f++) atomicOr(&T[f].iwork[IW_ERR], 1);
       }
-      L.ival[2] = idx;
-      L.ival[3] = f;
+      L.ival[2] = v;
     }
     __syncthreads();
-    const int idx = L.ival[2];
-    f = L.ival[3];
+    const int v = L.ival[2];
     __syncthreads();
-    if (f >= nfr) break;
+    if (v < 0) break;
+    const int f = v >> 16, sb = v & 0xffff;
     const BatchFrame &F = T[f];
     const Mav1FrameParams &fp = F.fp;
     if (staged < 0) stage_tabs(tid, RT);
@@ -2020,23 +2083,29 @@ __global__ __launch_bounds__(RT) void k_recon_intra_b(const BatchFrame *__restri
       stage_smats(fp, F.mats);
       staged = f;
     }
-    const int n = fp.sb_cols * fp.sb_rows;
-    const int *list = F.iwork + IW_HDR, *has = list + n;
-    int *done = F.iwork + IW_HDR + 2 * n;
-    const int sb = list[idx], r = sb / fp.sb_cols, c = sb - r * fp.sb_cols;
-    if (tid == 0) {
-      const int ts = (c / fp.tile_w_sb) * fp.tile_w_sb, te = d_min(fp.sb_cols, ts + fp.tile_w_sb);
-      const int dep[4] = {c > ts ? sb - 1 : -1, r > 0 && c > ts ? sb - fp.sb_cols - 1 : -1, r > 0 ? sb - fp.sb_cols : -1,
-                          r > 0 && c + 1 < te ? sb - fp.sb_cols + 1 : -1};
-      for (int k = 0; k < 4; k++)
-        if (dep[k] >= 0 && has[dep[k]] && !wait_flag(&done[dep[k]], maxSpin)) atomicOr(&F.iwork[IW_ERR], 1);
-    }
     __syncthreads();
+    const int r = sb / fp.sb_cols, c = sb - r * fp.sb_cols;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     intra_sb(fp, F.blocks, F.sbs[sb], F.src, F.rec, F.ref, F.gold, F.coefs, F.mats, r, c);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(&done[sb], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) {
+      /* the superblocks that read this one: queued when this was their last
+       * pending dependency */
+      const int n = fp.sb_cols * fp.sb_rows;
+      const int *has = F.iwork + IW_HDR + n;
+      int *cnt = F.iwork + IW_HDR + 3 * n;
+      const int cand[4] = {c + 1 < fp.sb_cols ? sb + 1 : -1, r + 1 < fp.sb_rows && c + 1 < fp.sb_cols ? sb + fp.sb_cols + 1 : -1,
+                           r + 1 < fp.sb_rows ? sb + fp.sb_cols : -1, r + 1 < fp.sb_rows && c > 0 ? sb + fp.sb_cols - 1 : -1};
+      for (int k = 0; k < 4; k++) {
+        const int d = cand[k];
+        if (d < 0 || !has[d]) continue;
+        int dep[4];
+        intra_deps(fp, d, dep);
+        if (dep[0] != sb && dep[1] != sb && dep[2] != sb && dep[3] != sb) continue;
+        if (__hip_atomic_fetch_add(&cnt[d], -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 1) queue_push(q, f << 16 | d);
+      }
+    }
     __syncthreads();
   }
 }
@@ -2058,14 +2127,19 @@ hipError_t launch_recon_inter_b(hipStream_t st, const BatchFrame *T, const Batch
   return hipGetLastError();
 }
 
-hipError_t launch_intra_sched_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {
-  (void)H;
+hipError_t launch_intra_sched_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int *q, int qcap) {
+  int need = 0;
+  for (int f = 0; f < n; f++) need += H[f].fp.sb_cols * H[f].fp.sb_rows;
+  if (need > qcap || n > (1 << 14)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_intra_sched_b, dim3(n), dim3(SCHED_T), 0, st, T);
+  hipLaunchKernelGGL(k_intra_queue_reset_b, dim3(std::min((qcap + 255) / 256, 1024)), dim3(256), 0, st, q, qcap);
+  hipLaunchKernelGGL(k_intra_ready_b, dim3(n), dim3(256), 0, st, T, q);
   return hipGetLastError();
 }
 
-hipError_t launch_recon_intra_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int workgroups) {
+hipError_t launch_recon_intra_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int *q,
+                                int workgroups) {
   (void)H;
-  hipLaunchKernelGGL(k_recon_intra_b, dim3(workgroups), dim3(RT), 0, st, T, n, 1 << 20);
+  hipLaunchKernelGGL(k_recon_intra_b, dim3(workgroups), dim3(RT), 0, st, T, n, q, 1 << 20);
   return hipGetLastError();
 }

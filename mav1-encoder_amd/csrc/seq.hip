@@ -158,6 +158,11 @@ struct mav1hip_batch {
    * pinned host mirror; a table is reused once the round that used it has
    * finished on the GPU */
   BatchFrame *d_tab = nullptr, *h_tab = nullptr;
+  /* per engine (whose rounds run beside each other's), the intra phase's
+   * ready queue: 4 header ints and one entry per superblock of a round (at
+   * most every stream's superblocks) */
+  int *d_iq[2] = {};
+  int iq_cap = 0;
   hipEvent_t tab_free[kTabs] = {};
   int tab_used[kTabs] = {};
   int tab_next = 0;
@@ -902,11 +907,11 @@ int round_late(mav1hip_batch *b, Round &R) {
   /* the scheduling pass beside the inter phase */
   BCHK(b, hipEventRecord(b->rec_go[e], st));
   BCHK(b, hipStreamWaitEvent(side, b->rec_go[e], 0));
-  LCHK(b, "intra_sched", launch_intra_sched_b(side, T, H, n));
+  LCHK(b, "intra_sched", launch_intra_sched_b(side, T, H, n, b->d_iq[e], b->iq_cap));
   BCHK(b, hipEventRecord(b->sched_done[e], side));
   LCHK(b, "recon_inter", launch_recon_inter_b(st, T, H, n, 2 * b->num_cu));
   BCHK(b, hipStreamWaitEvent(st, b->sched_done[e], 0));
-  LCHK(b, "recon_intra", launch_recon_intra_b(st, T, H, n, b->num_cu));
+  LCHK(b, "recon_intra", launch_recon_intra_b(st, T, H, n, b->d_iq[e], b->num_cu));
   const Frame &f0 = R.frames[0];
   const Mav1HipFilterPlan &pl = f0.plan;
   if (pl.filter_info) {
@@ -1008,6 +1013,7 @@ void batch_destroy(mav1hip_batch *b) {
       if (ev) (void)hipEventDestroy(ev);
   }
   (void)hipFree(b->d_tab);
+  for (auto *q : b->d_iq) (void)hipFree(q);
   if (b->h_tab) (void)hipHostFree(b->h_tab);
   delete b;
 }
@@ -1023,6 +1029,8 @@ int batch_setup(mav1hip_batch *b) {
     BCHK(b, hipEventCreateWithFlags(&b->sched_done[e], hipEventDisableTiming));
   }
   b->num_cu = ctx_num_cu(b->seqs[0]->eng[0].ctx);
+  for (auto *s : b->seqs) b->iq_cap += s->num_sbs;
+  for (auto &q : b->d_iq) BCHK(b, hipMalloc((void **)&q, sizeof(int) * (4 + (size_t)b->iq_cap)));
   return MAV1HIP_OK;
 }
 

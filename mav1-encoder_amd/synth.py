@@ -232,3 +232,15 @@ def config_synth(cfg, seed):
     if cfg == 3:
         return Synth(1920, 1080, seed, n_objects=16, max_vel=48, noise_sigma=3.0, scene_cut=32)
     raise ValueError(cfg)
+
+
+# The batched workload: 16 streams of uneven sizes, each from its own seed
+# (the generator of the 352x288 stream: 6 sinusoids, 4 rectangles moving up
+# to +-8 px/frame over the global pan, noise sigma 2).
+BATCH_STREAMS = [(1920, 1080)] * 4 + [(1280, 720)] * 4 + [(854, 480)] * 8
+
+
+def batch_synth(i, seed_base=400):
+    w, h = BATCH_STREAMS[i]
+    return Synth(w, h, seed_base + i, n_objects=4, max_vel=8, noise_sigma=2.0)
+
