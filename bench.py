@@ -31,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "mav1-encoder_amd"))
 
 import numpy as np  # noqa: E402
 
+import autotune  # noqa: E402
 import mav1hip  # noqa: E402
 import synth  # noqa: E402
 
@@ -428,9 +429,12 @@ def batched_leg(nframes, torch):
             seq.close()
         return dt, out
 
+    knobs = autotune.load_tuned()
+
     def batched(n, check=False):
         take = digest if check else (lambda r: None)
         b = mav1hip.Batch(cfgs)
+        b.set_knobs(knobs)
         out = [[] for _ in range(S)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -478,6 +482,7 @@ def batched_leg(nframes, torch):
         "speedup": round(tp / tb, 3),
         "outputs_equal": op == ob and all(len(o) == nframes for o in ob),
         "pruned": False,
+        "launch_shapes": knobs or "defaults (no mav1-encoder_amd/batch_tuned.json)",
         "note": "plain: each stream through its own sequence driver, one after another; batched: one launch per "
                 "stage over the 16 streams' frames at the same coding position",
     }

@@ -418,6 +418,12 @@ int mav1hip_batch_send_frame(mav1hip_batch *b, int32_t stream, const uint8_t *co
                              const int32_t strides[3]);
 int mav1hip_batch_send_device_frame(mav1hip_batch *b, int32_t stream, const uint8_t *frame);
 int mav1hip_batch_receive(mav1hip_batch *b, int32_t stream, Mav1HipCodedFrame *out, int32_t wait);
+/* Launch shapes of the batched form, chosen by mav1-encoder_amd/autotune.py
+ * (0 = the default): "intra_wg_key" / "intra_wg_inter", the persistent
+ * workgroups of the intra phase in rounds of intra frames (default: one per
+ * CU) and of inter frames (default: a quarter of the CUs); "inter_wg", those
+ * of the inter phase (default: two per CU). Results do not depend on them. */
+int mav1hip_batch_set_knob(mav1hip_batch *b, const char *name, int32_t value);
 /* The sequence of stream i (owned by the batch; for mav1hip_seq_engine). */
 mav1hip_seq *mav1hip_batch_stream(mav1hip_batch *b, int32_t stream);
 

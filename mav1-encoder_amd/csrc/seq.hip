@@ -170,6 +170,10 @@ struct mav1hip_batch {
    * beside the inter phase */
   hipEvent_t me_done[2][4] = {}, ana_done[2] = {}, rec_go[2] = {}, sched_done[2] = {};
   std::vector<Round> pending; /* second-engine rounds waiting for the next ALTREF round's early stages */
+  /* launch shapes (mav1hip_batch_set_knob): persistent workgroups of the
+   * intra phase in rounds of intra frames and of inter frames, and of the
+   * inter phase; 0 = the default */
+  int intra_wg_key = 0, intra_wg_inter = 0, inter_wg = 0;
 };
 
 namespace {
@@ -909,9 +913,16 @@ int round_late(mav1hip_batch *b, Round &R) {
   BCHK(b, hipStreamWaitEvent(side, b->rec_go[e], 0));
   LCHK(b, "intra_sched", launch_intra_sched_b(side, T, H, n, b->d_iq[e], b->iq_cap));
   BCHK(b, hipEventRecord(b->sched_done[e], side));
-  LCHK(b, "recon_inter", launch_recon_inter_b(st, T, H, n, 2 * b->num_cu));
+  const int interWg = b->inter_wg > 0 ? b->inter_wg : 2 * b->num_cu;
+  LCHK(b, "recon_inter", launch_recon_inter_b(st, T, H, n, interWg));
   BCHK(b, hipStreamWaitEvent(st, b->sched_done[e], 0));
-  LCHK(b, "recon_intra", launch_recon_intra_b(st, T, H, n, b->d_iq[e], b->num_cu));
+  /* an inter frame's intra superblocks form a few short chains: a quarter
+   * of the CUs serve them and leave the rest to the other engine's round
+   * and the transfers; intra frames use every CU */
+  const bool intraFrame = R.frames[0].fp.frame_is_intra != 0;
+  int intraWg = intraFrame ? b->intra_wg_key : b->intra_wg_inter;
+  if (intraWg <= 0) intraWg = intraFrame ? b->num_cu : std::max(1, b->num_cu / 4);
+  LCHK(b, "recon_intra", launch_recon_intra_b(st, T, H, n, b->d_iq[e], intraWg));
   const Frame &f0 = R.frames[0];
   const Mav1HipFilterPlan &pl = f0.plan;
   if (pl.filter_info) {
@@ -1248,6 +1259,19 @@ int mav1hip_batch_receive(mav1hip_batch *b, int32_t stream, Mav1HipCodedFrame *o
     }
   }
   return from_seq(b, stream, mav1hip_seq_receive(s, out, wait));
+}
+
+int mav1hip_batch_set_knob(mav1hip_batch *b, const char *name, int32_t value) {
+  if (!b || !name || value < 0 || value > 65536) return MAV1HIP_EINVAL;
+  if (!strcmp(name, "intra_wg_key"))
+    b->intra_wg_key = value;
+  else if (!strcmp(name, "intra_wg_inter"))
+    b->intra_wg_inter = value;
+  else if (!strcmp(name, "inter_wg"))
+    b->inter_wg = value;
+  else
+    return bfail(b, MAV1HIP_EINVAL, "unknown knob %s", name);
+  return MAV1HIP_OK;
 }
 
 mav1hip_seq *mav1hip_batch_stream(mav1hip_batch *b, int32_t stream) {

@@ -151,6 +151,7 @@ def _load():
         "mav1hip_batch_send_device_frame": (C.c_int, [P, i32, P]),
         "mav1hip_batch_receive": (C.c_int, [P, i32, P, i32]),
         "mav1hip_batch_stream": (P, [P, i32]),
+        "mav1hip_batch_set_knob": (C.c_int, [P, C.c_char_p, i32]),
         "mav1hip_y4m_open": (P, [C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                  C.c_char_p, C.c_size_t]),
         "mav1hip_y4m_read": (C.c_int, [P, P, P]),
@@ -462,6 +463,16 @@ class Batch:
 
     def receive(self, i, wait=True):
         return coded_dict(self.receive_raw(i, wait))
+
+    def set_knob(self, name, value):
+        """A launch shape of the batched form (mav1hip_batch_set_knob)."""
+        rc = lib().mav1hip_batch_set_knob(self.h, name.encode(), int(value))
+        if rc:
+            raise self._err("set_knob", rc)
+
+    def set_knobs(self, knobs):
+        for k, v in (knobs or {}).items():
+            self.set_knob(k, v)
 
 
 class Context:
