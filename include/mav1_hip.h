@@ -428,6 +428,45 @@ int mav1hip_batch_set_knob(mav1hip_batch *b, const char *name, int32_t value);
 mav1hip_seq *mav1hip_batch_stream(mav1hip_batch *b, int32_t stream);
 
 /* ------------------------------------------------------------------------
+ * Learned partition / mode pruner (training and inference on the GPU)
+ * ------------------------------------------------------------------------
+ * No counterpart in the reference, whose analysis searches every partition
+ * slot and inter candidate (src/kernels/k_analysis.h:256-311, 656-808). A
+ * small CNN per 64x64 superblock: input the superblock's luma / 255 and its
+ * residual against the co-located reference block / 255 (64 x 64 x 2
+ * floats, channels innermost) and the frame's quantiser q_idx / 255;
+ * three 3x3 stride-2 convolutions (32, 64, 64 channels, ReLU) on BF16
+ * operands with FP32 accumulation on the matrix cores, mean pooling, and
+ * two heads: MAV1HIP_PRUNER_PART_CLASSES partition logits and
+ * MAV1HIP_PRUNER_MODE_CLASSES mode logits (labels: mav1-encoder_amd/
+ * pruner_data.py). Parameters are one FP32 vector of
+ * mav1hip_pruner_num_params() values (layout in csrc/pruner.hip). */
+#define MAV1HIP_PRUNER_PART_CLASSES 4
+#define MAV1HIP_PRUNER_MODE_CLASSES 8
+typedef struct mav1hip_pruner mav1hip_pruner;
+int32_t mav1hip_pruner_num_params(void);
+/* Device buffers for batches of up to max_batch superblocks. */
+mav1hip_pruner *mav1hip_pruner_create(int32_t max_batch, char *err, size_t errlen);
+void mav1hip_pruner_destroy(mav1hip_pruner *p);
+const char *mav1hip_pruner_error(const mav1hip_pruner *p);
+/* Parameters in and out (host FP32); setting them resets the optimizer. */
+int mav1hip_pruner_set_params(mav1hip_pruner *p, const float *params);
+int mav1hip_pruner_get_params(mav1hip_pruner *p, float *params);
+/* A batch: x batch x 64 x 64 x 2 floats, q batch floats, labels batch x 2
+ * (partition class, mode class; -1 = no label; NULL = none). */
+int mav1hip_pruner_load(mav1hip_pruner *p, const float *x, const float *q, const int32_t *labels, int32_t batch);
+/* Forward pass of the loaded batch: logits (batch x 12: 4 partition, 8
+ * mode) and the mean loss (sum of the two heads' softmax cross-entropies)
+ * into host memory when not NULL. */
+int mav1hip_pruner_forward(mav1hip_pruner *p, float *logits, float *loss);
+/* Gradient of the mean loss for every parameter (after a forward pass);
+ * into host memory when not NULL. */
+int mav1hip_pruner_backward(mav1hip_pruner *p, float *grads);
+/* One Adam step (step = 1, 2, ...) with the last gradients. */
+int mav1hip_pruner_adam(mav1hip_pruner *p, float lr, float beta1, float beta2, float eps, int32_t step);
+int mav1hip_pruner_sync(mav1hip_pruner *p);
+
+/* ------------------------------------------------------------------------
  * Input side and quality measure (host only)
  * ------------------------------------------------------------------------ */
 /* The y4m reader of the command-line front end (tools/mav1enc.c:67-77):

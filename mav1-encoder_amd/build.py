@@ -12,7 +12,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmav1hip.so")
 
-HIP_SOURCES = ["api.hip", "me.hip", "analysis.hip", "recon.hip", "filters.hip", "lrsearch.hip", "noalloc.hip", "seq.hip"]
+HIP_SOURCES = ["api.hip", "me.hip", "analysis.hip", "recon.hip", "filters.hip", "lrsearch.hip", "noalloc.hip", "seq.hip", "pruner.hip"]
 # sources whose floating-point results must round exactly as written (no FMA contraction)
 STRICT_FP = {"lrsearch.hip"}
 C_SOURCES = ["params.c", "fwdmats.c", "y4m.c"]

@@ -152,6 +152,17 @@ def _load():
         "mav1hip_batch_receive": (C.c_int, [P, i32, P, i32]),
         "mav1hip_batch_stream": (P, [P, i32]),
         "mav1hip_batch_set_knob": (C.c_int, [P, C.c_char_p, i32]),
+        "mav1hip_pruner_num_params": (i32, []),
+        "mav1hip_pruner_create": (P, [i32, C.c_char_p, C.c_size_t]),
+        "mav1hip_pruner_destroy": (None, [P]),
+        "mav1hip_pruner_error": (C.c_char_p, [P]),
+        "mav1hip_pruner_set_params": (C.c_int, [P, P]),
+        "mav1hip_pruner_get_params": (C.c_int, [P, P]),
+        "mav1hip_pruner_load": (C.c_int, [P, P, P, P, i32]),
+        "mav1hip_pruner_forward": (C.c_int, [P, P, P]),
+        "mav1hip_pruner_backward": (C.c_int, [P, P]),
+        "mav1hip_pruner_adam": (C.c_int, [P, C.c_float, C.c_float, C.c_float, C.c_float, i32]),
+        "mav1hip_pruner_sync": (C.c_int, [P]),
         "mav1hip_y4m_open": (P, [C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                  C.c_char_p, C.c_size_t]),
         "mav1hip_y4m_read": (C.c_int, [P, P, P]),
@@ -473,6 +484,73 @@ class Batch:
     def set_knobs(self, knobs):
         for k, v in (knobs or {}).items():
             self.set_knob(k, v)
+
+
+class Pruner:
+    """The learned pruner's network on the GPU (mav1hip_pruner_*): FP32
+    parameters, BF16 MFMA convolutions, forward, backward and Adam."""
+
+    def __init__(self, max_batch=64):
+        err = C.create_string_buffer(512)
+        h = lib().mav1hip_pruner_create(max_batch, err, 512)
+        if not h:
+            raise HipError(err.value.decode())
+        self.h = h
+        self.n = lib().mav1hip_pruner_num_params()
+        self.batch = 0
+
+    def close(self):
+        if self.h:
+            lib().mav1hip_pruner_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, what, rc):
+        if rc:
+            raise HipError(f"{what}: {lib().mav1hip_pruner_error(self.h).decode()} (rc {rc})")
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == self.n
+        self._chk("set_params", lib().mav1hip_pruner_set_params(self.h, p.ctypes.data))
+
+    def get_params(self):
+        p = np.empty(self.n, np.float32)
+        self._chk("get_params", lib().mav1hip_pruner_get_params(self.h, p.ctypes.data))
+        return p
+
+    def load(self, x, q, labels=None):
+        x = np.ascontiguousarray(x, np.float32)
+        q = np.ascontiguousarray(q, np.float32)
+        self.batch = x.shape[0]
+        assert x.shape == (self.batch, 64, 64, 2) and q.shape == (self.batch,)
+        lab = None if labels is None else np.ascontiguousarray(labels, np.int32)
+        self._keep = (x, q, lab)
+        self._chk("load", lib().mav1hip_pruner_load(self.h, x.ctypes.data, q.ctypes.data,
+                                                    None if lab is None else lab.ctypes.data, self.batch))
+
+    def forward(self):
+        logits = np.empty((self.batch, 12), np.float32)
+        loss = np.empty(1, np.float32)
+        self._chk("forward", lib().mav1hip_pruner_forward(self.h, logits.ctypes.data, loss.ctypes.data))
+        return logits, float(loss[0])
+
+    def backward(self):
+        g = np.empty(self.n, np.float32)
+        self._chk("backward", lib().mav1hip_pruner_backward(self.h, g.ctypes.data))
+        return g
+
+    def train_step(self, step, lr=1e-3):
+        """forward, backward and one Adam step on the loaded batch, all on
+        the device; returns nothing (no host transfers)."""
+        self._chk("forward", lib().mav1hip_pruner_forward(self.h, None, None))
+        self._chk("backward", lib().mav1hip_pruner_backward(self.h, None))
+        self._chk("adam", lib().mav1hip_pruner_adam(self.h, lr, 0.9, 0.999, 1e-8, step))
 
 
 class Context:
