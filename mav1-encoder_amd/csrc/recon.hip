@@ -206,12 +206,15 @@ __shared__ RLds gL;
 /* Pointer p into tile k (0: reconstruction, 1: source) of `plane` with
  * p[y * S + x] = sample (x, y) of the plane, for the superblock holding mi
  * (miRow, miCol); S = the tile's row stride. */
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
 DEVI uint8_t *tile_ptr(int k, int plane, int miRow, int miCol, int &S) {
   const int sub = plane > 0;
   S = sub ? TS_C : TS_Y;
-  uint8_t *base = gL.tile[k] + (plane == 0 ? 0 : TILE_Y + (plane - 1) * TILE_C);
+  lds_u8 *base = (lds_u8 *)(gL.tile[k] + (plane == 0 ? 0 : TILE_Y + (plane - 1) * TILE_C));
   const int oy = (((miRow >> 4) * 64) >> sub) - 1, ox = (((miCol >> 4) * 64) >> sub) - 4;
-  return (uint8_t *)((uintptr_t)base - (uintptr_t)((long long)oy * S + ox));
+  /* the offset is taken off the 32-bit LDS address, so the pointer stays in
+   * the LDS address space and its accesses are LDS instructions */
+  return (uint8_t *)(lds_u8 *)(uintptr_t)((uint32_t)(uintptr_t)base - (uint32_t)(oy * S + ox));
 }
 
 DEVI TxCtx wg_ctx() {
@@ -298,6 +301,17 @@ DEVI int tile_start(const Mav1FrameParams &fp, int miCol) { return ((miCol >> 4)
 /* find_eob (k_recon.h:50-54): 1 + last scan position with a non-zero level. */
 DEVI int tx_eob(const TxCtx &c, const int32_t *q_, const int16_t *scan, int n) {
   const lds_i32 *q = lds(q_);
+  if (!c.wg) {
+    /* one wave: the last non-zero scan position from a ballot per 64
+     * positions (no cross-lane reduction) */
+    int e = 0;
+    for (int k0 = 0; k0 < n; k0 += 64) {
+      const int k = k0 + c.tid;
+      const unsigned long long nz = __ballot(k < n && q[scan[k]] != 0);
+      if (nz) e = k0 + 64 - __clzll(nz);
+    }
+    return e;
+  }
   int m = 0;
   for (int k = c.tid; k < n; k += c.nt)
     if (q[scan[k]]) m = k + 1;
