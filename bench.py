@@ -34,6 +34,7 @@ import numpy as np  # noqa: E402
 import autotune  # noqa: E402
 import mav1hip  # noqa: E402
 import synth  # noqa: E402
+import train_pruner  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 W, H = 1920, 1080
@@ -394,8 +395,18 @@ def batched_leg(nframes, torch):
         geo = synth.Geometry(w, h, cfgs[i].tile_cols_log2)
         dev.append([torch.from_numpy(synth.pad_frame(geo, frames[i * nframes + t])).to("cuda:0")
                     for t in range(nframes)])
+    ys = [[frames[i * nframes + t][0] for t in range(nframes)] for i in range(S)]
+    geos = [synth.Geometry(w, h, cfgs[i].tile_cols_log2) for i, (w, h) in enumerate(synth.BATCH_STREAMS)]
     del frames
     torch.cuda.synchronize()
+    psnr_log = []
+
+    def quality(i, r):
+        """luma PSNR of a coded frame against its source (the check passes)"""
+        fp = r.fp
+        fb = fp.offset[2] + fp.stride[2] * fp.plane_h[2]
+        rec = np.ctypeslib.as_array(C.cast(r.recon, C.POINTER(C.c_uint8)), (fb,))
+        psnr_log.append(mav1hip.psnr(synth.unpad_frame(geos[i], rec)[0], ys[i][r.disp]))
 
     def digest(r):
         fp = r.fp
@@ -430,48 +441,67 @@ def batched_leg(nframes, torch):
         return dt, out
 
     knobs = autotune.load_tuned()
+    weights = train_pruner.load_weights()
 
-    def batched(n, check=False):
-        take = digest if check else (lambda r: None)
+    def batched(n, check=False, tau=0.0):
         b = mav1hip.Batch(cfgs)
         b.set_knobs(knobs)
+        if tau > 0:
+            b.set_pruner(weights, tau)
+        def take(i, r):
+            if not check:
+                return None
+            quality(i, r)
+            return digest(r)
+
         out = [[] for _ in range(S)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for t in range(n):
             for i in range(S):
                 while not b.send_device(i, dev[i][t].data_ptr()):
-                    out[i].append(take(b.receive_raw(i, True)))
+                    out[i].append(take(i, b.receive_raw(i, True)))
             # the frames whose outputs are ready, without waiting
             for i in range(S):
                 while True:
                     r = b.receive_raw(i, False)
                     if r is None:
                         break
-                    out[i].append(take(r))
+                    out[i].append(take(i, r))
         for i in range(S):
             while not b.send(i, None):
-                out[i].append(take(b.receive_raw(i, True)))
+                out[i].append(take(i, b.receive_raw(i, True)))
         for i in range(S):
             while True:
                 r = b.receive_raw(i, True)
                 if r is None:
                     break
-                out[i].append(take(r))
+                out[i].append(take(i, r))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         b.close()
         return dt, out
 
+    tau = 0.1
     warm = min(nframes, 4)
     plain(warm)
     batched(warm)
+    if weights is not None:
+        batched(warm, tau=tau)
     # timed codings (no hashing inside the timed region), then the same
-    # codings again with every frame's outputs hashed
+    # codings again with every frame's outputs hashed and their luma PSNR
     tp, _ = plain(nframes)
     tb, _ = batched(nframes)
+    tq = batched(nframes, tau=tau)[0] if weights is not None else None
     _, op = plain(nframes, True)
+    psnr_log.clear()
     _, ob = batched(nframes, True)
+    psnr_b = float(np.mean(psnr_log))
+    psnr_log.clear()
+    psnr_q = None
+    if weights is not None:
+        _, oq = batched(nframes, True, tau=tau)
+        psnr_q = float(np.mean(psnr_log))
     total = S * nframes
     return {
         "workload": f"16 streams (4 x 1920x1080, 4 x 1280x720, 8 x 854x480), {nframes} frames each from its own "
@@ -481,7 +511,16 @@ def batched_leg(nframes, torch):
         "batched_aggregate_fps": round(total / tb, 3),
         "speedup": round(tp / tb, 3),
         "outputs_equal": op == ob and all(len(o) == nframes for o in ob),
-        "pruned": False,
+        "luma_psnr_db": round(psnr_b, 4),
+        "pruned": None if tq is None else {
+            "tau": tau,
+            "aggregate_fps": round(total / tq, 3),
+            "speedup_vs_plain": round(tp / tq, 3),
+            "luma_psnr_db": round(psnr_q, 4),
+            "luma_psnr_delta_db": round(psnr_q - psnr_b, 4),
+            "weights": "mav1-encoder_amd/pruner_weights.f32 (train_pruner.py)",
+            "note": "BD-rate needs bit counts (no entropy coder in this build): only the distortion at equal q",
+        },
         "launch_shapes": knobs or "defaults (no mav1-encoder_amd/batch_tuned.json)",
         "note": "plain: each stream through its own sequence driver, one after another; batched: one launch per "
                 "stage over the 16 streams' frames at the same coding position",

@@ -424,6 +424,14 @@ int mav1hip_batch_receive(mav1hip_batch *b, int32_t stream, Mav1HipCodedFrame *o
  * CU) and of inter frames (default: a quarter of the CUs); "inter_wg", those
  * of the inter phase (default: two per CU). Results do not depend on them. */
 int mav1hip_batch_set_knob(mav1hip_batch *b, const char *name, int32_t value);
+/* Pruning of the batched analysis by the learned pruner (the network of
+ * mav1hip_pruner_*, parameters as mav1hip_pruner_set_params takes them):
+ * per superblock wholly inside its frame, the analysis evaluates no level
+ * deeper than the smallest l whose cumulative partition probability reaches
+ * 1 - tau (the exhaustive search where the network is not that confident).
+ * params NULL or tau <= 0 turns it off (the default: outputs then equal the
+ * sequence driver's). Changes the coded decisions when on. */
+int mav1hip_batch_set_pruner(mav1hip_batch *b, const float *params, float tau);
 /* The sequence of stream i (owned by the batch; for mav1hip_seq_engine). */
 mav1hip_seq *mav1hip_batch_stream(mav1hip_batch *b, int32_t stream);
 

@@ -117,7 +117,7 @@ struct AnaLds {
 };
 
 template <int LEVEL>
-DEVI void analyze_level_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold, const int32_t *__restrict__ mvf, const int32_t *__restrict__ mvg, long long *__restrict__ costs, Mav1BlockInfo *__restrict__ infos, int nslots, const int blk_x, const int blk_y, const int blk_z) {
+DEVI void analyze_level_body(const Mav1FrameParams &fp, const uint8_t *__restrict__ src, const uint8_t *__restrict__ ref, const uint8_t *__restrict__ gold, const int32_t *__restrict__ mvf, const int32_t *__restrict__ mvg, long long *__restrict__ costs, Mav1BlockInfo *__restrict__ infos, int nslots, const int8_t *__restrict__ prune, const int blk_x, const int blk_y, const int blk_z) {
   constexpr int BW = 64 >> LEVEL, N = 1 << LEVEL;
   /* threads per item: four waves for the 64x64 level (510 items at 1080p),
    * two for the 32x32 level, one below */
@@ -134,6 +134,9 @@ DEVI void analyze_level_body(const Mav1FrameParams &fp, const uint8_t *__restric
 
   /* ---- item validity (analyze_item) ---- */
   bool skip = r >= fp.mi_rows || c >= fp.mi_cols;
+  /* the learned pruner's deepest level for this superblock (batched form
+   * only; an item it cuts is "not evaluated", as depth pruning leaves it) */
+  if (!skip && prune && LEVEL > prune[sb]) skip = true;
   if (!skip && fp.fast && LEVEL > 0 && fp.depth_thr) {
     int yy = i >> LEVEL, xx = i & (N - 1);
     for (int a = LEVEL - 1; a >= 0 && !skip; a--) {
@@ -393,7 +396,8 @@ __global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_an
                                                       const int32_t *__restrict__ mvf, const int32_t *__restrict__ mvg,
                                                       long long *__restrict__ costs, Mav1BlockInfo *__restrict__ infos,
                                                       int nslots) {
-  analyze_level_body<LEVEL>(fp, src, ref, gold, mvf, mvg, costs, infos, nslots, blockIdx.x, blockIdx.y, blockIdx.z);
+  analyze_level_body<LEVEL>(fp, src, ref, gold, mvf, mvg, costs, infos, nslots, nullptr, blockIdx.x, blockIdx.y,
+                            blockIdx.z);
 }
 
 /* Pass 2 (analyze_sb_table): one wave per superblock. The bottom-up DP runs
@@ -564,7 +568,8 @@ __global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_an
     const BatchFrame *__restrict__ T, const BatchGrid g) {
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
-  analyze_level_body<LEVEL>(F.fp, F.src, F.ref, F.gold, F.mvf, F.mvg, F.acosts, F.ainfos, g.aux[b.f], b.x, b.y, b.z);
+  analyze_level_body<LEVEL>(F.fp, F.src, F.ref, F.gold, F.mvf, F.mvg, F.acosts, F.ainfos, g.aux[b.f], F.prune, b.x, b.y,
+                            b.z);
 }
 
 hipError_t launch_analyze_b(hipStream_t st, int level, const BatchFrame *T, const BatchFrame *H, int n) {

@@ -47,6 +47,9 @@ struct BatchFrame {
   int *lrtype, *lrreruns;
   long long frame_bytes;
   int lr_units, lr_gen;
+  /* the deepest analysis level per superblock the pruner allows (NULL: no
+   * pruning) */
+  int8_t *prune;
 };
 
 struct BatchGrid {
@@ -121,5 +124,14 @@ hipError_t launch_lr_chain_b(hipStream_t st, const BatchFrame *T, const BatchFra
  * deblocked copy, 1 = reconstruction into the restoration output, 2 = the
  * CDEF choices cleared. */
 hipError_t launch_frame_op_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int which);
+
+/* The pruner on the superblocks of the round's frames (pruner.hip): every
+ * superblock's sample built from the frame's source and LAST reference in
+ * HBM, the network's forward pass on `st`, and into each frame's prune[]
+ * the deepest analysis level: the smallest level l whose cumulative
+ * partition probability reaches 1 - tau (3, no pruning, when none does or
+ * the superblock is not wholly inside the frame). */
+struct mav1hip_pruner;
+int pruner_infer_frames(mav1hip_pruner *p, hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, float tau);
 
 #endif

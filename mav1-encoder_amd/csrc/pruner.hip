@@ -35,6 +35,7 @@
 #include <string.h>
 
 #include "../../include/mav1_hip.h"
+#include "batch.h"
 
 namespace {
 
@@ -98,19 +99,24 @@ __global__ void k_input(const float *__restrict__ x, uint16_t *__restrict__ x0, 
 }
 
 /* Forward convolution, 3x3 stride 2 pad 1, bias and ReLU fused; out BF16.
- * Block: 4 waves, each a 16-pixel x 16-channel tile; grid (M / 64, Co / 16).
- * Ci % 4 == 0 (the 4 consecutive k of a lane share one tap), Co % 16 == 0. */
+ * Block: 4 waves, each 16 output pixels x all NT * 16 output channels (the
+ * pixels' im2col fragment gathered once per k step for every channel tile);
+ * grid M / 64. Ci % 4 == 0 (the 4 consecutive k of a lane share one tap). */
+template <int NT>
 __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
                                                   const float *__restrict__ bias, uint16_t *__restrict__ out, int B,
-                                                  int H, int W, int Ci, int Co, int Kp) {
+                                                  int H, int W, int Ci, int Kp) {
+  constexpr int Co = NT * 16;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int Ho = H / 2, Wo = W / 2, M = B * Ho * Wo;
-  const int m0 = (blockIdx.x * 4 + wave) * 16, n0 = blockIdx.y * 16;
+  const int m0 = (blockIdx.x * 4 + wave) * 16;
   const int r = lane & 15, kq = (lane >> 4) * 4;
   const int m = m0 + r;
   const int b = m / (Ho * Wo), rem = m - b * (Ho * Wo), oy = rem / Wo, ox = rem - oy * Wo;
   const int kreal = 9 * Ci;
-  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  f4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < Kp; k0 += 16) {
     const int k = k0 + kq;
     s4 a = {0, 0, 0, 0};
@@ -119,17 +125,20 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t *__restrict__ i
       const int iy = 2 * oy - 1 + tap / 3, ix = 2 * ox - 1 + tap % 3;
       if (iy >= 0 && iy < H && ix >= 0 && ix < W) a = *(const s4 *)(in + (((size_t)b * H + iy) * W + ix) * Ci + ci);
     }
-    const s4 bv = *(const s4 *)(w + (size_t)(n0 + r) * Kp + k);
-    acc = mfma16(a, bv, acc);
-  }
-  const int co = n0 + r;
-  const float bb = bias[co];
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int om = m0 + (lane >> 4) * 4 + j;
-    if (om < M) {
-      const float v = acc[j] + bb;
-      out[(size_t)om * Co + co] = f2bf(v > 0.f ? v : 0.f);
+    for (int t = 0; t < NT; t++) acc[t] = mfma16(a, *(const s4 *)(w + (size_t)(t * 16 + r) * Kp + k), acc[t]);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int co = t * 16 + r;
+    const float bb = bias[co];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int om = m0 + (lane >> 4) * 4 + j;
+      if (om < M) {
+        const float v = acc[t][j] + bb;
+        out[(size_t)om * Co + co] = f2bf(v > 0.f ? v : 0.f);
+      }
     }
   }
 }
@@ -357,6 +366,56 @@ __global__ void k_adam(float *__restrict__ P, const float *__restrict__ G, float
   P[i] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
 }
 
+/* A sample per superblock of the round's frames, straight from HBM: luma /
+ * 255 and (inter frames) luma minus the LAST reference's co-located luma /
+ * 255, as the training samples are built; q = base_q_idx / 255. */
+__global__ __launch_bounds__(256) void k_frame_input(const BatchFrame *__restrict__ T, const BatchGrid g,
+                                                     uint16_t *__restrict__ x0, float *__restrict__ q) {
+  const BatchIdx b = batch_idx(g);
+  const BatchFrame &F = T[b.f];
+  const Mav1FrameParams &fp = F.fp;
+  const int sr = b.x / fp.sb_cols, sc = b.x - sr * fp.sb_cols, S = fp.stride[0];
+  const uint8_t *Y = F.src + fp.offset[0];
+  const uint8_t *R = fp.frame_is_intra ? nullptr : F.ref + fp.offset[0];
+  for (int pix = threadIdx.x; pix < 4096; pix += 256) {
+    const int o = (sr * 64 + (pix >> 6)) * S + sc * 64 + (pix & 63);
+    const int l = Y[o];
+    uint16_t *d = x0 + ((size_t)blockIdx.x * 4096 + pix) * 4;
+    d[0] = f2bf((float)l / 255.f);
+    d[1] = R ? f2bf((float)(l - (int)R[o]) / 255.f) : 0;
+    d[2] = 0;
+    d[3] = 0;
+  }
+  if (threadIdx.x == 0) q[blockIdx.x] = (float)fp.base_q_idx / 255.f;
+}
+
+__global__ void k_prune_levels(const BatchFrame *__restrict__ T, const BatchGrid g, const float *__restrict__ logits,
+                               float tau) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.start[g.n]) return;
+  const BatchIdx b = batch_idx(g, s);
+  const BatchFrame &F = T[b.f];
+  const Mav1FrameParams &fp = F.fp;
+  const int sr = b.x / fp.sb_cols, sc = b.x - sr * fp.sb_cols;
+  int lvl = 3;
+  if ((sr + 1) * 16 <= fp.mi_rows && (sc + 1) * 16 <= fp.mi_cols) {
+    const float *lg = logits + (size_t)s * (NP + NM);
+    float mx = lg[0];
+    for (int j = 1; j < NP; j++) mx = fmaxf(mx, lg[j]);
+    float e[NP], se = 0.f;
+    for (int j = 0; j < NP; j++) se += (e[j] = expf(lg[j] - mx));
+    float cdf = 0.f;
+    for (int l = 0; l < 3; l++) {
+      cdf += e[l] / se;
+      if (cdf >= 1.f - tau) {
+        lvl = l;
+        break;
+      }
+    }
+  }
+  F.prune[b.x] = (int8_t)lvl;
+}
+
 }  // namespace
 
 struct mav1hip_pruner {
@@ -417,8 +476,13 @@ constexpr Layer L1 = {64, CI0, C1, K1}, L2 = {32, C1, C2, K2}, L3 = {16, C2, C3,
 
 int conv_fwd(mav1hip_pruner *p, const Layer &L, const uint16_t *in, const uint16_t *w, const float *bias, uint16_t *out) {
   const int M = p->batch * (L.H / 2) * (L.H / 2);
-  hipLaunchKernelGGL(k_conv_fwd, dim3((M + 63) / 64, L.Co / 16), dim3(256), 0, p->st, in, w, bias, out, p->batch, L.H,
-                     L.H, L.Ci, L.Co, L.Kp);
+  const dim3 grid((M + 63) / 64);
+  if (L.Co == 32)
+    hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, p->st, in, w, bias, out, p->batch, L.H, L.H, L.Ci, L.Kp);
+  else if (L.Co == 64)
+    hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, p->st, in, w, bias, out, p->batch, L.H, L.H, L.Ci, L.Kp);
+  else
+    return pfail(p, MAV1HIP_EINVAL, "conv: %d output channels", L.Co);
   PCHK(p, hipGetLastError());
   return MAV1HIP_OK;
 }
@@ -457,6 +521,33 @@ void destroy_pruner(mav1hip_pruner *p) {
 }
 
 }  // namespace
+
+int pruner_infer_frames(mav1hip_pruner *p, hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, float tau) {
+  if (!p || n < 1) return MAV1HIP_EINVAL;
+  BatchGrid g = batch_grid();
+  for (int f = 0; f < n; f++) batch_grid_add(g, H[f].fp.sb_cols * H[f].fp.sb_rows, 1, 1);
+  const int total = g.start[n];
+  if (total > p->max_batch) return pfail(p, MAV1HIP_EINVAL, "pruner: %d superblocks, room for %d", total, p->max_batch);
+  hipStream_t own = p->st;
+  p->st = st;
+  p->batch = total;
+  int rc = pack(p);
+  if (!rc) {
+    hipLaunchKernelGGL(k_frame_input, dim3(total), dim3(256), 0, st, T, g, p->x0, p->q);
+    rc = hipGetLastError() == hipSuccess ? MAV1HIP_OK : pfail(p, MAV1HIP_EDEVICE, "pruner input launch failed");
+  }
+  if (!rc) rc = conv_fwd(p, L1, p->x0, p->w1, p->P + P_B1, p->y1);
+  if (!rc) rc = conv_fwd(p, L2, p->y1, p->w2, p->P + P_B2, p->y2);
+  if (!rc) rc = conv_fwd(p, L3, p->y2, p->w3, p->P + P_B3, p->y3);
+  if (!rc) {
+    hipLaunchKernelGGL(k_head_fwd, dim3(total), dim3(64), 0, st, p->y3, p->q, p->P, p->labels, p->h, p->logits,
+                       p->lossb);
+    hipLaunchKernelGGL(k_prune_levels, dim3((total + 255) / 256), dim3(256), 0, st, T, g, p->logits, tau);
+    rc = hipGetLastError() == hipSuccess ? MAV1HIP_OK : pfail(p, MAV1HIP_EDEVICE, "pruner launch failed");
+  }
+  p->st = own;
+  return rc;
+}
 
 extern "C" {
 

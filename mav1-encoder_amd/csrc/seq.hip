@@ -174,6 +174,13 @@ struct mav1hip_batch {
    * intra phase in rounds of intra frames and of inter frames, and of the
    * inter phase; 0 = the default */
   int intra_wg_key = 0, intra_wg_inter = 0, inter_wg = 0;
+  /* the learned pruner, per engine (mav1hip_batch_set_pruner): its network,
+   * the per-superblock deepest levels, and an event after the table upload
+   * that its inference on the side stream waits for */
+  mav1hip_pruner *pruner[2] = {};
+  int8_t *d_prune[2] = {};
+  hipEvent_t tab_up[2] = {};
+  float prune_tau = 0.f;
 };
 
 namespace {
@@ -862,14 +869,30 @@ int round_early(mav1hip_batch *b, Round &R) {
   int rc = take_table(b, &R.tab);
   if (rc) return rc;
   BatchFrame *T = b->d_tab + (size_t)R.tab * kMaxBatch, *H = b->h_tab + (size_t)R.tab * kMaxBatch;
-  for (int k = 0; k < n; k++)
+  const bool prune = b->pruner[e] && b->prune_tau > 0.f;
+  for (int k = 0, off = 0; k < n; k++) {
     if (ctx_batch_frame(R.seqs[k]->eng[e].ctx, &H[k], R.frames[k].plan.restoration))
       return bfail(b, MAV1HIP_EINVAL, "batch: frame %d has no parameters", k);
+    H[k].prune = prune ? b->d_prune[e] + off : nullptr;
+    off += H[k].fp.sb_cols * H[k].fp.sb_rows;
+  }
   mav1hip_seq *s0 = R.seqs[0];
   hipStream_t st = s0->eng[e].st, side = ctx_side_stream(s0->eng[e].ctx);
   BCHK(b, hipMemcpyAsync(T, H, sizeof(BatchFrame) * n, hipMemcpyHostToDevice, st));
   const Frame &f0 = R.frames[0];
   const bool me = !f0.is_key;
+  if (prune) {
+    /* the network's levels before the analysis: on the side stream beside
+     * motion search (after the table and the frames it reads are in place),
+     * or in line on a key frame round */
+    hipStream_t pst = me ? side : st;
+    if (me) {
+      BCHK(b, hipEventRecord(b->tab_up[e], st));
+      BCHK(b, hipStreamWaitEvent(side, b->tab_up[e], 0));
+    }
+    const int prc = pruner_infer_frames(b->pruner[e], pst, T, H, n, b->prune_tau);
+    if (prc) return bfail(b, prc, "pruner: %s", mav1hip_pruner_error(b->pruner[e]));
+  }
   if (me) {
     const int turbo = f0.fp.turbo, nref = f0.fp.golden ? 2 : 1;
     const bool hp = (turbo & MAV1_T_ME_HPEL) != 0, dec = (turbo & MAV1_T_ME_DEC) != 0;
@@ -1025,6 +1048,11 @@ void batch_destroy(mav1hip_batch *b) {
   }
   (void)hipFree(b->d_tab);
   for (auto *q : b->d_iq) (void)hipFree(q);
+  for (int e = 0; e < 2; e++) {
+    if (b->pruner[e]) mav1hip_pruner_destroy(b->pruner[e]);
+    (void)hipFree(b->d_prune[e]);
+    if (b->tab_up[e]) (void)hipEventDestroy(b->tab_up[e]);
+  }
   if (b->h_tab) (void)hipHostFree(b->h_tab);
   delete b;
 }
@@ -1271,6 +1299,27 @@ int mav1hip_batch_set_knob(mav1hip_batch *b, const char *name, int32_t value) {
     b->inter_wg = value;
   else
     return bfail(b, MAV1HIP_EINVAL, "unknown knob %s", name);
+  return MAV1HIP_OK;
+}
+
+int mav1hip_batch_set_pruner(mav1hip_batch *b, const float *params, float tau) {
+  if (!b || tau >= 1.f) return MAV1HIP_EINVAL;
+  if (!params || tau <= 0.f) {
+    b->prune_tau = 0.f;
+    return MAV1HIP_OK;
+  }
+  for (int e = 0; e < 2; e++) {
+    if (!b->pruner[e]) {
+      char err[256] = {0};
+      b->pruner[e] = mav1hip_pruner_create(b->iq_cap, err, sizeof err);
+      if (!b->pruner[e]) return bfail(b, MAV1HIP_EDEVICE, "pruner: %s", err);
+      BCHK(b, hipMalloc((void **)&b->d_prune[e], (size_t)b->iq_cap));
+      BCHK(b, hipEventCreateWithFlags(&b->tab_up[e], hipEventDisableTiming));
+    }
+    const int rc = mav1hip_pruner_set_params(b->pruner[e], params);
+    if (rc) return bfail(b, rc, "pruner: %s", mav1hip_pruner_error(b->pruner[e]));
+  }
+  b->prune_tau = tau;
   return MAV1HIP_OK;
 }
 

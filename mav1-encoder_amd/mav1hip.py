@@ -152,6 +152,7 @@ def _load():
         "mav1hip_batch_receive": (C.c_int, [P, i32, P, i32]),
         "mav1hip_batch_stream": (P, [P, i32]),
         "mav1hip_batch_set_knob": (C.c_int, [P, C.c_char_p, i32]),
+        "mav1hip_batch_set_pruner": (C.c_int, [P, P, C.c_float]),
         "mav1hip_pruner_num_params": (i32, []),
         "mav1hip_pruner_create": (P, [i32, C.c_char_p, C.c_size_t]),
         "mav1hip_pruner_destroy": (None, [P]),
@@ -480,6 +481,15 @@ class Batch:
         rc = lib().mav1hip_batch_set_knob(self.h, name.encode(), int(value))
         if rc:
             raise self._err("set_knob", rc)
+
+    def set_pruner(self, params, tau):
+        """Prune the analysis with the learned pruner (params: FP32 vector)
+        at confidence 1 - tau; None or tau <= 0 turns it off."""
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        self._pruner_params = p
+        rc = lib().mav1hip_batch_set_pruner(self.h, None if p is None else p.ctypes.data, float(tau))
+        if rc:
+            raise self._err("set_pruner", rc)
 
     def set_knobs(self, knobs):
         for k, v in (knobs or {}).items():

@@ -6,6 +6,7 @@ checked against the oracle's serial driver directly."""
 import os
 import sys
 
+import numpy as np
 import pytest
 
 import mav1hip
@@ -132,3 +133,64 @@ def test_batch_rejects_too_many_streams(gpu):
     cfg = mav1hip.default_config(204, 116, turbo=1, qindex=90, keyint=0)
     with pytest.raises(mav1hip.HipError, match="1 to 32 streams"):
         mav1hip.Batch([cfg] * 33)
+
+
+def luma_psnr(cfg, planes, coded):
+    geo = synth.Geometry(cfg.width, cfg.height, cfg.tile_cols_log2)
+    out = []
+    for c in coded:
+        y = synth.unpad_frame(geo, c["recon"])[0]
+        out.append(mav1hip.psnr(y, planes[c["disp"]][0]))
+    return out
+
+
+def test_batch_pruned_analysis(gpu):
+    """The learned pruner in the batched analysis (trained weights, confidence
+    0.9): every stream codes, the coding is repeatable, and its luma PSNR stays
+    within 0.5 dB of the exhaustive search's on every frame; the pruner off
+    again gives the exhaustive search's frames exactly."""
+    import train_pruner
+    w = train_pruner.load_weights()
+    assert w is not None and w.size == mav1hip.lib().mav1hip_pruner_num_params()
+    print("\n[batch] 2 x 854x480 + 640x360 turbo q125, pruner at tau 0.1", flush=True)
+    sizes = [(854, 480, 31), (854, 480, 32), (640, 360, 33)]
+    cfgs = [mav1hip.default_config(sw, sh, turbo=1, qindex=125, keyint=0) for sw, sh, _ in sizes]
+    planes = [[synth.Synth(sw, sh, seed, n_objects=6, max_vel=12, noise_sigma=2.5).frame(t) for t in range(6)]
+              for sw, sh, seed in sizes]
+
+    def run(tau):
+        b = mav1hip.Batch(cfgs, dump_decisions=True)
+        b.set_pruner(w, tau)
+        out = [[] for _ in cfgs]
+        try:
+            for t in range(6):
+                for i in range(len(cfgs)):
+                    while not b.send(i, planes[i][t]):
+                        out[i].append(b.receive(i, True))
+            for i in range(len(cfgs)):
+                while not b.send(i, None):
+                    out[i].append(b.receive(i, True))
+            for i in range(len(cfgs)):
+                while True:
+                    r = b.receive(i, True)
+                    if r is None:
+                        break
+                    out[i].append(r)
+        finally:
+            b.close()
+        return out
+
+    plain = run_batch(cfgs, planes)
+    a, bb, off = run(0.1), run(0.1), run(0.0)
+    changed = 0
+    for i, cfg in enumerate(cfgs):
+        assert len(a[i]) == 6
+        for x, y in zip(a[i], bb[i]):
+            compare(x, y)
+        for x, y in zip(off[i], plain[i]):
+            compare(x, y)
+        pp, pa = luma_psnr(cfg, planes[i], plain[i]), luma_psnr(cfg, planes[i], a[i])
+        changed += sum(not np.array_equal(x["blocks"], y["blocks"]) for x, y in zip(a[i], plain[i]))
+        print(f"stream {i}: luma PSNR exhaustive {np.mean(pp):.3f} dB, pruned {np.mean(pa):.3f} dB", flush=True)
+        assert min(p - q for p, q in zip(pa, pp)) > -0.5
+    print(f"{changed} frames with other decisions", flush=True)
