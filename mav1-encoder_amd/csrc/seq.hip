@@ -166,9 +166,8 @@ struct mav1hip_batch {
   hipEvent_t tab_free[kTabs] = {};
   int tab_used[kTabs] = {};
   int tab_next = 0;
-  /* per engine: analysis beside motion search, the intra scheduling pass
-   * beside the inter phase */
-  hipEvent_t me_done[2][4] = {}, ana_done[2] = {}, rec_go[2] = {}, sched_done[2] = {};
+  /* per engine: analysis beside motion search */
+  hipEvent_t me_done[2][4] = {}, ana_done[2] = {};
   std::vector<Round> pending; /* second-engine rounds waiting for the next ALTREF round's early stages */
   /* launch shapes (mav1hip_batch_set_knob): persistent workgroups of the
    * intra phase in rounds of intra frames and of inter frames, and of the
@@ -929,16 +928,14 @@ int round_early(mav1hip_batch *b, Round &R) {
 int round_late(mav1hip_batch *b, Round &R) {
   const int n = (int)R.frames.size(), e = R.engine;
   mav1hip_seq *s0 = R.seqs[0];
-  hipStream_t st = s0->eng[e].st, side = ctx_side_stream(s0->eng[e].ctx);
+  hipStream_t st = s0->eng[e].st;
   const BatchFrame *T = b->d_tab + (size_t)R.tab * kMaxBatch, *H = b->h_tab + (size_t)R.tab * kMaxBatch;
-  /* the scheduling pass beside the inter phase */
-  BCHK(b, hipEventRecord(b->rec_go[e], st));
-  BCHK(b, hipStreamWaitEvent(side, b->rec_go[e], 0));
-  LCHK(b, "intra_sched", launch_intra_sched_b(side, T, H, n, b->d_iq[e], b->iq_cap));
-  BCHK(b, hipEventRecord(b->sched_done[e], side));
+  /* the scheduling pass and the ready queue in line before the inter phase
+   * (they read only the analysis' records): beside it they would wait for
+   * CUs, since the persistent inter phase holds every CU */
+  LCHK(b, "intra_sched", launch_intra_sched_b(st, T, H, n, b->d_iq[e], b->iq_cap));
   const int interWg = b->inter_wg > 0 ? b->inter_wg : 2 * b->num_cu;
   LCHK(b, "recon_inter", launch_recon_inter_b(st, T, H, n, interWg));
-  BCHK(b, hipStreamWaitEvent(st, b->sched_done[e], 0));
   /* an inter frame's intra superblocks form a few short chains: a quarter
    * of the CUs serve them and leave the rest to the other engine's round
    * and the transfers; intra frames use every CU */
@@ -1043,8 +1040,7 @@ void batch_destroy(mav1hip_batch *b) {
   for (int e = 0; e < 2; e++) {
     for (auto &ev : b->me_done[e])
       if (ev) (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : {b->ana_done[e], b->rec_go[e], b->sched_done[e]})
-      if (ev) (void)hipEventDestroy(ev);
+    if (b->ana_done[e]) (void)hipEventDestroy(b->ana_done[e]);
   }
   (void)hipFree(b->d_tab);
   for (auto *q : b->d_iq) (void)hipFree(q);
@@ -1064,8 +1060,6 @@ int batch_setup(mav1hip_batch *b) {
   for (int e = 0; e < 2; e++) {
     for (auto &ev : b->me_done[e]) BCHK(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     BCHK(b, hipEventCreateWithFlags(&b->ana_done[e], hipEventDisableTiming));
-    BCHK(b, hipEventCreateWithFlags(&b->rec_go[e], hipEventDisableTiming));
-    BCHK(b, hipEventCreateWithFlags(&b->sched_done[e], hipEventDisableTiming));
   }
   b->num_cu = ctx_num_cu(b->seqs[0]->eng[0].ctx);
   for (auto *s : b->seqs) b->iq_cap += s->num_sbs;
