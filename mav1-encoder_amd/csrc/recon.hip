@@ -1581,28 +1581,31 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
   RLds &L = gL;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int ntask = ncur + 2 * nprev;
-  /* P1: the luma predictions of this step's blocks, one wave each */
-  for (int t = wave; t < ncur; t += RW) {
-    const int i = L.pord[cur + t];
-    Mav1BlockInfo &b = L.sbb[i];
-    if (lane == 0) {
-      b.tx_depth = 0;
-      b.interp_filter &= 0x30;
-    }
-    const int av = L.pav[i];
-    si_predict(fp, b, 0, L.psw[t], av & 1, (av >> 1) & 1);
-  }
-  __syncthreads();
-  /* P2: the chroma predictions of the previous step's blocks (U's CfL alpha
-   * at once), on the waves after the luma ones */
-  for (int t = wave + ncur; t < ntask; t += RW) {
-    const int k = t - ncur, plane = 1 + (k & 1), i = L.pord[prev + (k >> 1)];
-    Mav1BlockInfo &b = L.sbb[i];
-    const int av = L.pav[i];
-    si_predict(fp, b, plane, L.psw[4 + k], (av >> (2 * plane)) & 1, (av >> (2 * plane + 1)) & 1);
-    if (plane == 1 && b.uv_mode == UV_CFL_PRED) {
-      const int a = si_cfl_apply(b, 1, L.psw[4 + k], 0);
-      if (lane == 0) b.cfl_alpha_u = a;
+  /* P1: the luma predictions of this step's blocks and the chroma
+   * predictions of the previous step's blocks (U's CfL alpha at once), one
+   * task per wave. They touch disjoint data: a luma prediction reads luma
+   * of lower levels and writes its own block; a chroma prediction reads
+   * chroma of lower levels and its own block's reconstructed luma (CfL),
+   * and writes its own chroma block; each task has its own SiWave. */
+  for (int t = wave; t < ntask; t += RW) {
+    if (t < ncur) {
+      const int i = L.pord[cur + t];
+      Mav1BlockInfo &b = L.sbb[i];
+      if (lane == 0) {
+        b.tx_depth = 0;
+        b.interp_filter &= 0x30;
+      }
+      const int av = L.pav[i];
+      si_predict(fp, b, 0, L.psw[t], av & 1, (av >> 1) & 1);
+    } else {
+      const int k = t - ncur, plane = 1 + (k & 1), i = L.pord[prev + (k >> 1)];
+      Mav1BlockInfo &b = L.sbb[i];
+      const int av = L.pav[i];
+      si_predict(fp, b, plane, L.psw[4 + k], (av >> (2 * plane)) & 1, (av >> (2 * plane + 1)) & 1);
+      if (plane == 1 && b.uv_mode == UV_CFL_PRED) {
+        const int a = si_cfl_apply(b, 1, L.psw[4 + k], 0);
+        if (lane == 0) b.cfl_alpha_u = a;
+      }
     }
   }
   __syncthreads();
