@@ -234,6 +234,14 @@ DEVI TxCtx wave_ctx() {
   return c;
 }
 DEVI long long wg_sum64(long long v) { return tx_sum64(wg_ctx(), v); }
+/* A workgroup barrier that orders LDS only: global stores before it (levels,
+ * block records) need not have completed, as nothing in the workgroup reads
+ * them back before the superblock is published with an agent-scope release. */
+DEVI void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 /* ---- scan / tx-set helpers (spec 5.11.47 / 5.11.48) ---- */
 DEVI const int16_t *scan_of(int txSz, int kind) {
@@ -270,11 +278,14 @@ DEVI int small_scan_off(int txSz) {
     default: return -1;
   }
 }
+DEVI int scan_kind(int t) {
+  return (t == V_DCT || t == V_ADST || t == V_FLIPADST) ? 1 : (t == H_DCT || t == H_ADST || t == H_FLIPADST) ? 2 : 0;
+}
 DEVI const int16_t *get_scan(int txSz, int t) {
   if (txSz == TX_16X64) return Default_Scan_16x32;
   if (txSz == TX_64X16) return Default_Scan_32x16;
   if (Tx_Size_Sqr_Up[txSz] == TX_64X64) return Default_Scan_32x32;
-  const int kind = (t == V_DCT || t == V_ADST || t == V_FLIPADST) ? 1 : (t == H_DCT || t == H_ADST || t == H_FLIPADST) ? 2 : 0;
+  const int kind = scan_kind(t);
   const int off = small_scan_off(txSz);
   if (off >= 0) return gL.scans + kind * SCAN_SMALL + off;
   return scan_of(txSz, kind);
@@ -366,7 +377,8 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
   tx_residual(c, spx, px, S, x, y, lw, lh, t, resid, rs);
   tx_forward<SMALL>(c, resid, rs, tmp, coef, txSz, t, mats, gL.smats);
   tx_quantize(c, coef, q, txSz, dcQ, acQ, gL.qinv[plane][0], gL.qinv[plane][1]);
-  const int16_t *scan = get_scan(txSz, t);
+  /* transforms up to 16x16: the scan staged in LDS, addressed as LDS */
+  const int16_t *scan = SMALL ? gL.scans + scan_kind(t) * SCAN_SMALL + small_scan_off(txSz) : get_scan(txSz, t);
   const int eob = tx_eob(c, q, scan, seg_eob(txSz));
   if (direct) {
     *costOut = 0;
@@ -1608,7 +1620,7 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
       }
     }
   }
-  __syncthreads();
+  lds_barrier();
   /* luma candidates: offsets per block */
   int off[5], total = 0;
   bool single = true;
@@ -1638,7 +1650,7 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
         lvl_chroma(fp, prev, t - ncur, coefs, mats);
       }
     }
-    __syncthreads();
+    lds_barrier();
     return;
   }
   /* several types: rounds of one candidate per wave, the best kept per
@@ -1661,7 +1673,7 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
         L.reob[wave] = eob;
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (tid < ncur) {
       const int k = tid;
       int win = -1;
@@ -1675,7 +1687,7 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
         }
       L.pwin[k] = win;
     }
-    __syncthreads();
+    lds_barrier();
     for (int k = 0; k < ncur; k++) {
       const int win = L.pwin[k];
       if (win < 0) continue;
@@ -1684,7 +1696,7 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
       const int n = d_min(32, Tx_Width[txSz]) * d_min(32, Tx_Height[txSz]);
       for (int e = tid; e < n; e += RT) L.pq[k][e] = L.u.sm[win].q[e];
     }
-    __syncthreads();
+    lds_barrier();
   }
   /* P3: the chosen luma types' reconstruction beside the chroma coding */
   for (int t = wave; t < ntask; t += RW) {
@@ -1693,7 +1705,7 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
     else
       lvl_chroma(fp, prev, t - ncur, coefs, mats);
   }
-  __syncthreads();
+  lds_barrier();
 }
 
 /* Codes the intra blocks of superblock (r, c) (recon_sb, k_recon.h:396-440):
