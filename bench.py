@@ -106,11 +106,11 @@ BH = [4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 64, 32, 64, 128, 64, 128, 16, 4, 32, 8,
 # default run, from the FETCH_SIZE and WRITE_SIZE passes of
 # profiles/recon_intra_1080p_turbo_stream_pmc.md, scaled by the known-size
 # copy's factors there (fetch x1.9992, write x1.0).
-TRAFFIC = {"recon_intra": 10252850}
+TRAFFIC = {"recon_intra": 9298938}
 # VALUBusy (%) per launch over the timed stream, from the counter pass of the
 # same summary (100 * SQ_ACTIVE_INST_VALU / CU_NUM / GRBM_GUI_ACTIVE).
-VALU_BUSY = {"recon_intra": 0.05, "lr_fit": 6.11, "recon_inter": 2.15, "me_level0": 2.40, "me_level2": 5.05,
-             "me_level3": 9.39, "analyze_level0": 3.55, "qpel_build": 6.44, "cdef": 7.44}
+VALU_BUSY = {"recon_intra": 0.05, "lr_fit": 6.17, "recon_inter": 2.23, "me_level0": 2.46, "me_level2": 5.59,
+             "me_level3": 11.42, "analyze_level0": 3.26, "qpel_build": 6.60, "cdef": 7.44}
 
 
 def intra_phase_bytes(fp, blocks, sbs):
