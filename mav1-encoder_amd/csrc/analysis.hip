@@ -568,7 +568,7 @@ __global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_an
     const BatchFrame *__restrict__ T, const BatchGrid g) {
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
-  analyze_level_body<LEVEL>(F.fp, F.src, F.ref, F.gold, F.mvf, F.mvg, F.acosts, F.ainfos, g.aux[b.f], F.prune, b.x, b.y,
+  analyze_level_body<LEVEL>(F.fp, gptr(F.src), gptr(F.ref), gptr(F.gold), gptr(F.mvf), gptr(F.mvg), gptr(F.acosts), gptr(F.ainfos), g.aux[b.f], F.prune, b.x, b.y,
                             b.z);
 }
 
@@ -591,7 +591,7 @@ hipError_t launch_analyze_b(hipStream_t st, int level, const BatchFrame *T, cons
 __global__ __launch_bounds__(64) void k_analyze_table_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
-  analyze_table_body(F.fp, F.acosts, F.ainfos, F.blocks, F.sbs, b.x, b.y, b.z);
+  analyze_table_body(F.fp, gptr(F.acosts), gptr(F.ainfos), gptr(F.blocks), gptr(F.sbs), b.x, b.y, b.z);
 }
 
 hipError_t launch_analyze_table_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n) {

@@ -93,6 +93,13 @@ __device__ __forceinline__ BatchIdx batch_idx(const BatchGrid &g, int b) {
   return r;
 }
 __device__ __forceinline__ BatchIdx batch_idx(const BatchGrid &g) { return batch_idx(g, blockIdx.x); }
+/* A table pointer as a global-memory pointer (the table holds device
+ * allocations), rebuilt from its integer value so the compiler keeps the
+ * address space and emits global rather than flat instructions. */
+template <class T>
+__device__ __forceinline__ T *gptr(T *p) {
+  return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
 #endif
 
 /* Batched launchers: T is the table in device memory, H the same table on

@@ -583,31 +583,31 @@ hipError_t launch_decision_grid(hipStream_t st, const Mav1FrameParams &fp, const
 
 __global__ __launch_bounds__(64) void k_filter_info_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
   BATCH_FRAME();
-  filter_info_body(F.fp, F.blocks, F.sbs, F.lfmask, b.x, b.y, b.z);
+  filter_info_body(F.fp, gptr(F.blocks), gptr(F.sbs), gptr(F.lfmask), b.x, b.y, b.z);
 }
 __global__ __launch_bounds__(64) void k_deblock_b(const BatchFrame *__restrict__ T, const BatchGrid g, int pass) {
   BATCH_FRAME();
-  deblock_body(F.fp, F.rec, F.lfmask, pass, b.x, b.y, b.z);
+  deblock_body(F.fp, gptr(F.rec), gptr(F.lfmask), pass, b.x, b.y, b.z);
 }
 __global__ __launch_bounds__(64) void k_cdef_dir_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
   BATCH_FRAME();
-  cdef_dir_body(F.fp, F.rec, F.lfmask, F.dirvar, b.x, b.y, b.z);
+  cdef_dir_body(F.fp, gptr(F.rec), gptr(F.lfmask), gptr(F.dirvar), b.x, b.y, b.z);
 }
 __global__ __launch_bounds__(64) void k_cdef_b(const BatchFrame *__restrict__ T, const BatchGrid g, int n) {
   BATCH_FRAME();
-  cdef_body(F.fp, F.rec, F.cdefbuf, F.frame_bytes, F.dirvar, n, b.x, b.y, b.z);
+  cdef_body(F.fp, gptr(F.rec), gptr(F.cdefbuf), F.frame_bytes, gptr(F.dirvar), n, b.x, b.y, b.z);
 }
 __global__ __launch_bounds__(256) void k_cdef_select_b(const BatchFrame *__restrict__ T, const BatchGrid g, int n) {
   BATCH_FRAME();
-  cdef_select_body(F.fp, F.src, F.cdefbuf, F.frame_bytes, n, F.rec, F.choice, b.x, b.y, b.z);
+  cdef_select_body(F.fp, gptr(F.src), gptr(F.cdefbuf), F.frame_bytes, n, gptr(F.rec), gptr(F.choice), b.x, b.y, b.z);
 }
 __global__ __launch_bounds__(64) void k_lr_apply_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
   BATCH_FRAME();
-  lr_apply_body(F.fp, F.rec, F.dbk, F.lrout, F.lrunits, F.lrtype, b.x, b.y, b.z);
+  lr_apply_body(F.fp, gptr(F.rec), gptr(F.dbk), gptr(F.lrout), gptr(F.lrunits), gptr(F.lrtype), b.x, b.y, b.z);
 }
 __global__ __launch_bounds__(64) void k_decision_grid_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
   BATCH_FRAME();
-  decision_grid_body(F.fp, F.blocks, F.sbs, F.decgrid, b.x, b.y, b.z);
+  decision_grid_body(F.fp, gptr(F.blocks), gptr(F.sbs), gptr(F.decgrid), b.x, b.y, b.z);
 }
 
 /* The driver's frame copies and clears, 16 bytes per thread (the buffers
@@ -618,11 +618,11 @@ __global__ __launch_bounds__(256) void k_frame_op_b(const BatchFrame *__restrict
   const long long o = ((long long)b.x * 256 + threadIdx.x) * 16;
   if (which == 2) {
     const long long nsb = (long long)F.fp.sb_cols * F.fp.sb_rows;
-    for (long long q = o; q < o + 16 && q < nsb; q++) F.choice[q] = 0;
+    for (long long q = o; q < o + 16 && q < nsb; q++) gptr(F.choice)[q] = 0;
     return;
   }
-  const uint8_t *src = F.rec;
-  uint8_t *dst = which == 0 ? F.dbk : F.lrout;
+  const uint8_t *src = gptr(F.rec);
+  uint8_t *dst = which == 0 ? gptr(F.dbk) : gptr(F.lrout);
   const long long n = F.frame_bytes;
   if (o + 16 <= n) {
     *(uint4 *)(dst + o) = *(const uint4 *)(src + o);

@@ -1210,7 +1210,7 @@ __global__ __launch_bounds__(LR_T) void k_lr_fit_b(const BatchFrame *__restrict_
   const int units = g.start[g.n], part = blockIdx.x / units;
   const BatchIdx b = batch_idx(g, blockIdx.x - part * units);
   const BatchFrame &F = T[b.f];
-  lr_fit_body(F.fp, F.src, F.rec, F.dbk, (LrUnitStat *)F.lrstats, F.lr_gen, maxSpin, b.x, part, 0);
+  lr_fit_body(F.fp, gptr(F.src), gptr(F.rec), gptr(F.dbk), (LrUnitStat *)gptr(F.lrstats), F.lr_gen, maxSpin, b.x, part, 0);
 }
 __global__ void k_lr_chain_init_b(const BatchFrame *__restrict__ T) {
   const int p = threadIdx.x;
@@ -1223,14 +1223,14 @@ __global__ void k_lr_chain_init_b(const BatchFrame *__restrict__ T) {
 __global__ __launch_bounds__(LR_T) void k_lr_path_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
-  lr_path_body(F.fp, F.src, F.rec, F.dbk, (LrUnitStat *)F.lrstats, (const LrChain *)F.lrchain, b.x, b.y, b.z);
+  lr_path_body(F.fp, gptr(F.src), gptr(F.rec), gptr(F.dbk), (LrUnitStat *)gptr(F.lrstats), (const LrChain *)gptr(F.lrchain), b.x, b.y, b.z);
 }
 __global__ __launch_bounds__(LR_T) void k_lr_chain_b(const BatchFrame *__restrict__ T, const BatchGrid g, int lrTest,
                                                      int stop) {
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
-  lr_chain_body(F.fp, F.src, F.rec, F.dbk, (const LrUnitStat *)F.lrstats, F.lrunits, F.lrtype, lrTest, F.lrreruns,
-                (LrChain *)F.lrchain, stop, b.x, b.y, b.z);
+  lr_chain_body(F.fp, gptr(F.src), gptr(F.rec), gptr(F.dbk), (const LrUnitStat *)gptr(F.lrstats), gptr(F.lrunits), gptr(F.lrtype), lrTest, gptr(F.lrreruns),
+                (LrChain *)gptr(F.lrchain), stop, b.x, b.y, b.z);
 }
 
 static BatchGrid unit_grid(const BatchFrame *H, int n) {

@@ -514,7 +514,7 @@ hipError_t launch_me_level(hipStream_t st, int level, const Mav1FrameParams &fp,
 __global__ __launch_bounds__(256) void k_qpel_build_b(const BatchFrame *__restrict__ T, const BatchGrid g, int which) {
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
-  qpel_build_body(F.fp, which ? F.gold : F.ref, F.planes + which * F.entry_bytes, b.x, b.y, b.z);
+  qpel_build_body(F.fp, which ? gptr(F.gold) : gptr(F.ref), gptr(F.planes) + which * F.entry_bytes, b.x, b.y, b.z);
 }
 
 hipError_t launch_qpel_build_b(hipStream_t st, const BatchFrame *T, const BatchFrame *H, int n, int which) {
@@ -531,8 +531,8 @@ __global__ __launch_bounds__(256) void k_dec_build_b(const BatchFrame *__restric
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
   const size_t n16 = (size_t)16 * F.fp.frame_w * F.fp.frame_h;
-  const uint8_t *frame = which == 2 ? F.src : which ? F.gold : F.ref;
-  uint8_t *dec = which == 2 ? F.planes + 2 * F.entry_bytes : F.planes + which * F.entry_bytes + n16;
+  const uint8_t *frame = which == 2 ? gptr(F.src) : which ? gptr(F.gold) : gptr(F.ref);
+  uint8_t *dec = which == 2 ? gptr(F.planes) + 2 * F.entry_bytes : gptr(F.planes) + which * F.entry_bytes + n16;
   dec_build_body(F.fp, frame, dec, b.x, b.y, b.z);
 }
 
@@ -550,8 +550,8 @@ template <int LEVEL, bool HP>
 __global__ __launch_bounds__(64) void k_me_level_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
-  me_level_body<LEVEL, HP>(F.fp, F.src, F.ref, F.gold, F.planes, (long)F.entry_bytes, F.planes + 2 * F.entry_bytes,
-                           F.mvf, F.mvg, g.aux[b.f], b.x, b.y, b.z);
+  me_level_body<LEVEL, HP>(F.fp, gptr(F.src), gptr(F.ref), gptr(F.gold), gptr(F.planes), (long)F.entry_bytes, gptr(F.planes) + 2 * F.entry_bytes,
+                           gptr(F.mvf), gptr(F.mvg), g.aux[b.f], b.x, b.y, b.z);
 }
 
 template <int LEVEL>

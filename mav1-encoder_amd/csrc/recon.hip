@@ -1964,12 +1964,12 @@ hipError_t launch_recon_intra(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
 __global__ __launch_bounds__(64) void k_edge_flags_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
-  edge_flags_body(F.fp, F.blocks, F.sbs, F.ys, F.uvs, b.x, b.y, b.z);
+  edge_flags_body(F.fp, gptr(F.blocks), gptr(F.sbs), gptr(F.ys), gptr(F.uvs), b.x, b.y, b.z);
 }
 __global__ __launch_bounds__(64) void k_edge_types_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
-  edge_types_body(F.fp, F.blocks, F.sbs, F.ys, F.uvs, b.x, b.y, b.z);
+  edge_types_body(F.fp, gptr(F.blocks), gptr(F.sbs), gptr(F.ys), gptr(F.uvs), b.x, b.y, b.z);
 }
 __global__ void k_recon_list_reset_b(const BatchFrame *__restrict__ T) {
   if (threadIdx.x < 2) T[blockIdx.x].rwork[threadIdx.x] = 0;
@@ -1977,7 +1977,7 @@ __global__ void k_recon_list_reset_b(const BatchFrame *__restrict__ T) {
 __global__ __launch_bounds__(64) void k_recon_list_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
   const BatchIdx b = batch_idx(g);
   const BatchFrame &F = T[b.f];
-  recon_list_body(F.fp, F.blocks, F.sbs, F.rwork, b.x, b.y, b.z);
+  recon_list_body(F.fp, gptr(F.blocks), gptr(F.sbs), gptr(F.rwork), b.x, b.y, b.z);
 }
 
 /* Inter phase over several frames: every workgroup takes the inter blocks
@@ -1990,7 +1990,7 @@ __global__ __launch_bounds__(RT) void k_recon_inter_b(const BatchFrame *__restri
   stage_tabs(tid, RT);
   for (int f = 0; f < nfr; f++) {
     const BatchFrame &F = T[f];
-    int *work = F.rwork;
+    int *work = gptr(F.rwork);
     const int count = work[0];
     bool staged = false;
     for (;;) {
@@ -2000,14 +2000,14 @@ __global__ __launch_bounds__(RT) void k_recon_inter_b(const BatchFrame *__restri
       __syncthreads();
       if (idx >= count) break;
       if (!staged) {
-        stage_smats(F.fp, F.mats);
+        stage_smats(F.fp, gptr(F.mats));
         __syncthreads();
         staged = true;
       }
-      Mav1BlockInfo *gb = F.blocks + work[2 + idx];
+      Mav1BlockInfo *gb = gptr(F.blocks) + work[2 + idx];
       if (tid < 27) ((int32_t *)&L.blk)[tid] = ((const int32_t *)gb)[tid];
       __syncthreads();
-      recon_block(F.fp, F.src, F.rec, false, F.ref, F.gold, F.coefs, F.mats, false, false);
+      recon_block(F.fp, gptr(F.src), gptr(F.rec), false, gptr(F.ref), gptr(F.gold), gptr(F.coefs), gptr(F.mats), false, false);
       if (tid < 27) ((int32_t *)gb)[tid] = ((const int32_t *)&L.blk)[tid];
       __syncthreads();
     }
@@ -2016,7 +2016,7 @@ __global__ __launch_bounds__(RT) void k_recon_inter_b(const BatchFrame *__restri
 
 __global__ __launch_bounds__(SCHED_T) void k_intra_sched_b(const BatchFrame *__restrict__ T) {
   const BatchFrame &F = T[blockIdx.x];
-  intra_sched_body(F.fp, F.blocks, F.sbs, F.iwork, 0, 0, 0);
+  intra_sched_body(F.fp, gptr(F.blocks), gptr(F.sbs), gptr(F.iwork), 0, 0, 0);
 }
 
 /* Intra phase over several frames: a dataflow queue. A superblock holding
@@ -2064,8 +2064,8 @@ __global__ __launch_bounds__(256) void k_intra_ready_b(const BatchFrame *__restr
   const BatchFrame &F = T[f];
   const Mav1FrameParams &fp = F.fp;
   const int n = fp.sb_cols * fp.sb_rows;
-  const int *has = F.iwork + IW_HDR + n;
-  int *cnt = F.iwork + IW_HDR + 3 * n;
+  const int *has = gptr(F.iwork) + IW_HDR + n;
+  int *cnt = gptr(F.iwork) + IW_HDR + 3 * n;
   int mine = 0;
   for (int sb = threadIdx.x; sb < n; sb += blockDim.x) {
     if (!has[sb]) continue;
@@ -2109,21 +2109,22 @@ __global__ __launch_bounds__(RT) void k_recon_intra_b(const BatchFrame *__restri
     const Mav1FrameParams &fp = F.fp;
     if (staged < 0) stage_tabs(tid, RT);
     if (staged != f) {
-      stage_smats(fp, F.mats);
+      stage_smats(fp, gptr(F.mats));
       staged = f;
     }
     __syncthreads();
     const int r = sb / fp.sb_cols, c = sb - r * fp.sb_cols;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    intra_sb(fp, F.blocks, F.sbs[sb], F.src, F.rec, F.ref, F.gold, F.coefs, F.mats, r, c);
+    intra_sb(fp, gptr(F.blocks), gptr(F.sbs)[sb], gptr(F.src), gptr(F.rec), gptr(F.ref), gptr(F.gold), gptr(F.coefs),
+             gptr(F.mats), r, c);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (tid == 0) {
       /* the superblocks that read this one: queued when this was their last
        * pending dependency */
       const int n = fp.sb_cols * fp.sb_rows;
-      const int *has = F.iwork + IW_HDR + n;
-      int *cnt = F.iwork + IW_HDR + 3 * n;
+      const int *has = gptr(F.iwork) + IW_HDR + n;
+      int *cnt = gptr(F.iwork) + IW_HDR + 3 * n;
       const int cand[4] = {c + 1 < fp.sb_cols ? sb + 1 : -1, r + 1 < fp.sb_rows && c + 1 < fp.sb_cols ? sb + fp.sb_cols + 1 : -1,
                            r + 1 < fp.sb_rows ? sb + fp.sb_cols : -1, r + 1 < fp.sb_rows && c > 0 ? sb + fp.sb_cols - 1 : -1};
       for (int k = 0; k < 4; k++) {
