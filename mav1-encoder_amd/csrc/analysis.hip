@@ -396,8 +396,8 @@ __global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_an
                                                       const int32_t *__restrict__ mvf, const int32_t *__restrict__ mvg,
                                                       long long *__restrict__ costs, Mav1BlockInfo *__restrict__ infos,
                                                       int nslots) {
-  analyze_level_body<LEVEL>(fp, src, ref, gold, mvf, mvg, costs, infos, nslots, nullptr, blockIdx.x, blockIdx.y,
-                            blockIdx.z);
+  analyze_level_body<LEVEL>(fp, src, ref, gold, mvf, mvg, costs, infos, nslots, nullptr,
+                            xcd_item(blockIdx.x, gridDim.x, (1 << (2 * LEVEL)) * nslots), blockIdx.y, blockIdx.z);
 }
 
 /* Pass 2 (analyze_sb_table): one wave per superblock. The bottom-up DP runs
@@ -566,7 +566,8 @@ hipError_t launch_analyze_table(hipStream_t st, const Mav1FrameParams &fp, const
 template <int LEVEL>
 __global__ __launch_bounds__(LEVEL == 0 ? 256 : LEVEL == 1 ? 128 : 64) void k_analyze_level_b(
     const BatchFrame *__restrict__ T, const BatchGrid g) {
-  const BatchIdx b = batch_idx(g);
+  /* runs of 64 items: the frames' slot counts may differ */
+  const BatchIdx b = batch_idx(g, xcd_item(blockIdx.x, gridDim.x, 64));
   const BatchFrame &F = T[b.f];
   analyze_level_body<LEVEL>(F.fp, gptr(F.src), gptr(F.ref), gptr(F.gold), gptr(F.mvf), gptr(F.mvg), gptr(F.acosts), gptr(F.ainfos), g.aux[b.f], F.prune, b.x, b.y,
                             b.z);

@@ -27,6 +27,20 @@ DEVI int d_round2signed(int x, int n) { return x >= 0 ? d_round2(x, n) : -d_roun
 DEVI int d_floorlog2(unsigned x) { return 31 - __clz(x); }
 DEVI int d_node_base(int level) { return ((1 << (2 * level)) - 1) / 3; }
 
+/* XCD-aware work order: the GPU hands workgroup b of a launch to XCD b % 8
+ * (round robin), and each XCD has its own L2. Workgroup b works on item
+ * xcd_item(b, n, k): the items are dealt to the XCDs in runs of k
+ * consecutive items (a superblock's nodes), so the items that share
+ * reference rows meet in one L2 while the runs stay spread over the frame
+ * (balanced). A bijection of [0, n); items past the last whole round of 8
+ * runs keep their index. */
+DEVI int xcd_item(int b, int n, int k) {
+  const int full = n / (8 * k) * (8 * k);
+  if (b >= full) return b;
+  const int x = b & 7, i = b >> 3;
+  return ((i / k) * 8 + x) * k + i % k;
+}
+
 /* wave64 reductions (every lane gets the result) */
 DEVI int wave_sum(int v) {
 #pragma unroll

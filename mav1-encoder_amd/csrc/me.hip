@@ -478,7 +478,8 @@ __global__ __launch_bounds__(64) void k_me_level(const Mav1FrameParams fp, const
                                                  const uint8_t *__restrict__ planes, long entryBytes,
                                                  const uint8_t *__restrict__ srcDec, int32_t *__restrict__ field0,
                                                  int32_t *__restrict__ field1, int nodesPerRef) {
-  me_level_body<LEVEL, HP>(fp, src, ref0, ref1, planes, entryBytes, srcDec, field0, field1, nodesPerRef, blockIdx.x, blockIdx.y, blockIdx.z);
+  me_level_body<LEVEL, HP>(fp, src, ref0, ref1, planes, entryBytes, srcDec, field0, field1, nodesPerRef,
+                           xcd_item(blockIdx.x, gridDim.x, 1 << (2 * LEVEL)), blockIdx.y, blockIdx.z);
 }
 
 template <int LEVEL>
@@ -548,7 +549,7 @@ hipError_t launch_dec_build_b(hipStream_t st, const BatchFrame *T, const BatchFr
 
 template <int LEVEL, bool HP>
 __global__ __launch_bounds__(64) void k_me_level_b(const BatchFrame *__restrict__ T, const BatchGrid g) {
-  const BatchIdx b = batch_idx(g);
+  const BatchIdx b = batch_idx(g, xcd_item(blockIdx.x, gridDim.x, 1 << (2 * LEVEL)));
   const BatchFrame &F = T[b.f];
   me_level_body<LEVEL, HP>(F.fp, gptr(F.src), gptr(F.ref), gptr(F.gold), gptr(F.planes), (long)F.entry_bytes, gptr(F.planes) + 2 * F.entry_bytes,
                            gptr(F.mvf), gptr(F.mvg), g.aux[b.f], b.x, b.y, b.z);
