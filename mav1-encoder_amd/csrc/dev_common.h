@@ -16,6 +16,13 @@
 #define DEVI __device__ __forceinline__
 
 DEVI int d_min(int a, int b) { return a < b ? a : b; }
+/* The value unchanged, opaque to the optimiser: per-lane values derived
+ * from it are computed where they are used instead of being hoisted out of
+ * the enclosing loops and held (or spilled) across the whole kernel. */
+DEVI int d_local(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
 DEVI int d_max(int a, int b) { return a > b ? a : b; }
 DEVI int d_abs(int a) { return a < 0 ? -a : a; }
 DEVI int d_clip(int lo, int hi, int v) { return v < lo ? lo : (v > hi ? hi : v); }

@@ -17,11 +17,19 @@ __host__ __device__ constexpr int cbrev(int bits, int x) {
   for (int i = 0; i < bits; i++) r |= ((x >> i) & 1) << (bits - 1 - i);
   return r;
 }
+/* The rotation B(a, b, angle, flip) in 32-bit arithmetic, equal to the
+ * 64-bit products of the spec: every value a rotation reads is a 1-D
+ * transform input (dequantised coefficients are clipped to 16 bits, the
+ * column inputs clamped to 16 bits) or a Hadamard output (clamped to r = 16
+ * bits) -- no rotation of the DCT / ADST networks reads another rotation's
+ * output -- so |T[a]|, |T[b]| <= 2^15, |c|, |s| <= 4096 and every sum stays
+ * below 2^28: the 24-bit multiplies (v_mul_i32_i24) and 32-bit sums are
+ * exact. */
 DEVI void tB(int32_t *T, int a, int b, int angle, int flip) {
   const int c = c128(angle), s = c128(angle - 64);
-  long long x = (long long)T[a] * c - (long long)T[b] * s;
-  long long y = (long long)T[a] * s + (long long)T[b] * c;
-  int32_t xa = (int32_t)((x + 2048) >> 12), yb = (int32_t)((y + 2048) >> 12);
+  const int32_t x = __mul24(T[a], c) - __mul24(T[b], s);
+  const int32_t y = __mul24(T[a], s) + __mul24(T[b], c);
+  const int32_t xa = (x + 2048) >> 12, yb = (y + 2048) >> 12;
   T[a] = flip ? yb : xa;
   T[b] = flip ? xa : yb;
 }
@@ -201,19 +209,21 @@ DEVI void t_adst_out(int32_t *T) {
   for (int i = 0; i < N0; i++) T[i] = (i & 1) ? -c[adst_out_idx(i, n)] : c[adst_out_idx(i, n)];
 }
 
+/* 7.13.2.6 in 32-bit arithmetic: inputs within 16 bits, so the largest sum
+ * (the fourth output's) is below 10950 * 2^15 < 2^29. */
 DEVI void t_adst4(int32_t *T) {
-  long long s0 = 1321LL * T[0], s1 = 2482LL * T[0], s2 = 3344LL * T[1], s3 = 3803LL * T[2], s4 = 1321LL * T[2],
-            s5 = 2482LL * T[3], s6 = 3803LL * T[3], b7 = (long long)T[0] - T[2] + T[3];
+  int32_t s0 = __mul24(1321, T[0]), s1 = __mul24(2482, T[0]), s2 = __mul24(3344, T[1]), s3 = __mul24(3803, T[2]),
+          s4 = __mul24(1321, T[2]), s5 = __mul24(2482, T[3]), s6 = __mul24(3803, T[3]), b7 = T[0] - T[2] + T[3];
   s0 += s3;
   s1 -= s4;
   s3 = s2;
-  s2 = 3344LL * b7;
+  s2 = __mul24(3344, b7);
   s0 += s5;
   s1 -= s6;
-  T[0] = (int32_t)((s0 + s3 + 2048) >> 12);
-  T[1] = (int32_t)((s1 + s3 + 2048) >> 12);
-  T[2] = (int32_t)((s2 + 2048) >> 12);
-  T[3] = (int32_t)((s0 + s1 - s3 + 2048) >> 12);
+  T[0] = (s0 + s3 + 2048) >> 12;
+  T[1] = (s1 + s3 + 2048) >> 12;
+  T[2] = (s2 + 2048) >> 12;
+  T[3] = (s0 + s1 - s3 + 2048) >> 12;
 }
 DEVI void t_adst8(int32_t *T, int r) {
   t_adst_in<3>(T);
@@ -262,9 +272,9 @@ template <int n>
 DEVI void t_identity(int32_t *T) {
 #pragma unroll
   for (int i = 0; i < (1 << n); i++) {
-    if constexpr (n == 2) T[i] = (int32_t)(((long long)T[i] * 5793 + 2048) >> 12);
+    if constexpr (n == 2) T[i] = (__mul24(T[i], 5793) + 2048) >> 12; /* 16-bit inputs: below 2^29 */
     else if constexpr (n == 3) T[i] = T[i] * 2;
-    else if constexpr (n == 4) T[i] = (int32_t)(((long long)T[i] * 11586 + 2048) >> 12);
+    else if constexpr (n == 4) T[i] = (__mul24(T[i], 11586) + 2048) >> 12;
     else T[i] = T[i] * 4;
   }
 }
@@ -392,7 +402,7 @@ DEVI void itx_rows(const int32_t *dq_, int qs, int32_t *res_, int rs, int h, int
     for (int j = 0; j < W; j++) {
       int32_t v = (i < 32 && j < 32) ? dq[i * qs + j] : 0;
       nz |= v;
-      T[j] = rect ? (int32_t)(((long long)v * 2896 + 2048) >> 12) : v;
+      T[j] = rect ? (__mul24(v, 2896) + 2048) >> 12 : v;
     }
     if (nz) { /* every 1-D kernel maps an all-zero row to zeros */
       t_1d<KIND, N>(T, 16);
@@ -486,7 +496,11 @@ DEVI void tx_inverse(const TxCtx &c, const int32_t *dq, int qs, int32_t *res, in
  * holds h x Min(32,w) (row stride tw). One thread per output in both
  * passes; the sums of a fixed length are unrolled by up to 16 so the matrix
  * loads issue back to back. smats (may be NULL): the packed short matrices
- * in LDS, used for lengths up to 16. */
+ * in LDS, used for lengths up to 16.
+ * Widths: residuals are within +-255 and no matrix row has an L1 norm above
+ * 1.49e6 (entries below 2^20), so a row sum stays below 2^28.5 and takes
+ * 24-bit multiplies into 32 bits, exactly; a row output is then below 5800
+ * in magnitude and the column sums (up to 2^33) take 64-bit multiply-adds. */
 template <int W, class MP>
 DEVI void fwd_rows(const TxCtx &c, const int32_t *resid_, int rs, int32_t *tmp_, int h, int ltw, MP Mh) {
   const int tw = 1 << ltw;
@@ -496,10 +510,10 @@ DEVI void fwd_rows(const TxCtx &c, const int32_t *resid_, int rs, int32_t *tmp_,
     const int i = k >> ltw, kk = k & (tw - 1);
     const lds_i32 *rr = resid + i * rs;
     const MP m = Mh + kk * W;
-    long long s = 0;
+    int s = 0;
 #pragma unroll 16
-    for (int j = 0; j < W; j++) s += (long long)rr[j] * m[j];
-    tmp[k] = (int32_t)((s + 32768) >> 16);
+    for (int j = 0; j < W; j++) s += __mul24(rr[j], m[j]);
+    tmp[k] = (s + 32768) >> 16;
   }
 }
 template <int H, class MP>
@@ -549,17 +563,77 @@ DEVI void tx_forward_m(const TxCtx &c, const int32_t *resid, int rs, int32_t *tm
   tx_sync(c);
 }
 
+/* One wave's forward transform of a (1 << LH) x (1 << LW) block, both
+ * lengths at most 16, with the sizes known at compile time: every LDS
+ * access is a fixed offset from a per-lane base, so the loads of a sum issue
+ * back to back. The row pass writes tmp transposed (column kk at kk * H) so
+ * that the column pass also reads consecutive words. Same sums and
+ * roundings as fwd_rows / fwd_cols. */
+template <int LW, int LH>
+DEVI void sfwd(const lds_i32 *resid, int rs, lds_i32 *tmp, lds_i32 *coef, const lds_i32 *Mh, const lds_i32 *Mv,
+               int sh, int lane_) {
+  constexpr int W = 1 << LW, H = 1 << LH, NO = W * H, IT = (NO + 63) / 64;
+  const int lane = d_local(lane_);
+  constexpr bool RECT = LW - LH == 1 || LH - LW == 1;
+#pragma unroll 1
+  for (int t = 0; t < IT; t++) {
+    const int k = lane + 64 * t;
+    if (NO % 64 == 0 || k < NO) {
+      const int i = k >> LW, kk = k & (W - 1);
+      const lds_i32 *rr = resid + i * rs, *m = Mh + kk * W;
+      int s = 0;
+#pragma unroll
+      for (int j = 0; j < W; j++) s += __mul24(rr[j], m[j]);
+      tmp[kk * H + i] = (s + 32768) >> 16;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 1
+  for (int t = 0; t < IT; t++) {
+    const int k = lane + 64 * t;
+    if (NO % 64 == 0 || k < NO) {
+      const int k2 = k >> LW, kk = k & (W - 1);
+      const lds_i32 *col = tmp + kk * H, *m = Mv + k2 * H;
+      long long s = 0;
+#pragma unroll
+      for (int i = 0; i < H; i++) s += (long long)m[i] * col[i];
+      s *= (1ll << sh);
+      if constexpr (RECT) s = ((s + 2048) >> 12) * 5793;
+      coef[k] = (int32_t)(s >= 0 ? (s + (1ll << 19)) >> 20 : -((-s + (1ll << 19)) >> 20));
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /* smats (may be NULL): the packed short matrices in LDS, used when both
  * lengths are at most 16; otherwise the full matrices in global memory.
- * SMALL: both lengths are at most 16 and smats is given. */
+ * SMALL: both lengths are at most 16, smats is given and c is one wave. */
 template <bool SMALL = false>
 DEVI void tx_forward(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int32_t *coef, int txSz, int txType,
                      const int32_t *__restrict__ mats, const int32_t *smats) {
   const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
   const int rk = row_kind(txType), ck = col_kind(txType);
   if constexpr (SMALL) {
-    tx_forward_m<const lds_i32 *, true>(c, resid, rs, tmp, coef, txSz, lds(smats) + smat_off(rk, lw),
-                                        lds(smats) + smat_off(ck, lh));
+    /* one wave (c.nt == 64) */
+    const lds_i32 *Mh = lds(smats) + smat_off(rk, lw), *Mv = lds(smats) + smat_off(ck, lh);
+    const int sh = Transform_Row_Shift[txSz] + 4;
+    const lds_i32 *r = lds(resid);
+    lds_i32 *tp = lds(tmp), *co = lds(coef);
+    switch (lw * 4 + lh) {
+      case 10: sfwd<2, 2>(r, rs, tp, co, Mh, Mv, sh, c.tid); break;
+      case 11: sfwd<2, 3>(r, rs, tp, co, Mh, Mv, sh, c.tid); break;
+      case 12: sfwd<2, 4>(r, rs, tp, co, Mh, Mv, sh, c.tid); break;
+      case 14: sfwd<3, 2>(r, rs, tp, co, Mh, Mv, sh, c.tid); break;
+      case 15: sfwd<3, 3>(r, rs, tp, co, Mh, Mv, sh, c.tid); break;
+      case 16: sfwd<3, 4>(r, rs, tp, co, Mh, Mv, sh, c.tid); break;
+      case 18: sfwd<4, 2>(r, rs, tp, co, Mh, Mv, sh, c.tid); break;
+      case 19: sfwd<4, 3>(r, rs, tp, co, Mh, Mv, sh, c.tid); break;
+      default: sfwd<4, 4>(r, rs, tp, co, Mh, Mv, sh, c.tid); break;
+    }
   } else {
     if (smats && lw <= 4 && lh <= 4)
       tx_forward_m(c, resid, rs, tmp, coef, txSz, lds(smats) + smat_off(rk, lw), lds(smats) + smat_off(ck, lh));
@@ -569,22 +643,39 @@ DEVI void tx_forward(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp,
 }
 
 /* The quantiser's reciprocal of q: floor(2^44 / q) + 1. */
-DEVI unsigned long long tx_qinv(int q) { return (1ull << 44) / (unsigned)q + 1; }
+/* The quantiser's reciprocal for tx_quantize: 1 / q rounded to float. */
+DEVI float tx_qrcp(int q) { return 1.0f / (float)q; }
+
+/* floor(t / q) for 0 <= t < 2^19 and 4 <= q < 2^16 (which the reference
+ * computes as t * (2^44 / q + 1) >> 44): t * rq is within 2^-5 of t / q
+ * (t and q exact in float, rq and the product one rounding each), so its
+ * truncation is off by at most one, which the remainder corrects. */
+DEVI int tx_qdiv(int t, int q, float rq) {
+  int l = (int)((float)t * rq);
+  const int r = t - __mul24(l, q);
+  return l + (r >= q) - (r < 0);
+}
 
 /* Dead-zone quantiser (k_fwd.h:66-90) into the decoder layout (stride tw);
- * invDc / invAc: tx_qinv of dcQ / acQ. */
-DEVI void tx_quantize(const TxCtx &c, const int32_t *coef_, int32_t *q_, int txSz, int dcQ, int acQ,
-                      unsigned long long invDc, unsigned long long invAc) {
+ * rDc / rAc: tx_qrcp of dcQ / acQ. Coefficients are below 2^21 in
+ * magnitude, so t stays below 2^19; larger ones take an integer division. */
+DEVI void tx_quantize(const TxCtx &c, const int32_t *coef_, int32_t *q_, int txSz, int dcQ, int acQ, float rDc,
+                      float rAc) {
   const lds_i32 *coef = lds(coef_);
   lds_i32 *q = lds(q_);
   const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
   for (int k = c.tid; k < th * tw; k += c.nt) {
     const bool dc = k == 0;
     const int qv = dc ? dcQ : acQ, rnd = dc ? 26 : 22;
-    const int v = coef[k];
-    long long a = (long long)d_abs(v) * denom;
-    long long t = (a + (((long long)qv * rnd) >> 2)) >> 4;
-    int level = t < (1ll << 19) ? (int)(((unsigned long long)t * (dc ? invDc : invAc)) >> 44) : (int)(t / qv);
+    const int v = coef[k], av = d_abs(v);
+    int level;
+    if (av < (1 << 21)) {
+      const int t = (av * denom + ((qv * rnd) >> 2)) >> 4;
+      level = tx_qdiv(t, qv, dc ? rDc : rAc);
+    } else {
+      const long long t = ((long long)av * denom + ((qv * rnd) >> 2)) >> 4;
+      level = (int)d_min(t / qv, 0xFFFFll);
+    }
     if (level > 0xFFFF) level = 0xFFFF;
     q[k] = v < 0 ? -level : level;
   }
@@ -596,11 +687,14 @@ DEVI void tx_dequant(const TxCtx &c, const int32_t *q_, int32_t *dq_, int qs, in
   const lds_i32 *q = lds(q_);
   lds_i32 *dq = lds(dq_);
   const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
+  const int ltw = d_min(5, (int)Tx_Width_Log2[txSz]);
   for (int k = c.tid; k < th * tw; k += c.nt) {
-    int i = k / tw, j = k - i * tw;
-    long long d = (long long)q[k] * (k == 0 ? dcQ : acQ), ad = d < 0 ? -d : d;
-    long long d2 = (ad & 0xFFFFFF) >> (denom >> 1);
-    dq[i * qs + j] = d_clip(-(1 << 15), (1 << 15) - 1, (int)(d < 0 ? -d2 : d2));
+    const int i = k >> ltw, j = k & (tw - 1);
+    /* levels are at most 0xFFFF and quantisers at most 1828: the product
+     * stays below 2^27 */
+    const int d = __mul24(q[k], k == 0 ? dcQ : acQ), ad = d < 0 ? -d : d;
+    const int d2 = (ad & 0xFFFFFF) >> (denom >> 1);
+    dq[i * qs + j] = d_clip(-(1 << 15), (1 << 15) - 1, d < 0 ? -d2 : d2);
   }
   tx_sync(c);
 }

@@ -174,7 +174,7 @@ struct RLds {
   uint8_t tsrc[64 * 64], tpred[64 * 64]; /* the transform block's source and prediction */
   int32_t smats[TX_SMATS];
   int16_t scans[3 * SCAN_SMALL]; /* scans of the transforms up to 16x16: default, mrow, mcol */
-  unsigned long long qinv[3][2]; /* per plane the quantiser reciprocals of dc_q / ac_q */
+  float qrcp[3][2];         /* per plane the quantiser reciprocals of dc_q / ac_q */
   int ival[4];
   Mav1BlockInfo blk;
   /* intra phase: the superblock's reconstruction with the row above it and
@@ -217,9 +217,11 @@ DEVI uint8_t *tile_ptr(int k, int plane, int miRow, int miCol, int &S) {
   return (uint8_t *)(lds_u8 *)(uintptr_t)((uint32_t)(uintptr_t)base - (uint32_t)(oy * S + ox));
 }
 
+/* The lane index is opaque (d_local) so that the address arithmetic derived
+ * from it stays inside each transform stage. */
 DEVI TxCtx wg_ctx() {
   TxCtx c;
-  c.tid = threadIdx.x;
+  c.tid = d_local(threadIdx.x);
   c.nt = RT;
   c.wg = true;
   c.red = gL.red;
@@ -227,7 +229,7 @@ DEVI TxCtx wg_ctx() {
 }
 DEVI TxCtx wave_ctx() {
   TxCtx c;
-  c.tid = threadIdx.x & 63;
+  c.tid = d_local(threadIdx.x & 63);
   c.nt = 64;
   c.wg = false;
   c.red = nullptr;
@@ -376,7 +378,7 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
   const int dcQ = fp.dc_q[plane], acQ = fp.ac_q[plane];
   tx_residual(c, spx, px, S, x, y, lw, lh, t, resid, rs);
   tx_forward<SMALL>(c, resid, rs, tmp, coef, txSz, t, mats, gL.smats);
-  tx_quantize(c, coef, q, txSz, dcQ, acQ, gL.qinv[plane][0], gL.qinv[plane][1]);
+  tx_quantize(c, coef, q, txSz, dcQ, acQ, gL.qrcp[plane][0], gL.qrcp[plane][1]);
   /* transforms up to 16x16: the scan staged in LDS, addressed as LDS */
   const int16_t *scan = SMALL ? gL.scans + scan_kind(t) * SCAN_SMALL + small_scan_off(txSz) : get_scan(txSz, t);
   const int eob = tx_eob(c, q, scan, seg_eob(txSz));
@@ -1329,7 +1331,7 @@ DEVI void stage_smats(const Mav1FrameParams &fp, const int32_t *__restrict__ mat
   }
   if (threadIdx.x < 6) {
     const int p = threadIdx.x >> 1;
-    gL.qinv[p][threadIdx.x & 1] = tx_qinv((threadIdx.x & 1) ? fp.ac_q[p] : fp.dc_q[p]);
+    gL.qrcp[p][threadIdx.x & 1] = tx_qrcp((threadIdx.x & 1) ? fp.ac_q[p] : fp.dc_q[p]);
   }
 }
 

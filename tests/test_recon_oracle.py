@@ -94,6 +94,32 @@ def test_quantiser_reciprocal_equals_division():
         assert np.array_equal(fast, t // np.uint64(q)), f"q {q}"
 
 
+def test_quantiser_float_division_is_exact():
+    """The device quantiser's division (tx_qdiv in csrc/dev_tx.h): the
+    truncation of float(t) * (1.0f / q), both products rounded to nearest
+    float as the GPU's v_mul_f32 / division do, corrected by the remainder
+    t - l q once in either direction, equals t // q for every t < 2^19 and
+    every q of the 8-bit tables; without the correction it does not."""
+    import re
+    import os
+    txt = open(os.path.join(libs.ROOT, "mav1-encoder_amd", "csrc", "av1_tables.h")).read()
+    def table(name):
+        body = re.search(name + r"\[\d+\] = \{(.*?)\};", txt, re.S).group(1)
+        return [int(v) for v in body.replace("\n", " ").split(",") if v.strip()]
+    qs = sorted(set(table("Dc_Qlookup")[:256] + table("Ac_Qlookup")[:256]))
+    t = np.arange(1 << 19, dtype=np.int64)
+    tf = t.astype(np.float32)
+    corrected_any = False
+    for q in qs:
+        rq = np.float32(1.0) / np.float32(q)
+        l0 = (tf * rq).astype(np.int64)  # truncation toward zero, t >= 0
+        r = t - l0 * q
+        l = l0 + (r >= q) - (r < 0)
+        assert np.array_equal(l, t // q), f"q {q}"
+        corrected_any |= not np.array_equal(l0, t // q)
+    assert corrected_any
+
+
 def test_inverse_dct_is_the_scaled_orthonormal_dct():
     """Closed form: the integer inverse DCT of length N applied to a unit
     coefficient at k approximates sqrt(2/N) c_k cos(pi (2i+1) k / 2N) * 2^-?
