@@ -607,7 +607,7 @@ int mav1hip_launch_recon(mav1hip_ctx *c) {
   HIPCHK(c, hipStreamWaitEvent(c->stream, c->sched_done, 0));
   return timed(c, "recon_intra", [&] {
     return launch_recon_intra(c->stream, fp, c->d_blocks, c->d_sbs, c->b_src, c->d_rec, c->b_ref, c->b_gold,
-                              c->d_coefs, c->d_mats, c->d_iwork, c->num_cu);
+                              c->d_coefs, c->d_mats, c->d_iwork, intra_workgroups(fp, c->num_cu));
   });
 }
 

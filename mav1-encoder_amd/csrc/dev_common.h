@@ -16,6 +16,10 @@
 #define DEVI __device__ __forceinline__
 
 DEVI int d_min(int a, int b) { return a < b ? a : b; }
+/* v, which the caller knows to be equal in every active lane, as a scalar
+ * (v_readfirstlane): branches on it become scalar branches and values
+ * derived from it stay in scalar registers. */
+DEVI int d_uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 /* The value unchanged, opaque to the optimiser: per-lane values derived
  * from it are computed where they are used instead of being hoisted out of
  * the enclosing loops and held (or spilled) across the whole kernel. */

@@ -476,15 +476,17 @@ DEVI void itx_cols(int32_t *res_, int rs, int w, int tid, int nt) {
  * are compiled in (smaller code where the small-block path runs). */
 template <bool SMALL = false>
 DEVI void tx_inverse(const TxCtx &c, const int32_t *dq, int qs, int32_t *res, int rs, int txSz, int txType) {
-  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
+  txSz = d_uni(txSz);
+  txType = d_uni(txType);
+  const int lw = d_uni(Tx_Width_Log2[txSz]), lh = d_uni(Tx_Height_Log2[txSz]);
   const int rect = (lw - lh == 1) || (lh - lw == 1);
   const int rk = row_kind(txType), ck = col_kind(txType);
   if constexpr (SMALL) {
-    ITX_DISPATCH_SMALL(itx_rows, rk, lw, dq, qs, res, rs, 1 << lh, rect, (int)Transform_Row_Shift[txSz], c.tid, c.nt);
+    ITX_DISPATCH_SMALL(itx_rows, rk, lw, dq, qs, res, rs, 1 << lh, rect, d_uni(Transform_Row_Shift[txSz]), c.tid, c.nt);
     tx_sync(c);
     ITX_DISPATCH_SMALL(itx_cols, ck, lh, res, rs, 1 << lw, c.tid, c.nt);
   } else {
-    ITX_DISPATCH(itx_rows, rk, lw, dq, qs, res, rs, 1 << lh, rect, (int)Transform_Row_Shift[txSz], c.tid, c.nt);
+    ITX_DISPATCH(itx_rows, rk, lw, dq, qs, res, rs, 1 << lh, rect, d_uni(Transform_Row_Shift[txSz]), c.tid, c.nt);
     tx_sync(c);
     ITX_DISPATCH(itx_cols, ck, lh, res, rs, 1 << lw, c.tid, c.nt);
   }
@@ -541,7 +543,7 @@ DEVI int smat_off(int kind, int n) { return kind * 336 + (n == 2 ? 0 : n == 3 ? 
 template <class MP, bool SMALL = false>
 DEVI void tx_forward_m(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int32_t *coef, int txSz, MP Mh,
                        MP Mv) {
-  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], h = 1 << lh;
+  const int lw = d_uni(Tx_Width_Log2[txSz]), lh = d_uni(Tx_Height_Log2[txSz]), h = 1 << lh;
   const int ltw = d_min(5, lw), th = d_min(32, h);
   if (lw == 2) fwd_rows<4>(c, resid, rs, tmp, h, ltw, Mh);
   else if (lw == 3) fwd_rows<8>(c, resid, rs, tmp, h, ltw, Mh);
@@ -551,7 +553,7 @@ DEVI void tx_forward_m(const TxCtx &c, const int32_t *resid, int rs, int32_t *tm
     else fwd_rows<64>(c, resid, rs, tmp, h, ltw, Mh);
   }
   tx_sync(c);
-  const int sh = Transform_Row_Shift[txSz] + 4;
+  const int sh = d_uni(Transform_Row_Shift[txSz]) + 4;
   const bool rect = (lw - lh == 1) || (lh - lw == 1);
   if (lh == 2) fwd_cols<4>(c, tmp, coef, th, ltw, Mv, sh, rect);
   else if (lh == 3) fwd_cols<8>(c, tmp, coef, th, ltw, Mv, sh, rect);
@@ -615,12 +617,14 @@ DEVI void sfwd(const lds_i32 *resid, int rs, lds_i32 *tmp, lds_i32 *coef, const 
 template <bool SMALL = false>
 DEVI void tx_forward(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int32_t *coef, int txSz, int txType,
                      const int32_t *__restrict__ mats, const int32_t *smats) {
-  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz];
+  txSz = d_uni(txSz);
+  txType = d_uni(txType);
+  const int lw = d_uni(Tx_Width_Log2[txSz]), lh = d_uni(Tx_Height_Log2[txSz]);
   const int rk = row_kind(txType), ck = col_kind(txType);
   if constexpr (SMALL) {
     /* one wave (c.nt == 64) */
     const lds_i32 *Mh = lds(smats) + smat_off(rk, lw), *Mv = lds(smats) + smat_off(ck, lh);
-    const int sh = Transform_Row_Shift[txSz] + 4;
+    const int sh = d_uni(Transform_Row_Shift[txSz]) + 4;
     const lds_i32 *r = lds(resid);
     lds_i32 *tp = lds(tmp), *co = lds(coef);
     switch (lw * 4 + lh) {
@@ -663,7 +667,8 @@ DEVI void tx_quantize(const TxCtx &c, const int32_t *coef_, int32_t *q_, int txS
                       float rAc) {
   const lds_i32 *coef = lds(coef_);
   lds_i32 *q = lds(q_);
-  const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
+  txSz = d_uni(txSz);
+  const int tw = d_min(32, d_uni(Tx_Width[txSz])), th = d_min(32, d_uni(Tx_Height[txSz])), denom = dq_denom(txSz);
   for (int k = c.tid; k < th * tw; k += c.nt) {
     const bool dc = k == 0;
     const int qv = dc ? dcQ : acQ, rnd = dc ? 26 : 22;
@@ -686,8 +691,9 @@ DEVI void tx_quantize(const TxCtx &c, const int32_t *coef_, int32_t *q_, int txS
 DEVI void tx_dequant(const TxCtx &c, const int32_t *q_, int32_t *dq_, int qs, int txSz, int dcQ, int acQ) {
   const lds_i32 *q = lds(q_);
   lds_i32 *dq = lds(dq_);
-  const int tw = d_min(32, (int)Tx_Width[txSz]), th = d_min(32, (int)Tx_Height[txSz]), denom = dq_denom(txSz);
-  const int ltw = d_min(5, (int)Tx_Width_Log2[txSz]);
+  txSz = d_uni(txSz);
+  const int tw = d_min(32, d_uni(Tx_Width[txSz])), th = d_min(32, d_uni(Tx_Height[txSz])), denom = dq_denom(txSz);
+  const int ltw = d_min(5, d_uni(Tx_Width_Log2[txSz]));
   for (int k = c.tid; k < th * tw; k += c.nt) {
     const int i = k >> ltw, j = k & (tw - 1);
     /* levels are at most 0xFFFF and quantisers at most 1828: the product

@@ -42,6 +42,7 @@ hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
 /* Intra phase work space (ints) and whether the frame fits its scheduler. */
 size_t intra_work_ints(const Mav1FrameParams &fp);
 int intra_sched_fits(const Mav1FrameParams &fp);
+int intra_workgroups(const Mav1FrameParams &fp, int num_cu);
 /* The intra phase: the scheduling pass, then `workgroups` persistent
  * workgroups over the superblocks holding intra blocks; iwork[2] becomes
  * non-zero if a dependency wait ran out. */

@@ -374,14 +374,22 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
                   int x, int y, int txSz, int plane, int t, bool direct, int32_t *resid, int rs, int32_t *tmp,
                   int32_t *coef, int32_t *q, int32_t *dq, int qs, const int32_t *__restrict__ mats,
                   long long *costOut) {
-  const int lw = Tx_Width_Log2[txSz], lh = Tx_Height_Log2[txSz], w = 1 << lw, h = 1 << lh;
+  /* one transform block per call: its size, plane, type and placement are
+   * the same in every lane */
+  txSz = d_uni(txSz);
+  plane = d_uni(plane);
+  t = d_uni(t);
+  x = d_uni(x);
+  y = d_uni(y);
+  S = d_uni(S);
+  const int lw = d_uni(Tx_Width_Log2[txSz]), lh = d_uni(Tx_Height_Log2[txSz]), w = 1 << lw, h = 1 << lh;
   const int dcQ = fp.dc_q[plane], acQ = fp.ac_q[plane];
   tx_residual(c, spx, px, S, x, y, lw, lh, t, resid, rs);
   tx_forward<SMALL>(c, resid, rs, tmp, coef, txSz, t, mats, gL.smats);
   tx_quantize(c, coef, q, txSz, dcQ, acQ, gL.qrcp[plane][0], gL.qrcp[plane][1]);
   /* transforms up to 16x16: the scan staged in LDS, addressed as LDS */
   const int16_t *scan = SMALL ? gL.scans + scan_kind(t) * SCAN_SMALL + small_scan_off(txSz) : get_scan(txSz, t);
-  const int eob = tx_eob(c, q, scan, seg_eob(txSz));
+  const int eob = d_uni(tx_eob(c, q, scan, d_uni(seg_eob(txSz))));
   if (direct) {
     *costOut = 0;
     return eob;
@@ -635,6 +643,20 @@ DEVI void edge_upsample(const TxCtx &c, int *e, int *dup, int numPx) {
 DEVI void intra_tx_predict(const TxCtx &c, uint8_t *px, int S, int sx, int sy, int lw, int lh, int mode, int delta,
                            int hl, int ha, int har, int hbl, int maxX, int maxY, int edgeFilter, int filterType,
                            int *eA, int *eL, int *ecp) {
+  /* one transform block per team: the block's parameters are the same in
+   * every lane */
+  S = d_uni(S);
+  sx = d_uni(sx);
+  sy = d_uni(sy);
+  lw = d_uni(lw);
+  lh = d_uni(lh);
+  mode = d_uni(mode);
+  delta = d_uni(delta);
+  hl = d_uni(hl);
+  ha = d_uni(ha);
+  har = d_uni(har);
+  hbl = d_uni(hbl);
+  filterType = d_uni(filterType);
   const int tid = c.tid, w = 1 << lw, h = 1 << lh;
   int *A = eA + EDGE_NEG, *Lc = eL + EDGE_NEG;
   /* 7.11.2 steps 1-3: the neighbouring samples */
@@ -652,7 +674,10 @@ DEVI void intra_tx_predict(const TxCtx &c, uint8_t *px, int S, int sx, int sy, i
   }
   IntraMode im;
   intra_mode_setup(im, mode, delta, lw, lh);
-  im.corner = A[-1];
+  im.pAngle = d_uni(im.pAngle);
+  im.dx = d_uni(im.dx);
+  im.dy = d_uni(im.dy);
+  im.corner = d_uni(A[-1]);
   if (mode >= V_PRED && mode <= D67_PRED && edgeFilter) {
     const int pa = im.pAngle;
     if (pa != 90 && pa != 180) {
@@ -1087,15 +1112,18 @@ struct SiGeo {
   const uint8_t *spx;
 };
 DEVI SiGeo si_geo(const Mav1BlockInfo &b, int plane) {
+  /* one block per wave: every field is the same in each lane */
   SiGeo g;
+  plane = d_uni(plane);
+  const int bs = d_uni(b.bsize), mr = d_uni(b.mi_row), mc = d_uni(b.mi_col);
   g.sub = plane > 0;
-  g.txSz = plane ? uv_tx_size(b.bsize) : Max_Tx_Size_Rect[b.bsize];
-  g.lw = Tx_Width_Log2[g.txSz];
-  g.lh = Tx_Height_Log2[g.txSz];
-  g.sx = (b.mi_col >> g.sub) * 4;
-  g.sy = (b.mi_row >> g.sub) * 4;
-  g.px = tile_ptr(0, plane, b.mi_row, b.mi_col, g.S);
-  g.spx = tile_ptr(1, plane, b.mi_row, b.mi_col, g.S);
+  g.txSz = d_uni(plane ? uv_tx_size(bs) : Max_Tx_Size_Rect[bs]);
+  g.lw = d_uni(Tx_Width_Log2[g.txSz]);
+  g.lh = d_uni(Tx_Height_Log2[g.txSz]);
+  g.sx = (mc >> g.sub) * 4;
+  g.sy = (mr >> g.sub) * 4;
+  g.px = tile_ptr(0, plane, mr, mc, g.S);
+  g.spx = tile_ptr(1, plane, mr, mc, g.S);
   return g;
 }
 
@@ -1106,24 +1134,27 @@ DEVI void si_predict(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane, SiW
   const TxCtx c = wave_ctx();
   const int lane = c.tid;
   const SiGeo g = si_geo(b, plane);
-  const int sub = g.sub, stepX = Tx_Width[g.txSz] >> 2, stepY = Tx_Height[g.txSz] >> 2;
+  plane = d_uni(plane);
+  const int sub = g.sub, stepX = d_uni(Tx_Width[g.txSz]) >> 2, stepY = d_uni(Tx_Height[g.txSz]) >> 2;
   const int sbr = ((g.sy << sub) >> 2) & 15, sbc = ((g.sx << sub) >> 2) & 15;
-  const int availU = b.mi_row > 0, availL = b.mi_col > tile_start(fp, b.mi_col);
-  const int isCfl = plane > 0 && b.uv_mode == UV_CFL_PRED;
-  intra_tx_predict(c, g.px, g.S, g.sx, g.sy, g.lw, g.lh, plane == 0 ? b.y_mode : (isCfl ? DC_PRED : b.uv_mode),
-                   plane == 0 ? b.angle_y : b.angle_uv, availL, availU,
-                   har >= 0 ? har : gL.bd[BDI(plane, (sbr >> sub) - 1, (sbc >> sub) + stepX)],
-                   hbl >= 0 ? hbl : gL.bd[BDI(plane, (sbr >> sub) + stepY, (sbc >> sub) - 1)],
+  const int mr = d_uni(b.mi_row), mc = d_uni(b.mi_col);
+  const int availU = mr > 0, availL = mc > tile_start(fp, mc);
+  const int uvMode = d_uni(b.uv_mode);
+  const int isCfl = plane > 0 && uvMode == UV_CFL_PRED;
+  intra_tx_predict(c, g.px, g.S, g.sx, g.sy, g.lw, g.lh, d_uni(plane == 0 ? b.y_mode : (isCfl ? DC_PRED : uvMode)),
+                   d_uni(plane == 0 ? b.angle_y : b.angle_uv), availL, availU,
+                   d_uni(har >= 0 ? har : gL.bd[BDI(plane, (sbr >> sub) - 1, (sbc >> sub) + stepX)]),
+                   d_uni(hbl >= 0 ? hbl : gL.bd[BDI(plane, (sbr >> sub) + stepY, (sbc >> sub) - 1)]),
                    ((fp.mi_cols * 4) >> sub) - 1, ((fp.mi_rows * 4) >> sub) - 1, fp.enable_intra_edge_filter,
-                   (b.interp_filter >> (plane == 0 ? 4 : 5)) & 1, W.eA, W.eL, W.ecp);
+                   d_uni((b.interp_filter >> (plane == 0 ? 4 : 5)) & 1), W.eA, W.eL, W.ecp);
   if (!isCfl) return;
   /* CfL (7.11.5) over the DC prediction: the luma average of each sample,
    * then one alpha per lane */
   const int w = 1 << g.lw, h = 1 << g.lh, n = w * h;
   int LS;
-  const uint8_t *luma = tile_ptr(0, 0, b.mi_row, b.mi_col, LS);
-  const int lumaTx = Max_Tx_Size_Rect[b.bsize];
-  const int maxLW = b.mi_col * 4 + Tx_Width[lumaTx], maxLH = b.mi_row * 4 + Tx_Height[lumaTx];
+  const uint8_t *luma = tile_ptr(0, 0, mr, mc, LS);
+  const int lumaTx = d_uni(Max_Tx_Size_Rect[d_uni(b.bsize)]);
+  const int maxLW = mc * 4 + d_uni(Tx_Width[lumaTx]), maxLH = mr * 4 + d_uni(Tx_Height[lumaTx]);
   const int dcv = g.px[g.sy * g.S + g.sx];
   int sum = 0;
   for (int k = lane; k < n; k += 64) {
@@ -1171,7 +1202,7 @@ DEVI int si_cfl_apply(const Mav1BlockInfo &b, int plane, SiWave &W, int excludeZ
       idx = oi;
     }
   }
-  const int alpha = idx - 16, avg = W.cflAvg, dcv = W.cflDc;
+  const int alpha = d_uni(idx) - 16, avg = d_uni(W.cflAvg), dcv = d_uni(W.cflDc);
   tx_sync(c);
   for (int k = lane; k < n; k += 64) {
     const int i = k >> g.lw, j = k & (w - 1);
@@ -1187,15 +1218,15 @@ DEVI void si_code_chroma(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane,
                          const int32_t *__restrict__ mats, SmallTx &T, bool mark = true) {
   const TxCtx c = wave_ctx();
   const SiGeo g = si_geo(b, plane);
-  const int set = get_tx_set(g.txSz, 0, fp.reduced_tx_set);
-  int t = Mode_To_Txfm[b.uv_mode];
-  if (Tx_Size_Sqr_Up[g.txSz] > TX_32X32 || !in_set(set, t, 0)) t = DCT_DCT;
+  const int set = d_uni(get_tx_set(g.txSz, 0, fp.reduced_tx_set));
+  int t = d_uni(Mode_To_Txfm[d_uni(b.uv_mode)]);
+  if (d_uni(Tx_Size_Sqr_Up[g.txSz]) > TX_32X32 || !d_uni(in_set(set, t, 0))) t = DCT_DCT;
   long long cost;
   const int eob = tx_eval<true>(c, fp, g.spx, g.px, g.S, g.sx, g.sy, g.txSz, plane, t, true, T.resid, SRS, T.tmp, T.coef,
                           T.q, T.dq, SQS, mats, &cost);
   const int type = eob > 0 ? t : DCT_DCT;
   const int tw = d_min(32, 1 << g.lw), th = d_min(32, 1 << g.lh);
-  int32_t *co = coefs + b.coef_off[plane];
+  int32_t *co = coefs + d_uni(b.coef_off[plane]);
   for (int k = c.tid; k < tw * th; k += 64) co[k] = T.q[k];
   if (eob > 0) {
     tx_dequant(c, T.q, T.dq, SQS, g.txSz, fp.dc_q[plane], fp.ac_q[plane]);
@@ -1560,7 +1591,9 @@ DEVI void lvl_luma_out(const Mav1FrameParams &fp, Mav1BlockInfo &b, const int32_
   const TxCtx c = wave_ctx();
   const SiGeo g = si_geo(b, 0);
   const int tw = d_min(32, 1 << g.lw), th = d_min(32, 1 << g.lh);
-  int32_t *co = coefs + b.coef_off[0];
+  type = d_uni(type);
+  eob = d_uni(eob);
+  int32_t *co = coefs + d_uni(b.coef_off[0]);
   for (int k = c.tid; k < tw * th; k += 64) co[k] = q[k];
   if (eob > 0) {
     tx_dequant(c, q, W.dq, SQS, g.txSz, fp.dc_q[0], fp.ac_q[0]);
@@ -1579,10 +1612,11 @@ DEVI void lvl_luma_out(const Mav1FrameParams &fp, Mav1BlockInfo &b, const int32_
 DEVI void lvl_chroma(const Mav1FrameParams &fp, int prev, int k, int32_t *__restrict__ coefs,
                      const int32_t *__restrict__ mats) {
   RLds &L = gL;
-  const int plane = 1 + (k & 1), wave = threadIdx.x >> 6;
-  Mav1BlockInfo &b = L.sbb[L.pord[prev + (k >> 1)]];
-  if (plane == 2 && b.uv_mode == UV_CFL_PRED) {
-    const int a = si_cfl_apply(b, 2, L.psw[4 + k], b.cfl_alpha_u == 0);
+  k = d_uni(k);
+  const int plane = 1 + (k & 1), wave = d_uni(threadIdx.x >> 6);
+  Mav1BlockInfo &b = L.sbb[d_uni(L.pord[prev + (k >> 1)])];
+  if (plane == 2 && d_uni(b.uv_mode) == UV_CFL_PRED) {
+    const int a = si_cfl_apply(b, 2, L.psw[4 + k], d_uni(b.cfl_alpha_u) == 0);
     if ((threadIdx.x & 63) == 0) b.cfl_alpha_v = a;
   }
   si_code_chroma(fp, b, plane, coefs, mats, L.u.sm[wave], false);
@@ -1593,7 +1627,11 @@ DEVI void lvl_chroma(const Mav1FrameParams &fp, int prev, int k, int32_t *__rest
 DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const int32_t *__restrict__ mats, int cur,
                    int ncur, int prev, int nprev) {
   RLds &L = gL;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, wave = d_uni(tid >> 6), lane = tid & 63;
+  cur = d_uni(cur);
+  ncur = d_uni(ncur);
+  prev = d_uni(prev);
+  nprev = d_uni(nprev);
   const int ntask = ncur + 2 * nprev;
   /* P1: the luma predictions of this step's blocks and the chroma
    * predictions of the previous step's blocks (U's CfL alpha at once), one
@@ -1603,20 +1641,20 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
    * and writes its own chroma block; each task has its own SiWave. */
   for (int t = wave; t < ntask; t += RW) {
     if (t < ncur) {
-      const int i = L.pord[cur + t];
+      const int i = d_uni(L.pord[cur + t]);
       Mav1BlockInfo &b = L.sbb[i];
       if (lane == 0) {
         b.tx_depth = 0;
         b.interp_filter &= 0x30;
       }
-      const int av = L.pav[i];
+      const int av = d_uni(L.pav[i]);
       si_predict(fp, b, 0, L.psw[t], av & 1, (av >> 1) & 1);
     } else {
-      const int k = t - ncur, plane = 1 + (k & 1), i = L.pord[prev + (k >> 1)];
+      const int k = t - ncur, plane = 1 + (k & 1), i = d_uni(L.pord[prev + (k >> 1)]);
       Mav1BlockInfo &b = L.sbb[i];
-      const int av = L.pav[i];
+      const int av = d_uni(L.pav[i]);
       si_predict(fp, b, plane, L.psw[4 + k], (av >> (2 * plane)) & 1, (av >> (2 * plane + 1)) & 1);
-      if (plane == 1 && b.uv_mode == UV_CFL_PRED) {
+      if (plane == 1 && d_uni(b.uv_mode) == UV_CFL_PRED) {
         const int a = si_cfl_apply(b, 1, L.psw[4 + k], 0);
         if (lane == 0) b.cfl_alpha_u = a;
       }
@@ -1629,7 +1667,7 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
   for (int k = 0; k < 4; k++) {
     off[k] = total;
     if (k < ncur) {
-      const int nc = lvl_ncands(fp, Max_Tx_Size_Rect[L.sbb[L.pord[cur + k]].bsize]);
+      const int nc = d_uni(lvl_ncands(fp, d_uni(Max_Tx_Size_Rect[d_uni(L.sbb[d_uni(L.pord[cur + k])].bsize)])));
       total += nc;
       single = single && nc == 1;
     }
@@ -1641,7 +1679,7 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
     const TxCtx c = wave_ctx();
     for (int t = wave; t < ntask; t += RW) {
       if (t < ncur) {
-        Mav1BlockInfo &b = L.sbb[L.pord[cur + t]];
+        Mav1BlockInfo &b = L.sbb[d_uni(L.pord[cur + t])];
         const SiGeo g = si_geo(b, 0);
         SmallTx &W = L.u.sm[wave];
         long long cost;
@@ -1663,7 +1701,7 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
     if (t < total) {
       int k = 0;
       while (t >= off[k + 1]) k++;
-      Mav1BlockInfo &b = L.sbb[L.pord[cur + k]];
+      Mav1BlockInfo &b = L.sbb[d_uni(L.pord[cur + k])];
       const SiGeo g = si_geo(b, 0);
       SmallTx &W = L.u.sm[wave];
       long long cost;
@@ -1691,10 +1729,10 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
     }
     lds_barrier();
     for (int k = 0; k < ncur; k++) {
-      const int win = L.pwin[k];
+      const int win = d_uni(L.pwin[k]);
       if (win < 0) continue;
-      const Mav1BlockInfo &b = L.sbb[L.pord[cur + k]];
-      const int txSz = Max_Tx_Size_Rect[b.bsize];
+      const Mav1BlockInfo &b = L.sbb[d_uni(L.pord[cur + k])];
+      const int txSz = d_uni(Max_Tx_Size_Rect[d_uni(b.bsize)]);
       const int n = d_min(32, Tx_Width[txSz]) * d_min(32, Tx_Height[txSz]);
       for (int e = tid; e < n; e += RT) L.pq[k][e] = L.u.sm[win].q[e];
     }
@@ -1703,7 +1741,8 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
   /* P3: the chosen luma types' reconstruction beside the chroma coding */
   for (int t = wave; t < ntask; t += RW) {
     if (t < ncur)
-      lvl_luma_out(fp, L.sbb[L.pord[cur + t]], L.pq[t], L.ptype[t], L.peob[t], coefs, L.u.sm[wave]);
+      lvl_luma_out(fp, L.sbb[d_uni(L.pord[cur + t])], L.pq[t], d_uni(L.ptype[t]), d_uni(L.peob[t]), coefs,
+                   L.u.sm[wave]);
     else
       lvl_chroma(fp, prev, t - ncur, coefs, mats);
   }
@@ -1760,12 +1799,12 @@ DEVI void intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks
   if (L.pok) {
     lvl_schedule(fp, nb);
     int prev = 0, nprev = 0, cur = 0;
-    const int pn = L.pn;
+    const int pn = d_uni(L.pn);
     for (;;) {
       int ncur = 0;
       if (cur < pn) {
-        const int lv = L.plv[L.pord[cur]];
-        while (ncur < LVL_MAXW && cur + ncur < pn && L.plv[L.pord[cur + ncur]] == lv) ncur++;
+        const int lv = d_uni(L.plv[d_uni(L.pord[cur])]);
+        while (ncur < LVL_MAXW && cur + ncur < pn && d_uni(L.plv[d_uni(L.pord[cur + ncur])]) == lv) ncur++;
       }
       if (!ncur && !nprev) break;
       lvl_step(fp, coefs, mats, cur, ncur, prev, nprev);
@@ -1943,6 +1982,18 @@ hipError_t launch_recon_inter(hipStream_t st, const Mav1FrameParams &fp, Mav1Blo
 
 /* list, has, done and (batched form) the pending-dependency counts */
 size_t intra_work_ints(const Mav1FrameParams &fp) { return IW_HDR + 4 * (size_t)fp.sb_cols * fp.sb_rows; }
+/* Workgroups for the intra phase of one frame: twice the superblocks the
+ * dependency order lets run at once (per tile column, one superblock in
+ * every second column of a diagonal c + 2 r), at least 16, at most the CU
+ * count. More only hold CUs while they wait, which the other streams'
+ * kernels could use. */
+int intra_workgroups(const Mav1FrameParams &fp, int num_cu) {
+  const int tiles = (fp.sb_cols + fp.tile_w_sb - 1) / fp.tile_w_sb;
+  const int per = ((fp.tile_w_sb < 2 * fp.sb_rows ? fp.tile_w_sb : 2 * fp.sb_rows) + 1) / 2;
+  const int wg = 2 * tiles * per < 16 ? 16 : 2 * tiles * per;
+  return wg < num_cu ? wg : num_cu;
+}
+
 int intra_sched_fits(const Mav1FrameParams &fp) {
   return (fp.sb_cols + 2 * fp.sb_rows) * fp.sb_rows <= SCHED_KEYS && fp.sb_cols * fp.sb_rows < 65535;
 }
