@@ -342,6 +342,55 @@ struct TxCtx {
   long long *red;
 };
 
+/* Wave reductions through DPP lane swaps instead of LDS permutes, for a
+ * wave whose 64 lanes are all active; the result is a scalar (every lane
+ * gets it). Within each row of 16 lanes: swaps across 1 and 2 (quad
+ * permutes), the half-row mirror and the row mirror leave the row's total in
+ * every lane of the row; the four row totals are then read out. */
+#define DPP_QX1 0xB1   /* quad_perm [1, 0, 3, 2] */
+#define DPP_QX2 0x4E   /* quad_perm [2, 3, 0, 1] */
+#define DPP_HMIRROR 0x141
+#define DPP_MIRROR 0x140
+template <int CTRL>
+DEVI int dpp_t(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+DEVI int wave_max_dpp(int v) {
+  v = d_max(v, dpp_t<DPP_QX1>(v));
+  v = d_max(v, dpp_t<DPP_QX2>(v));
+  v = d_max(v, dpp_t<DPP_HMIRROR>(v));
+  v = d_max(v, dpp_t<DPP_MIRROR>(v));
+  return d_max(d_max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               d_max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+DEVI int wave_sum_dpp(int v) {
+  v += dpp_t<DPP_QX1>(v);
+  v += dpp_t<DPP_QX2>(v);
+  v += dpp_t<DPP_HMIRROR>(v);
+  v += dpp_t<DPP_MIRROR>(v);
+  return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
+         __builtin_amdgcn_readlane(v, 48);
+}
+template <int CTRL>
+DEVI long long dpp_t64(long long v) {
+  const int lo = dpp_t<CTRL>((int)(unsigned)(unsigned long long)v), hi = dpp_t<CTRL>((int)(v >> 32));
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+DEVI long long wave_sum64_dpp(long long v) {
+  v += dpp_t64<DPP_QX1>(v);
+  v += dpp_t64<DPP_QX2>(v);
+  v += dpp_t64<DPP_HMIRROR>(v);
+  v += dpp_t64<DPP_MIRROR>(v);
+  long long t = 0;
+#pragma unroll
+  for (int r = 0; r < 64; r += 16) {
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)(unsigned long long)v, r);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(v >> 32), r);
+    t += (long long)(((unsigned long long)hi << 32) | lo);
+  }
+  return t;
+}
+
 DEVI void tx_sync(const TxCtx &c) {
   if (c.wg) {
     __syncthreads();
@@ -352,7 +401,7 @@ DEVI void tx_sync(const TxCtx &c) {
   }
 }
 DEVI long long tx_sum64(const TxCtx &c, long long v) {
-  v = wave_sum64(v);
+  v = wave_sum64_dpp(v);
   if (!c.wg) return v;
   if ((c.tid & 63) == 0) c.red[c.tid >> 6] = v;
   __syncthreads();

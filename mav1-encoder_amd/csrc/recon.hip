@@ -196,6 +196,8 @@ struct RLds {
   int peob[4], ptype[4], pwin[4];
   long long rcost[RW];      /* this round's candidates: cost and eob per wave */
   int reob[RW];
+  int ufl[64];              /* per block: U's CfL alpha is set (V's CfL waits for it) */
+  int lerr;                 /* a wait for ufl ran out */
 };
 
 /* The working set of the workgroup that reconstructs a block / superblock.
@@ -1165,7 +1167,7 @@ DEVI void si_predict(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane, SiW
     W.cfl[k] = v;
     sum += v;
   }
-  const int avg = d_round2(wave_sum(sum), g.lw + g.lh);
+  const int avg = d_round2(wave_sum_dpp(sum), g.lw + g.lh);
   tx_sync(c);
   if (lane < 33) {
     const int a = lane - 16;
@@ -1534,16 +1536,16 @@ DEVI void lvl_schedule(const Mav1FrameParams &fp, int n) {
     }
     for (int i = 0; i < n; i++) {
       const Mav1BlockInfo &b = L.sbb[i];
-      if (b.is_inter) continue;
-      const int r = b.mi_row & 15, c = b.mi_col & 15, h = Num_4x4_Blocks_High[b.bsize], w = Num_4x4_Blocks_Wide[b.bsize];
+      if (d_uni(b.is_inter)) continue;
+      const int r = d_uni(b.mi_row) & 15, c = d_uni(b.mi_col) & 15, bs = d_uni(b.bsize);
+      const int h = d_uni(Num_4x4_Blocks_High[bs]), w = d_uni(Num_4x4_Blocks_Wide[bs]);
       int m = 0;
       if (j < i && intraJ) {
         const bool left = c - 1 >= cj && c - 1 < cj + wj && r - 1 < rj + hj && r + 2 * h > rj;
         const bool above = r - 1 >= rj && r - 1 < rj + hj && c - 1 < cj + wj && c + 2 * w > cj;
         if (left || above) m = myLv;
       }
-#pragma unroll
-      for (int sh = 32; sh >= 1; sh >>= 1) m = d_max(m, __shfl_xor(m, sh));
+      m = wave_max_dpp(m);
       if (j == i) myLv = m + 1;
     }
     if (j < n) L.plv[j] = myLv;
@@ -1616,10 +1618,51 @@ DEVI void lvl_chroma(const Mav1FrameParams &fp, int prev, int k, int32_t *__rest
   const int plane = 1 + (k & 1), wave = d_uni(threadIdx.x >> 6);
   Mav1BlockInfo &b = L.sbb[d_uni(L.pord[prev + (k >> 1)])];
   if (plane == 2 && d_uni(b.uv_mode) == UV_CFL_PRED) {
+    /* U's alpha (another wave's task of the same step) decides whether V
+     * may take alpha 0 */
+    const int i = d_uni(L.pord[prev + (k >> 1)]);
+    int spin = 0;
+    while (!d_uni(__hip_atomic_load(&L.ufl[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) && spin < (1 << 24)) {
+      __builtin_amdgcn_s_sleep(1);
+      spin++;
+    }
+    if (spin == (1 << 24) && (threadIdx.x & 63) == 0) L.lerr = 1;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     const int a = si_cfl_apply(b, 2, L.psw[4 + k], d_uni(b.cfl_alpha_u) == 0);
     if ((threadIdx.x & 63) == 0) b.cfl_alpha_v = a;
   }
   si_code_chroma(fp, b, plane, coefs, mats, L.u.sm[wave], false);
+}
+
+/* The prediction of task t of a step (luma of block pord[cur + t] for
+ * t < ncur, else U / V of block pord[prev + (t - ncur) / 2]) on one wave;
+ * U's CfL alpha is chosen at once and published in ufl. */
+DEVI void lvl_predict(const Mav1FrameParams &fp, int t, int cur, int ncur, int prev) {
+  RLds &L = gL;
+  const int lane = threadIdx.x & 63;
+  if (t < ncur) {
+    const int i = d_uni(L.pord[cur + t]);
+    Mav1BlockInfo &b = L.sbb[i];
+    if (lane == 0) {
+      b.tx_depth = 0;
+      b.interp_filter &= 0x30;
+    }
+    const int av = d_uni(L.pav[i]);
+    si_predict(fp, b, 0, L.psw[t], av & 1, (av >> 1) & 1);
+  } else {
+    const int k = t - ncur, plane = 1 + (k & 1), i = d_uni(L.pord[prev + (k >> 1)]);
+    Mav1BlockInfo &b = L.sbb[i];
+    const int av = d_uni(L.pav[i]);
+    si_predict(fp, b, plane, L.psw[4 + k], (av >> (2 * plane)) & 1, (av >> (2 * plane + 1)) & 1);
+    if (plane == 1 && d_uni(b.uv_mode) == UV_CFL_PRED) {
+      const int a = si_cfl_apply(b, 1, L.psw[4 + k], 0);
+      if (lane == 0) {
+        b.cfl_alpha_u = a;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __hip_atomic_store(&L.ufl[i], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+  }
 }
 
 /* One step of the level schedule: luma of blocks pord[cur .. cur + ncur)
@@ -1633,34 +1676,6 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
   prev = d_uni(prev);
   nprev = d_uni(nprev);
   const int ntask = ncur + 2 * nprev;
-  /* P1: the luma predictions of this step's blocks and the chroma
-   * predictions of the previous step's blocks (U's CfL alpha at once), one
-   * task per wave. They touch disjoint data: a luma prediction reads luma
-   * of lower levels and writes its own block; a chroma prediction reads
-   * chroma of lower levels and its own block's reconstructed luma (CfL),
-   * and writes its own chroma block; each task has its own SiWave. */
-  for (int t = wave; t < ntask; t += RW) {
-    if (t < ncur) {
-      const int i = d_uni(L.pord[cur + t]);
-      Mav1BlockInfo &b = L.sbb[i];
-      if (lane == 0) {
-        b.tx_depth = 0;
-        b.interp_filter &= 0x30;
-      }
-      const int av = d_uni(L.pav[i]);
-      si_predict(fp, b, 0, L.psw[t], av & 1, (av >> 1) & 1);
-    } else {
-      const int k = t - ncur, plane = 1 + (k & 1), i = d_uni(L.pord[prev + (k >> 1)]);
-      Mav1BlockInfo &b = L.sbb[i];
-      const int av = d_uni(L.pav[i]);
-      si_predict(fp, b, plane, L.psw[4 + k], (av >> (2 * plane)) & 1, (av >> (2 * plane + 1)) & 1);
-      if (plane == 1 && d_uni(b.uv_mode) == UV_CFL_PRED) {
-        const int a = si_cfl_apply(b, 1, L.psw[4 + k], 0);
-        if (lane == 0) b.cfl_alpha_u = a;
-      }
-    }
-  }
-  lds_barrier();
   /* luma candidates: offsets per block */
   int off[5], total = 0;
   bool single = true;
@@ -1673,11 +1688,19 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
     }
   }
   off[4] = total;
+  /* The tasks: the luma of this step's blocks (t < ncur) and U / V of the
+   * previous step's blocks, one per wave. Their predictions touch disjoint
+   * data: a luma prediction reads luma of lower levels and writes its own
+   * block; a chroma prediction reads chroma of lower levels and its own
+   * block's reconstructed luma (CfL), and writes its own chroma block; each
+   * task has its own SiWave. The coding of a task reads only its own
+   * prediction, except V's CfL, which waits for U's alpha (ufl). */
   if (single) {
-    /* P3: one type per block: evaluation and reconstruction on the block's
-     * wave, beside the chroma coding */
+    /* one type per block: each wave predicts and codes its tasks end to
+     * end, with no barrier between prediction and coding */
     const TxCtx c = wave_ctx();
     for (int t = wave; t < ntask; t += RW) {
+      lvl_predict(fp, t, cur, ncur, prev);
       if (t < ncur) {
         Mav1BlockInfo &b = L.sbb[d_uni(L.pord[cur + t])];
         const SiGeo g = si_geo(b, 0);
@@ -1693,6 +1716,9 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
     lds_barrier();
     return;
   }
+  /* several types: the predictions first, behind a barrier the rounds */
+  for (int t = wave; t < ntask; t += RW) lvl_predict(fp, t, cur, ncur, prev);
+  lds_barrier();
   /* several types: rounds of one candidate per wave, the best kept per
    * block (strictly smaller cost wins, candidates in list order) */
   if (tid < ncur) L.pcost[tid] = -1;
@@ -1750,8 +1776,10 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
 }
 
 /* Codes the intra blocks of superblock (r, c) (recon_sb, k_recon.h:396-440):
- * BlockDecoded flags, LDS tiles, blocks in decode order, tiles back. */
-DEVI void intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo &S,
+ * BlockDecoded flags, LDS tiles, blocks in decode order, tiles back.
+ * Returns non-zero when a wait inside ran out (the results are then not
+ * to be used). */
+DEVI int intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo &S,
                    const uint8_t *__restrict__ src, uint8_t *rec, const uint8_t *__restrict__ ref,
                    const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs, const int32_t *__restrict__ mats,
                    int r, int c) {
@@ -1787,8 +1815,14 @@ DEVI void intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks
    * takes the small-block path and no block is smaller than 8x8 */
   const int nb = S.num_blocks;
   for (int q = tid; q < nb * 27; q += RT) ((int32_t *)L.sbb)[q] = ((const int32_t *)(blocks + S.first_block))[q];
-  for (int q = tid; q < 64; q += RT) L.pav[q] = 0;
-  if (tid == 0) L.pok = 1;
+  for (int q = tid; q < 64; q += RT) {
+    L.pav[q] = 0;
+    L.ufl[q] = 0;
+  }
+  if (tid == 0) {
+    L.pok = 1;
+    L.lerr = 0;
+  }
   __syncthreads();
   if (tid < nb) {
     const Mav1BlockInfo &b = L.sbb[tid];
@@ -1879,6 +1913,7 @@ DEVI void intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks
       *(uint32_t *)(rec + fp.offset[p] + (y0 + ty) * stride + x0 + tx) = tr[(ty + 1) * (TS / 4) + 1 + tx / 4];
     }
   }
+  return L.lerr;
 }
 
 __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Mav1BlockInfo *__restrict__ blocks,
@@ -1913,10 +1948,11 @@ __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Ma
     }
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    intra_sb(fp, blocks, sbs[sb], src, rec, ref, gold, coefs, mats, r, c);
+    const int err = intra_sb(fp, blocks, sbs[sb], src, rec, ref, gold, coefs, mats, r, c);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (tid == 0) {
+      if (err) atomicOr(&iwork[IW_ERR], 1);
       __hip_atomic_store(&done[sb], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       L.ival[2] = atomicAdd(&iwork[IW_NEXT], 1);
     }
@@ -2168,11 +2204,12 @@ __global__ __launch_bounds__(RT) void k_recon_intra_b(const BatchFrame *__restri
     __syncthreads();
     const int r = sb / fp.sb_cols, c = sb - r * fp.sb_cols;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    intra_sb(fp, gptr(F.blocks), gptr(F.sbs)[sb], gptr(F.src), gptr(F.rec), gptr(F.ref), gptr(F.gold), gptr(F.coefs),
-             gptr(F.mats), r, c);
+    const int err = intra_sb(fp, gptr(F.blocks), gptr(F.sbs)[sb], gptr(F.src), gptr(F.rec), gptr(F.ref), gptr(F.gold),
+                             gptr(F.coefs), gptr(F.mats), r, c);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (tid == 0) {
+      if (err) atomicOr(&gptr(F.iwork)[IW_ERR], 1);
       /* the superblocks that read this one: queued when this was their last
        * pending dependency */
       const int n = fp.sb_cols * fp.sb_rows;
