@@ -614,15 +614,45 @@ DEVI void tx_forward_m(const TxCtx &c, const int32_t *resid, int rs, int32_t *tm
   tx_sync(c);
 }
 
+/* The quantiser's reciprocal for tx_quantize: 1 / q rounded to float. */
+DEVI float tx_qrcp(int q) { return 1.0f / (float)q; }
+
+/* floor(t / q) for 0 <= t < 2^19 and 4 <= q < 2^16 (which the reference
+ * computes as t * (2^44 / q + 1) >> 44): t * rq is within 2^-5 of t / q
+ * (t and q exact in float, rq and the product one rounding each), so its
+ * truncation is off by at most one, which the remainder corrects. */
+DEVI int tx_qdiv(int t, int q, float rq) {
+  int l = (int)((float)t * rq);
+  const int r = t - __mul24(l, q);
+  return l + (r >= q) - (r < 0);
+}
+
+/* One coefficient v through the dead-zone quantiser (k_fwd.h:66-90): the
+ * signed level. Coefficients are below 2^21 in magnitude, so t stays below
+ * 2^19; larger ones take an integer division. */
+DEVI int tx_quant1(int v, bool dc, int dcQ, int acQ, float rDc, float rAc, int denom) {
+  const int qv = dc ? dcQ : acQ, rnd = dc ? 26 : 22, av = d_abs(v);
+  int level;
+  if (av < (1 << 21)) {
+    const int t = (av * denom + ((qv * rnd) >> 2)) >> 4;
+    level = tx_qdiv(t, qv, dc ? rDc : rAc);
+  } else {
+    const long long t = ((long long)av * denom + ((qv * rnd) >> 2)) >> 4;
+    level = t / qv > 0xFFFF ? 0xFFFF : (int)(t / qv);
+  }
+  if (level > 0xFFFF) level = 0xFFFF;
+  return v < 0 ? -level : level;
+}
+
 /* One wave's forward transform of a (1 << LH) x (1 << LW) block, both
  * lengths at most 16, with the sizes known at compile time: every LDS
  * access is a fixed offset from a per-lane base, so the loads of a sum issue
  * back to back. The row pass writes tmp transposed (column kk at kk * H) so
  * that the column pass also reads consecutive words. Same sums and
  * roundings as fwd_rows / fwd_cols. */
-template <int LW, int LH>
+template <int LW, int LH, bool QUANT = false>
 DEVI void sfwd(const lds_i32 *resid, int rs, lds_i32 *tmp, lds_i32 *coef, const lds_i32 *Mh, const lds_i32 *Mv,
-               int sh, int lane_) {
+               int sh, int lane_, int dcQ = 0, int acQ = 0, float rDc = 0.f, float rAc = 0.f) {
   constexpr int W = 1 << LW, H = 1 << LH, NO = W * H, IT = (NO + 63) / 64;
   const int lane = d_local(lane_);
   constexpr bool RECT = LW - LH == 1 || LH - LW == 1;
@@ -652,12 +682,40 @@ DEVI void sfwd(const lds_i32 *resid, int rs, lds_i32 *tmp, lds_i32 *coef, const 
       for (int i = 0; i < H; i++) s += (long long)m[i] * col[i];
       s *= (1ll << sh);
       if constexpr (RECT) s = ((s + 2048) >> 12) * 5793;
-      coef[k] = (int32_t)(s >= 0 ? (s + (1ll << 19)) >> 20 : -((-s + (1ll << 19)) >> 20));
+      const int v = (int32_t)(s >= 0 ? (s + (1ll << 19)) >> 20 : -((-s + (1ll << 19)) >> 20));
+      /* QUANT: the level at once (sizes up to 16 have no dequantiser denominator) */
+      coef[k] = QUANT ? tx_quant1(v, k == 0, dcQ, acQ, rDc, rAc, 1) : v;
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* One wave's forward transform and quantiser of a block of at most 16x16:
+ * levels into q (stride 1 << lw). */
+DEVI void tx_forward_quant_small(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int32_t *q, int txSz,
+                                 int txType, const int32_t *smats, int dcQ, int acQ, float rDc, float rAc) {
+  txSz = d_uni(txSz);
+  txType = d_uni(txType);
+  const int lw = d_uni(Tx_Width_Log2[txSz]), lh = d_uni(Tx_Height_Log2[txSz]);
+  const lds_i32 *Mh = lds(smats) + smat_off(row_kind(txType), lw), *Mv = lds(smats) + smat_off(col_kind(txType), lh);
+  const int sh = d_uni(Transform_Row_Shift[txSz]) + 4;
+  const lds_i32 *r = lds(resid);
+  lds_i32 *tp = lds(tmp), *qo = lds(q);
+#define SFWDQ(a, b) sfwd<a, b, true>(r, rs, tp, qo, Mh, Mv, sh, c.tid, dcQ, acQ, rDc, rAc)
+  switch (lw * 4 + lh) {
+    case 10: SFWDQ(2, 2); break;
+    case 11: SFWDQ(2, 3); break;
+    case 12: SFWDQ(2, 4); break;
+    case 14: SFWDQ(3, 2); break;
+    case 15: SFWDQ(3, 3); break;
+    case 16: SFWDQ(3, 4); break;
+    case 18: SFWDQ(4, 2); break;
+    case 19: SFWDQ(4, 3); break;
+    default: SFWDQ(4, 4); break;
+  }
+#undef SFWDQ
 }
 
 /* smats (may be NULL): the packed short matrices in LDS, used when both
@@ -696,43 +754,15 @@ DEVI void tx_forward(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp,
 }
 
 /* The quantiser's reciprocal of q: floor(2^44 / q) + 1. */
-/* The quantiser's reciprocal for tx_quantize: 1 / q rounded to float. */
-DEVI float tx_qrcp(int q) { return 1.0f / (float)q; }
-
-/* floor(t / q) for 0 <= t < 2^19 and 4 <= q < 2^16 (which the reference
- * computes as t * (2^44 / q + 1) >> 44): t * rq is within 2^-5 of t / q
- * (t and q exact in float, rq and the product one rounding each), so its
- * truncation is off by at most one, which the remainder corrects. */
-DEVI int tx_qdiv(int t, int q, float rq) {
-  int l = (int)((float)t * rq);
-  const int r = t - __mul24(l, q);
-  return l + (r >= q) - (r < 0);
-}
-
 /* Dead-zone quantiser (k_fwd.h:66-90) into the decoder layout (stride tw);
- * rDc / rAc: tx_qrcp of dcQ / acQ. Coefficients are below 2^21 in
- * magnitude, so t stays below 2^19; larger ones take an integer division. */
+ * rDc / rAc: tx_qrcp of dcQ / acQ. */
 DEVI void tx_quantize(const TxCtx &c, const int32_t *coef_, int32_t *q_, int txSz, int dcQ, int acQ, float rDc,
                       float rAc) {
   const lds_i32 *coef = lds(coef_);
   lds_i32 *q = lds(q_);
   txSz = d_uni(txSz);
   const int tw = d_min(32, d_uni(Tx_Width[txSz])), th = d_min(32, d_uni(Tx_Height[txSz])), denom = dq_denom(txSz);
-  for (int k = c.tid; k < th * tw; k += c.nt) {
-    const bool dc = k == 0;
-    const int qv = dc ? dcQ : acQ, rnd = dc ? 26 : 22;
-    const int v = coef[k], av = d_abs(v);
-    int level;
-    if (av < (1 << 21)) {
-      const int t = (av * denom + ((qv * rnd) >> 2)) >> 4;
-      level = tx_qdiv(t, qv, dc ? rDc : rAc);
-    } else {
-      const long long t = ((long long)av * denom + ((qv * rnd) >> 2)) >> 4;
-      level = (int)d_min(t / qv, 0xFFFFll);
-    }
-    if (level > 0xFFFF) level = 0xFFFF;
-    q[k] = v < 0 ? -level : level;
-  }
+  for (int k = c.tid; k < th * tw; k += c.nt) q[k] = tx_quant1(coef[k], k == 0, dcQ, acQ, rDc, rAc, denom);
   tx_sync(c);
 }
 
@@ -751,6 +781,69 @@ DEVI void tx_dequant(const TxCtx &c, const int32_t *q_, int32_t *dq_, int qs, in
     const int d2 = (ad & 0xFFFFFF) >> (denom >> 1);
     dq[i * qs + j] = d_clip(-(1 << 15), (1 << 15) - 1, d < 0 ? -d2 : d2);
   }
+  tx_sync(c);
+}
+
+/* One wave's dequantisation, inverse transform and reconstruction of a
+ * block of at most 16x16 (tx_dequant, tx_inverse and the addition into the
+ * prediction, fused): the row pass dequantises the levels q (stride
+ * 1 << lw) as it loads them, the column pass adds its outputs to px in the
+ * type's flipped order, clipped. res: scratch for the row outputs (row
+ * stride rs). Same values as the three passes. */
+template <int KIND, int N>
+DEVI void sitx_rows_dq(const lds_i32 *q, lds_i32 *res, int rs, int h, int rect, int rowShift, int dcQ, int acQ,
+                       int tid) {
+  constexpr int W = 1 << N;
+  for (int i = tid; i < h; i += 64) {
+    int32_t T[W];
+    int nz = 0;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      const int k = i * W + j, d = __mul24(q[k], k == 0 ? dcQ : acQ), ad = (d < 0 ? -d : d) & 0xFFFFFF;
+      const int v = d_clip(-(1 << 15), (1 << 15) - 1, d < 0 ? -ad : ad);
+      nz |= v;
+      T[j] = rect ? (__mul24(v, 2896) + 2048) >> 12 : v;
+    }
+    if (nz) {
+      t_1d<KIND, N>(T, 16);
+#pragma unroll
+      for (int j = 0; j < W; j++) res[i * rs + j] = d_clip(-(1 << 15), (1 << 15) - 1, d_round2(T[j], rowShift));
+    } else {
+#pragma unroll
+      for (int j = 0; j < W; j++) res[i * rs + j] = 0;
+    }
+  }
+}
+template <int KIND, int N>
+DEVI void sitx_cols_add(const lds_i32 *res, int rs, int w, uint8_t *px, int S, int x, int y, int fud, int flr,
+                        int tid) {
+  constexpr int H = 1 << N;
+  for (int j = tid; j < w; j += 64) {
+    int32_t T[H];
+#pragma unroll
+    for (int i = 0; i < H; i++) T[i] = res[i * rs + j];
+    t_1d<KIND, N>(T, 16);
+    uint8_t *col = px + y * S + x + (flr ? w - 1 - j : j);
+#pragma unroll
+    for (int i = 0; i < H; i++) {
+      uint8_t *o = col + (fud ? H - 1 - i : i) * S;
+      *o = (uint8_t)d_clip1((int)*o + d_round2(T[i], 4));
+    }
+  }
+}
+DEVI void tx_recon_small(const TxCtx &c, const int32_t *q, int32_t *res, int rs, int txSz, int txType, int dcQ,
+                         int acQ, uint8_t *px, int S, int x, int y) {
+  txSz = d_uni(txSz);
+  txType = d_uni(txType);
+  const int lw = d_uni(Tx_Width_Log2[txSz]), lh = d_uni(Tx_Height_Log2[txSz]);
+  const int rect = (lw - lh == 1) || (lh - lw == 1);
+  const int rk = row_kind(txType), ck = col_kind(txType);
+  const lds_i32 *ql = lds(q);
+  lds_i32 *rl = lds(res);
+  ITX_DISPATCH_SMALL(sitx_rows_dq, rk, lw, ql, rl, rs, 1 << lh, rect, d_uni(Transform_Row_Shift[txSz]), dcQ, acQ,
+                     c.tid);
+  tx_sync(c);
+  ITX_DISPATCH_SMALL(sitx_cols_add, ck, lh, rl, rs, 1 << lw, px, S, x, y, flip_ud(txType), flip_lr(txType), c.tid);
   tx_sync(c);
 }
 

@@ -387,8 +387,12 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
   const int lw = d_uni(Tx_Width_Log2[txSz]), lh = d_uni(Tx_Height_Log2[txSz]), w = 1 << lw, h = 1 << lh;
   const int dcQ = fp.dc_q[plane], acQ = fp.ac_q[plane];
   tx_residual(c, spx, px, S, x, y, lw, lh, t, resid, rs);
-  tx_forward<SMALL>(c, resid, rs, tmp, coef, txSz, t, mats, gL.smats);
-  tx_quantize(c, coef, q, txSz, dcQ, acQ, gL.qrcp[plane][0], gL.qrcp[plane][1]);
+  if constexpr (SMALL) {
+    tx_forward_quant_small(c, resid, rs, tmp, q, txSz, t, gL.smats, dcQ, acQ, gL.qrcp[plane][0], gL.qrcp[plane][1]);
+  } else {
+    tx_forward<SMALL>(c, resid, rs, tmp, coef, txSz, t, mats, gL.smats);
+    tx_quantize(c, coef, q, txSz, dcQ, acQ, gL.qrcp[plane][0], gL.qrcp[plane][1]);
+  }
   /* transforms up to 16x16: the scan staged in LDS, addressed as LDS */
   const int16_t *scan = SMALL ? gL.scans + scan_kind(t) * SCAN_SMALL + small_scan_off(txSz) : get_scan(txSz, t);
   const int eob = d_uni(tx_eob(c, q, scan, d_uni(seg_eob(txSz))));
@@ -1230,11 +1234,8 @@ DEVI void si_code_chroma(const Mav1FrameParams &fp, Mav1BlockInfo &b, int plane,
   const int tw = d_min(32, 1 << g.lw), th = d_min(32, 1 << g.lh);
   int32_t *co = coefs + d_uni(b.coef_off[plane]);
   for (int k = c.tid; k < tw * th; k += 64) co[k] = T.q[k];
-  if (eob > 0) {
-    tx_dequant(c, T.q, T.dq, SQS, g.txSz, fp.dc_q[plane], fp.ac_q[plane]);
-    tx_inverse<true>(c, T.dq, SQS, T.resid, SRS, g.txSz, type);
-    tx_add(c, g.px, g.S, g.sx, g.sy, g.lw, g.lh, type, T.resid, SRS);
-  }
+  if (eob > 0)
+    tx_recon_small(c, T.q, T.resid, SRS, g.txSz, type, fp.dc_q[plane], fp.ac_q[plane], g.px, g.S, g.sx, g.sy);
   if (c.tid == 0) {
     b.eob[plane] = eob > 0;
     if (mark) si_mark(fp, b, plane);
@@ -1597,11 +1598,7 @@ DEVI void lvl_luma_out(const Mav1FrameParams &fp, Mav1BlockInfo &b, const int32_
   eob = d_uni(eob);
   int32_t *co = coefs + d_uni(b.coef_off[0]);
   for (int k = c.tid; k < tw * th; k += 64) co[k] = q[k];
-  if (eob > 0) {
-    tx_dequant(c, q, W.dq, SQS, g.txSz, fp.dc_q[0], fp.ac_q[0]);
-    tx_inverse<true>(c, W.dq, SQS, W.resid, SRS, g.txSz, type);
-    tx_add(c, g.px, g.S, g.sx, g.sy, g.lw, g.lh, type, W.resid, SRS);
-  }
+  if (eob > 0) tx_recon_small(c, q, W.resid, SRS, g.txSz, type, fp.dc_q[0], fp.ac_q[0], g.px, g.S, g.sx, g.sy);
   if (c.tid == 0) {
     b.tx_types_lo = (int32_t)(((uint32_t)b.tx_types_lo & ~15u) | (uint32_t)type);
     b.tx_type = type;
