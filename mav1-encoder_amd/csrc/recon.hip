@@ -198,6 +198,11 @@ struct RLds {
   int reob[RW];
   int ufl[64];              /* per block: U's CfL alpha is set (V's CfL waits for it) */
   int lerr;                 /* a wait for ufl ran out */
+  /* the dataflow form: per block the earlier intra blocks its prediction
+   * reads (bit j: block j), and the blocks whose luma / U / V are done */
+  unsigned long long dmask[64];
+  unsigned long long ldm, udm, vdm;
+  int psingle;              /* every intra block of the superblock has a single transform type */
 };
 
 /* The working set of the workgroup that reconstructs a block / superblock.
@@ -1540,14 +1545,18 @@ DEVI void lvl_schedule(const Mav1FrameParams &fp, int n) {
       if (d_uni(b.is_inter)) continue;
       const int r = d_uni(b.mi_row) & 15, c = d_uni(b.mi_col) & 15, bs = d_uni(b.bsize);
       const int h = d_uni(Num_4x4_Blocks_High[bs]), w = d_uni(Num_4x4_Blocks_Wide[bs]);
-      int m = 0;
+      bool dep = false;
       if (j < i && intraJ) {
         const bool left = c - 1 >= cj && c - 1 < cj + wj && r - 1 < rj + hj && r + 2 * h > rj;
         const bool above = r - 1 >= rj && r - 1 < rj + hj && c - 1 < cj + wj && c + 2 * w > cj;
-        if (left || above) m = myLv;
+        dep = left || above;
       }
-      m = wave_max_dpp(m);
-      if (j == i) myLv = m + 1;
+      const unsigned long long dm = __ballot(dep);
+      const int m = wave_max_dpp(dep ? myLv : 0);
+      if (j == i) {
+        myLv = m + 1;
+        L.dmask[i] = dm;
+      }
     }
     if (j < n) L.plv[j] = myLv;
   }
@@ -1772,6 +1781,99 @@ DEVI void lvl_step(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const
   lds_barrier();
 }
 
+/* ---- the level schedule as a dataflow ----
+ * When every intra block of the superblock has a single transform type,
+ * the steps give way to per-block dependencies: luma of block i waits for
+ * the luma of the blocks in dmask[i] (those its prediction reads); U (V)
+ * of block i waits for its own luma and the U (V) of the blocks in
+ * dmask[i] (the chroma prediction reads the same neighbours at half
+ * resolution); V's CfL waits for U's alpha. Waves 0-3 take the luma tasks
+ * in schedule order (task u on wave u % 4), waves 4-7 the chroma tasks
+ * (U then V per block, task ci on wave 4 + ci % 4). Every task waits only
+ * for tasks earlier in its kind's order, and luma waits for no chroma, so
+ * the earliest unfinished task can always run: every wait ends. Waits are
+ * still bounded (a wait that runs out sets lerr). Results equal the steps'. */
+#define LVL_SPIN (1 << 20)
+DEVI unsigned long long lds_load64(const unsigned long long *p) {
+  const unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return ((unsigned long long)(unsigned)d_uni((int)(v >> 32)) << 32) | (unsigned)d_uni((int)(unsigned)v);
+}
+/* Waits until (*m & need) == need and bit `own` of *own (if own) is set. */
+DEVI void lvl_wait(const unsigned long long *m, unsigned long long need, const unsigned long long *own, int i) {
+  for (int it = 0; it < LVL_SPIN; it++) {
+    if ((lds_load64(m) & need) == need && (!own || ((lds_load64(own) >> i) & 1))) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+      return;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  if ((threadIdx.x & 63) == 0) gL.lerr = 1;
+}
+DEVI void lvl_publish(unsigned long long *m, int i) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_or(m, 1ull << i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+DEVI void lvl_dataflow(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, const int32_t *__restrict__ mats) {
+  RLds &L = gL;
+  const int wave = d_uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int pn = d_uni(L.pn);
+  const TxCtx c = wave_ctx();
+  SmallTx &W = L.u.sm[wave];
+  SiWave &P = L.psw[wave];
+  if (wave < 4) {
+    for (int u = wave; u < pn; u += 4) {
+      const int i = d_uni(L.pord[u]);
+      Mav1BlockInfo &b = L.sbb[i];
+      lvl_wait(&L.ldm, L.dmask[i], nullptr, 0);
+      if (lane == 0) {
+        b.tx_depth = 0;
+        b.interp_filter &= 0x30;
+      }
+      const int av = d_uni(L.pav[i]);
+      si_predict(fp, b, 0, P, av & 1, (av >> 1) & 1);
+      const SiGeo g = si_geo(b, 0);
+      long long cost;
+      const int eob = tx_eval<true>(c, fp, g.spx, g.px, g.S, g.sx, g.sy, g.txSz, 0, DCT_DCT, true, W.resid, SRS, W.tmp,
+                                    W.coef, W.q, W.dq, SQS, mats, &cost);
+      lvl_luma_out(fp, b, W.q, DCT_DCT, eob, coefs, W);
+      lvl_publish(&L.ldm, i);
+    }
+  } else {
+    for (int ci = wave - 4; ci < 2 * pn; ci += 4) {
+      const int u = ci >> 1, plane = 1 + (ci & 1), i = d_uni(L.pord[u]);
+      Mav1BlockInfo &b = L.sbb[i];
+      unsigned long long *dn = plane == 1 ? &L.udm : &L.vdm;
+      lvl_wait(dn, L.dmask[i], &L.ldm, i);
+      const int av = d_uni(L.pav[i]);
+      si_predict(fp, b, plane, P, (av >> (2 * plane)) & 1, (av >> (2 * plane + 1)) & 1);
+      if (d_uni(b.uv_mode) == UV_CFL_PRED) {
+        if (plane == 1) {
+          const int a = si_cfl_apply(b, 1, P, 0);
+          if (lane == 0) {
+            b.cfl_alpha_u = a;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            __hip_atomic_store(&L.ufl[i], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        } else {
+          int spin = 0;
+          while (!d_uni(__hip_atomic_load(&L.ufl[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) &&
+                 spin < LVL_SPIN) {
+            __builtin_amdgcn_s_sleep(1);
+            spin++;
+          }
+          if (spin == LVL_SPIN && lane == 0) L.lerr = 1;
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+          const int a = si_cfl_apply(b, 2, P, d_uni(b.cfl_alpha_u) == 0);
+          if (lane == 0) b.cfl_alpha_v = a;
+        }
+      }
+      si_code_chroma(fp, b, plane, coefs, mats, W, false);
+      lvl_publish(dn, i);
+    }
+  }
+  __syncthreads();
+}
+
 /* Codes the intra blocks of superblock (r, c) (recon_sb, k_recon.h:396-440):
  * BlockDecoded flags, LDS tiles, blocks in decode order, tiles back.
  * Returns non-zero when a wait inside ran out (the results are then not
@@ -1819,15 +1921,24 @@ DEVI int intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks,
   if (tid == 0) {
     L.pok = 1;
     L.lerr = 0;
+    L.psingle = 1;
+    L.ldm = L.udm = L.vdm = 0;
   }
   __syncthreads();
   if (tid < nb) {
     const Mav1BlockInfo &b = L.sbb[tid];
     const int bs = b.bsize;
     if (bs < BLOCK_8X8 || bs == BLOCK_4X16 || bs == BLOCK_16X4 || (!b.is_inter && !si_eligible(fp, b))) L.pok = 0;
+    if (!b.is_inter && lvl_ncands(fp, Max_Tx_Size_Rect[bs]) != 1) L.psingle = 0;
   }
   __syncthreads();
-  if (L.pok) {
+  if (L.pok && L.psingle) {
+    lvl_schedule(fp, nb);
+    lvl_dataflow(fp, coefs, mats);
+    for (int q = tid; q < nb * 27; q += RT)
+      if (!L.sbb[q / 27].is_inter) ((int32_t *)(blocks + S.first_block))[q] = ((const int32_t *)L.sbb)[q];
+    __syncthreads();
+  } else if (L.pok) {
     lvl_schedule(fp, nb);
     int prev = 0, nprev = 0, cur = 0;
     const int pn = d_uni(L.pn);
