@@ -30,8 +30,8 @@ import synth  # noqa: E402
 TUNED = os.path.join(HERE, "batch_tuned.json")
 
 CANDIDATES = {
-    "intra_wg_inter": [16, 32, 64, 128, 256],
-    "intra_wg_key": [128, 192, 256],
+    "intra_wg_inter": [16, 32, 64, 128],
+    "intra_wg_key": [32, 64, 128, 192],
     "inter_wg": [128, 256, 512],
 }
 
