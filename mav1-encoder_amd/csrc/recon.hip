@@ -1500,11 +1500,15 @@ DEVI bool wait_flag(const int *flag, int maxSpin) {
  * where an earlier block of the superblock covers the cell. Covering whole
  * blocks equals marking their transform blocks inside the frame, and a cell
  * outside the frame never changes a prediction (the edge fetch clamps to
- * the frame). Each step predicts the luma of up to four blocks of one
- * level, one wave each; then, behind a barrier, the chroma of the previous
- * step's blocks (U and V a wave each; CfL reads the luma finished one step
- * earlier); then the luma transforms and reconstructions beside the chroma
- * coding. Results equal the decode-order pipeline's. */
+ * the frame). When every intra block has a single transform type the
+ * blocks run as a dataflow over these dependencies (lvl_dataflow below).
+ * Otherwise each step takes the luma of up to four blocks of one level and
+ * U and V of the previous step's blocks (CfL reads the luma finished one
+ * step earlier), one task per wave: the predictions, behind a barrier the
+ * luma transform types in rounds of one candidate per wave, then the chosen
+ * types' reconstruction beside the chroma coding (lvl_step). Results equal
+ * the decode-order pipeline's. The level pass also records per block the
+ * mask of the earlier intra blocks it reads (dmask). */
 /* luma blocks coded side by side in one step */
 #define LVL_MAXW 4
 DEVI void lvl_schedule(const Mav1FrameParams &fp, int n) {
