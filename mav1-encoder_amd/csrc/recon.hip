@@ -1883,14 +1883,12 @@ DEVI void lvl_dataflow(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, c
   __syncthreads();
 }
 
-/* Codes the intra blocks of superblock (r, c) (recon_sb, k_recon.h:396-440):
- * BlockDecoded flags, LDS tiles, blocks in decode order, tiles back.
- * Returns non-zero when a wait inside ran out (the results are then not
- * to be used). */
-DEVI int intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo &S,
-                   const uint8_t *__restrict__ src, uint8_t *rec, const uint8_t *__restrict__ ref,
-                   const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs, const int32_t *__restrict__ mats,
-                   int r, int c) {
+/* The intra blocks of superblock (r, c) (recon_sb, k_recon.h:396-440) in
+ * two parts. intra_sb_prep: what does not read the neighbours'
+ * reconstruction (BlockDecoded flags, source tile, block records, the level
+ * schedule), so it can run while the neighbours are still being coded. */
+DEVI void intra_sb_prep(const Mav1FrameParams &fp, const Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo &S,
+                        const uint8_t *__restrict__ src, int r, int c) {
   RLds &L = gL;
   const int tid = threadIdx.x;
   const int c0 = 0, cl = c;
@@ -1904,19 +1902,15 @@ DEVI int intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks,
     if (y == (16 >> sub) && x == -1) v = 0;
     if (y <= (16 >> sub) && x <= (16 >> sub)) L.bd[q] = v;
   }
-  /* the superblock's tiles: reconstruction (with the row above and the
-   * columns left) and source, as dwords */
+  /* the superblock's source tile, as dwords */
   for (int p = 0; p < 3; p++) {
     const int sub = p > 0, TS = sub ? TS_C : TS_Y, n = 64 >> sub, wpr = TS / 4;
     const int oy = ((r * 64) >> sub) - 1, ox = (((c0 + cl) * 64) >> sub) - 4, stride = fp.stride[p];
-    uint32_t *tr = (uint32_t *)(L.tile[0] + (p == 0 ? 0 : TILE_Y + (p - 1) * TILE_C));
     uint32_t *ts = (uint32_t *)(L.tile[1] + (p == 0 ? 0 : TILE_Y + (p - 1) * TILE_C));
     for (int q = tid; q < (n + 1) * wpr; q += RT) {
       const int ty = q / wpr, tx = (q - ty * wpr) * 4, Y = oy + ty, X = ox + tx;
-      if ((ty > 0 && tx >= 4 + n) || Y < 0 || X < 0 || X >= stride) continue;
-      const int o = fp.offset[p] + Y * stride + X;
-      tr[q] = *(const uint32_t *)(rec + o);
-      if (ty > 0 && tx >= 4) ts[q] = *(const uint32_t *)(src + o);
+      if (ty == 0 || tx < 4 || tx >= 4 + n || Y < 0 || X < 0 || X >= stride) continue;
+      ts[q] = *(const uint32_t *)(src + fp.offset[p] + Y * stride + X);
     }
   }
   /* the block records; the level schedule applies when every intra block
@@ -1941,14 +1935,38 @@ DEVI int intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks,
     if (!b.is_inter && lvl_ncands(fp, Max_Tx_Size_Rect[bs]) != 1) L.psingle = 0;
   }
   __syncthreads();
+  if (L.pok) lvl_schedule(fp, nb);
+}
+
+/* intra_sb: after intra_sb_prep and once the neighbouring superblocks are
+ * done, the reconstruction tile (with the row above and the columns left),
+ * the blocks, the tiles back. Returns non-zero when a wait inside ran out
+ * (the results are then not to be used). */
+DEVI int intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks, const Mav1SbInfo &S,
+                  const uint8_t *__restrict__ src, uint8_t *rec, const uint8_t *__restrict__ ref,
+                  const uint8_t *__restrict__ gold, int32_t *__restrict__ coefs, const int32_t *__restrict__ mats,
+                  int r, int c) {
+  RLds &L = gL;
+  const int tid = threadIdx.x;
+  const int c0 = 0, cl = c;
+  for (int p = 0; p < 3; p++) {
+    const int sub = p > 0, TS = sub ? TS_C : TS_Y, n = 64 >> sub, wpr = TS / 4;
+    const int oy = ((r * 64) >> sub) - 1, ox = (((c0 + cl) * 64) >> sub) - 4, stride = fp.stride[p];
+    uint32_t *tr = (uint32_t *)(L.tile[0] + (p == 0 ? 0 : TILE_Y + (p - 1) * TILE_C));
+    for (int q = tid; q < (n + 1) * wpr; q += RT) {
+      const int ty = q / wpr, tx = (q - ty * wpr) * 4, Y = oy + ty, X = ox + tx;
+      if ((ty > 0 && tx >= 4 + n) || Y < 0 || X < 0 || X >= stride) continue;
+      tr[q] = *(const uint32_t *)(rec + fp.offset[p] + Y * stride + X);
+    }
+  }
+  const int nb = S.num_blocks;
+  __syncthreads();
   if (L.pok && L.psingle) {
-    lvl_schedule(fp, nb);
     lvl_dataflow(fp, coefs, mats);
     for (int q = tid; q < nb * 27; q += RT)
       if (!L.sbb[q / 27].is_inter) ((int32_t *)(blocks + S.first_block))[q] = ((const int32_t *)L.sbb)[q];
     __syncthreads();
   } else if (L.pok) {
-    lvl_schedule(fp, nb);
     int prev = 0, nprev = 0, cur = 0;
     const int pn = d_uni(L.pn);
     for (;;) {
@@ -2055,6 +2073,9 @@ __global__ __launch_bounds__(RT) void k_recon_intra(const Mav1FrameParams fp, Ma
     __syncthreads();
     if (idx >= iwork[IW_COUNT]) break;
     const int sb = list[idx], r = sb / fp.sb_cols, c = sb - r * fp.sb_cols;
+    /* everything that does not read the neighbours' reconstruction first,
+     * beside the wait for them */
+    intra_sb_prep(fp, blocks, sbs[sb], src, r, c);
     if (tid == 0) {
       /* the neighbours whose reconstruction this superblock reads */
       const int ts = (c / fp.tile_w_sb) * fp.tile_w_sb, te = d_min(fp.sb_cols, ts + fp.tile_w_sb);
@@ -2321,6 +2342,7 @@ __global__ __launch_bounds__(RT) void k_recon_intra_b(const BatchFrame *__restri
     __syncthreads();
     const int r = sb / fp.sb_cols, c = sb - r * fp.sb_cols;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    intra_sb_prep(fp, gptr(F.blocks), gptr(F.sbs)[sb], gptr(F.src), r, c);
     const int err = intra_sb(fp, gptr(F.blocks), gptr(F.sbs)[sb], gptr(F.src), gptr(F.rec), gptr(F.ref), gptr(F.gold),
                              gptr(F.coefs), gptr(F.mats), r, c);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
