@@ -329,6 +329,7 @@ DEVI int dq_denom(int tx) {
 /* The transform working arrays live in LDS; the stage functions receive
  * them as generic pointers and address them through 32-bit LDS pointers. */
 typedef __attribute__((address_space(3))) int32_t lds_i32;
+typedef __attribute__((address_space(3))) int16_t lds_i16;
 DEVI lds_i32 *lds(int32_t *p) { return (lds_i32 *)p; }
 DEVI const lds_i32 *lds(const int32_t *p) { return (const lds_i32 *)p; }
 
@@ -651,8 +652,8 @@ DEVI int tx_quant1(int v, bool dc, int dcQ, int acQ, float rDc, float rAc, int d
  * that the column pass also reads consecutive words. Same sums and
  * roundings as fwd_rows / fwd_cols. */
 template <int LW, int LH, bool QUANT = false>
-DEVI void sfwd(const lds_i32 *resid, int rs, lds_i32 *tmp, lds_i32 *coef, const lds_i32 *Mh, const lds_i32 *Mv,
-               int sh, int lane_, int dcQ = 0, int acQ = 0, float rDc = 0.f, float rAc = 0.f) {
+DEVI int sfwd(const lds_i32 *resid, int rs, lds_i32 *tmp, lds_i32 *coef, const lds_i32 *Mh, const lds_i32 *Mv, int sh,
+              int lane_, int dcQ = 0, int acQ = 0, float rDc = 0.f, float rAc = 0.f, const lds_i16 *iscan = nullptr) {
   constexpr int W = 1 << LW, H = 1 << LH, NO = W * H, IT = (NO + 63) / 64;
   const int lane = d_local(lane_);
   constexpr bool RECT = LW - LH == 1 || LH - LW == 1;
@@ -671,6 +672,7 @@ DEVI void sfwd(const lds_i32 *resid, int rs, lds_i32 *tmp, lds_i32 *coef, const 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int eob = 0;
 #pragma unroll 1
   for (int t = 0; t < IT; t++) {
     const int k = lane + 64 * t;
@@ -683,19 +685,30 @@ DEVI void sfwd(const lds_i32 *resid, int rs, lds_i32 *tmp, lds_i32 *coef, const 
       s *= (1ll << sh);
       if constexpr (RECT) s = ((s + 2048) >> 12) * 5793;
       const int v = (int32_t)(s >= 0 ? (s + (1ll << 19)) >> 20 : -((-s + (1ll << 19)) >> 20));
-      /* QUANT: the level at once (sizes up to 16 have no dequantiser denominator) */
-      coef[k] = QUANT ? tx_quant1(v, k == 0, dcQ, acQ, rDc, rAc, 1) : v;
+      /* QUANT: the level at once (sizes up to 16 have no dequantiser
+       * denominator), and the eob: 1 + the largest scan position (iscan)
+       * of a non-zero level */
+      if constexpr (QUANT) {
+        const int lv = tx_quant1(v, k == 0, dcQ, acQ, rDc, rAc, 1);
+        coef[k] = lv;
+        if (lv) eob = d_max(eob, iscan[k] + 1);
+      } else {
+        coef[k] = v;
+      }
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return QUANT ? wave_max_dpp(eob) : 0;
 }
 
 /* One wave's forward transform and quantiser of a block of at most 16x16:
- * levels into q (stride 1 << lw). */
-DEVI void tx_forward_quant_small(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int32_t *q, int txSz,
-                                 int txType, const int32_t *smats, int dcQ, int acQ, float rDc, float rAc) {
+ * levels into q (stride 1 << lw); returns the eob for the inverse scan
+ * iscan (the scan position of each coefficient). The whole wave calls it. */
+DEVI int tx_forward_quant_small(const TxCtx &c, const int32_t *resid, int rs, int32_t *tmp, int32_t *q, int txSz,
+                                int txType, const int32_t *smats, int dcQ, int acQ, float rDc, float rAc,
+                                const int16_t *iscan) {
   txSz = d_uni(txSz);
   txType = d_uni(txType);
   const int lw = d_uni(Tx_Width_Log2[txSz]), lh = d_uni(Tx_Height_Log2[txSz]);
@@ -703,17 +716,18 @@ DEVI void tx_forward_quant_small(const TxCtx &c, const int32_t *resid, int rs, i
   const int sh = d_uni(Transform_Row_Shift[txSz]) + 4;
   const lds_i32 *r = lds(resid);
   lds_i32 *tp = lds(tmp), *qo = lds(q);
-#define SFWDQ(a, b) sfwd<a, b, true>(r, rs, tp, qo, Mh, Mv, sh, c.tid, dcQ, acQ, rDc, rAc)
+  const lds_i16 *is = (const lds_i16 *)iscan;
+#define SFWDQ(a, b) return sfwd<a, b, true>(r, rs, tp, qo, Mh, Mv, sh, c.tid, dcQ, acQ, rDc, rAc, is)
   switch (lw * 4 + lh) {
-    case 10: SFWDQ(2, 2); break;
-    case 11: SFWDQ(2, 3); break;
-    case 12: SFWDQ(2, 4); break;
-    case 14: SFWDQ(3, 2); break;
-    case 15: SFWDQ(3, 3); break;
-    case 16: SFWDQ(3, 4); break;
-    case 18: SFWDQ(4, 2); break;
-    case 19: SFWDQ(4, 3); break;
-    default: SFWDQ(4, 4); break;
+    case 10: SFWDQ(2, 2);
+    case 11: SFWDQ(2, 3);
+    case 12: SFWDQ(2, 4);
+    case 14: SFWDQ(3, 2);
+    case 15: SFWDQ(3, 3);
+    case 16: SFWDQ(3, 4);
+    case 18: SFWDQ(4, 2);
+    case 19: SFWDQ(4, 3);
+    default: SFWDQ(4, 4);
   }
 #undef SFWDQ
 }

@@ -174,6 +174,7 @@ struct RLds {
   uint8_t tsrc[64 * 64], tpred[64 * 64]; /* the transform block's source and prediction */
   int32_t smats[TX_SMATS];
   int16_t scans[3 * SCAN_SMALL]; /* scans of the transforms up to 16x16: default, mrow, mcol */
+  int16_t iscans[3 * SCAN_SMALL]; /* their inverses: the scan position of each coefficient */
   float qrcp[3][2];         /* per plane the quantiser reciprocals of dc_q / ac_q */
   int ival[4];
   Mav1BlockInfo blk;
@@ -392,15 +393,19 @@ NOINL int tx_eval(const TxCtx &c, const Mav1FrameParams &fp, const uint8_t *__re
   const int lw = d_uni(Tx_Width_Log2[txSz]), lh = d_uni(Tx_Height_Log2[txSz]), w = 1 << lw, h = 1 << lh;
   const int dcQ = fp.dc_q[plane], acQ = fp.ac_q[plane];
   tx_residual(c, spx, px, S, x, y, lw, lh, t, resid, rs);
+  /* transforms up to 16x16: the scans staged in LDS, addressed as LDS; the
+   * one-wave path takes the eob from the quantiser (the largest inverse-scan
+   * position of a non-zero level, plus one: tx_eob's value) */
+  const int16_t *scan = SMALL ? gL.scans + scan_kind(t) * SCAN_SMALL + small_scan_off(txSz) : get_scan(txSz, t);
+  int eob;
   if constexpr (SMALL) {
-    tx_forward_quant_small(c, resid, rs, tmp, q, txSz, t, gL.smats, dcQ, acQ, gL.qrcp[plane][0], gL.qrcp[plane][1]);
+    eob = d_uni(tx_forward_quant_small(c, resid, rs, tmp, q, txSz, t, gL.smats, dcQ, acQ, gL.qrcp[plane][0],
+                                       gL.qrcp[plane][1], gL.iscans + scan_kind(t) * SCAN_SMALL + small_scan_off(txSz)));
   } else {
     tx_forward<SMALL>(c, resid, rs, tmp, coef, txSz, t, mats, gL.smats);
     tx_quantize(c, coef, q, txSz, dcQ, acQ, gL.qrcp[plane][0], gL.qrcp[plane][1]);
+    eob = d_uni(tx_eob(c, q, scan, d_uni(seg_eob(txSz))));
   }
-  /* transforms up to 16x16: the scan staged in LDS, addressed as LDS */
-  const int16_t *scan = SMALL ? gL.scans + scan_kind(t) * SCAN_SMALL + small_scan_off(txSz) : get_scan(txSz, t);
-  const int eob = d_uni(tx_eob(c, q, scan, d_uni(seg_eob(txSz))));
   if (direct) {
     *costOut = 0;
     return eob;
@@ -1366,7 +1371,9 @@ DEVI void stage_smats(const Mav1FrameParams &fp, const int32_t *__restrict__ mat
     const int kind = q / SCAN_SMALL, r = q - kind * SCAN_SMALL;
     int k = 8;
     while (small_scan_off(sizes[k]) > r) k--;
-    gL.scans[q] = scan_of(sizes[k], kind)[r - small_scan_off(sizes[k])];
+    const int off = small_scan_off(sizes[k]), v = scan_of(sizes[k], kind)[r - off];
+    gL.scans[q] = v;
+    gL.iscans[kind * SCAN_SMALL + off + v] = r - off;
   }
   if (threadIdx.x < 6) {
     const int p = threadIdx.x >> 1;
