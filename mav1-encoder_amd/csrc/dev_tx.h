@@ -629,14 +629,14 @@ DEVI int tx_qdiv(int t, int q, float rq) {
 }
 
 /* One coefficient v through the dead-zone quantiser (k_fwd.h:66-90): the
- * signed level. Coefficients are below 2^21 in magnitude, so t stays below
- * 2^19; larger ones take an integer division. */
+ * signed level. As in the reference, t below 2^19 (every coefficient of an
+ * 8-bit residual) takes the reciprocal, larger t an integer division. */
 DEVI int tx_quant1(int v, bool dc, int dcQ, int acQ, float rDc, float rAc, int denom) {
   const int qv = dc ? dcQ : acQ, rnd = dc ? 26 : 22, av = d_abs(v);
   int level;
   if (av < (1 << 21)) {
     const int t = (av * denom + ((qv * rnd) >> 2)) >> 4;
-    level = tx_qdiv(t, qv, dc ? rDc : rAc);
+    level = t < (1 << 19) ? tx_qdiv(t, qv, dc ? rDc : rAc) : t / qv;
   } else {
     const long long t = ((long long)av * denom + ((qv * rnd) >> 2)) >> 4;
     level = t / qv > 0xFFFF ? 0xFFFF : (int)(t / qv);
