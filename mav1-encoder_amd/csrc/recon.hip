@@ -1851,8 +1851,19 @@ DEVI void lvl_dataflow(const Mav1FrameParams &fp, int32_t *__restrict__ coefs, c
       long long cost;
       const int eob = tx_eval<true>(c, fp, g.spx, g.px, g.S, g.sx, g.sy, g.txSz, 0, DCT_DCT, true, W.resid, SRS, W.tmp,
                                     W.coef, W.q, W.dq, SQS, mats, &cost);
-      lvl_luma_out(fp, b, W.q, DCT_DCT, eob, coefs, W);
+      /* the reconstruction first: the blocks waiting for this one read only
+       * its pixels; then the levels out and the block's transform fields
+       * (lvl_luma_out's work, in that order) */
+      if (eob > 0) tx_recon_small(c, W.q, W.resid, SRS, g.txSz, DCT_DCT, fp.dc_q[0], fp.ac_q[0], g.px, g.S, g.sx, g.sy);
       lvl_publish(&L.ldm, i);
+      const int tw = d_min(32, 1 << g.lw), th = d_min(32, 1 << g.lh);
+      int32_t *co = coefs + d_uni(b.coef_off[0]);
+      for (int k = c.tid; k < tw * th; k += 64) co[k] = W.q[k];
+      if (lane == 0) {
+        b.tx_types_lo = (int32_t)(((uint32_t)b.tx_types_lo & ~15u) | (uint32_t)DCT_DCT);
+        b.tx_type = DCT_DCT;
+        b.eob[0] = eob > 0;
+      }
     }
   } else {
     for (int ci = wave - LVL_LW; ci < 2 * pn; ci += RW - LVL_LW) {
