@@ -1967,32 +1967,15 @@ DEVI int intra_sb(const Mav1FrameParams &fp, Mav1BlockInfo *__restrict__ blocks,
   RLds &L = gL;
   const int tid = threadIdx.x;
   const int c0 = 0, cl = c;
-  /* the three planes' tiles as one index space, every thread's loads issued
-   * before its LDS stores (one memory round trip instead of one per row
-   * band): this load sits on the chain of dependent superblocks */
-  {
-    constexpr int N0 = TILE_Y / 4, N1 = TILE_C / 4, NT = N0 + 2 * N1, PER = (NT + RT - 1) / RT;
-    uint32_t v[PER];
-    int dst[PER];
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-      const int g = tid + k * RT;
-      dst[k] = -1;
-      if (g < NT) {
-        const int p = g < N0 ? 0 : g < N0 + N1 ? 1 : 2, q = g - (p == 0 ? 0 : p == 1 ? N0 : N0 + N1);
-        const int sub = p > 0, n = 64 >> sub, wpr = (sub ? TS_C : TS_Y) / 4;
-        const int ty = q / wpr, tx = (q - ty * wpr) * 4;
-        const int Y = ((r * 64) >> sub) - 1 + ty, X = (((c0 + cl) * 64) >> sub) - 4 + tx, stride = fp.stride[p];
-        if (!((ty > 0 && tx >= 4 + n) || Y < 0 || X < 0 || X >= stride)) {
-          v[k] = *(const uint32_t *)(rec + fp.offset[p] + Y * stride + X);
-          dst[k] = g;
-        }
-      }
+  for (int p = 0; p < 3; p++) {
+    const int sub = p > 0, TS = sub ? TS_C : TS_Y, n = 64 >> sub, wpr = TS / 4;
+    const int oy = ((r * 64) >> sub) - 1, ox = (((c0 + cl) * 64) >> sub) - 4, stride = fp.stride[p];
+    uint32_t *tr = (uint32_t *)(L.tile[0] + (p == 0 ? 0 : TILE_Y + (p - 1) * TILE_C));
+    for (int q = tid; q < (n + 1) * wpr; q += RT) {
+      const int ty = q / wpr, tx = (q - ty * wpr) * 4, Y = oy + ty, X = ox + tx;
+      if ((ty > 0 && tx >= 4 + n) || Y < 0 || X < 0 || X >= stride) continue;
+      tr[q] = *(const uint32_t *)(rec + fp.offset[p] + Y * stride + X);
     }
-    uint32_t *tr = (uint32_t *)L.tile[0];
-#pragma unroll
-    for (int k = 0; k < PER; k++)
-      if (dst[k] >= 0) tr[dst[k]] = v[k];
   }
   const int nb = S.num_blocks;
   __syncthreads();
