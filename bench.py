@@ -106,11 +106,11 @@ BH = [4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 64, 32, 64, 128, 64, 128, 16, 4, 32, 8,
 # default run, from the FETCH_SIZE and WRITE_SIZE passes of
 # profiles/recon_intra_1080p_turbo_stream_pmc.md, scaled by the known-size
 # copy's factors there (fetch x1.9992, write x1.0).
-TRAFFIC = {"recon_intra": 4875125}
+TRAFFIC = {"recon_intra": 4950880}
 # VALUBusy (%) per launch over the timed stream, from the counter pass of the
 # same summary (100 * SQ_ACTIVE_INST_VALU / CU_NUM / GRBM_GUI_ACTIVE).
-VALU_BUSY = {"recon_intra": 0.07, "lr_fit": 6.17, "recon_inter": 2.20, "me_level0": 2.46, "me_level2": 5.62,
-             "me_level3": 11.46, "analyze_level0": 3.24, "qpel_build": 6.58, "cdef": 7.43}
+VALU_BUSY = {"recon_intra": 0.07, "lr_fit": 6.19, "recon_inter": 2.22, "me_level0": 2.46, "me_level2": 5.62,
+             "me_level3": 11.50, "analyze_level0": 3.22, "qpel_build": 6.60, "cdef": 7.44}
 
 
 def intra_phase_bytes(fp, blocks, sbs):
